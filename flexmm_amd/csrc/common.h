@@ -1,0 +1,162 @@
+// Shared host/device helpers for the flexmm_amd HIP library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+namespace fmm {
+
+// ---------------------------------------------------------------------------
+// Error handling: every C-ABI entry point returns 0 on success, a negative
+// FMM_E_* code for an argument error and a positive hipError_t for a runtime
+// failure; the message is kept per thread and read by fmm2d_last_error().
+enum {
+  FMM_OK = 0,
+  FMM_E_ARG = -1,       // invalid argument (null pointer, bad size ...)
+  FMM_E_STATE = -2,     // call out of order (no tree, no expansions ...)
+  FMM_E_LIMIT = -3,     // a documented limit was exceeded
+  FMM_E_INTERNAL = -4,  // internal invariant violated
+};
+
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+void clear_error();
+const char* last_error();
+
+struct Failure {
+  int code;
+};
+
+#define FMM_HIP(expr)                                                             \
+  do {                                                                            \
+    hipError_t e__ = (expr);                                                      \
+    if (e__ != hipSuccess) {                                                      \
+      ::fmm::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e__),    \
+                       __FILE__, __LINE__);                                       \
+      throw ::fmm::Failure{(int)e__};                                             \
+    }                                                                             \
+  } while (0)
+
+#define FMM_REQUIRE(cond, code, ...)                                              \
+  do {                                                                            \
+    if (!(cond)) {                                                                \
+      ::fmm::set_error(__VA_ARGS__);                                              \
+      throw ::fmm::Failure{(code)};                                               \
+    }                                                                             \
+  } while (0)
+
+#define FMM_LAUNCH_CHECK() FMM_HIP(hipGetLastError())
+
+// Device buffer that only grows; reused across calls so that steady-state
+// calls allocate nothing.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  // Growing frees the old block; the device is synchronised first so that no
+  // queued kernel still uses it.
+  void ensure(size_t nbytes) {
+    if (nbytes <= bytes) return;
+    if (ptr) {
+      FMM_HIP(hipDeviceSynchronize());
+      FMM_HIP(hipFree(ptr));
+    }
+    ptr = nullptr;
+    bytes = 0;
+    size_t want = nbytes < 256 ? 256 : nbytes;
+    FMM_HIP(hipMalloc(&ptr, want));
+    bytes = want;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(ptr); }
+  ~DevBuf() { release(); }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
+// Pinned host staging for the few scalars the host must read back.
+struct HostBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t nbytes) {
+    if (nbytes <= bytes) return;
+    if (ptr) FMM_HIP(hipHostFree(ptr));
+    ptr = nullptr;
+    FMM_HIP(hipHostMalloc(&ptr, nbytes, hipHostMallocDefault));
+    bytes = nbytes;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(ptr); }
+  ~HostBuf() {
+    if (ptr) (void)hipHostFree(ptr);
+  }
+  HostBuf() = default;
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+};
+
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------------------
+// Table-driven FP64 logarithm (P. T. P. Tang, "Table-driven implementation of
+// the logarithm function in IEEE floating-point arithmetic", ACM TOMS 16(4),
+// 1990), in the FMA variant: x = m 2^e with m in [1, 2); j = top 7 fraction
+// bits of m; r = fma(m, 1/c_j, -1) with |r| < 2^-8; log(x) = e ln2 - log(1/c_j)
+// + log1p(r), log1p by its Taylor polynomial to r^7.  Table entries are
+// {1/c_j, -log(1/c_j)} with c_j = 1 + (j + 1/2)/128, computed on the host in
+// extended precision.  Accuracy ~2 ulp for normal x; log(0) = -inf.
+constexpr int kLogTableSize = 128;
+
+struct LogTableHost {
+  static void fill(double2* out);  // out[kLogTableSize]
+};
+
+#if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
+__device__ __forceinline__ double log_tab(double x, const double2* __restrict__ tab) {
+  const double LN2_HI = 6.93147180369123816490e-01;  // ln 2 with 32 significant bits
+  const double LN2_LO = 1.90821492927058770002e-10;  // ln 2 - LN2_HI
+  double m = __builtin_amdgcn_frexp_mant(x);         // [0.5, 1)
+  int e = __builtin_amdgcn_frexp_exp(x);
+  m = m + m;                                          // [1, 2)
+  double ed = (double)(e - 1);
+  int hi = __double2hiint(m);
+  int j = (hi >> 13) & (kLogTableSize - 1);
+  double2 t = tab[j];
+  double r = __builtin_fma(m, t.x, -1.0);
+  double p = __builtin_fma(r, 1.0 / 7.0, -1.0 / 6.0);
+  p = __builtin_fma(r, p, 1.0 / 5.0);
+  p = __builtin_fma(r, p, -1.0 / 4.0);
+  p = __builtin_fma(r, p, 1.0 / 3.0);
+  p = __builtin_fma(r, p, -0.5);
+  double r2 = r * r;
+  double lp = __builtin_fma(r2, p, r);
+  double res = __builtin_fma(ed, LN2_HI, t.y) + __builtin_fma(ed, LN2_LO, lp);
+  return x == 0.0 ? -__builtin_inf() : res;
+}
+#endif
+
+// ---------------------------------------------------------------------------
+// Kernel identifiers shared with the Python registry.
+enum KernelId : int {
+  KER_LAPLACE = 0,   // -log(d^2)/(4 pi)
+  KER_MODHELM = 1,   // K0(k r)/(2 pi)
+};
+enum EvalId : int {
+  EVAL_POT = 0,      // 1 output: u
+  EVAL_GRAD = 1,     // 3 outputs: u, du/dtx, du/dty
+};
+enum Precision : int {
+  PREC_FP64 = 0,
+  PREC_MIXED = 1,
+};
+
+}  // namespace fmm

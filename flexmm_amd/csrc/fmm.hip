@@ -1,0 +1,941 @@
+// KIFMM passes on the device and the C-ABI of libflexmm_amd.
+//
+//   upward    S2M (check potential of leaf sources on the large circle) and
+//             M2M (children's equivalent densities to the parent's check
+//             surface), then the pseudo-inverse apply q = V diag(1/s) U^T b;
+//             fmm.py:73-124, helpers.py:159-173, 240-248.
+//   downward  L2L (parent's local density to the child's check surface), the
+//             M2L translations of the 40-offset ring, then the same
+//             pseudo-inverse; fmm.py:125-166.
+//   evaluate  target location, targets grouped by leaf, L2T + P2P fused in one
+//             pass; fmm.py:182-199, helpers.py:180-235.
+// Dense contractions run on FP64 MFMA (v_mfma_f64_16x16x4f64) with the
+// operator block staged in LDS; expansions are rows of PD doubles (PD = 32 or
+// 64, p padded with zeros).
+#include "tree.h"
+
+#include <cstdarg>
+#include <cstring>
+#include <exception>
+
+#include "../../include/flexmm_amd.h"
+
+namespace fmm {
+
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+void clear_error() { g_err.clear(); }
+const char* last_error() { return g_err.c_str(); }
+
+void LogTableHost::fill(double2* out) {
+  for (int j = 0; j < kLogTableSize; ++j) {
+    long double c = 1.0L + ((long double)j + 0.5L) / kLogTableSize;
+    double inv = (double)(1.0L / c);
+    long double lg = -logl((long double)inv);
+    out[j] = make_double2(inv, (double)lg);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Operator layout per level: OP_NBLK blocks of [PD][PD] doubles (row-major,
+// zero padded), then the large-circle coordinates lx[PD], ly[PD].
+enum {
+  OP_PINV1 = 0,   // diag(1/s) U^T
+  OP_PINV2 = 1,   // V
+  OP_M2M = 2,     // 4 blocks: M2MC[:, c p:(c+1) p]
+  OP_L2L = 6,     // 4 blocks: M2MC[:, c p:(c+1) p]^T
+  OP_M2L = 10,    // 40 blocks in slot order
+  OP_NBLK = 50,
+};
+inline int64_t ops_per_level(int PD) { return (int64_t)OP_NBLK * PD * PD + 2 * PD; }
+
+constexpr double kInv4Pi = 0.07957747154594767;   // 1/(4 pi)
+constexpr double kInv2Pi = 0.15915494309189535;   // 1/(2 pi)
+constexpr int LDA = 68;   // LDS row stride of an A block: 68 = 4 (mod 32) spreads an MFMA A
+                          // fragment (16 rows x 4 columns) over distinct LDS banks
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+struct SlotTable {
+  int ox[kNumM2L], oy[kNumM2L];
+};
+__constant__ SlotTable c_slots;
+
+struct Plan {
+  int kernel_id = KER_LAPLACE;
+  double kparam = 0.0;
+  int p = 0, PD = 0, precision = PREC_FP64;
+  Tree tree;
+  DevBuf ops;
+  int ops_levels = 0;
+  std::vector<char> ops_set;
+  DevBuf logtab;
+  DevBuf tau_s, check, mult, le;
+  bool have_exp = false;
+  // targets
+  DevBuf tnode, tcnt, toff, tcur, tperm, tsx, tsy, icnt, ioff, items, ntot;
+  PrimWork pw;
+  HostBuf hbuf;
+  int device = 0;
+  // optional phase timing with HIP events on the caller's stream
+  bool profile = false;
+  hipEvent_t ev[2 * FMM2D_NUM_PHASES] = {};
+  bool ev_used[FMM2D_NUM_PHASES] = {};
+  ~Plan() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  void mark(int phase, bool end, hipStream_t s) {
+    if (!profile) return;
+    hipEvent_t& e = ev[2 * phase + (end ? 1 : 0)];
+    if (!e) FMM_HIP(hipEventCreate(&e));
+    FMM_HIP(hipEventRecord(e, s));
+    if (end) ev_used[phase] = true;
+  }
+};
+
+namespace {
+
+__device__ __forceinline__ void load_logtab(double2* tab, const double2* __restrict__ g) {
+  for (int i = threadIdx.x; i < kLogTableSize; i += blockDim.x) tab[i] = g[i];
+}
+
+// ---------------------------------------------------------------------------
+// S2M: one wave per leaf, lane k = check point k on the large circle.
+template <int PD>
+__global__ __launch_bounds__(64) void k_s2m(const int* __restrict__ upnodes,
+                                            const int* __restrict__ nlevel,
+                                            const int* __restrict__ bot, const int* __restrict__ top,
+                                            const int* __restrict__ row,
+                                            const double* __restrict__ xmid,
+                                            const double* __restrict__ ymid,
+                                            const double* __restrict__ sx,
+                                            const double* __restrict__ sy,
+                                            const double* __restrict__ tau,
+                                            const double* __restrict__ ops, int64_t opsz, int p,
+                                            const double2* __restrict__ logtab,
+                                            double* __restrict__ check) {
+  __shared__ double2 tab[kLogTableSize];
+  load_logtab(tab, logtab);
+  __syncthreads();
+  const int g = upnodes[blockIdx.x];
+  const int d = nlevel[g];
+  const int k = threadIdx.x;
+  const int r = row[g];
+  if (r < 0 || d < 2) return;   // block-uniform
+  const double* lx = ops + d * opsz + (int64_t)OP_NBLK * PD * PD;
+  const double* ly = lx + PD;
+  const double ax = k < p ? lx[k] : 1.0;
+  const double ay = k < p ? ly[k] : 1.0;
+  const double cx = xmid[g], cy = ymid[g];
+  const int b = __builtin_amdgcn_readfirstlane(bot[g]);
+  const int t = __builtin_amdgcn_readfirstlane(top[g]);
+  double acc = 0.0;
+  for (int j = b; j < t; ++j) {
+    const double dx = ax - (sx[j] - cx);   // source relative to the box centre, helpers.py:168-169
+    const double dy = ay - (sy[j] - cy);
+    const double d2 = __builtin_fma(dy, dy, dx * dx);
+    acc = __builtin_fma(tau[j], log_tab(d2, tab), acc);
+  }
+  if (k < PD) check[(int64_t)r * PD + k] = k < p ? -kInv4Pi * acc : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// MFMA helpers.  For v_mfma_f64_16x16x4f64 lane l holds A[l%16][l/16],
+// B[l/16][l%16] and D[4v + l/16][l%16] (v = 0..3).  A PD-row result therefore
+// sits in lane l as rows 4m + l/16 (m = 4 rb + v), which is exactly the
+// B-operand layout of the next product: results chain without shuffles.
+template <int PD>
+__device__ __forceinline__ void lds_load_A(double* __restrict__ A, const double* __restrict__ G) {
+  for (int idx = threadIdx.x; idx < PD * PD; idx += blockDim.x) {
+    int r = idx / PD;
+    int c = idx - r * PD;
+    A[r * LDA + c] = G[idx];
+  }
+}
+
+template <int PD>
+__device__ __forceinline__ void mfma_block(d4 (&acc)[PD / 16], const double* __restrict__ A,
+                                           const double (&b)[PD / 4]) {
+  constexpr int RB = PD / 16, KS = PD / 4;
+  const int l = threadIdx.x & 63;
+  const double* a0 = A + (l & 15) * LDA + (l >> 4);
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+      acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[rb * 16 * LDA + ks * 4], b[ks], acc[rb], 0,
+                                                     0, 0);
+}
+
+template <int PD>
+__device__ __forceinline__ void zero_acc(d4 (&acc)[PD / 16]) {
+#pragma unroll
+  for (int rb = 0; rb < PD / 16; ++rb) acc[rb] = d4{0.0, 0.0, 0.0, 0.0};
+}
+
+// acc <- pinv2 * (pinv1 * acc), both operators staged through A.
+template <int PD>
+__device__ __forceinline__ void apply_pinv(d4 (&acc)[PD / 16], double* __restrict__ A,
+                                           const double* __restrict__ opsL) {
+  constexpr int RB = PD / 16, KS = PD / 4;
+#pragma unroll
+  for (int ph = 0; ph < 2; ++ph) {
+    double b[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) b[ks] = acc[ks >> 2][ks & 3];
+    __syncthreads();
+    lds_load_A<PD>(A, opsL + (int64_t)(OP_PINV1 + ph) * PD * PD);
+    __syncthreads();
+    zero_acc<PD>(acc);
+    mfma_block<PD>(acc, A, b);
+  }
+  (void)RB;
+}
+
+// ---------------------------------------------------------------------------
+// Upward, one level: rows [row0, row0 + nrows); 64 rows per block, 16 per wave.
+template <int PD>
+__global__ __launch_bounds__(256) void k_upward(int64_t row0, int nrows, int with_m2m,
+                                               const int* __restrict__ rownode,
+                                               const int* __restrict__ flags,
+                                               const int* __restrict__ child,
+                                               const int* __restrict__ row,
+                                               const double* __restrict__ check,
+                                               const double* __restrict__ opsL,
+                                               double* __restrict__ mult) {
+  __shared__ double A[PD * LDA];
+  constexpr int RB = PD / 16, KS = PD / 4;
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int col = l & 15, kq = l >> 4;
+  const int rl = blockIdx.x * 64 + w * 16 + col;
+  const bool active = rl < nrows;
+  const int64_t r = row0 + rl;
+  const int g = active ? rownode[r] : 0;
+  const int f = active ? flags[g] : 0;
+  int crow = -1;
+  if (active && (f & NF_CHILDREN)) crow = row[child[g]];
+  d4 acc[RB];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      acc[rb][v] = (active && (f & NF_UP)) ? check[r * PD + 16 * rb + 4 * v + kq] : 0.0;
+  if (with_m2m) {
+    for (int c = 0; c < 4; ++c) {
+      __syncthreads();
+      lds_load_A<PD>(A, opsL + (int64_t)(OP_M2M + c) * PD * PD);
+      __syncthreads();
+      double b[KS];
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks)
+        b[ks] = crow >= 0 ? mult[(int64_t)(crow + c) * PD + 4 * ks + kq] : 0.0;
+      mfma_block<PD>(acc, A, b);
+    }
+  }
+  apply_pinv<PD>(acc, A, opsL);
+  if (active) {
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) mult[r * PD + 16 * rb + 4 * v + kq] = acc[rb][v];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Downward, one level.  Blocks are tiled by child position c so that all
+// columns of a block share the 27 M2L offsets that can occur for c; block
+// (tile, c) handles the c-th child of sibling groups [64 tile, 64 tile + 64).
+template <int PD>
+__global__ __launch_bounds__(256) void k_downward(int g0, int ngroups, int do_l2l,
+                                                 const int* __restrict__ flags,
+                                                 const int* __restrict__ parent,
+                                                 const int* __restrict__ m2l,
+                                                 const double* __restrict__ mult,
+                                                 const double* __restrict__ opsL,
+                                                 const double* __restrict__ opsParent,
+                                                 double* __restrict__ le) {
+  __shared__ double A[PD * LDA];
+  constexpr int RB = PD / 16, KS = PD / 4;
+  const int c = blockIdx.x & 3;
+  const int tile = blockIdx.x >> 2;
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int col = l & 15, kq = l >> 4;
+  const int q = tile * 64 + w * 16 + col;
+  const int g = g0 + 4 * q + c;
+  const bool active = q < ngroups && !(flags[q < ngroups ? g : g0] & NF_FAKE);
+  d4 acc[RB];
+  zero_acc<PD>(acc);
+  if (do_l2l) {
+    __syncthreads();
+    // L2LC of the parent's width (fmm.py:142,162)
+    lds_load_A<PD>(A, opsParent + (int64_t)(OP_L2L + c) * PD * PD);
+    __syncthreads();
+    const int pg = active ? parent[g] : -1;
+    double b[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) b[ks] = pg >= 0 ? le[(int64_t)pg * PD + 4 * ks + kq] : 0.0;
+    mfma_block<PD>(acc, A, b);
+  }
+  const int cx = c >> 1, cy = c & 1;
+  const int* lst = m2l + (int64_t)g * kNumM2L;
+  for (int s = 0; s < kNumM2L; ++s) {
+    const int ox = c_slots.ox[s], oy = c_slots.oy[s];
+    if (ox < -2 - cx || ox > 3 - cx || oy < -2 - cy || oy > 3 - cy) continue;   // block-uniform
+    const int src = active ? lst[s] : -1;
+    __syncthreads();
+    lds_load_A<PD>(A, opsL + (int64_t)(OP_M2L + s) * PD * PD);
+    __syncthreads();
+    if (__ballot(src >= 0) == 0) continue;   // wave-uniform
+    double b[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) b[ks] = src >= 0 ? mult[(int64_t)src * PD + 4 * ks + kq] : 0.0;
+    mfma_block<PD>(acc, A, b);
+  }
+  apply_pinv<PD>(acc, A, opsL);
+  if (active) {
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) le[(int64_t)g * PD + 16 * rb + 4 * v + kq] = acc[rb][v];
+  }
+}
+
+__global__ void k_gather_tau(const double* __restrict__ tau, const uint32_t* __restrict__ perm,
+                             int64_t n, double* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = tau[perm[i]];
+}
+
+// ---------------------------------------------------------------------------
+// Targets: locate, count per evaluation box, scatter into box order.
+__global__ __launch_bounds__(256) void k_locate_count(const double* __restrict__ tx,
+                                                     const double* __restrict__ ty, int64_t nt,
+                                                     const int* __restrict__ flags,
+                                                     const int* __restrict__ child,
+                                                     const double* __restrict__ xmid,
+                                                     const double* __restrict__ ymid,
+                                                     const int* __restrict__ evalid,
+                                                     int* __restrict__ tbox, int* __restrict__ tcnt) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nt) return;
+  const double x = tx[i], y = ty[i];
+  int g = 0;
+  for (int it = 0; it < kMaxLevels && !(flags[g] & NF_LEAF); ++it)
+    g = child[g] + 2 * (x > xmid[g]) + (y > ymid[g]);   // tree.py:720-725
+  const int e = evalid[g];
+  tbox[i] = e;
+  atomicAdd(&tcnt[e], 1);
+}
+
+__global__ __launch_bounds__(256) void k_target_scatter(const double* __restrict__ tx,
+                                                       const double* __restrict__ ty, int64_t nt,
+                                                       const int* __restrict__ tbox,
+                                                       const int* __restrict__ toff,
+                                                       int* __restrict__ tcur,
+                                                       int* __restrict__ tperm,
+                                                       double* __restrict__ tsx,
+                                                       double* __restrict__ tsy) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nt) return;
+  const int e = tbox[i];
+  const int pos = toff[e] + atomicAdd(&tcur[e], 1);
+  tperm[pos] = (int)i;
+  tsx[pos] = tx[i];
+  tsy[pos] = ty[i];
+}
+
+constexpr int kTgtChunk = 64;
+
+__global__ __launch_bounds__(256) void k_item_counts(const int* __restrict__ tcnt, int ne,
+                                                    int* __restrict__ icnt) {
+  int b = blockIdx.x * 256 + threadIdx.x;
+  if (b < ne) icnt[b] = (tcnt[b] + kTgtChunk - 1) / kTgtChunk;
+}
+
+__global__ __launch_bounds__(256) void k_make_items(const int* __restrict__ icnt,
+                                                   const int* __restrict__ ioff, int ne,
+                                                   int2* __restrict__ items) {
+  int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= ne) return;
+  const int n = icnt[b], o = ioff[b];
+  for (int k = 0; k < n; ++k) items[o + k] = make_int2(b, k);
+}
+
+// ---------------------------------------------------------------------------
+// L2T + P2P, one wave per (box, chunk of 64 targets); lane = target.
+template <int PD, int NOUT>
+__global__ __launch_bounds__(64) void k_eval(
+    const int2* __restrict__ items, const int* __restrict__ nitems,
+    const int* __restrict__ evalnode, const int* __restrict__ toff,
+    const int* __restrict__ tperm, const double* __restrict__ tsx, const double* __restrict__ tsy,
+    const int* __restrict__ nlevel, const double* __restrict__ xmid,
+    const double* __restrict__ ymid, const int* __restrict__ coll, const int* __restrict__ bot,
+    const int* __restrict__ top, const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ tau, const double* __restrict__ le,
+    const double* __restrict__ ops, int64_t opsz, int p, const double2* __restrict__ logtab,
+    int check, int64_t nt, double* __restrict__ out) {
+  __shared__ double2 tab[kLogTableSize];
+  if ((int)blockIdx.x >= *nitems) return;
+  load_logtab(tab, logtab);
+  __syncthreads();
+  const int2 it = items[blockIdx.x];
+  const int g = evalnode[it.x];
+  const int t0 = toff[it.x] + it.y * kTgtChunk;
+  const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
+  const int t = t0 + threadIdx.x;
+  const bool act = t < t1;
+  const double x = act ? tsx[t] : 0.0;
+  const double y = act ? tsy[t] : 0.0;
+  double u = 0.0, ux = 0.0, uy = 0.0;
+  // L2T: helpers.py:219-235 (the equivalent density sits on the large circle)
+  const int d = nlevel[g];
+  if (d >= 2) {
+    const double xr = x - xmid[g];
+    const double yr = y - ymid[g];
+    const double* lx = ops + d * opsz + (int64_t)OP_NBLK * PD * PD;
+    const double* ly = lx + PD;
+    const double* lg = le + (int64_t)g * PD;
+    for (int k = 0; k < p; ++k) {
+      const double dx = xr - lx[k];
+      const double dy = yr - ly[k];
+      const double d2 = __builtin_fma(dy, dy, dx * dx);
+      const double q = lg[k];
+      u = __builtin_fma(q, log_tab(d2, tab), u);
+      if (NOUT == 3) {
+        const double qi = q / d2;
+        ux = __builtin_fma(qi, dx, ux);
+        uy = __builtin_fma(qi, dy, uy);
+      }
+    }
+  }
+  // P2P over the leaf's colleagues: helpers.py:180-217
+  const int* cl = coll + (int64_t)g * 9;
+  for (int qn = 0; qn < 9; ++qn) {
+    const int c = cl[qn];
+    if (c < 0) continue;
+    const int b = __builtin_amdgcn_readfirstlane(bot[c]);
+    const int e = __builtin_amdgcn_readfirstlane(top[c]);
+    for (int j = b; j < e; ++j) {
+      const double sxj = sx[j], syj = sy[j], tj = tau[j];
+      const double dx = x - sxj;
+      const double dy = y - syj;
+      const double d2 = __builtin_fma(dy, dy, dx * dx);
+      const bool skip = check && sxj == x && syj == y;
+      const double lg2 = log_tab(d2, tab);
+      u = skip ? u : __builtin_fma(tj, lg2, u);
+      if (NOUT == 3) {
+        const double qi = skip ? 0.0 : tj / d2;
+        ux = __builtin_fma(qi, dx, ux);
+        uy = __builtin_fma(qi, dy, uy);
+      }
+    }
+  }
+  if (act) {
+    const int64_t o = tperm[t];
+    out[o] = -kInv4Pi * u;
+    if (NOUT == 3) {
+      out[nt + o] = -kInv2Pi * ux;
+      out[2 * nt + o] = -kInv2Pi * uy;
+    }
+  }
+}
+
+// Algorithmic work counts (for the roofline accounting).
+__global__ __launch_bounds__(256) void k_eval_stats(const int* __restrict__ evalnode, int ne,
+                                                   const int* __restrict__ tcnt,
+                                                   const int* __restrict__ coll,
+                                                   const int* __restrict__ bot,
+                                                   const int* __restrict__ top,
+                                                   const int* __restrict__ nlevel,
+                                                   unsigned long long* __restrict__ out) {
+  int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= ne) return;
+  const int c = tcnt[b];
+  if (c == 0) return;
+  const int g = evalnode[b];
+  long long ns = 0;
+  for (int q = 0; q < 9; ++q) {
+    int cc = coll[(int64_t)g * 9 + q];
+    if (cc >= 0) ns += top[cc] - bot[cc];
+  }
+  atomicAdd(&out[0], (unsigned long long)((long long)c * ns));
+  if (nlevel[g] >= 2) atomicAdd(&out[1], (unsigned long long)c);
+  atomicAdd(&out[2], (unsigned long long)c);
+}
+
+__global__ __launch_bounds__(256) void k_tree_stats(int NN, const int* __restrict__ nlevel,
+                                                   const int* __restrict__ flags,
+                                                   const int* __restrict__ m2l,
+                                                   const int* __restrict__ row,
+                                                   const int* __restrict__ child,
+                                                   const int* __restrict__ bot,
+                                                   const int* __restrict__ top,
+                                                   unsigned long long* __restrict__ out) {
+  int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= NN) return;
+  const int d = nlevel[g], f = flags[g];
+  if (d < 2) return;
+  if (!(f & NF_FAKE)) {
+    int np = 0;
+    for (int s = 0; s < kNumM2L; ++s) np += m2l[(int64_t)g * kNumM2L + s] >= 0;
+    atomicAdd(&out[3], (unsigned long long)np);
+    atomicAdd(&out[6], 1ull);
+    if (d >= 3) atomicAdd(&out[8], 1ull);
+  }
+  if ((f & NF_UP) && row[g] >= 0) atomicAdd(&out[4], (unsigned long long)(top[g] - bot[g]));
+  if ((f & NF_CHILDREN) && row[g] >= 0 && row[child[g]] >= 0) atomicAdd(&out[7], 1ull);
+}
+
+inline unsigned grid256(int64_t n) { return (unsigned)(n <= 0 ? 1 : ceil_div(n, 256)); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side of the passes.
+static void check_ops(const Plan& P) {
+  const Tree& t = P.tree;
+  FMM_REQUIRE(t.built, FMM_E_STATE, "no tree: call fmm2d_tree_build first");
+  for (int d = 2; d < t.L; ++d)
+    FMM_REQUIRE(d < P.ops_levels && P.ops_set[d], FMM_E_STATE,
+                "operators for level %d not set (fmm2d_set_operators)", d);
+}
+
+static void upward(Plan& P, const double* tau, hipStream_t s) {
+  Tree& t = P.tree;
+  check_ops(P);
+  const int PD = P.PD;
+  const int64_t opsz = ops_per_level(PD);
+  P.tau_s.ensure(t.n * sizeof(double));
+  hipLaunchKernelGGL(k_gather_tau, dim3(grid256(t.n)), dim3(256), 0, s, tau, t.perm.as<uint32_t>(),
+                     t.n, P.tau_s.as<double>());
+  FMM_LAUNCH_CHECK();
+  const int64_t nr = t.n_rows;
+  P.check.ensure((nr + 1) * PD * sizeof(double));
+  P.mult.ensure((nr + 1) * PD * sizeof(double));
+  if (t.L <= 2 || nr == 0) {
+    P.have_exp = true;
+    return;
+  }
+  P.mark(FMM2D_PHASE_S2M, false, s);
+  if (t.n_up > 0) {
+    auto kern = PD == 32 ? k_s2m<32> : k_s2m<64>;
+    hipLaunchKernelGGL(kern, dim3(t.n_up), dim3(64), 0, s, t.upnodes.as<int>(), t.nlevel.as<int>(),
+                       t.bot.as<int>(), t.top.as<int>(), t.row.as<int>(), t.xmid.as<double>(),
+                       t.ymid.as<double>(), t.sx.as<double>(), t.sy.as<double>(),
+                       P.tau_s.as<double>(), P.ops.as<double>(), opsz, P.p,
+                       P.logtab.as<double2>(), P.check.as<double>());
+    FMM_LAUNCH_CHECK();
+  }
+  P.mark(FMM2D_PHASE_S2M, true, s);
+  for (int d = t.L - 1; d >= 2; --d) {
+    const int64_t r0 = t.row_off[d];
+    const int nrows = (int)(t.row_off[d + 1] - r0);
+    if (nrows == 0) continue;
+    const double* opsL = P.ops.as<double>() + d * opsz;
+    auto kern = PD == 32 ? k_upward<32> : k_upward<64>;
+    hipLaunchKernelGGL(kern, dim3(ceil_div(nrows, 64)), dim3(256), 0, s, r0, nrows,
+                       d < t.L - 1 ? 1 : 0, t.rownode.as<int>(), t.flags.as<int>(),
+                       t.child.as<int>(), t.row.as<int>(), P.check.as<double>(), opsL,
+                       P.mult.as<double>());
+    FMM_LAUNCH_CHECK();
+  }
+  P.have_exp = false;
+}
+
+static void downward(Plan& P, hipStream_t s) {
+  Tree& t = P.tree;
+  check_ops(P);
+  const int PD = P.PD;
+  const int64_t opsz = ops_per_level(PD);
+  P.le.ensure((t.n_nodes + 1) * PD * sizeof(double));
+  FMM_HIP(hipMemsetAsync(P.le.ptr, 0, (t.n_nodes + 1) * PD * sizeof(double), s));
+  for (int d = 2; d < t.L; ++d) {
+    const int ngroups = (int)(t.lev[d].n_nodes / 4);
+    if (ngroups == 0) continue;
+    const double* opsL = P.ops.as<double>() + d * opsz;
+    auto kern = PD == 32 ? k_downward<32> : k_downward<64>;
+    hipLaunchKernelGGL(kern, dim3(4 * ceil_div(ngroups, 64)), dim3(256), 0, s,
+                       (int)t.lev[d].node_off, ngroups, d >= 3 ? 1 : 0, t.flags.as<int>(),
+                       t.parent.as<int>(), t.m2l.as<int>(), P.mult.as<double>(), opsL,
+                       opsL - opsz, P.le.as<double>());
+    FMM_LAUNCH_CHECK();
+  }
+  P.have_exp = true;
+}
+
+static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, int ev, int check,
+                     double* out, hipStream_t s) {
+  Tree& t = P.tree;
+  FMM_REQUIRE(P.have_exp, FMM_E_STATE, "no expansions: call fmm2d_upward and fmm2d_downward first");
+  FMM_REQUIRE(ev == EVAL_POT || ev == EVAL_GRAD, FMM_E_ARG, "unknown evaluator %d", ev);
+  FMM_REQUIRE(nt >= 0 && nt < ((int64_t)1 << 31), FMM_E_LIMIT, "too many targets");
+  if (nt == 0) return;
+  const int64_t ne = t.n_eval;
+  P.tnode.ensure(nt * sizeof(int));
+  P.tcnt.ensure((ne + 1) * sizeof(int));
+  P.toff.ensure((ne + 1) * sizeof(int));
+  P.tcur.ensure((ne + 1) * sizeof(int));
+  P.tperm.ensure(nt * sizeof(int));
+  P.tsx.ensure(nt * sizeof(double));
+  P.tsy.ensure(nt * sizeof(double));
+  P.icnt.ensure((ne + 1) * sizeof(int));
+  P.ioff.ensure((ne + 1) * sizeof(int));
+  const int64_t icap = ne + nt / kTgtChunk + 1;
+  P.items.ensure(icap * sizeof(int2));
+  P.ntot.ensure(4 * sizeof(int));
+  FMM_HIP(hipMemsetAsync(P.tcnt.ptr, 0, (ne + 1) * sizeof(int), s));
+  FMM_HIP(hipMemsetAsync(P.tcur.ptr, 0, (ne + 1) * sizeof(int), s));
+  hipLaunchKernelGGL(k_locate_count, dim3(grid256(nt)), dim3(256), 0, s, tx, ty, nt,
+                     t.flags.as<int>(), t.child.as<int>(), t.xmid.as<double>(), t.ymid.as<double>(),
+                     t.evalid.as<int>(), P.tnode.as<int>(), P.tcnt.as<int>());
+  FMM_LAUNCH_CHECK();
+  // toff has ne + 1 entries: scanning ne + 1 counts (the last is zero) gives toff[ne] = nt.
+  exclusive_scan_i32(P.tcnt.as<int>(), P.toff.as<int>(), ne + 1, nullptr, P.pw, s);
+  hipLaunchKernelGGL(k_target_scatter, dim3(grid256(nt)), dim3(256), 0, s, tx, ty, nt,
+                     P.tnode.as<int>(), P.toff.as<int>(), P.tcur.as<int>(), P.tperm.as<int>(),
+                     P.tsx.as<double>(), P.tsy.as<double>());
+  FMM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_item_counts, dim3(grid256(ne)), dim3(256), 0, s, P.tcnt.as<int>(), (int)ne,
+                     P.icnt.as<int>());
+  FMM_LAUNCH_CHECK();
+  exclusive_scan_i32(P.icnt.as<int>(), P.ioff.as<int>(), ne, P.ntot.as<int>(), P.pw, s);
+  hipLaunchKernelGGL(k_make_items, dim3(grid256(ne)), dim3(256), 0, s, P.icnt.as<int>(),
+                     P.ioff.as<int>(), (int)ne, P.items.as<int2>());
+  FMM_LAUNCH_CHECK();
+  const int PD = P.PD;
+  const int64_t opsz = ops_per_level(PD);
+  auto kern = ev == EVAL_POT ? (PD == 32 ? k_eval<32, 1> : k_eval<64, 1>)
+                             : (PD == 32 ? k_eval<32, 3> : k_eval<64, 3>);
+  P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
+  hipLaunchKernelGGL(kern, dim3(icap), dim3(64), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
+                     t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(), P.tsx.as<double>(),
+                     P.tsy.as<double>(), t.nlevel.as<int>(), t.xmid.as<double>(),
+                     t.ymid.as<double>(), t.coll.as<int>(), t.bot.as<int>(), t.top.as<int>(),
+                     t.sx.as<double>(), t.sy.as<double>(), P.tau_s.as<double>(),
+                     P.le.as<double>(), P.ops.as<double>(), opsz, P.p, P.logtab.as<double2>(),
+                     check, nt, out);
+  FMM_LAUNCH_CHECK();
+  P.mark(FMM2D_PHASE_EVAL_KERNEL, true, s);
+}
+
+}  // namespace fmm
+
+// ===========================================================================
+// C-ABI
+using fmm::Plan;
+
+#define ABI_BEGIN        \
+  fmm::clear_error();    \
+  try {
+#define ABI_END                                                   \
+  }                                                               \
+  catch (const fmm::Failure& f) {                                 \
+    return f.code;                                                \
+  }                                                               \
+  catch (const std::exception& e) {                               \
+    fmm::set_error("internal error: %s", e.what());               \
+    return fmm::FMM_E_INTERNAL;                                   \
+  }                                                               \
+  catch (...) {                                                   \
+    fmm::set_error("internal error");                             \
+    return fmm::FMM_E_INTERNAL;                                   \
+  }                                                               \
+  return fmm::FMM_OK;
+
+#define NEED(ptr) FMM_REQUIRE((ptr) != nullptr, fmm::FMM_E_ARG, "%s is null", #ptr)
+
+static Plan* as_plan(void* h) {
+  FMM_REQUIRE(h != nullptr, fmm::FMM_E_ARG, "plan handle is null");
+  return static_cast<Plan*>(h);
+}
+
+extern "C" {
+
+int fmm2d_abi_version(void) { return FMM2D_ABI_VERSION; }
+
+const char* fmm2d_last_error(void) { return fmm::last_error(); }
+
+int fmm2d_plan_create(int kernel_id, const double* kparams, int nkparams, int p, int precision,
+                      void** out_plan) {
+  ABI_BEGIN
+  NEED(out_plan);
+  *out_plan = nullptr;
+  FMM_REQUIRE(kernel_id == fmm::KER_LAPLACE, fmm::FMM_E_ARG, "unknown kernel id %d", kernel_id);
+  FMM_REQUIRE(p >= 4 && p <= 64, fmm::FMM_E_LIMIT, "Nequiv = %d outside [4, 64]", p);
+  FMM_REQUIRE(precision == fmm::PREC_FP64, fmm::FMM_E_ARG, "unknown precision %d", precision);
+  FMM_REQUIRE(nkparams >= 0 && (nkparams == 0 || kparams), fmm::FMM_E_ARG, "bad kernel params");
+  Plan* P = new Plan();
+  try {
+    P->kernel_id = kernel_id;
+    P->kparam = nkparams > 0 ? kparams[0] : 0.0;
+    P->p = p;
+    P->PD = p <= 32 ? 32 : 64;
+    P->precision = precision;
+    FMM_HIP(hipGetDevice(&P->device));
+    fmm::SlotTable st;
+    for (int s = 0; s < fmm::kNumM2L; ++s) fmm::m2l_slot_offset(s, &st.ox[s], &st.oy[s]);
+    FMM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fmm::c_slots), &st, sizeof st));
+    double2 tab[fmm::kLogTableSize];
+    fmm::LogTableHost::fill(tab);
+    P->logtab.ensure(sizeof tab);
+    FMM_HIP(hipMemcpy(P->logtab.ptr, tab, sizeof tab, hipMemcpyHostToDevice));
+  } catch (...) {
+    delete P;
+    throw;
+  }
+  *out_plan = P;
+  ABI_END
+}
+
+int fmm2d_plan_destroy(void* plan) {
+  ABI_BEGIN
+  if (plan) {
+    (void)hipDeviceSynchronize();
+    delete static_cast<Plan*>(plan);
+  }
+  ABI_END
+}
+
+int fmm2d_tree_build(void* plan, const double* x, const double* y, int64_t n, const double* bbox,
+                     int64_t ncut, void* stream) {
+  ABI_BEGIN
+  Plan* P = as_plan(plan);
+  NEED(x);
+  NEED(y);
+  NEED(bbox);
+  FMM_REQUIRE(ncut >= 1 && ncut <= (1 << 30), fmm::FMM_E_ARG, "Ncutoff %lld out of range",
+              (long long)ncut);
+  const double w0 = bbox[1] - bbox[0];   // tree.py:484 (the x extent sets the width)
+  P->have_exp = false;
+  P->mark(FMM2D_PHASE_TREE, false, (hipStream_t)stream);
+  fmm::tree_build(P->tree, x, y, n, bbox[0], bbox[2], w0, (int)ncut, (hipStream_t)stream);
+  P->mark(FMM2D_PHASE_TREE, true, (hipStream_t)stream);
+  ABI_END
+}
+
+int fmm2d_tree_info(void* plan, int64_t* out, int64_t nout) {
+  ABI_BEGIN
+  Plan* P = as_plan(plan);
+  NEED(out);
+  const fmm::Tree& t = P->tree;
+  FMM_REQUIRE(t.built, fmm::FMM_E_STATE, "no tree");
+  const int L = t.L;
+  FMM_REQUIRE(nout >= 9 + 3 * (int64_t)L, fmm::FMM_E_ARG, "tree_info needs %d slots",
+              9 + 3 * L);
+  out[0] = L;
+  out[1] = t.n_nodes;
+  out[2] = t.n_rows;
+  out[3] = t.n_eval;
+  out[4] = t.n_up;
+  out[5] = t.n;
+  out[6] = P->PD;
+  out[7] = t.ncut;
+  for (int d = 0; d < L; ++d) {
+    out[8 + 3 * d] = t.lev[d].node_off;
+    out[8 + 3 * d + 1] = t.lev[d].n_nodes;
+    out[8 + 3 * d + 2] = t.row_off[d];
+  }
+  out[8 + 3 * L] = t.row_off[L];
+  ABI_END
+}
+
+int fmm2d_tree_export(void* plan, int which, void* dst, int64_t bytes, void* stream) {
+  ABI_BEGIN
+  Plan* P = as_plan(plan);
+  NEED(dst);
+  const fmm::Tree& t = P->tree;
+  FMM_REQUIRE(t.built, fmm::FMM_E_STATE, "no tree");
+  const void* src = nullptr;
+  int64_t need = 0;
+  const int64_t n = t.n, NN = t.n_nodes;
+  switch (which) {
+    case FMM2D_TREE_KEYS: src = t.keys.ptr; need = n * 8; break;
+    case FMM2D_TREE_PERM: src = t.perm.ptr; need = n * 4; break;
+    case FMM2D_TREE_SX: src = t.sx.ptr; need = n * 8; break;
+    case FMM2D_TREE_SY: src = t.sy.ptr; need = n * 8; break;
+    case FMM2D_TREE_CODE: src = t.code.ptr; need = NN * 8; break;
+    case FMM2D_TREE_LEVEL: src = t.nlevel.ptr; need = NN * 4; break;
+    case FMM2D_TREE_BOT: src = t.bot.ptr; need = NN * 4; break;
+    case FMM2D_TREE_TOP: src = t.top.ptr; need = NN * 4; break;
+    case FMM2D_TREE_PARENT: src = t.parent.ptr; need = NN * 4; break;
+    case FMM2D_TREE_CHILD: src = t.child.ptr; need = NN * 4; break;
+    case FMM2D_TREE_FLAGS: src = t.flags.ptr; need = NN * 4; break;
+    case FMM2D_TREE_COLLEAGUES: src = t.coll.ptr; need = NN * 9 * 4; break;
+    case FMM2D_TREE_XMID: src = t.xmid.ptr; need = NN * 8; break;
+    case FMM2D_TREE_YMID: src = t.ymid.ptr; need = NN * 8; break;
+    case FMM2D_TREE_ROW: src = t.row.ptr; need = NN * 4; break;
+    case FMM2D_TREE_M2L: src = t.m2l.ptr; need = NN * fmm::kNumM2L * 4; break;
+    case FMM2D_TREE_EVALID: src = t.evalid.ptr; need = NN * 4; break;
+    default: FMM_REQUIRE(false, fmm::FMM_E_ARG, "unknown tree array %d", which);
+  }
+  FMM_REQUIRE(bytes >= need, fmm::FMM_E_ARG, "tree array %d needs %lld bytes", which,
+              (long long)need);
+  if (need > 0)
+    FMM_HIP(hipMemcpyAsync(dst, src, need, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  ABI_END
+}
+
+int64_t fmm2d_operator_size(int p) {
+  if (p < 4 || p > 64) return -1;
+  return fmm::ops_per_level(p <= 32 ? 32 : 64);
+}
+
+int fmm2d_set_operators(void* plan, int level, const double* dev_ops, int64_t n, void* stream) {
+  ABI_BEGIN
+  Plan* P = as_plan(plan);
+  NEED(dev_ops);
+  const int64_t opsz = fmm::ops_per_level(P->PD);
+  FMM_REQUIRE(n == opsz, fmm::FMM_E_ARG, "operator block has %lld values, expected %lld",
+              (long long)n, (long long)opsz);
+  FMM_REQUIRE(level >= 0 && level < fmm::kMaxLevels, fmm::FMM_E_ARG, "bad level %d", level);
+  hipStream_t s = (hipStream_t)stream;
+  if (level >= P->ops_levels) {
+    // grow to all levels at once, keeping what is set
+    const int nl = fmm::kMaxLevels;
+    fmm::DevBuf nb;
+    nb.ensure(nl * opsz * sizeof(double));
+    if (P->ops_levels > 0)
+      FMM_HIP(hipMemcpyAsync(nb.ptr, P->ops.ptr, P->ops_levels * opsz * sizeof(double),
+                             hipMemcpyDeviceToDevice, s));
+    FMM_HIP(hipStreamSynchronize(s));
+    std::swap(nb.ptr, P->ops.ptr);
+    std::swap(nb.bytes, P->ops.bytes);
+    P->ops_levels = nl;
+    P->ops_set.resize(nl, 0);
+  }
+  FMM_HIP(hipMemcpyAsync(P->ops.as<double>() + level * opsz, dev_ops, opsz * sizeof(double),
+                         hipMemcpyDeviceToDevice, s));
+  P->ops_set[level] = 1;
+  ABI_END
+}
+
+int fmm2d_clear_operators(void* plan) {
+  ABI_BEGIN
+  Plan* P = as_plan(plan);
+  std::fill(P->ops_set.begin(), P->ops_set.end(), 0);
+  ABI_END
+}
+
+int fmm2d_upward(void* plan, const double* tau, int64_t n, void* stream) {
+  ABI_BEGIN
+  Plan* P = as_plan(plan);
+  NEED(tau);
+  FMM_REQUIRE(P->tree.built, fmm::FMM_E_STATE, "no tree");
+  FMM_REQUIRE(n == P->tree.n, fmm::FMM_E_ARG, "tau has %lld entries, tree has %lld sources",
+              (long long)n, (long long)P->tree.n);
+  P->mark(FMM2D_PHASE_UPWARD, false, (hipStream_t)stream);
+  fmm::upward(*P, tau, (hipStream_t)stream);
+  P->mark(FMM2D_PHASE_UPWARD, true, (hipStream_t)stream);
+  ABI_END
+}
+
+int fmm2d_downward(void* plan, void* stream) {
+  ABI_BEGIN
+  Plan* P = as_plan(plan);
+  FMM_REQUIRE(P->tree.built && P->tau_s.ptr, fmm::FMM_E_STATE, "call fmm2d_upward first");
+  P->mark(FMM2D_PHASE_DOWNWARD, false, (hipStream_t)stream);
+  fmm::downward(*P, (hipStream_t)stream);
+  P->mark(FMM2D_PHASE_DOWNWARD, true, (hipStream_t)stream);
+  ABI_END
+}
+
+int fmm2d_evaluate(void* plan, const double* tx, const double* ty, int64_t nt, int evaluator,
+                   int check, double* out, void* stream) {
+  ABI_BEGIN
+  Plan* P = as_plan(plan);
+  NEED(tx);
+  NEED(ty);
+  NEED(out);
+  P->mark(FMM2D_PHASE_EVALUATE, false, (hipStream_t)stream);
+  fmm::evaluate(*P, tx, ty, nt, evaluator, check, out, (hipStream_t)stream);
+  P->mark(FMM2D_PHASE_EVALUATE, true, (hipStream_t)stream);
+  ABI_END
+}
+
+int fmm2d_work_counts(void* plan, int64_t* out, int n) {
+  ABI_BEGIN
+  Plan* P = as_plan(plan);
+  NEED(out);
+  FMM_REQUIRE(n >= 9, fmm::FMM_E_ARG, "work_counts needs 9 slots");
+  const fmm::Tree& t = P->tree;
+  FMM_REQUIRE(t.built, fmm::FMM_E_STATE, "no tree");
+  fmm::DevBuf acc;
+  acc.ensure(16 * sizeof(unsigned long long));
+  FMM_HIP(hipMemset(acc.ptr, 0, 16 * sizeof(unsigned long long)));
+  if (P->tcnt.ptr && P->have_exp)
+    hipLaunchKernelGGL(fmm::k_eval_stats, dim3(fmm::grid256(t.n_eval)), dim3(256), 0, 0,
+                       t.evalnode.as<int>(), (int)t.n_eval, P->tcnt.as<int>(), t.coll.as<int>(),
+                       t.bot.as<int>(), t.top.as<int>(), t.nlevel.as<int>(),
+                       acc.as<unsigned long long>());
+  hipLaunchKernelGGL(fmm::k_tree_stats, dim3(fmm::grid256(t.n_nodes)), dim3(256), 0, 0,
+                     (int)t.n_nodes, t.nlevel.as<int>(), t.flags.as<int>(), t.m2l.as<int>(),
+                     t.row.as<int>(), t.child.as<int>(), t.bot.as<int>(), t.top.as<int>(),
+                     acc.as<unsigned long long>());
+  FMM_LAUNCH_CHECK();
+  unsigned long long h[16];
+  FMM_HIP(hipMemcpy(h, acc.ptr, sizeof h, hipMemcpyDeviceToHost));
+  for (int i = 0; i < 9; ++i) out[i] = (int64_t)h[i];
+  out[5] = t.n_rows;
+  ABI_END
+}
+
+int fmm2d_set_profiling(void* plan, int enable) {
+  ABI_BEGIN
+  Plan* P = as_plan(plan);
+  P->profile = enable != 0;
+  for (auto& u : P->ev_used) u = false;
+  ABI_END
+}
+
+int fmm2d_phase_times(void* plan, double* ms, int n) {
+  ABI_BEGIN
+  Plan* P = as_plan(plan);
+  NEED(ms);
+  FMM_REQUIRE(n >= FMM2D_NUM_PHASES, fmm::FMM_E_ARG, "phase_times needs %d slots",
+              FMM2D_NUM_PHASES);
+  for (int i = 0; i < FMM2D_NUM_PHASES; ++i) {
+    ms[i] = -1.0;
+    if (!P->ev_used[i]) continue;
+    FMM_HIP(hipEventSynchronize(P->ev[2 * i + 1]));
+    float t = 0.f;
+    FMM_HIP(hipEventElapsedTime(&t, P->ev[2 * i], P->ev[2 * i + 1]));
+    ms[i] = t;
+  }
+  ABI_END
+}
+
+int fmm2d_export_expansions(void* plan, int which, double* dst, int64_t nvals, void* stream) {
+  ABI_BEGIN
+  Plan* P = as_plan(plan);
+  NEED(dst);
+  const fmm::Tree& t = P->tree;
+  FMM_REQUIRE(t.built, fmm::FMM_E_STATE, "no tree");
+  const void* src = nullptr;
+  int64_t need = 0;
+  switch (which) {
+    case FMM2D_EXP_CHECK: src = P->check.ptr; need = t.n_rows * P->PD; break;
+    case FMM2D_EXP_MULTIPOLE: src = P->mult.ptr; need = t.n_rows * P->PD; break;
+    case FMM2D_EXP_LOCAL: src = P->le.ptr; need = t.n_nodes * P->PD; break;
+    case FMM2D_EXP_TAU_SORTED: src = P->tau_s.ptr; need = t.n; break;
+    default: FMM_REQUIRE(false, fmm::FMM_E_ARG, "unknown expansion array %d", which);
+  }
+  FMM_REQUIRE(src != nullptr || need == 0, fmm::FMM_E_STATE, "expansion array %d not computed",
+              which);
+  FMM_REQUIRE(nvals >= need, fmm::FMM_E_ARG, "expansion array %d needs %lld values", which,
+              (long long)need);
+  if (need > 0)
+    FMM_HIP(hipMemcpyAsync(dst, src, need * sizeof(double), hipMemcpyDeviceToDevice,
+                           (hipStream_t)stream));
+  ABI_END
+}
+
+}  // extern "C"

@@ -1,0 +1,335 @@
+// Device-wide primitives: exclusive scan and segmented MSD radix sort.
+#include "prims.h"
+
+namespace fmm {
+
+namespace {
+
+constexpr int SCAN_BS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_BS * SCAN_ITEMS;
+
+__global__ __launch_bounds__(SCAN_BS) void k_scan_block_sums(const int* __restrict__ in, int64_t n,
+                                                            int* __restrict__ sums) {
+  __shared__ int lds[SCAN_BS / 64];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  int acc = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; ++i) {
+    int64_t idx = base + (int64_t)i * SCAN_BS + threadIdx.x;
+    if (idx < n) acc += in[idx];
+  }
+  __shared__ int tot;
+  (void)block_exclusive_scan<SCAN_BS>(acc, lds, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_sums(int* __restrict__ sums, int64_t nb,
+                                                   int* __restrict__ total) {
+  __shared__ int lds[1024 / 64];
+  __shared__ int tot;
+  int carry = 0;
+  for (int64_t base = 0; base < nb; base += 1024) {
+    int64_t i = base + threadIdx.x;
+    int v = i < nb ? sums[i] : 0;
+    int ex = block_exclusive_scan<1024>(v, lds, &tot);
+    if (i < nb) sums[i] = carry + ex;
+    __syncthreads();
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ __launch_bounds__(SCAN_BS) void k_scan_apply(const int* in, int* out, int64_t n,
+                                                       const int* __restrict__ sums) {
+  __shared__ int lds[SCAN_BS / 64];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  int v[SCAN_ITEMS];
+  int acc = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; ++i) {
+    int64_t idx = base + i;
+    v[i] = idx < n ? in[idx] : 0;
+    acc += v[i];
+  }
+  int ex = block_exclusive_scan<SCAN_BS>(acc, lds, nullptr) + sums[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; ++i) {
+    int64_t idx = base + i;
+    if (idx < n) out[idx] = ex;
+    ex += v[i];
+  }
+}
+
+}  // namespace
+
+void exclusive_scan_i32(const int* in, int* out, int64_t n, int* total_dev, PrimWork& w,
+                        hipStream_t s) {
+  if (n <= 0) {
+    if (total_dev) FMM_HIP(hipMemsetAsync(total_dev, 0, sizeof(int), s));
+    return;
+  }
+  int64_t nb = ceil_div(n, SCAN_TILE);
+  w.blocksums.ensure(nb * sizeof(int));
+  int* sums = w.blocksums.as<int>();
+  hipLaunchKernelGGL(k_scan_block_sums, dim3(nb), dim3(SCAN_BS), 0, s, in, n, sums);
+  FMM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nb, total_dev);
+  FMM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BS), 0, s, in, out, n, sums);
+  FMM_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Segmented MSD radix sort.
+namespace {
+
+constexpr int CHUNK = 4096;
+constexpr int SORT_BS = 256;
+
+enum { C_NEXT = 0, C_FINAL = 1, C_CHUNKS = 2, C_ACTIVE = 3, C_COUNT = 4 };
+
+__global__ __launch_bounds__(1024) void k_make_chunks(const int2* __restrict__ seg, int nseg,
+                                                     int4* __restrict__ chunks,
+                                                     int* __restrict__ segChunk0,
+                                                     int* __restrict__ counters) {
+  __shared__ int lds[1024 / 64];
+  __shared__ int tot;
+  int carry = 0;
+  for (int base = 0; base < nseg; base += 1024) {
+    int si = base + threadIdx.x;
+    int2 sg = si < nseg ? seg[si] : make_int2(0, 0);
+    int cnt = si < nseg ? (int)ceil_div(sg.y, CHUNK) : 0;
+    int ex = block_exclusive_scan<1024>(cnt, lds, &tot);
+    if (si < nseg) {
+      int c0 = carry + ex;
+      segChunk0[si] = c0;
+      for (int c = 0; c < cnt; ++c) {
+        int st = sg.x + c * CHUNK;
+        int ln = min(CHUNK, sg.y - c * CHUNK);
+        chunks[c0 + c] = make_int4(si, st, ln, 0);
+      }
+    }
+    __syncthreads();
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counters[C_CHUNKS] = carry;
+}
+
+__global__ __launch_bounds__(SORT_BS) void k_chunk_hist(const uint64_t* __restrict__ keys,
+                                                       const int4* __restrict__ chunks,
+                                                       const int* __restrict__ counters, int shift,
+                                                       int* __restrict__ chunkHist) {
+  __shared__ int hist[256];
+  const int b = blockIdx.x;
+  if (b >= counters[C_CHUNKS]) return;
+  int4 ch = chunks[b];
+  hist[threadIdx.x] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < ch.z; i += SORT_BS) {
+    int d = (int)((keys[ch.y + i] >> shift) & 255u);
+    atomicAdd(&hist[d], 1);
+  }
+  __syncthreads();
+  chunkHist[(int64_t)b * 256 + threadIdx.x] = hist[threadIdx.x];
+}
+
+__global__ __launch_bounds__(SORT_BS) void k_seg_offsets(
+    const int2* __restrict__ seg, const int* __restrict__ segChunk0, int* __restrict__ chunkHist,
+    int threshold, int last_round, int2* __restrict__ segNext, int2* __restrict__ fin,
+    int* __restrict__ counters) {
+  __shared__ int lds[SORT_BS / 64];
+  const int si = blockIdx.x;
+  const int d = threadIdx.x;
+  int2 sg = seg[si];
+  int c0 = segChunk0[si];
+  int nc = (int)ceil_div(sg.y, CHUNK);
+  int run = 0;
+  for (int c = c0; c < c0 + nc; ++c) {
+    int h = chunkHist[(int64_t)c * 256 + d];
+    chunkHist[(int64_t)c * 256 + d] = run;
+    run += h;
+  }
+  int base = block_exclusive_scan<SORT_BS>(run, lds, nullptr);
+  int start = sg.x + base;
+  for (int c = c0; c < c0 + nc; ++c) chunkHist[(int64_t)c * 256 + d] += start;
+  if (run > threshold && !last_round) {
+    int idx = atomicAdd(&counters[C_NEXT], 1);
+    segNext[idx] = make_int2(start, run);
+    atomicAdd(&counters[C_ACTIVE], run);
+  } else if (run >= 2 && run <= kLocalSortMax) {
+    int idx = atomicAdd(&counters[C_FINAL], 1);
+    fin[idx] = make_int2(start, run);
+  }
+}
+
+__global__ __launch_bounds__(SORT_BS) void k_chunk_scatter(
+    const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+    const int4* __restrict__ chunks, const int* __restrict__ counters, int shift,
+    const int* __restrict__ chunkHist, uint64_t* __restrict__ keysB, uint32_t* __restrict__ valsB) {
+  __shared__ int off[256];
+  const int b = blockIdx.x;
+  if (b >= counters[C_CHUNKS]) return;
+  int4 ch = chunks[b];
+  off[threadIdx.x] = chunkHist[(int64_t)b * 256 + threadIdx.x];
+  __syncthreads();
+  for (int i = threadIdx.x; i < ch.z; i += SORT_BS) {
+    uint64_t k = keys[ch.y + i];
+    int d = (int)((k >> shift) & 255u);
+    int pos = atomicAdd(&off[d], 1);
+    keysB[pos] = k;
+    valsB[pos] = vals[ch.y + i];
+  }
+}
+
+__global__ __launch_bounds__(SORT_BS) void k_chunk_copy(const uint64_t* __restrict__ keysB,
+                                                       const uint32_t* __restrict__ valsB,
+                                                       const int4* __restrict__ chunks,
+                                                       const int* __restrict__ counters,
+                                                       uint64_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ vals) {
+  const int b = blockIdx.x;
+  if (b >= counters[C_CHUNKS]) return;
+  int4 ch = chunks[b];
+  for (int i = threadIdx.x; i < ch.z; i += SORT_BS) {
+    keys[ch.y + i] = keysB[ch.y + i];
+    vals[ch.y + i] = valsB[ch.y + i];
+  }
+}
+
+__global__ void k_sort_init(int2* __restrict__ seg0, int2* __restrict__ fin,
+                            int* __restrict__ counters, int n, int T) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int i = 0; i < C_COUNT; ++i) counters[i] = 0;
+  if (n > T) {
+    seg0[0] = make_int2(0, n);
+  } else {
+    fin[0] = make_int2(0, n);
+    counters[C_FINAL] = 1;
+  }
+}
+
+__device__ __forceinline__ bool pair_gt(uint64_t ka, uint32_t va, uint64_t kb, uint32_t vb) {
+  return ka > kb || (ka == kb && va > vb);
+}
+
+__global__ __launch_bounds__(SORT_BS) void k_local_sort(uint64_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ vals,
+                                                       const int2* __restrict__ fin) {
+  __shared__ uint64_t sk[kLocalSortMax];
+  __shared__ uint32_t sv[kLocalSortMax];
+  int2 sg = fin[blockIdx.x];
+  int P = 1;
+  while (P < sg.y) P <<= 1;
+  for (int i = threadIdx.x; i < P; i += SORT_BS) {
+    if (i < sg.y) {
+      sk[i] = keys[sg.x + i];
+      sv[i] = vals[sg.x + i];
+    } else {
+      sk[i] = ~0ull;
+      sv[i] = 0xffffffffu;
+    }
+  }
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < P; i += SORT_BS) {
+        int ixj = i ^ j;
+        if (ixj > i) {
+          bool up = (i & k) == 0;
+          uint64_t ka = sk[i], kb = sk[ixj];
+          uint32_t va = sv[i], vb = sv[ixj];
+          bool gt = pair_gt(ka, va, kb, vb);
+          if (gt == up) {
+            sk[i] = kb;
+            sk[ixj] = ka;
+            sv[i] = vb;
+            sv[ixj] = va;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < sg.y; i += SORT_BS) {
+    keys[sg.x + i] = sk[i];
+    vals[sg.x + i] = sv[i];
+  }
+}
+
+}  // namespace
+
+void sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, PrimWork& w,
+                    hipStream_t s) {
+  if (n <= 1) return;
+  FMM_REQUIRE(n < (int64_t)1 << 31, FMM_E_LIMIT, "sort: n = %lld exceeds 2^31-1", (long long)n);
+  int T = threshold < 1 ? 1 : threshold;
+  if (T > kLocalSortMax) T = kLocalSortMax;
+  const int64_t segcap = n / 2 + 2;
+  w.keysB.ensure(n * sizeof(uint64_t));
+  w.valsB.ensure(n * sizeof(uint32_t));
+  w.seg[0].ensure(segcap * sizeof(int2));
+  w.seg[1].ensure(segcap * sizeof(int2));
+  w.fin.ensure(segcap * sizeof(int2));
+  w.counters.ensure(C_COUNT * sizeof(int));
+  w.hcounters.ensure(C_COUNT * sizeof(int));
+  int* cnt = w.counters.as<int>();
+  int* hc = w.hcounters.as<int>();
+  hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(64), 0, s, w.seg[0].as<int2>(), w.fin.as<int2>(),
+                     cnt, (int)n, T);
+  FMM_LAUNCH_CHECK();
+
+  int nseg = 0;
+  int64_t active = 0;
+  int nfinal = 0;
+  int cur = 0;
+  if (n > T) {
+    nseg = 1;
+    active = n;
+  } else {
+    nfinal = 1;
+  }
+  for (int r = 0; r < 8 && nseg > 0; ++r) {
+    const int shift = 56 - 8 * r;
+    const int64_t cbound = nseg + active / CHUNK + 1;
+    w.chunks.ensure(cbound * sizeof(int4));
+    w.chunkHist.ensure(cbound * 256 * sizeof(int));
+    w.segChunk0.ensure((int64_t)nseg * sizeof(int));
+    int2* segCur = w.seg[cur].as<int2>();
+    int2* segNext = w.seg[cur ^ 1].as<int2>();
+    FMM_HIP(hipMemsetAsync(cnt + C_NEXT, 0, sizeof(int), s));
+    FMM_HIP(hipMemsetAsync(cnt + C_ACTIVE, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_make_chunks, dim3(1), dim3(1024), 0, s, segCur, nseg,
+                       w.chunks.as<int4>(), w.segChunk0.as<int>(), cnt);
+    FMM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_chunk_hist, dim3(cbound), dim3(SORT_BS), 0, s, keys, w.chunks.as<int4>(),
+                       cnt, shift, w.chunkHist.as<int>());
+    FMM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_seg_offsets, dim3(nseg), dim3(SORT_BS), 0, s, segCur,
+                       w.segChunk0.as<int>(), w.chunkHist.as<int>(), T, r == 7 ? 1 : 0, segNext,
+                       w.fin.as<int2>(), cnt);
+    FMM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_chunk_scatter, dim3(cbound), dim3(SORT_BS), 0, s, keys, vals,
+                       w.chunks.as<int4>(), cnt, shift, w.chunkHist.as<int>(),
+                       w.keysB.as<uint64_t>(), w.valsB.as<uint32_t>());
+    FMM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_chunk_copy, dim3(cbound), dim3(SORT_BS), 0, s, w.keysB.as<uint64_t>(),
+                       w.valsB.as<uint32_t>(), w.chunks.as<int4>(), cnt, keys, vals);
+    FMM_LAUNCH_CHECK();
+    FMM_HIP(hipMemcpyAsync(hc, cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, s));
+    FMM_HIP(hipStreamSynchronize(s));
+    nseg = hc[C_NEXT];
+    active = hc[C_ACTIVE];
+    nfinal = hc[C_FINAL];
+    cur ^= 1;
+  }
+  if (nfinal > 0) {
+    hipLaunchKernelGGL(k_local_sort, dim3(nfinal), dim3(SORT_BS), 0, s, keys, vals,
+                       w.fin.as<int2>());
+    FMM_LAUNCH_CHECK();
+  }
+}
+
+}  // namespace fmm

@@ -1,0 +1,837 @@
+// Adaptive quadtree of the reference (flexmm/tree.py), built on the device.
+//
+// Every point gets a 31-level Morton key from the reference's own FP64
+// midpoint recurrence (tree.py:169-176, 301-304: xmid = xmin + w/2, children
+// start at xmid - w/2 or xmid, class 2*(x > xmid) + (y > ymid), tree.py:32-34),
+// so the classification of every point at every level is the reference's
+// bit for bit.  After a segmented MSD radix sort by key, the tree is a set of
+// sorted code lists:
+//   S_d   cells of depth d that are split (refinement: > ncut points,
+//         tree.py:204; plus the minimal corner 2:1 balance, tree.py:529-541);
+//   X_d   real leaves with a split same-level neighbour (tree.py:542-553,
+//         385-397), which get four fake children;
+//   P_d   = S_d u X_d, the nodes of depth d that have children.
+// Level d+1 holds the children of P_d in Morton order.  Colleagues, parents,
+// point ranges and M2L partners are binary searches in these sorted lists.
+#include "tree.h"
+
+#include <cstring>
+
+namespace fmm {
+
+namespace {
+
+constexpr int BS = 256;
+
+__device__ __forceinline__ int clz64(uint64_t v) { return v ? __clzll((long long)v) : 64; }
+
+__host__ __device__ inline uint64_t spread_bits(uint32_t v) {
+  uint64_t x = v;
+  x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+  x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+  x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+  x = (x | (x << 2)) & 0x3333333333333333ull;
+  x = (x | (x << 1)) & 0x5555555555555555ull;
+  return x;
+}
+__host__ __device__ inline uint32_t compact_bits(uint64_t x) {
+  x &= 0x5555555555555555ull;
+  x = (x | (x >> 1)) & 0x3333333333333333ull;
+  x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+  x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+  x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+  x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+  return (uint32_t)x;
+}
+// Morton code of a cell: the digit of level l (from the top) is 2*xbit + ybit.
+__host__ __device__ inline uint64_t morton(uint32_t ix, uint32_t iy) {
+  return (spread_bits(ix) << 1) | spread_bits(iy);
+}
+__host__ __device__ inline void demorton(uint64_t c, uint32_t* ix, uint32_t* iy) {
+  *ix = compact_bits(c >> 1);
+  *iy = compact_bits(c);
+}
+
+__device__ __forceinline__ int64_t bsearch_u64(const uint64_t* __restrict__ a, int64_t n,
+                                               uint64_t v) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    uint64_t am = a[mid];
+    if (am < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return (lo < n && a[lo] == v) ? lo : -1;
+}
+
+__device__ __forceinline__ int64_t lower_bound_u64(const uint64_t* __restrict__ a, int64_t n,
+                                                   uint64_t v) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (a[mid] < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <class T>
+__device__ __forceinline__ int find_level(const T* __restrict__ off, int L, int64_t g) {
+  int d = 0;
+  while (d + 1 < L && g >= (int64_t)off[d + 1]) ++d;
+  return d;
+}
+
+// --- points ----------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_point_keys(const double* __restrict__ x,
+                                                   const double* __restrict__ y, int n,
+                                                   double xmin0, double ymin0, double w0,
+                                                   uint64_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals) {
+  int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  const double px = x[i], py = y[i];
+  double xmin = xmin0, ymin = ymin0, w = w0;
+  uint64_t key = 0;
+  for (int l = 1; l <= kMaxDepth; ++l) {
+    const double h = 0.5 * w;
+    const double xm = xmin + h;
+    const double ym = ymin + h;
+    const int bx = px > xm;
+    const int by = py > ym;
+    key |= (uint64_t)(2 * bx + by) << (64 - 2 * l);
+    xmin = bx ? xm : xm - h;
+    ymin = by ? ym : ym - h;
+    w = h;
+  }
+  keys[i] = key;
+  vals[i] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(BS) void k_gather_sorted(const double* __restrict__ x,
+                                                      const double* __restrict__ y,
+                                                      const uint32_t* __restrict__ perm, int n,
+                                                      double* __restrict__ sx,
+                                                      double* __restrict__ sy) {
+  int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  uint32_t j = perm[i];
+  sx[i] = x[j];
+  sy[i] = y[j];
+}
+
+// --- refinement: split cells from the sorted keys ----------------------------
+// Position i starts the cells of depths s(i)+1.. (s = common depth with key
+// i-1); such a cell has more than ncut points iff key i+ncut is still inside
+// it, i.e. depth <= c(i) (common depth with key i+ncut).
+__global__ __launch_bounds__(BS) void k_split_pos(const uint64_t* __restrict__ keys, int n,
+                                                  int ncut, int* __restrict__ flag,
+                                                  int2* __restrict__ sc, int* __restrict__ dcount) {
+  int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  uint64_t k = keys[i];
+  int s = (i == 0) ? -1 : min(clz64(k ^ keys[i - 1]) >> 1, kMaxDepth);
+  int c = ((int64_t)i + ncut < n) ? min(clz64(k ^ keys[i + ncut]) >> 1, kMaxDepth - 1) : -1;
+  int f = s < c;
+  flag[i] = f;
+  sc[i] = make_int2(s, c);
+  if (f)
+    for (int d = s + 1; d <= c; ++d) atomicAdd(&dcount[d], 1);
+}
+
+__global__ __launch_bounds__(BS) void k_compact_pos(const int* __restrict__ flag,
+                                                    const int* __restrict__ rank,
+                                                    const int2* __restrict__ sc, int n,
+                                                    int* __restrict__ ppos, int2* __restrict__ psc) {
+  int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n || !flag[i]) return;
+  int r = rank[i];
+  ppos[r] = i;
+  psc[r] = sc[i];
+}
+
+__global__ __launch_bounds__(1024) void k_depth_lists(const int* __restrict__ ppos,
+                                                     const int2* __restrict__ psc, int np,
+                                                     const uint64_t* __restrict__ keys,
+                                                     const int64_t* __restrict__ soff,
+                                                     uint64_t* __restrict__ sref) {
+  __shared__ int lds[1024 / 64];
+  __shared__ int tot;
+  const int d = blockIdx.x;
+  uint64_t* out = sref + soff[d];
+  int carry = 0;
+  for (int base = 0; base < np; base += 1024) {
+    int j = base + threadIdx.x;
+    int pred = 0;
+    uint64_t code = 0;
+    if (j < np) {
+      int2 v = psc[j];
+      pred = (v.x < d) && (d <= v.y);
+      if (pred && d > 0) code = keys[ppos[j]] >> (64 - 2 * d);
+    }
+    int ex = block_exclusive_scan<1024>(pred, lds, &tot);
+    if (pred) out[carry + ex] = code;
+    __syncthreads();
+    carry += tot;
+    __syncthreads();
+  }
+}
+
+// --- 2:1 balance ---------------------------------------------------------------
+// A leaf at depth d is split iff a same-level neighbour has a grandchild; the
+// minimal balanced refinement is S_d = Sref_d u N(parents(S_{d+1})), with
+// N the 3x3 neighbourhood.  This kernel counts neighbours missing from Sref,
+// which is zero exactly when the refinement tree is already balanced.
+__global__ __launch_bounds__(BS) void k_balance_check(const uint64_t* __restrict__ sref,
+                                                      const int64_t* __restrict__ soff, int L,
+                                                      int* __restrict__ rcount) {
+  int64_t e = soff[2] + (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (e >= soff[L - 1]) return;
+  int D = 2;
+  while (e >= soff[D + 1]) ++D;
+  uint64_t s = sref[e];
+  if (e > soff[D] && (sref[e - 1] >> 2) == (s >> 2)) return;
+  const int d = D - 1;
+  uint64_t pc = s >> 2;
+  uint32_t ix, iy;
+  demorton(pc, &ix, &iy);
+  const int64_t lim = (int64_t)1 << d;
+  const uint64_t* lst = sref + soff[d];
+  const int64_t nl = soff[d + 1] - soff[d];
+  int miss = 0;
+  for (int k = 0; k < 9; ++k) {
+    int64_t nx = (int64_t)ix + k / 3 - 1, ny = (int64_t)iy + k % 3 - 1;
+    if (nx < 0 || ny < 0 || nx >= lim || ny >= lim) continue;
+    if (bsearch_u64(lst, nl, morton((uint32_t)nx, (uint32_t)ny)) < 0) ++miss;
+  }
+  if (miss) atomicAdd(rcount, miss);
+}
+
+__global__ __launch_bounds__(BS) void k_bal_candidates(const uint64_t* __restrict__ sref_d,
+                                                       int64_t nref,
+                                                       const uint64_t* __restrict__ s_next,
+                                                       int64_t nnext, int d,
+                                                       uint64_t* __restrict__ cand,
+                                                       uint32_t* __restrict__ vals) {
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const int sh = 64 - 2 * d;
+  if (i < nref) {
+    cand[i] = sref_d[i] << sh;
+    vals[i] = 0;
+    return;
+  }
+  int64_t j = i - nref;
+  if (j >= 9 * nnext) return;
+  int64_t e = j / 9;
+  int k = (int)(j % 9);
+  uint64_t s = s_next[e];
+  uint64_t out = ~0ull;
+  if (e == 0 || (s_next[e - 1] >> 2) != (s >> 2)) {
+    uint32_t ix, iy;
+    demorton(s >> 2, &ix, &iy);
+    int64_t lim = (int64_t)1 << d;
+    int64_t nx = (int64_t)ix + k / 3 - 1, ny = (int64_t)iy + k % 3 - 1;
+    if (nx >= 0 && ny >= 0 && nx < lim && ny < lim) out = morton((uint32_t)nx, (uint32_t)ny) << sh;
+  }
+  cand[i] = out;
+  vals[i] = 0;
+}
+
+__global__ __launch_bounds__(BS) void k_unique_flags(const uint64_t* __restrict__ cand, int64_t n,
+                                                     int* __restrict__ flag) {
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  uint64_t c = cand[i];
+  flag[i] = (c != ~0ull) && (i == 0 || cand[i - 1] != c);
+}
+
+__global__ __launch_bounds__(BS) void k_unique_scatter(const uint64_t* __restrict__ cand,
+                                                       const int* __restrict__ flag,
+                                                       const int* __restrict__ rank, int64_t n,
+                                                       int d, uint64_t* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i >= n || !flag[i]) return;
+  out[rank[i]] = cand[i] >> (64 - 2 * d);
+}
+
+// --- nodes with children (split or X-list) -----------------------------------
+// Real nodes of level d are the children of S_{d-1} (the root for d = 0).
+__global__ __launch_bounds__(BS) void k_real_flags(const uint64_t* __restrict__ s_all,
+                                                   const int64_t* __restrict__ soff,
+                                                   const int64_t* __restrict__ roff, int L,
+                                                   uint64_t* __restrict__ rcode,
+                                                   int* __restrict__ hasc, int* __restrict__ kind,
+                                                   int* __restrict__ pcount) {
+  int64_t r = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (r >= roff[L]) return;
+  int d = 0;
+  while (r >= roff[d + 1]) ++d;
+  int64_t k = r - roff[d];
+  uint64_t code = d == 0 ? 0 : ((s_all[soff[d - 1] + k / 4] << 2) | (uint64_t)(k & 3));
+  int inS = 0, isX = 0;
+  if (d <= L - 2) {
+    const uint64_t* lst = s_all + soff[d];
+    const int64_t nl = soff[d + 1] - soff[d];
+    inS = bsearch_u64(lst, nl, code) >= 0;
+    if (!inS && d >= 1) {
+      uint32_t ix, iy;
+      demorton(code, &ix, &iy);
+      const int64_t lim = (int64_t)1 << d;
+      for (int q = 0; q < 9 && !isX; ++q) {
+        if (q == 4) continue;
+        int64_t nx = (int64_t)ix + q / 3 - 1, ny = (int64_t)iy + q % 3 - 1;
+        if (nx < 0 || ny < 0 || nx >= lim || ny >= lim) continue;
+        isX = bsearch_u64(lst, nl, morton((uint32_t)nx, (uint32_t)ny)) >= 0;
+      }
+    }
+  }
+  rcode[r] = code;
+  hasc[r] = inS || isX;
+  kind[r] = isX;
+  if (inS || isX) atomicAdd(&pcount[d], 1);
+}
+
+__global__ __launch_bounds__(BS) void k_p_scatter(const uint64_t* __restrict__ rcode,
+                                                  const int* __restrict__ hasc,
+                                                  const int* __restrict__ kind,
+                                                  const int* __restrict__ rank, int64_t nr,
+                                                  uint64_t* __restrict__ p_code,
+                                                  int* __restrict__ p_kind) {
+  int64_t r = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (r >= nr || !hasc[r]) return;
+  int q = rank[r];
+  p_code[q] = rcode[r];
+  p_kind[q] = kind[r];
+}
+
+// --- final node arrays ---------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_nodes_a(const uint64_t* __restrict__ p_code,
+                                                const int* __restrict__ p_kind,
+                                                const int64_t* __restrict__ poff,
+                                                const int* __restrict__ levoff, int L,
+                                                uint64_t* __restrict__ code,
+                                                int* __restrict__ nlev, int* __restrict__ flags) {
+  int g = blockIdx.x * BS + threadIdx.x;
+  if (g >= levoff[L]) return;
+  int d = find_level(levoff, L, g);
+  int k = g - levoff[d];
+  uint64_t c = 0;
+  int fake = 0;
+  if (d > 0) {
+    int64_t pj = poff[d - 1] + k / 4;
+    c = (p_code[pj] << 2) | (uint64_t)(k & 3);
+    fake = p_kind[pj] == 1;
+  }
+  code[g] = c;
+  nlev[g] = d;
+  flags[g] = fake ? NF_FAKE : 0;
+}
+
+__global__ __launch_bounds__(BS) void k_nodes_b(
+    const uint64_t* __restrict__ p_code, const int* __restrict__ p_kind,
+    const int64_t* __restrict__ poff, const int* __restrict__ levoff, int L,
+    const uint64_t* __restrict__ keys, int npts, double xmin0, double ymin0, double w0,
+    const uint64_t* __restrict__ code, int* __restrict__ flags, int* __restrict__ bot,
+    int* __restrict__ top, int* __restrict__ parent, int* __restrict__ child,
+    int* __restrict__ coll, double* __restrict__ xmid, double* __restrict__ ymid) {
+  int g = blockIdx.x * BS + threadIdx.x;
+  if (g >= levoff[L]) return;
+  const int d = find_level(levoff, L, g);
+  const uint64_t c = code[g];
+  const int fake = flags[g] & NF_FAKE;
+  // children
+  int ch = -1, xl = 0;
+  if (!fake && d <= L - 2) {
+    int64_t pos = bsearch_u64(p_code + poff[d], poff[d + 1] - poff[d], c);
+    if (pos >= 0) {
+      ch = levoff[d + 1] + 4 * (int)pos;
+      xl = p_kind[poff[d] + pos] == 1;
+    }
+  }
+  const int leaf = !fake && (ch < 0 || xl);
+  // point range of the cell
+  int b = 0, t = npts;
+  if (d > 0) {
+    const int sh = 64 - 2 * d;
+    b = (int)lower_bound_u64(keys, npts, c << sh);
+    const uint64_t last = ((uint64_t)1 << (2 * d)) - 1;
+    t = (c == last) ? npts : (int)lower_bound_u64(keys, npts, (c + 1) << sh);
+  }
+  const int has_src = t > b;
+  int f = fake ? NF_FAKE : 0;
+  if (leaf) f |= NF_LEAF;
+  if (xl) f |= NF_XLIST;
+  if (ch >= 0) f |= NF_CHILDREN;
+  if (has_src) f |= NF_SOURCE;
+  if (((leaf && !xl) || fake) && has_src) f |= NF_UP;   // tree.py:557-560
+  if ((!leaf || xl) && !fake) f |= NF_DOWN;             // tree.py:562 (fake nodes have leaf=False)
+  flags[g] = f;
+  bot[g] = b;
+  top[g] = t;
+  child[g] = ch;
+  // centre by the reference recurrence
+  double xm0 = xmin0, ym0 = ymin0, w = w0;
+  for (int l = 1; l <= d; ++l) {
+    const double h = 0.5 * w;
+    const double xm = xm0 + h, ym = ym0 + h;
+    const int dig = (int)((c >> (2 * (d - l))) & 3);
+    xm0 = (dig & 2) ? xm : xm - h;
+    ym0 = (dig & 1) ? ym : ym - h;
+    w = h;
+  }
+  xmid[g] = xm0 + 0.5 * w;
+  ymid[g] = ym0 + 0.5 * w;
+  // colleagues and parent
+  uint32_t ix, iy;
+  demorton(c, &ix, &iy);
+  const int64_t lim = (int64_t)1 << d;
+  const uint64_t* lst = code + levoff[d];
+  const int64_t nl = levoff[d + 1] - levoff[d];
+  for (int q = 0; q < 9; ++q) {
+    int64_t nx = (int64_t)ix + q / 3 - 1, ny = (int64_t)iy + q % 3 - 1;
+    int r = -1;
+    if (nx >= 0 && ny >= 0 && nx < lim && ny < lim) {
+      int64_t pos = (q == 4) ? (g - levoff[d]) : bsearch_u64(lst, nl, morton((uint32_t)nx, (uint32_t)ny));
+      if (pos >= 0) r = levoff[d] + (int)pos;
+    }
+    coll[(int64_t)g * 9 + q] = r;
+  }
+  int par = -1;
+  if (d > 0) {
+    int64_t pos = bsearch_u64(code + levoff[d - 1], levoff[d] - levoff[d - 1], c >> 2);
+    if (pos >= 0) par = levoff[d - 1] + (int)pos;
+  }
+  parent[g] = par;
+}
+
+// --- density rows: sibling groups (levels >= 2) holding a source ------------------
+__global__ __launch_bounds__(BS) void k_group_flags(const int* __restrict__ flags, int g0, int ngroups,
+                                                    const int* __restrict__ levoff, int L,
+                                                    int* __restrict__ gflag,
+                                                    int* __restrict__ lcount) {
+  int q = blockIdx.x * BS + threadIdx.x;
+  if (q >= ngroups) return;
+  int g = g0 + 4 * q;
+  int any = 0;
+  for (int j = 0; j < 4; ++j) any |= flags[g + j] & NF_SOURCE;
+  gflag[q] = any ? 1 : 0;
+  if (any) atomicAdd(&lcount[find_level(levoff, L, g)], 4);
+}
+
+__global__ __launch_bounds__(BS) void k_rows(const int* __restrict__ gflag,
+                                             const int* __restrict__ grank, int g0, int nnodes,
+                                             int* __restrict__ row, int* __restrict__ rownode) {
+  int g = blockIdx.x * BS + threadIdx.x;
+  if (g >= nnodes) return;
+  int r = -1;
+  if (g >= g0) {
+    int q = (g - g0) / 4;
+    if (gflag[q]) r = 4 * grank[q] + (g - g0) % 4;
+  }
+  row[g] = r;
+  if (r >= 0) rownode[r] = g;
+}
+
+// --- M2L partners: children of the parent's colleagues, outside the 3x3 --------
+// (tree.py:399-435).  For child position (cx, cy) the parent's 3x3 covers
+// offsets [-2-cx, 3-cx] x [-2-cy, 3-cy].
+__global__ __launch_bounds__(BS) void k_m2l_lists(const uint64_t* __restrict__ code,
+                                                  const int* __restrict__ flags,
+                                                  const int* __restrict__ row,
+                                                  const int* __restrict__ levoff, int L,
+                                                  int* __restrict__ m2l) {
+  int g = levoff[2] + blockIdx.x * BS + threadIdx.x;
+  if (g >= levoff[L]) return;
+  const int d = find_level(levoff, L, g);
+  int* out = m2l + (int64_t)g * kNumM2L;
+  if (flags[g] & NF_FAKE) {
+    for (int s = 0; s < kNumM2L; ++s) out[s] = -1;
+    return;
+  }
+  uint32_t ix, iy;
+  demorton(code[g], &ix, &iy);
+  const int cx = ix & 1, cy = iy & 1;
+  const int64_t lim = (int64_t)1 << d;
+  const uint64_t* lst = code + levoff[d];
+  const int64_t nl = levoff[d + 1] - levoff[d];
+  for (int s = 0; s < kNumM2L; ++s) {
+    int ox, oy;
+    m2l_slot_offset(s, &ox, &oy);
+    int r = -1;
+    if (ox >= -2 - cx && ox <= 3 - cx && oy >= -2 - cy && oy <= 3 - cy) {
+      int64_t nx = (int64_t)ix + ox, ny = (int64_t)iy + oy;
+      if (nx >= 0 && ny >= 0 && nx < lim && ny < lim) {
+        int64_t pos = bsearch_u64(lst, nl, morton((uint32_t)nx, (uint32_t)ny));
+        if (pos >= 0) r = row[levoff[d] + pos];
+      }
+    }
+    out[s] = r;
+  }
+}
+
+__global__ __launch_bounds__(BS) void k_flag_bit(const int* __restrict__ flags, int n, int bit,
+                                                 int* __restrict__ out) {
+  int g = blockIdx.x * BS + threadIdx.x;
+  if (g >= n) return;
+  out[g] = (flags[g] & bit) ? 1 : 0;
+}
+
+// S2M boxes: compute_upwards nodes that own a multipole row (levels >= 2;
+// the upward pass stops at level 2, fmm.py:98).
+__global__ __launch_bounds__(BS) void k_up_flag(const int* __restrict__ flags,
+                                                const int* __restrict__ row, int n,
+                                                int* __restrict__ out) {
+  int g = blockIdx.x * BS + threadIdx.x;
+  if (g >= n) return;
+  out[g] = ((flags[g] & NF_UP) && row[g] >= 0) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(BS) void k_list_scatter(const int* __restrict__ flag,
+                                                     const int* __restrict__ rank, int n,
+                                                     int* __restrict__ list,
+                                                     int* __restrict__ idx_of) {
+  int g = blockIdx.x * BS + threadIdx.x;
+  if (g >= n) return;
+  if (flag[g]) {
+    list[rank[g]] = g;
+    if (idx_of) idx_of[g] = rank[g];
+  } else if (idx_of) {
+    idx_of[g] = -1;
+  }
+}
+
+__global__ __launch_bounds__(BS) void k_locate(const double* __restrict__ tx,
+                                               const double* __restrict__ ty, int64_t nt,
+                                               const int* __restrict__ flags,
+                                               const int* __restrict__ child,
+                                               const double* __restrict__ xmid,
+                                               const double* __restrict__ ymid,
+                                               int* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i >= nt) return;
+  const double x = tx[i], y = ty[i];
+  int g = 0;
+  for (int it = 0; it < kMaxLevels && !(flags[g] & NF_LEAF); ++it)
+    g = child[g] + 2 * (x > xmid[g]) + (y > ymid[g]);   // tree.py:720-725
+  out[i] = g;
+}
+
+inline unsigned grid_for(int64_t n) { return (unsigned)(n <= 0 ? 1 : ceil_div(n, BS)); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmin, double ymin,
+                double w0, int ncut, hipStream_t s) {
+  FMM_REQUIRE(n >= 1 && n < ((int64_t)1 << 31) - 1, FMM_E_LIMIT,
+              "tree: number of sources %lld outside [1, 2^31-2]", (long long)n);
+  FMM_REQUIRE(ncut >= 1, FMM_E_ARG, "tree: Ncutoff must be >= 1 (got %d)", ncut);
+  FMM_REQUIRE(w0 > 0.0 && std::isfinite(w0), FMM_E_ARG, "tree: root width must be positive");
+  t.built = false;
+  t.n = n;
+  t.ncut = ncut;
+  t.xmin = xmin;
+  t.ymin = ymin;
+  t.w0 = w0;
+  const int nn = (int)n;
+
+  // -- sorted points ----------------------------------------------------------
+  t.keys.ensure(n * sizeof(uint64_t));
+  t.perm.ensure(n * sizeof(uint32_t));
+  t.sx.ensure(n * sizeof(double));
+  t.sy.ensure(n * sizeof(double));
+  uint64_t* keys = t.keys.as<uint64_t>();
+  uint32_t* perm = t.perm.as<uint32_t>();
+  hipLaunchKernelGGL(k_point_keys, dim3(grid_for(n)), dim3(BS), 0, s, x, y, nn, xmin, ymin, w0,
+                     keys, perm);
+  FMM_LAUNCH_CHECK();
+  sort_pairs_u64(keys, perm, n, ncut < kLocalSortMax ? ncut : kLocalSortMax, t.pw, s);
+  hipLaunchKernelGGL(k_gather_sorted, dim3(grid_for(n)), dim3(BS), 0, s, x, y, perm, nn,
+                     t.sx.as<double>(), t.sy.as<double>());
+  FMM_LAUNCH_CHECK();
+
+  // -- counters ------------------------------------------------------------------
+  enum { CNT_D = 0, CNT_P = 32, CNT_R = 33, CNT_PC = 34, CNT_ROWS = 66, CNT_EVAL = 67,
+         CNT_UP = 68, CNT_UNIQ = 69, CNT_LR = 72, CNT_N = 104 };
+  t.w_cnt.ensure(CNT_N * sizeof(int));
+  t.h_cnt.ensure(CNT_N * sizeof(int));
+  int* cnt = t.w_cnt.as<int>();
+  int* hc = t.h_cnt.as<int>();
+  FMM_HIP(hipMemsetAsync(cnt, 0, CNT_N * sizeof(int), s));
+  // Small host->device uploads go through distinct regions of a pinned buffer
+  // so that the host never rewrites memory a queued copy still reads.
+  constexpr int kUpRegion = kMaxLevels + 2;
+  t.h_up.ensure(8 * kUpRegion * sizeof(int64_t));
+  int64_t* hup = t.h_up.as<int64_t>();
+  int up_region = 0;
+  auto upload = [&](void* dst, const void* src, size_t bytes) {
+    FMM_REQUIRE(up_region < 8 && bytes <= kUpRegion * sizeof(int64_t), FMM_E_INTERNAL,
+                "tree: upload staging exhausted");
+    int64_t* slot = hup + (int64_t)up_region * kUpRegion;
+    ++up_region;
+    std::memcpy(slot, src, bytes);
+    FMM_HIP(hipMemcpyAsync(dst, slot, bytes, hipMemcpyHostToDevice, s));
+  };
+
+  // -- split cells of the refinement --------------------------------------------
+  t.w_flag.ensure(n * sizeof(int));
+  t.w_rank.ensure(n * sizeof(int));
+  t.w_a.ensure(n * sizeof(int2));
+  int* flag = t.w_flag.as<int>();
+  int* rank = t.w_rank.as<int>();
+  int2* sc = t.w_a.as<int2>();
+  hipLaunchKernelGGL(k_split_pos, dim3(grid_for(n)), dim3(BS), 0, s, keys, nn, ncut, flag, sc,
+                     cnt + CNT_D);
+  FMM_LAUNCH_CHECK();
+  exclusive_scan_i32(flag, rank, n, cnt + CNT_P, t.pw, s);
+  FMM_HIP(hipMemcpyAsync(hc, cnt, CNT_N * sizeof(int), hipMemcpyDeviceToHost, s));
+  FMM_HIP(hipStreamSynchronize(s));
+  const int np = hc[CNT_P];
+  int maxd = -1;
+  int64_t dcount[kMaxLevels];
+  for (int d = 0; d < kMaxLevels; ++d) {
+    dcount[d] = hc[CNT_D + d];
+    if (dcount[d] > 0) maxd = d;
+  }
+  const int L = maxd + 2;   // deepest split depth + children level + 1
+  t.L = L;
+  FMM_REQUIRE(L <= kMaxLevels, FMM_E_INTERNAL, "tree: %d levels exceed the depth cap", L);
+
+  // split-cell lists per depth (S_ref)
+  t.s_off.assign(L + 1, 0);
+  for (int d = 0; d < L; ++d) t.s_off[d + 1] = t.s_off[d] + (d <= L - 2 ? dcount[d] : 0);
+  const int64_t ns_total = t.s_off[L];
+  t.w_b.ensure((L + 1) * sizeof(int64_t) * 3 + 64);
+  int64_t* d_soff = t.w_b.as<int64_t>();
+  t.sref_all.ensure((ns_total + 1) * sizeof(uint64_t));
+  if (np > 0) {
+    t.w_c.ensure((int64_t)(np + 2) * (sizeof(int) + sizeof(int2)) + 16);
+    int* ppos = t.w_c.as<int>();
+    int2* psc = reinterpret_cast<int2*>(ppos + np + (np & 1));
+    hipLaunchKernelGGL(k_compact_pos, dim3(grid_for(n)), dim3(BS), 0, s, flag, rank, sc, nn, ppos,
+                       psc);
+    FMM_LAUNCH_CHECK();
+    upload(d_soff, t.s_off.data(), (L + 1) * sizeof(int64_t));
+    hipLaunchKernelGGL(k_depth_lists, dim3(L - 1), dim3(1024), 0, s, ppos, psc, np, keys, d_soff,
+                       t.sref_all.as<uint64_t>());
+    FMM_LAUNCH_CHECK();
+  }
+
+  // -- 2:1 balance ---------------------------------------------------------------
+  bool balanced = true;
+  if (L >= 4) {
+    int64_t ne = t.s_off[L - 1] - t.s_off[2];
+    if (ne > 0) {
+      hipLaunchKernelGGL(k_balance_check, dim3(grid_for(ne)), dim3(BS), 0, s,
+                         t.sref_all.as<uint64_t>(), d_soff, L, cnt + CNT_R);
+      FMM_LAUNCH_CHECK();
+      FMM_HIP(hipMemcpyAsync(hc + CNT_R, cnt + CNT_R, sizeof(int), hipMemcpyDeviceToHost, s));
+      FMM_HIP(hipStreamSynchronize(s));
+      balanced = hc[CNT_R] == 0;
+    }
+  }
+  std::vector<int64_t> soff = t.s_off;
+  if (balanced) {
+    t.s_all.ensure((ns_total + 1) * sizeof(uint64_t));
+    if (ns_total > 0)
+      FMM_HIP(hipMemcpyAsync(t.s_all.ptr, t.sref_all.ptr, ns_total * sizeof(uint64_t),
+                             hipMemcpyDeviceToDevice, s));
+  } else {
+    // Bottom-up: S_d = sort_unique(Sref_d u N(parents(S_{d+1}))) for d = L-3 .. 1.
+    std::vector<DevBuf> lists(L);
+    std::vector<int64_t> sz(L, 0);
+    for (int d = 0; d < L; ++d) sz[d] = t.s_off[d + 1] - t.s_off[d];
+    auto copy_ref = [&](int d) {
+      lists[d].ensure((sz[d] + 1) * sizeof(uint64_t));
+      if (sz[d] > 0)
+        FMM_HIP(hipMemcpyAsync(lists[d].ptr, t.sref_all.as<uint64_t>() + t.s_off[d],
+                               sz[d] * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    };
+    copy_ref(0);
+    if (L - 2 >= 1) copy_ref(L - 2);
+    for (int d = L - 3; d >= 1; --d) {
+      const int64_t nref = t.s_off[d + 1] - t.s_off[d];
+      const int64_t nnext = sz[d + 1];
+      const int64_t nc = nref + 9 * nnext;
+      DevBuf candk, candv, fl, rk;
+      candk.ensure((nc + 1) * sizeof(uint64_t));
+      candv.ensure((nc + 1) * sizeof(uint32_t));
+      fl.ensure((nc + 1) * sizeof(int));
+      rk.ensure((nc + 1) * sizeof(int));
+      hipLaunchKernelGGL(k_bal_candidates, dim3(grid_for(nc)), dim3(BS), 0, s,
+                         t.sref_all.as<uint64_t>() + t.s_off[d], nref,
+                         lists[d + 1].as<uint64_t>(), nnext, d, candk.as<uint64_t>(),
+                         candv.as<uint32_t>());
+      FMM_LAUNCH_CHECK();
+      sort_pairs_u64(candk.as<uint64_t>(), candv.as<uint32_t>(), nc, 1024, t.pw, s);
+      hipLaunchKernelGGL(k_unique_flags, dim3(grid_for(nc)), dim3(BS), 0, s, candk.as<uint64_t>(),
+                         nc, fl.as<int>());
+      FMM_LAUNCH_CHECK();
+      exclusive_scan_i32(fl.as<int>(), rk.as<int>(), nc, cnt + CNT_UNIQ, t.pw, s);
+      FMM_HIP(hipMemcpyAsync(hc + CNT_UNIQ, cnt + CNT_UNIQ, sizeof(int), hipMemcpyDeviceToHost, s));
+      FMM_HIP(hipStreamSynchronize(s));
+      sz[d] = hc[CNT_UNIQ];
+      lists[d].ensure((sz[d] + 1) * sizeof(uint64_t));
+      hipLaunchKernelGGL(k_unique_scatter, dim3(grid_for(nc)), dim3(BS), 0, s,
+                         candk.as<uint64_t>(), fl.as<int>(), rk.as<int>(), nc, d,
+                         lists[d].as<uint64_t>());
+      FMM_LAUNCH_CHECK();
+      FMM_HIP(hipStreamSynchronize(s));
+    }
+    soff.assign(L + 1, 0);
+    for (int d = 0; d < L; ++d) soff[d + 1] = soff[d] + (d <= L - 2 ? sz[d] : 0);
+    t.s_all.ensure((soff[L] + 1) * sizeof(uint64_t));
+    for (int d = 0; d <= L - 2; ++d)
+      if (sz[d] > 0)
+        FMM_HIP(hipMemcpyAsync(t.s_all.as<uint64_t>() + soff[d], lists[d].ptr,
+                               sz[d] * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    FMM_HIP(hipStreamSynchronize(s));
+  }
+  t.s_off = soff;
+
+  // -- nodes with children (P_d = S_d u X_d) ----------------------------------------
+  std::vector<int64_t> roff(L + 1, 0);
+  roff[1] = 1;
+  for (int d = 1; d < L; ++d) roff[d + 1] = roff[d] + 4 * (soff[d] - soff[d - 1]);
+  const int64_t nr = roff[L];
+  int64_t* d_soff2 = d_soff;
+  int64_t* d_roff = d_soff + (L + 1);
+  int64_t* d_poff = d_soff + 2 * (L + 1);
+  upload(d_soff2, soff.data(), (L + 1) * sizeof(int64_t));
+  upload(d_roff, roff.data(), (L + 1) * sizeof(int64_t));
+  t.w_c.ensure(nr * (sizeof(uint64_t) + 3 * sizeof(int)) + 64);
+  uint64_t* rcode = t.w_c.as<uint64_t>();
+  int* hasc = reinterpret_cast<int*>(rcode + nr);
+  int* kind = hasc + nr;
+  int* rrank = kind + nr;
+  hipLaunchKernelGGL(k_real_flags, dim3(grid_for(nr)), dim3(BS), 0, s, t.s_all.as<uint64_t>(),
+                     d_soff2, d_roff, L, rcode, hasc, kind, cnt + CNT_PC);
+  FMM_LAUNCH_CHECK();
+  exclusive_scan_i32(hasc, rrank, nr, nullptr, t.pw, s);
+  FMM_HIP(hipMemcpyAsync(hc + CNT_PC, cnt + CNT_PC, kMaxLevels * sizeof(int),
+                         hipMemcpyDeviceToHost, s));
+  FMM_HIP(hipStreamSynchronize(s));
+  t.p_off.assign(L + 1, 0);
+  for (int d = 0; d < L; ++d) t.p_off[d + 1] = t.p_off[d] + hc[CNT_PC + d];
+  FMM_REQUIRE(hc[CNT_PC + L - 1] == 0, FMM_E_INTERNAL, "tree: deepest level has children");
+  const int64_t np_total = t.p_off[L];
+  t.p_code.ensure((np_total + 1) * sizeof(uint64_t));
+  t.p_kind.ensure((np_total + 1) * sizeof(int));
+  hipLaunchKernelGGL(k_p_scatter, dim3(grid_for(nr)), dim3(BS), 0, s, rcode, hasc, kind, rrank, nr,
+                     t.p_code.as<uint64_t>(), t.p_kind.as<int>());
+  FMM_LAUNCH_CHECK();
+  upload(d_poff, t.p_off.data(), (L + 1) * sizeof(int64_t));
+
+  // -- final node arrays ------------------------------------------------------------
+  t.lev.assign(L, TreeLevel());
+  std::vector<int> levoff(L + 1, 0);
+  int64_t off = 0;
+  for (int d = 0; d < L; ++d) {
+    int64_t nd = d == 0 ? 1 : 4 * (t.p_off[d] - t.p_off[d - 1]);
+    t.lev[d].node_off = off;
+    t.lev[d].n_nodes = nd;
+    t.lev[d].width = std::ldexp(w0, -d);
+    levoff[d] = (int)off;
+    off += nd;
+  }
+  levoff[L] = (int)off;
+  FMM_REQUIRE(off < ((int64_t)1 << 31) / kNumM2L, FMM_E_LIMIT, "tree: too many nodes (%lld)",
+              (long long)off);
+  const int NN = (int)off;
+  t.n_nodes = off;
+  t.levoff.ensure((L + 1) * sizeof(int));
+  upload(t.levoff.ptr, levoff.data(), (L + 1) * sizeof(int));
+  int* d_levoff = t.levoff.as<int>();
+  t.code.ensure(off * sizeof(uint64_t));
+  t.nlevel.ensure(off * sizeof(int));
+  t.bot.ensure(off * sizeof(int));
+  t.top.ensure(off * sizeof(int));
+  t.parent.ensure(off * sizeof(int));
+  t.child.ensure(off * sizeof(int));
+  t.flags.ensure(off * sizeof(int));
+  t.coll.ensure(off * 9 * sizeof(int));
+  t.xmid.ensure(off * sizeof(double));
+  t.ymid.ensure(off * sizeof(double));
+  t.row.ensure(off * sizeof(int));
+  t.m2l.ensure(off * kNumM2L * sizeof(int));
+  t.evalid.ensure(off * sizeof(int));
+  hipLaunchKernelGGL(k_nodes_a, dim3(grid_for(off)), dim3(BS), 0, s, t.p_code.as<uint64_t>(),
+                     t.p_kind.as<int>(), d_poff, d_levoff, L, t.code.as<uint64_t>(),
+                     t.nlevel.as<int>(), t.flags.as<int>());
+  FMM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_nodes_b, dim3(grid_for(off)), dim3(BS), 0, s, t.p_code.as<uint64_t>(),
+                     t.p_kind.as<int>(), d_poff, d_levoff, L, keys, nn, xmin, ymin, w0,
+                     t.code.as<uint64_t>(), t.flags.as<int>(), t.bot.as<int>(), t.top.as<int>(),
+                     t.parent.as<int>(), t.child.as<int>(), t.coll.as<int>(), t.xmid.as<double>(),
+                     t.ymid.as<double>());
+  FMM_LAUNCH_CHECK();
+
+  // -- density rows ------------------------------------------------------------------
+  t.w_flag.ensure((off + 1) * sizeof(int));
+  t.w_rank.ensure((off + 1) * sizeof(int));
+  flag = t.w_flag.as<int>();
+  rank = t.w_rank.as<int>();
+  const int g0 = L > 2 ? levoff[2] : NN;
+  const int ngroups = (NN - g0) / 4;
+  if (ngroups > 0) {
+    hipLaunchKernelGGL(k_group_flags, dim3(grid_for(ngroups)), dim3(BS), 0, s, t.flags.as<int>(),
+                       g0, ngroups, d_levoff, L, flag, cnt + CNT_LR);
+    FMM_LAUNCH_CHECK();
+    exclusive_scan_i32(flag, rank, ngroups, cnt + CNT_ROWS, t.pw, s);
+  }
+  t.rownode.ensure((4 * (int64_t)ngroups + 1) * sizeof(int));
+  hipLaunchKernelGGL(k_rows, dim3(grid_for(off)), dim3(BS), 0, s, flag, rank, g0, NN,
+                     t.row.as<int>(), t.rownode.as<int>());
+  FMM_LAUNCH_CHECK();
+  if (L > 2) {
+    hipLaunchKernelGGL(k_m2l_lists, dim3(grid_for(NN - g0)), dim3(BS), 0, s,
+                       t.code.as<uint64_t>(), t.flags.as<int>(), t.row.as<int>(), d_levoff, L,
+                       t.m2l.as<int>());
+    FMM_LAUNCH_CHECK();
+  }
+
+  // -- evaluation boxes and S2M boxes ---------------------------------------------------
+  t.w_a.ensure((off + 1) * 2 * sizeof(int));
+  int* f2 = t.w_a.as<int>();
+  int* r2 = f2 + off;
+  hipLaunchKernelGGL(k_flag_bit, dim3(grid_for(off)), dim3(BS), 0, s, t.flags.as<int>(), NN,
+                     (int)NF_LEAF, flag);
+  FMM_LAUNCH_CHECK();
+  exclusive_scan_i32(flag, rank, off, cnt + CNT_EVAL, t.pw, s);
+  hipLaunchKernelGGL(k_up_flag, dim3(grid_for(off)), dim3(BS), 0, s, t.flags.as<int>(),
+                     t.row.as<int>(), NN, f2);
+  FMM_LAUNCH_CHECK();
+  exclusive_scan_i32(f2, r2, off, cnt + CNT_UP, t.pw, s);
+  FMM_HIP(hipMemcpyAsync(hc + CNT_ROWS, cnt + CNT_ROWS, (CNT_N - CNT_ROWS) * sizeof(int),
+                         hipMemcpyDeviceToHost, s));
+  FMM_HIP(hipStreamSynchronize(s));
+  t.n_rows = ngroups > 0 ? 4 * (int64_t)hc[CNT_ROWS] : 0;
+  t.row_off.assign(L + 1, 0);
+  for (int d = 0; d < L; ++d) t.row_off[d + 1] = t.row_off[d] + hc[CNT_LR + d];
+  FMM_REQUIRE(t.row_off[L] == t.n_rows, FMM_E_INTERNAL, "tree: row count mismatch");
+  t.n_eval = hc[CNT_EVAL];
+  t.n_up = hc[CNT_UP];
+  t.evalnode.ensure((t.n_eval + 1) * sizeof(int));
+  t.upnodes.ensure((t.n_up + 1) * sizeof(int));
+  hipLaunchKernelGGL(k_list_scatter, dim3(grid_for(off)), dim3(BS), 0, s, flag, rank, NN,
+                     t.evalnode.as<int>(), t.evalid.as<int>());
+  FMM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_list_scatter, dim3(grid_for(off)), dim3(BS), 0, s, f2, r2, NN,
+                     t.upnodes.as<int>(), (int*)nullptr);
+  FMM_LAUNCH_CHECK();
+  t.built = true;
+}
+
+void tree_locate(const Tree& t, const double* tx, const double* ty, int64_t nt, int* out_node,
+                 hipStream_t s) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_locate, dim3(grid_for(nt)), dim3(BS), 0, s, tx, ty, nt, t.flags.as<int>(),
+                     t.child.as<int>(), t.xmid.as<double>(), t.ymid.as<double>(), out_node);
+  FMM_LAUNCH_CHECK();
+}
+
+}  // namespace fmm

@@ -1,0 +1,122 @@
+"""Kernel registry: the GPU stand-ins for the reference's user kernel functions.
+
+The reference accepts any numba-jitted ``kernel_eval(sx, sy, tx, ty)`` and
+inlines it into its loops (flexmm/kifmm2d/scalar/fmm.py:38,
+helpers.py:115-153).  A GPU evaluator cannot compile arbitrary Python, so
+``kernel_eval`` here must be one of the descriptors below.  Each is still a
+plain callable with the reference's argument convention (FP64, NumPy
+broadcasting; ``dx = tx - sx``), used on the host for the per-width
+precomputation, and carries the id of its hand-written device code.
+Anything else passed as ``kernel_eval`` raises ``NotImplementedError``.
+"""
+import numpy as np
+
+from . import _lib
+
+FOUR_PI = 4.0 * np.pi
+TWO_PI = 2.0 * np.pi
+
+
+class Kernel:
+    """A scalar kernel K(s, t) with a device implementation."""
+
+    def __init__(self, name, kernel_id, params, fn, doc):
+        self.name = name
+        self.kernel_id = kernel_id
+        self.params = tuple(float(p) for p in params)
+        self._fn = fn
+        self.__doc__ = doc
+
+    def __call__(self, sx, sy, tx, ty):
+        return self._fn(np.asarray(sx, dtype=np.float64), np.asarray(sy, dtype=np.float64),
+                        np.asarray(tx, dtype=np.float64), np.asarray(ty, dtype=np.float64))
+
+    @property
+    def key(self):
+        return (self.kernel_id, self.params)
+
+    def __repr__(self):
+        return "Kernel(%s%s)" % (self.name, "" if not self.params else ", %s" % (self.params,))
+
+
+class Evaluator:
+    """A named evaluator (the pair KAS1/KAS2 of register_evaluator,
+    helpers.py:176-238): which outputs the device L2T/P2P pass produces."""
+
+    def __init__(self, name, kernel, evaluator_id, outputs, fn, doc):
+        self.name = name
+        self.kernel = kernel
+        self.evaluator_id = evaluator_id
+        self.outputs = outputs
+        self._fn = fn
+        self.__doc__ = doc
+
+    def __call__(self, sx, sy, tx, ty):
+        return self._fn(np.asarray(sx, dtype=np.float64), np.asarray(sy, dtype=np.float64),
+                        np.asarray(tx, dtype=np.float64), np.asarray(ty, dtype=np.float64))
+
+    def __repr__(self):
+        return "Evaluator(%s, outputs=%d)" % (self.name, self.outputs)
+
+
+def _laplace(sx, sy, tx, ty):
+    dx = tx - sx
+    dy = ty - sy
+    return -np.log(dx * dx + dy * dy) / FOUR_PI
+
+
+def _laplace_grad(sx, sy, tx, ty):
+    dx = tx - sx
+    dy = ty - sy
+    d2 = dx * dx + dy * dy
+    return (-np.log(d2) / FOUR_PI, -dx / (TWO_PI * d2), -dy / (TWO_PI * d2))
+
+
+laplace = Kernel(
+    "laplace", _lib.KERNEL_LAPLACE, (), _laplace,
+    "Laplace kernel -log(|t-s|^2)/(4 pi) (examples/pure_kifmm_laplace.py:24-29).")
+
+laplace_potential = Evaluator(
+    "laplace_potential", laplace, _lib.EVAL_POTENTIAL, 1, lambda *a: (_laplace(*a),),
+    "Potential u (the default 'potential_evaluation', fmm.py:63).")
+
+laplace_gradient = Evaluator(
+    "laplace_gradient", laplace, _lib.EVAL_GRADIENT, 3, _laplace_grad,
+    "(u, du/dx_t, du/dy_t) (examples/pure_kifmm_laplace_gradient.py:32-55).")
+
+KERNELS = {k.name: k for k in (laplace,)}
+EVALUATORS = {e.name: e for e in (laplace_potential, laplace_gradient)}
+
+
+def resolve_kernel(kernel_eval):
+    if isinstance(kernel_eval, Kernel):
+        return kernel_eval
+    if isinstance(kernel_eval, str) and kernel_eval in KERNELS:
+        return KERNELS[kernel_eval]
+    raise NotImplementedError(
+        "kernel_eval=%r has no device implementation; pass one of flexmm_amd.kernels (%s)"
+        % (kernel_eval, ", ".join(sorted(KERNELS))))
+
+
+def default_evaluator(kernel):
+    for e in EVALUATORS.values():
+        if e.kernel is kernel and e.evaluator_id == _lib.EVAL_POTENTIAL:
+            return e
+    raise NotImplementedError("no potential evaluator for %r" % (kernel,))
+
+
+def resolve_evaluator(kernel, KAS1, KAS2, outputs):
+    """Map register_evaluator's (KAS1, KAS2, outputs) to a device evaluator."""
+    cands = [e for e in (KAS1, KAS2) if e is not None]
+    if not cands or not all(isinstance(e, Evaluator) for e in cands):
+        raise NotImplementedError(
+            "register_evaluator needs flexmm_amd.kernels evaluators (%s), got %r"
+            % (", ".join(sorted(EVALUATORS)), (KAS1, KAS2)))
+    ev = cands[0]
+    if any(e is not ev for e in cands):
+        raise NotImplementedError("KAS1 and KAS2 must be the same device evaluator")
+    if ev.kernel is not kernel:
+        raise ValueError("evaluator %s belongs to kernel %s, not %s" % (ev.name, ev.kernel.name, kernel.name))
+    if outputs != ev.outputs:
+        raise ValueError("evaluator %s produces %d outputs, not %d" % (ev.name, ev.outputs, outputs))
+    return ev

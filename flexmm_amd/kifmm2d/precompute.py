@@ -1,0 +1,98 @@
+"""Per-width FP64 operators of the KIFMM, computed once on the host.
+
+Mirrors ``Precomputation`` of the reference (flexmm/kifmm2d/scalar/helpers.py:
+250-317) and packs the result into the device layout of
+``fmm2d_set_operators`` (include/flexmm_amd.h):
+
+* equivalent/check circles: p equi-angular points on radii
+  r1 = w sqrt(2)/2 and r2 = w (4 - sqrt(2))/2 (helpers.py:5-15);
+* A = K(small circle -> large circle); its pseudo-inverse
+  V diag(1/s) U^T with singular values below 1e-15 dropped
+  (SVD_Solver, helpers.py:240-248) serves both S2L and L2S (helpers.py:263-267);
+* M2MC = K(children's small circles -> parent's large circle) (p x 4p,
+  children ordered (-,-), (-,+), (+,-), (+,+); helpers.py:270-284), L2LC = M2MC^T;
+* the 40 M2L matrices K(small circle shifted by (ox, oy) w -> small circle)
+  (helpers.py:288-296), stored dense.  The reference applies some of them in
+  SVD-compressed form (rank at absolute tolerance 1e-14, helpers.py:17-36); the
+  dense matrix differs from that product by at most the dropped singular
+  values, below 1e-14.
+
+These are setup computations of the plan (once per width, cached), not part
+of the evaluation path.
+"""
+import numpy as np
+
+PINV_TOL = 1e-15   # helpers.py:241
+OP_NBLK = 50
+
+
+def padded_dim(p):
+    if p < 4 or p > 64:
+        raise ValueError("Nequiv = %d outside [4, 64]" % p)
+    return 32 if p <= 32 else 64
+
+
+def m2l_slots():
+    """(ox, oy) of the 40 M2L slots: (ii, jj) over the 7x7 grid row-major,
+    skipping the 3x3 centre; offset (ii - 3, jj - 3) (tree.py:431-435)."""
+    out = []
+    for ii in range(7):
+        for jj in range(7):
+            if 2 <= ii <= 4 and 2 <= jj <= 4:
+                continue
+            out.append((ii - 3, jj - 3))
+    return out
+
+
+def circles(width, p):
+    theta = np.linspace(0, 2 * np.pi, p, endpoint=False)
+    r1 = 0.5 * width * np.sqrt(2)
+    r2 = 0.5 * width * (4 - np.sqrt(2))
+    return (r1 * np.cos(theta), r1 * np.sin(theta), r2 * np.cos(theta), r2 * np.sin(theta))
+
+
+def kernel_matrix(K, sx, sy, tx, ty):
+    """M[j, i] = K(s_i -> t_j)."""
+    return K(sx[None, :], sy[None, :], tx[:, None], ty[:, None])
+
+
+class LevelOperators:
+    """FP64 operators of one box width."""
+
+    def __init__(self, kernel, p, width):
+        self.p = p
+        self.width = float(width)
+        sx, sy, lx, ly = circles(width, p)
+        A = kernel_matrix(kernel, sx, sy, lx, ly)
+        U, S, VH = np.linalg.svd(A)
+        SI = np.where(S < PINV_TOL, 0.0, 1.0 / np.where(S < PINV_TOL, 1.0, S))
+        self.pinv1 = SI[:, None] * U.T
+        self.pinv2 = VH.T
+        self.pinv_rank = int((S >= PINV_TOL).sum())
+        ssx, ssy, _, _ = circles(0.5 * width, p)
+        q = 0.25 * width
+        cex = np.concatenate([ssx - q, ssx - q, ssx + q, ssx + q])
+        cey = np.concatenate([ssy - q, ssy + q, ssy - q, ssy + q])
+        self.M2MC = kernel_matrix(kernel, cex, cey, lx, ly)
+        self.m2l = [kernel_matrix(kernel, sx + ox * width, sy + oy * width, sx, sy)
+                    for ox, oy in m2l_slots()]
+        self.small = (sx, sy)
+        self.large = (lx, ly)
+
+    def pack(self):
+        """Flat FP64 array in the fmm2d_set_operators layout."""
+        p = self.p
+        PD = padded_dim(p)
+        blocks = np.zeros((OP_NBLK, PD, PD))
+        blocks[0, :p, :p] = self.pinv1
+        blocks[1, :p, :p] = self.pinv2
+        for c in range(4):
+            m = self.M2MC[:, c * p:(c + 1) * p]
+            blocks[2 + c, :p, :p] = m
+            blocks[6 + c, :p, :p] = m.T
+        for s, m in enumerate(self.m2l):
+            blocks[10 + s, :p, :p] = m
+        circ = np.zeros((2, PD))
+        circ[0, :p] = self.large[0]
+        circ[1, :p] = self.large[1]
+        return np.concatenate([blocks.ravel(), circ.ravel()])

@@ -1,0 +1,151 @@
+/*
+ * flexmm_amd -- C-ABI of the MI355X (gfx950) 2D kernel-independent FMM.
+ *
+ * The reference (dbstein/flexmm, kifmm2d path) exposes a Python class, not an
+ * FFI; each entry point below replaces one step of that class, cited as
+ * file:line of the reference.  All pointers named x, y, tau, tx, ty, out, dst
+ * are DEVICE pointers (for example torch.Tensor.data_ptr() of a CUDA/HIP
+ * tensor); `stream` is a hipStream_t (torch.cuda.current_stream().cuda_stream)
+ * or NULL for the default stream.  Work is enqueued on that stream; the tree
+ * build synchronises it to size its per-level arrays.
+ *
+ * Return value: 0 on success, < 0 for an argument/state/limit error
+ * (FMM2D_E_*), > 0 for a HIP runtime error (hipError_t).  The message of the
+ * last failure on the calling thread is returned by fmm2d_last_error().
+ * Nothing is thrown across the ABI.  A plan is not thread-safe; distinct
+ * plans may be used concurrently.
+ */
+#ifndef FLEXMM_AMD_H
+#define FLEXMM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FMM2D_ABI_VERSION 1
+
+/* error codes (negative) */
+#define FMM2D_E_ARG (-1)
+#define FMM2D_E_STATE (-2)
+#define FMM2D_E_LIMIT (-3)
+#define FMM2D_E_INTERNAL (-4)
+
+/* kernels (the reference's user kernel_eval, fmm.py:38; examples/pure_kifmm_laplace.py:24-29) */
+#define FMM2D_KERNEL_LAPLACE 0 /* -log(|x-y|^2)/(4 pi) */
+
+/* evaluators (register_evaluator, helpers.py:176-238) */
+#define FMM2D_EVAL_POTENTIAL 0 /* 1 output: u */
+#define FMM2D_EVAL_GRADIENT 1  /* 3 outputs: u, du/dx_t, du/dy_t
+                                  (examples/pure_kifmm_laplace_gradient.py:32-55) */
+
+/* precision */
+#define FMM2D_PREC_FP64 0
+
+/* tree arrays for fmm2d_tree_export */
+#define FMM2D_TREE_KEYS 0       /* u64[n]   Morton keys of the sorted sources            */
+#define FMM2D_TREE_PERM 1       /* u32[n]   tree position -> input index (Tree.ordv)      */
+#define FMM2D_TREE_SX 2         /* f64[n]   sorted x (Tree.x)                             */
+#define FMM2D_TREE_SY 3         /* f64[n]   sorted y (Tree.y)                             */
+#define FMM2D_TREE_CODE 4       /* u64[NN]  Morton code of each node in its level          */
+#define FMM2D_TREE_LEVEL 5      /* i32[NN]                                                */
+#define FMM2D_TREE_BOT 6        /* i32[NN]  Level.bot_ind                                 */
+#define FMM2D_TREE_TOP 7        /* i32[NN]  Level.top_ind                                 */
+#define FMM2D_TREE_PARENT 8     /* i32[NN]  global parent index                           */
+#define FMM2D_TREE_CHILD 9      /* i32[NN]  global first child or -1 (Level.children_ind) */
+#define FMM2D_TREE_FLAGS 10     /* i32[NN]  bits: 1 leaf, 2 fake_leaf, 4 Xlist,
+                                   8 compute_upwards, 16 compute_downwards,
+                                   32 has children, 64 has sources                       */
+#define FMM2D_TREE_COLLEAGUES 11 /* i32[NN*9] colleagues, slot (dx+1)*3+(dy+1)           */
+#define FMM2D_TREE_XMID 12      /* f64[NN]                                                */
+#define FMM2D_TREE_YMID 13      /* f64[NN]                                                */
+#define FMM2D_TREE_ROW 14       /* i32[NN]  multipole row (this_density_ind) or -1        */
+#define FMM2D_TREE_M2L 15       /* i32[NN*40] multipole rows of the interaction list     */
+#define FMM2D_TREE_EVALID 16    /* i32[NN]  index among leaves or -1                      */
+
+/* expansion arrays for fmm2d_export_expansions (rows of PD doubles) */
+#define FMM2D_EXP_CHECK 0       /* S2M check potentials (upward partial multipoles)   */
+#define FMM2D_EXP_MULTIPOLE 1   /* equivalent densities (FMM.multipoles)              */
+#define FMM2D_EXP_LOCAL 2       /* local equivalent densities (FMM.Local_Expansions)  */
+#define FMM2D_EXP_TAU_SORTED 3  /* charges in tree order (FMM.tau_ordered)            */
+
+/* phases timed by fmm2d_set_profiling / fmm2d_phase_times */
+#define FMM2D_PHASE_TREE 0        /* fmm2d_tree_build                         */
+#define FMM2D_PHASE_UPWARD 1      /* fmm2d_upward                             */
+#define FMM2D_PHASE_DOWNWARD 2    /* fmm2d_downward                           */
+#define FMM2D_PHASE_EVALUATE 3    /* fmm2d_evaluate (locate, sort, L2T + P2P) */
+#define FMM2D_PHASE_EVAL_KERNEL 4 /* the fused L2T + P2P kernel alone         */
+#define FMM2D_PHASE_S2M 5         /* the S2M kernel alone                     */
+#define FMM2D_NUM_PHASES 6
+
+int fmm2d_abi_version(void);
+const char* fmm2d_last_error(void);
+
+/* Create/destroy a plan: one FMM for one kernel and Nequiv = p (4 <= p <= 64).
+ * Replaces FMM.__init__'s helper lookup (fmm.py:38-43, 61-65). */
+int fmm2d_plan_create(int kernel_id, const double* kparams, int nkparams, int p, int precision,
+                      void** plan);
+int fmm2d_plan_destroy(void* plan);
+
+/* Build the quadtree of n sources in the square box bbox = [xmin, xmax, ymin,
+ * ymax] (width xmax - xmin) with leaf capacity ncut.
+ * Replaces Tree(x, y, Ncutoff, bbox) (fmm.py:68-72, tree.py:451-518). */
+int fmm2d_tree_build(void* plan, const double* x, const double* y, int64_t n, const double* bbox,
+                     int64_t ncut, void* stream);
+
+/* Host summary: out[0..7] = levels L, nodes, multipole rows, leaves, S2M boxes,
+ * sources, row stride PD, ncut; then per level d: out[8+3d] node offset,
+ * out[9+3d] node count, out[10+3d] first multipole row; out[8+3L] = rows.
+ * Needs nout >= 9 + 3L.  Replaces the Tree.Levels attributes (tree.py:275-308). */
+int fmm2d_tree_info(void* plan, int64_t* out, int64_t nout);
+
+/* Copy one tree array (FMM2D_TREE_*) into a device buffer of `bytes` bytes. */
+int fmm2d_tree_export(void* plan, int which, void* dst, int64_t bytes, void* stream);
+
+/* Doubles per level of the operator block for Nequiv = p, or -1. */
+int64_t fmm2d_operator_size(int p);
+
+/* Set the precomputed FP64 operators of one tree level from a device array of
+ * fmm2d_operator_size(p) doubles (copied; the caller may free it afterwards):
+ * 50 blocks of [PD][PD] -- diag(1/s)U^T, V, the 4 M2M column blocks of M2MC,
+ * their 4 transposes (L2L), the 40 M2L matrices in slot order -- then the
+ * large-circle x and y (PD each).  PD = 32 for p <= 32, else 64.
+ * Replaces Precomputation (helpers.py:250-317). */
+int fmm2d_set_operators(void* plan, int level, const double* dev_ops, int64_t n, void* stream);
+int fmm2d_clear_operators(void* plan);
+
+/* Upward pass from charges tau (input order, n values).
+ * Replaces FMM.upwards_pass (fmm.py:73-124). */
+int fmm2d_upward(void* plan, const double* tau, int64_t n, void* stream);
+
+/* Downward pass.  Replaces FMM.downwards_pass (fmm.py:125-166). */
+int fmm2d_downward(void* plan, void* stream);
+
+/* Evaluate at nt targets into out[outputs][nt] (input target order).
+ * check != 0 skips sources that coincide exactly with the target.
+ * Replaces FMM.evaluate_to_points (fmm.py:182-199). */
+int fmm2d_evaluate(void* plan, const double* tx, const double* ty, int64_t nt, int evaluator,
+                   int check, double* out, void* stream);
+
+/* Copy an expansion array (FMM2D_EXP_*) into a device buffer of nvals doubles. */
+int fmm2d_export_expansions(void* plan, int which, double* dst, int64_t nvals, void* stream);
+
+/* Phase timing with HIP events recorded on the caller's stream (off by
+ * default).  fmm2d_phase_times waits for the events and writes the last
+ * duration of each FMM2D_PHASE_* in milliseconds (-1 if not recorded). */
+int fmm2d_set_profiling(void* plan, int enable);
+
+/* Algorithmic work of the current tree and the last evaluation (synchronous):
+ * out[0] P2P pairs, [1] targets with an L2T (leaf level >= 2), [2] targets,
+ * [3] M2L box pairs, [4] S2M source points, [5] multipole rows (upward
+ * pseudo-inverse applies), [6] local expansions (downward pseudo-inverse
+ * applies), [7] M2M parents, [8] L2L children.  Needs n >= 9. */
+int fmm2d_work_counts(void* plan, int64_t* out, int n);
+int fmm2d_phase_times(void* plan, double* ms, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLEXMM_AMD_H */

@@ -1,0 +1,102 @@
+"""FMM parity: device potentials against the oracle restatement (FP64, 1e-12
+relative in max norm, the error metric of examples/pure_kifmm_laplace.py:149-151)
+and against exact FP64 direct sums."""
+import numpy as np
+import pytest
+
+from oracle import direct as odirect
+from oracle import kernels as okern
+from oracle import kifmm2d as okifmm
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+PARITY_TOL = 1e-12
+
+
+def _fmm(x, y, ncut, p, bbox=None):
+    from flexmm_amd import kernels
+    from flexmm_amd.kifmm2d import FMM, the_help
+    the_help.clear()
+    return FMM(x, y, kernels.laplace, ncut, p, bbox=bbox)
+
+
+def _oracle(x, y, ncut, p, bbox=None):
+    return okifmm.FMM(x, y, okern.laplace, ncut, p, bbox=bbox)
+
+
+@pytest.mark.parametrize("p", [32, 60])
+def test_self_eval_uniform(p):
+    x, y, tau = H.uniform(30000, 1)
+    print("self-eval uniform p", p, flush=True)
+    f = _fmm(x, y, 64, p)
+    f.build_expansions(tau)
+    u = f.source_evaluation(x, y)[0]
+    o = _oracle(x, y, 64, p)
+    o.build_expansions(tau)
+    uo = o.source_evaluation(x, y)[0]
+    err = H.rel_max_err(u, uo)
+    print("p", p, "err vs oracle %.3e" % err)
+    assert err < PARITY_TOL
+
+
+def test_target_eval_clustered_grid():
+    x, y, tau = H.clustered(30000, 5, 0.01, 3)
+    tx, ty = H.grid(120)
+    print("clustered grid", flush=True)
+    f = _fmm(x, y, 50, 60, bbox=[0, 1, 0, 1])
+    f.build_expansions(tau)
+    u = f.target_evaluation(tx, ty)[0]
+    o = _oracle(x, y, 50, 60, bbox=[0, 1, 0, 1])
+    o.build_expansions(tau)
+    uo = o.target_evaluation(tx, ty)[0]
+    err = H.rel_max_err(u, uo)
+    print("err vs oracle %.3e" % err)
+    assert err < PARITY_TOL
+    ref = odirect.direct(okern.laplace, x, y, tau, tx[::7], ty[::7])[0]
+    assert H.rel_max_err(u[::7], ref) < 1e-12
+
+
+def test_gradient_evaluator():
+    from flexmm_amd import kernels
+    x, y, tau = H.uniform(20000, 8)
+    tx, ty, _ = H.uniform(5000, 9)
+    f = _fmm(x, y, 64, 60)
+    f.register_evaluator(kernels.laplace_gradient, kernels.laplace_gradient, "gradient", 3)
+    f.build_expansions(tau)
+    g = f.evaluate_to_points(tx, ty, "gradient", False)
+    o = _oracle(x, y, 64, 60)
+    o.register_evaluator(okern.laplace_gradient, "gradient", 3)
+    o.build_expansions(tau)
+    go = o.evaluate_to_points(tx, ty, "gradient", False)
+    for k in range(3):
+        err = H.rel_max_err(g[k], go[k])
+        print("output", k, "err %.3e" % err)
+        assert err < PARITY_TOL
+    ref = odirect.direct(okern.laplace_gradient, x, y, tau, tx, ty)
+    for k in range(3):
+        assert H.rel_max_err(g[k], ref[k]) < 1e-11
+
+
+@pytest.mark.parametrize("n,ncut", [(50, 64), (100, 30), (300, 40)])
+def test_shallow_trees_are_direct(n, ncut):
+    x, y, tau = H.uniform(n, n)
+    f = _fmm(x, y, ncut, 32)
+    f.build_expansions(tau)
+    u = f.source_evaluation(x, y)[0]
+    o = _oracle(x, y, ncut, 32)
+    o.build_expansions(tau)
+    uo = o.source_evaluation(x, y)[0]
+    assert H.rel_max_err(u, uo) < PARITY_TOL
+
+
+def test_direct_sum_p64():
+    x, y, tau = H.uniform(40000, 12)
+    f = _fmm(x, y, 64, 64)
+    f.build_expansions(tau)
+    u = f.source_evaluation(x, y)[0]
+    sel = np.arange(0, x.size, 37)
+    ref = odirect.direct(okern.laplace, x, y, tau, x[sel], y[sel], check=True)[0]
+    err = H.rel_max_err(u[sel], ref)
+    print("err vs direct %.3e" % err)
+    assert err < 5e-13

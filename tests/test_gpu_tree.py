@@ -1,0 +1,90 @@
+"""Quadtree parity: the device tree against the oracle restatement of
+flexmm/tree.py, compared canonically (nodes keyed by (level, ix, iy); flags,
+point sets, colleagues, M2L partners, and the FP64 node centres bit for bit)."""
+import numpy as np
+import pytest
+
+from oracle import tree as otree
+from oracle import kifmm2d as okifmm
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _build(x, y, ncut, p=32, bbox=None):
+    from flexmm_amd import kernels
+    from flexmm_amd.kifmm2d import FMM, the_help
+    the_help.clear()
+    return FMM(x, y, kernels.laplace, ncut, p, bbox=bbox)
+
+
+def _oracle_tree(x, y, ncut, bbox):
+    return otree.Tree(x, y, ncut, bbox)
+
+
+CASES = {
+    "uniform_20k": lambda: (*H.uniform(20000, 11)[:2], 64, None),
+    "uniform_100k_c1": lambda: (*H.uniform(100000, 1)[:2], 64, None),
+    "clustered_30k": lambda: (*H.clustered(30000, 5, 0.01, 3)[:2], 50, [0, 1, 0, 1]),
+    "clustered_deep": lambda: (*H.clustered(40000, 3, 0.001, 4)[:2], 20, [0, 1, 0, 1]),
+    "tiny_root_leaf": lambda: (*H.uniform(50, 5)[:2], 64, None),
+    "small_two_levels": lambda: (*H.uniform(100, 6)[:2], 30, None),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_tree_matches_oracle(case):
+    x, y, ncut, bbox = CASES[case]()
+    print("case", case, "n", x.size, flush=True)
+    f = _build(x, y, ncut, bbox=bbox)
+    ora = _oracle_tree(x, y, ncut, f.bbox)
+    assert f.bbox == okifmm.get_bbox(x, y, bbox)
+    n = H.compare_canonical(H.gpu_canonical(f), ora.canonical())
+    assert n == sum(l.n_node for l in ora.Levels)
+
+
+def test_tree_dyadic_ties():
+    """Points exactly on cell mid-lines go to the low child (x > xmid is strict)."""
+    g = np.arange(0, 64) / 64.0
+    x, y = np.meshgrid(g, g, indexing="ij")
+    x = np.concatenate([x.ravel(), [0.5, 0.25, 0.75]])
+    y = np.concatenate([y.ravel(), [0.5, 0.5, 0.125]])
+    f = _build(x, y, 8, bbox=[0, 1, 0, 1])
+    ora = _oracle_tree(x, y, 8, f.bbox)
+    H.compare_canonical(H.gpu_canonical(f), ora.canonical())
+
+
+def test_tree_duplicates_below_cutoff():
+    x, y, _ = H.uniform(5000, 7)
+    x = np.concatenate([x, np.full(10, x[0])])
+    y = np.concatenate([y, np.full(10, y[0])])
+    f = _build(x, y, 32, bbox=None)
+    ora = _oracle_tree(x, y, 32, f.bbox)
+    H.compare_canonical(H.gpu_canonical(f), ora.canonical())
+
+
+def test_tree_uniform_counts_full():
+    """A full uniform tree: levels 0..L-1 hold 4^d nodes and the M2L lists hold
+    the closed-form number of pairs."""
+    x, y, _ = H.uniform(262144, 21)
+    # 256 points per level-5 box on average (all split), 64 per level-6 leaf
+    f = _build(x, y, 150, bbox=[0, 1, 0, 1])
+    t = f.tree
+    assert t.level_nodes == [4 ** d for d in range(t.levels)]
+    m2l = f.tree_array("m2l").cpu().numpy().reshape(-1, 40)
+    # per level: sum over boxes of the ring partners that lie inside the domain
+    for d in range(2, t.levels):
+        n = 2 ** d
+        g0 = t.node_off[d]
+        cnt = int((m2l[g0:g0 + 4 ** d] >= 0).sum())
+        ix, iy = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+        expect = 0
+        for ox in range(-3, 4):
+            for oy in range(-3, 4):
+                if abs(ox) <= 1 and abs(oy) <= 1:
+                    continue
+                cx, cy = ix & 1, iy & 1
+                ok = (ox >= -2 - cx) & (ox <= 3 - cx) & (oy >= -2 - cy) & (oy <= 3 - cy)
+                ok &= (ix + ox >= 0) & (ix + ox < n) & (iy + oy >= 0) & (iy + oy < n)
+                expect += int(ok.sum())
+        assert cnt == expect, (d, cnt, expect)
