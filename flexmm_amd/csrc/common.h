@@ -109,37 +109,40 @@ __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + 
 // ---------------------------------------------------------------------------
 // Table-driven FP64 logarithm (P. T. P. Tang, "Table-driven implementation of
 // the logarithm function in IEEE floating-point arithmetic", ACM TOMS 16(4),
-// 1990), in the FMA variant: x = m 2^e with m in [1, 2); j = top 7 fraction
-// bits of m; r = fma(m, 1/c_j, -1) with |r| < 2^-8; log(x) = e ln2 - log(1/c_j)
-// + log1p(r), log1p by its Taylor polynomial to r^7.  Table entries are
-// {1/c_j, -log(1/c_j)} with c_j = 1 + (j + 1/2)/128, computed on the host in
-// extended precision.  Accuracy ~2 ulp for normal x; log(0) = -inf.
-constexpr int kLogTableSize = 128;
+// 1990), in an FMA variant split for accumulation:
+//   x = m 2^e, m in [0.5, 1) (frexp);  j = top 8 fraction bits of m;
+//   r = fma(m, T_j, -1) with T_j = 2/c_j, c_j = 1 + (j + 1/2)/256, |r| < 2^-9;
+//   log(x) = e ln2 + t,  t = -log(T_j) + log1p(r),
+// with log1p by its Taylor polynomial to r^5 (truncation < 1e-17).  Callers
+// accumulate sum(w e) and sum(w t) separately and multiply the first by ln 2
+// once.  The table {T_j, -log(T_j)} is filled on the host in extended precision.
+constexpr int kLogTableSize = 256;
+constexpr double kLn2 = 0.6931471805599453;
 
 struct LogTableHost {
   static void fill(double2* out);  // out[kLogTableSize]
 };
 
 #if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
+// log(x) = e * ln2 + t (x > 0).  For x == 0, e = 0 and t is finite garbage:
+// callers handle coincident points themselves.
+__device__ __forceinline__ void log_split(double x, const double2* __restrict__ tab, double& e,
+                                          double& t) {
+  const double m = __builtin_amdgcn_frexp_mant(x);
+  e = (double)__builtin_amdgcn_frexp_exp(x);
+  const int j = (__double2hiint(m) >> 12) & (kLogTableSize - 1);
+  const double2 c = tab[j];
+  const double r = __builtin_fma(m, c.x, -1.0);
+  double q = __builtin_fma(r, 0.2, -0.25);
+  q = __builtin_fma(r, q, 1.0 / 3.0);
+  q = __builtin_fma(r, q, -0.5);
+  t = __builtin_fma(r * r, q, r) + c.y;
+}
+
 __device__ __forceinline__ double log_tab(double x, const double2* __restrict__ tab) {
-  const double LN2_HI = 6.93147180369123816490e-01;  // ln 2 with 32 significant bits
-  const double LN2_LO = 1.90821492927058770002e-10;  // ln 2 - LN2_HI
-  double m = __builtin_amdgcn_frexp_mant(x);         // [0.5, 1)
-  int e = __builtin_amdgcn_frexp_exp(x);
-  m = m + m;                                          // [1, 2)
-  double ed = (double)(e - 1);
-  int hi = __double2hiint(m);
-  int j = (hi >> 13) & (kLogTableSize - 1);
-  double2 t = tab[j];
-  double r = __builtin_fma(m, t.x, -1.0);
-  double p = __builtin_fma(r, 1.0 / 7.0, -1.0 / 6.0);
-  p = __builtin_fma(r, p, 1.0 / 5.0);
-  p = __builtin_fma(r, p, -1.0 / 4.0);
-  p = __builtin_fma(r, p, 1.0 / 3.0);
-  p = __builtin_fma(r, p, -0.5);
-  double r2 = r * r;
-  double lp = __builtin_fma(r2, p, r);
-  double res = __builtin_fma(ed, LN2_HI, t.y) + __builtin_fma(ed, LN2_LO, lp);
+  double e, t;
+  log_split(x, tab, e, t);
+  const double res = __builtin_fma(e, kLn2, t);
   return x == 0.0 ? -__builtin_inf() : res;
 }
 #endif

@@ -38,9 +38,9 @@ const char* last_error() { return g_err.c_str(); }
 void LogTableHost::fill(double2* out) {
   for (int j = 0; j < kLogTableSize; ++j) {
     long double c = 1.0L + ((long double)j + 0.5L) / kLogTableSize;
-    double inv = (double)(1.0L / c);
-    long double lg = -logl((long double)inv);
-    out[j] = make_double2(inv, (double)lg);
+    double T = (double)(2.0L / c);
+    long double lg = -logl((long double)T);
+    out[j] = make_double2(T, (double)lg);
   }
 }
 
@@ -78,7 +78,7 @@ struct Plan {
   int ops_levels = 0;
   std::vector<char> ops_set;
   DevBuf logtab;
-  DevBuf tau_s, check, mult, le;
+  DevBuf tau_s, src, check, mult, le, ple;
   bool have_exp = false;
   // targets
   DevBuf tnode, tcnt, toff, tcur, tperm, tsx, tsy, icnt, ioff, items, ntot;
@@ -109,7 +109,33 @@ __device__ __forceinline__ void load_logtab(double2* tab, const double2* __restr
 }
 
 // ---------------------------------------------------------------------------
-// S2M: one wave per leaf, lane k = check point k on the large circle.
+// Sources in tree order packed as (x, y, tau, 0): one 32-byte record per
+// source, read by wave-uniform (scalar) loads in the S2M and P2P loops.
+__global__ void k_pack_sources(const double* __restrict__ tau, const uint32_t* __restrict__ perm,
+                               const double* __restrict__ sx, const double* __restrict__ sy,
+                               int64_t n, double* __restrict__ tau_s,
+                               double4* __restrict__ packed) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double t = tau[perm[i]];
+  tau_s[i] = t;
+  packed[i] = make_double4(sx[i], sy[i], t, 0.0);
+}
+
+// Accumulate w * log(|t - s|^2) = w * (e ln2 + lt) as (ae += w e, at += w lt).
+struct LogAcc {
+  double ae = 0.0, at = 0.0;
+  __device__ __forceinline__ void add(double w, double d2, const double2* __restrict__ tab) {
+    double e, lt;
+    log_split(d2, tab, e, lt);
+    ae = __builtin_fma(w, e, ae);
+    at = __builtin_fma(w, lt, at);
+  }
+  __device__ __forceinline__ double value() const { return __builtin_fma(ae, kLn2, at); }
+};
+
+// S2M: one wave per leaf, lane k = check point k on the large circle
+// (helpers.py:159-173; the source is taken relative to the box centre).
 template <int PD>
 __global__ __launch_bounds__(64) void k_s2m(const int* __restrict__ upnodes,
                                             const int* __restrict__ nlevel,
@@ -117,9 +143,7 @@ __global__ __launch_bounds__(64) void k_s2m(const int* __restrict__ upnodes,
                                             const int* __restrict__ row,
                                             const double* __restrict__ xmid,
                                             const double* __restrict__ ymid,
-                                            const double* __restrict__ sx,
-                                            const double* __restrict__ sy,
-                                            const double* __restrict__ tau,
+                                            const double4* __restrict__ src,
                                             const double* __restrict__ ops, int64_t opsz, int p,
                                             const double2* __restrict__ logtab,
                                             double* __restrict__ check) {
@@ -138,14 +162,26 @@ __global__ __launch_bounds__(64) void k_s2m(const int* __restrict__ upnodes,
   const double cx = xmid[g], cy = ymid[g];
   const int b = __builtin_amdgcn_readfirstlane(bot[g]);
   const int t = __builtin_amdgcn_readfirstlane(top[g]);
-  double acc = 0.0;
-  for (int j = b; j < t; ++j) {
-    const double dx = ax - (sx[j] - cx);   // source relative to the box centre, helpers.py:168-169
-    const double dy = ay - (sy[j] - cy);
-    const double d2 = __builtin_fma(dy, dy, dx * dx);
-    acc = __builtin_fma(tau[j], log_tab(d2, tab), acc);
+  LogAcc acc;
+  int j = b;
+  for (; j + 4 <= t; j += 4) {
+    double4 s4[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) s4[u] = src[j + u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double dx = ax - (s4[u].x - cx);
+      const double dy = ay - (s4[u].y - cy);
+      acc.add(s4[u].z, __builtin_fma(dy, dy, dx * dx), tab);
+    }
   }
-  if (k < PD) check[(int64_t)r * PD + k] = k < p ? -kInv4Pi * acc : 0.0;
+  for (; j < t; ++j) {
+    const double4 sj = src[j];
+    const double dx = ax - (sj.x - cx);
+    const double dy = ay - (sj.y - cy);
+    acc.add(sj.z, __builtin_fma(dy, dy, dx * dx), tab);
+  }
+  if (k < PD) check[(int64_t)r * PD + k] = k < p ? -kInv4Pi * acc.value() : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -153,19 +189,45 @@ __global__ __launch_bounds__(64) void k_s2m(const int* __restrict__ upnodes,
 // B[l/16][l%16] and D[4v + l/16][l%16] (v = 0..3).  A PD-row result therefore
 // sits in lane l as rows 4m + l/16 (m = 4 rb + v), which is exactly the
 // B-operand layout of the next product: results chain without shuffles.
+//
+// A block size: 4 waves, 16 columns (boxes) per wave, every wave computes all
+// PD rows of its 16 columns.  A sequence of "phases" acc += A_i * B_i runs
+// through a double-buffered LDS stage for A_i: while phase i multiplies, the
+// 16-byte global loads of A_{i+1} and the B gathers of phase i+1 are in
+// flight, and one barrier per phase separates the buffers.
 template <int PD>
-__device__ __forceinline__ void lds_load_A(double* __restrict__ A, const double* __restrict__ G) {
-  for (int idx = threadIdx.x; idx < PD * PD; idx += blockDim.x) {
-    int r = idx / PD;
-    int c = idx - r * PD;
-    A[r * LDA + c] = G[idx];
+struct Mfma {
+  static constexpr int RB = PD / 16;     // 16-row blocks
+  static constexpr int KS = PD / 4;      // k-steps of 4
+  static constexpr int LDA = PD + 4;     // = 4 (mod 32): the 16 rows x 4 columns of an A
+                                         // fragment fall into distinct LDS banks
+  static constexpr int NA2 = PD * PD / 2 / 256;   // double2 per thread for one A block
+};
+
+typedef double dv2 __attribute__((ext_vector_type(2)));
+
+template <int PD>
+struct AStage {
+  dv2 v[Mfma<PD>::NA2];
+  __device__ __forceinline__ void load(const double* __restrict__ G) {
+    const dv2* G2 = reinterpret_cast<const dv2*>(G);
+#pragma unroll
+    for (int i = 0; i < Mfma<PD>::NA2; ++i) v[i] = G2[threadIdx.x + i * 256];
   }
-}
+  __device__ __forceinline__ void store(double* __restrict__ A) const {
+#pragma unroll
+    for (int i = 0; i < Mfma<PD>::NA2; ++i) {
+      const int idx = 2 * (threadIdx.x + i * 256);
+      const int r = idx / PD, c = idx - r * PD;
+      *reinterpret_cast<dv2*>(A + r * Mfma<PD>::LDA + c) = v[i];
+    }
+  }
+};
 
 template <int PD>
 __device__ __forceinline__ void mfma_block(d4 (&acc)[PD / 16], const double* __restrict__ A,
                                            const double (&b)[PD / 4]) {
-  constexpr int RB = PD / 16, KS = PD / 4;
+  constexpr int RB = Mfma<PD>::RB, KS = Mfma<PD>::KS, LDA = Mfma<PD>::LDA;
   const int l = threadIdx.x & 63;
   const double* a0 = A + (l & 15) * LDA + (l >> 4);
 #pragma unroll
@@ -182,137 +244,211 @@ __device__ __forceinline__ void zero_acc(d4 (&acc)[PD / 16]) {
   for (int rb = 0; rb < PD / 16; ++rb) acc[rb] = d4{0.0, 0.0, 0.0, 0.0};
 }
 
-// acc <- pinv2 * (pinv1 * acc), both operators staged through A.
+// acc (D layout) from / to a row of PD doubles
 template <int PD>
-__device__ __forceinline__ void apply_pinv(d4 (&acc)[PD / 16], double* __restrict__ A,
-                                           const double* __restrict__ opsL) {
-  constexpr int RB = PD / 16, KS = PD / 4;
+__device__ __forceinline__ void load_acc(d4 (&acc)[PD / 16], const double* __restrict__ rowp) {
+  const int kq = (threadIdx.x & 63) >> 4;
 #pragma unroll
-  for (int ph = 0; ph < 2; ++ph) {
-    double b[KS];
+  for (int rb = 0; rb < PD / 16; ++rb)
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) b[ks] = acc[ks >> 2][ks & 3];
-    __syncthreads();
-    lds_load_A<PD>(A, opsL + (int64_t)(OP_PINV1 + ph) * PD * PD);
-    __syncthreads();
-    zero_acc<PD>(acc);
-    mfma_block<PD>(acc, A, b);
+    for (int v = 0; v < 4; ++v) acc[rb][v] = rowp ? rowp[16 * rb + 4 * v + kq] : 0.0;
+}
+template <int PD>
+__device__ __forceinline__ void store_acc(const d4 (&acc)[PD / 16], double* __restrict__ rowp) {
+  const int kq = (threadIdx.x & 63) >> 4;
+#pragma unroll
+  for (int rb = 0; rb < PD / 16; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) rowp[16 * rb + 4 * v + kq] = acc[rb][v];
+}
+
+// One phase: operator block A (PD x PD, global) and either a per-lane source
+// row (B column = PD doubles, or null for zeros) or the accumulator itself
+// (chain: acc <- A * acc).
+struct Phase {
+  const double* A;
+  const double* brow;
+  bool chain;
+};
+
+// `next()` returns the phases in order (it may keep state).
+template <int PD, class Next>
+__device__ __forceinline__ void run_phases(int nph, d4 (&acc)[PD / 16], double* __restrict__ buf0,
+                                           double* __restrict__ buf1, Next& next) {
+  constexpr int KS = Mfma<PD>::KS;
+  if (nph <= 0) return;
+  const int kq = (threadIdx.x & 63) >> 4;
+  AStage<PD> st;
+  double bn[KS];
+  Phase ps = next();
+  st.load(ps.A);
+  bool chain_n = ps.chain;
+  bool any_n = ps.chain || __ballot(ps.brow != nullptr) != 0;
+  if (!ps.chain) {
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) bn[ks] = ps.brow ? ps.brow[4 * ks + kq] : 0.0;
   }
-  (void)RB;
+  for (int i = 0; i < nph; ++i) {
+    double* A = (i & 1) ? buf1 : buf0;
+    st.store(A);
+    const bool chain = chain_n, any = any_n;
+    double bc[KS];
+    if (chain) {
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) bc[ks] = acc[ks >> 2][ks & 3];
+      zero_acc<PD>(acc);
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) bc[ks] = bn[ks];
+    }
+    __syncthreads();
+    if (i + 1 < nph) {
+      ps = next();
+      st.load(ps.A);
+      chain_n = ps.chain;
+      any_n = ps.chain || __ballot(ps.brow != nullptr) != 0;
+      if (!ps.chain) {
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) bn[ks] = ps.brow ? ps.brow[4 * ks + kq] : 0.0;
+      }
+    }
+    if (any) mfma_block<PD>(acc, A, bc);
+  }
 }
 
 // ---------------------------------------------------------------------------
-// Upward, one level: rows [row0, row0 + nrows); 64 rows per block, 16 per wave.
+// Upward, one level: rows [row0, row0 + nrows); 64 rows (boxes) per block.
+// check = S2M result (leaves) or M2M of the four children (parents), then
+// multipole = pinv2 * (pinv1 * check)  (fmm.py:98-122).
 template <int PD>
-__global__ __launch_bounds__(256) void k_upward(int64_t row0, int nrows, int with_m2m,
-                                               const int* __restrict__ rownode,
-                                               const int* __restrict__ flags,
-                                               const int* __restrict__ child,
-                                               const int* __restrict__ row,
-                                               const double* __restrict__ check,
-                                               const double* __restrict__ opsL,
-                                               double* __restrict__ mult) {
-  __shared__ double A[PD * LDA];
-  constexpr int RB = PD / 16, KS = PD / 4;
+__global__ __launch_bounds__(256) void k_mult_level(int64_t row0, int nrows, int with_m2m,
+                                                   const int* __restrict__ rownode,
+                                                   const int* __restrict__ flags,
+                                                   const int* __restrict__ child,
+                                                   const int* __restrict__ row,
+                                                   const double* __restrict__ check,
+                                                   const double* __restrict__ opsL,
+                                                   double* __restrict__ mult) {
+  __shared__ __attribute__((aligned(16))) double buf[2][PD * Mfma<PD>::LDA];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int col = l & 15, kq = l >> 4;
-  const int rl = blockIdx.x * 64 + w * 16 + col;
+  const int rl = blockIdx.x * 64 + w * 16 + (l & 15);
   const bool active = rl < nrows;
   const int64_t r = row0 + rl;
   const int g = active ? rownode[r] : 0;
   const int f = active ? flags[g] : 0;
-  int crow = -1;
-  if (active && (f & NF_CHILDREN)) crow = row[child[g]];
-  d4 acc[RB];
-#pragma unroll
-  for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-      acc[rb][v] = (active && (f & NF_UP)) ? check[r * PD + 16 * rb + 4 * v + kq] : 0.0;
-  if (with_m2m) {
-    for (int c = 0; c < 4; ++c) {
-      __syncthreads();
-      lds_load_A<PD>(A, opsL + (int64_t)(OP_M2M + c) * PD * PD);
-      __syncthreads();
-      double b[KS];
-#pragma unroll
-      for (int ks = 0; ks < KS; ++ks)
-        b[ks] = crow >= 0 ? mult[(int64_t)(crow + c) * PD + 4 * ks + kq] : 0.0;
-      mfma_block<PD>(acc, A, b);
-    }
-  }
-  apply_pinv<PD>(acc, A, opsL);
-  if (active) {
-#pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) mult[r * PD + 16 * rb + 4 * v + kq] = acc[rb][v];
-  }
+  const int crow = (active && (f & NF_CHILDREN)) ? row[child[g]] : -1;
+  d4 acc[PD / 16];
+  load_acc<PD>(acc, (active && (f & NF_UP)) ? check + r * PD : nullptr);
+  const int nm = with_m2m ? 4 : 0;
+  int i = 0;
+  auto next = [&]() -> Phase {
+    const int k = i++;
+    if (k < nm)
+      return Phase{opsL + (int64_t)(OP_M2M + k) * PD * PD,
+                   crow >= 0 ? mult + (int64_t)(crow + k) * PD : nullptr, false};
+    return Phase{opsL + (int64_t)(OP_PINV1 + (k - nm)) * PD * PD, nullptr, true};
+  };
+  run_phases<PD>(nm + 2, acc, buf[0], buf[1], next);
+  if (active) store_acc<PD>(acc, mult + r * PD);
 }
 
 // ---------------------------------------------------------------------------
-// Downward, one level.  Blocks are tiled by child position c so that all
-// columns of a block share the 27 M2L offsets that can occur for c; block
-// (tile, c) handles the c-th child of sibling groups [64 tile, 64 tile + 64).
+// M2L of all levels in one launch (the translations only read multipoles).
+// Blocks of a level are tiled by child position c so that all columns of a
+// block share the 27 offsets that can occur for c; block (tile, c) handles the
+// c-th child of sibling groups [64 tile, 64 tile + 64).  Offsets absent from
+// every column of a block are skipped.
+struct LevelTable {
+  int nlev;
+  int d[kMaxLevels];
+  int g0[kMaxLevels];
+  int ngroups[kMaxLevels];
+  int boff[kMaxLevels + 1];
+};
+
+struct ValidSlots {
+  int n[4];
+  int slot[4][kNumM2L];
+};
+__constant__ ValidSlots c_valid;
+
 template <int PD>
-__global__ __launch_bounds__(256) void k_downward(int g0, int ngroups, int do_l2l,
-                                                 const int* __restrict__ flags,
-                                                 const int* __restrict__ parent,
-                                                 const int* __restrict__ m2l,
-                                                 const double* __restrict__ mult,
-                                                 const double* __restrict__ opsL,
-                                                 const double* __restrict__ opsParent,
-                                                 double* __restrict__ le) {
-  __shared__ double A[PD * LDA];
-  constexpr int RB = PD / 16, KS = PD / 4;
-  const int c = blockIdx.x & 3;
-  const int tile = blockIdx.x >> 2;
+__global__ __launch_bounds__(256) void k_m2l_all(LevelTable lt, const int* __restrict__ flags,
+                                                const int* __restrict__ m2l,
+                                                const double* __restrict__ mult,
+                                                const double* __restrict__ ops, int64_t opsz,
+                                                double* __restrict__ ple) {
+  __shared__ __attribute__((aligned(16))) double buf[2][PD * Mfma<PD>::LDA];
+  __shared__ unsigned long long smask;
+  int li = 0;
+  while (li + 1 < lt.nlev && (int)blockIdx.x >= lt.boff[li + 1]) ++li;
+  const int lb = blockIdx.x - lt.boff[li];
+  const int c = lb & 3, tile = lb >> 2;
+  const int ngroups = lt.ngroups[li];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int col = l & 15, kq = l >> 4;
-  const int q = tile * 64 + w * 16 + col;
-  const int g = g0 + 4 * q + c;
-  const bool active = q < ngroups && !(flags[q < ngroups ? g : g0] & NF_FAKE);
-  d4 acc[RB];
-  zero_acc<PD>(acc);
-  if (do_l2l) {
-    __syncthreads();
-    // L2LC of the parent's width (fmm.py:142,162)
-    lds_load_A<PD>(A, opsParent + (int64_t)(OP_L2L + c) * PD * PD);
-    __syncthreads();
-    const int pg = active ? parent[g] : -1;
-    double b[KS];
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) b[ks] = pg >= 0 ? le[(int64_t)pg * PD + 4 * ks + kq] : 0.0;
-    mfma_block<PD>(acc, A, b);
-  }
-  const int cx = c >> 1, cy = c & 1;
+  const int q = tile * 64 + w * 16 + (l & 15);
+  const int g = lt.g0[li] + 4 * q + c;
+  const bool active = q < ngroups && !(flags[q < ngroups ? g : lt.g0[li]] & NF_FAKE);
   const int* lst = m2l + (int64_t)g * kNumM2L;
-  for (int s = 0; s < kNumM2L; ++s) {
-    const int ox = c_slots.ox[s], oy = c_slots.oy[s];
-    if (ox < -2 - cx || ox > 3 - cx || oy < -2 - cy || oy > 3 - cy) continue;   // block-uniform
+  const int nv = c_valid.n[c];
+  if (threadIdx.x == 0) smask = 0ull;
+  __syncthreads();
+  unsigned long long wm = 0ull;
+  for (int i = 0; i < nv; ++i) {
+    const int src = active ? lst[c_valid.slot[c][i]] : -1;
+    if (__ballot(src >= 0)) wm |= 1ull << i;
+  }
+  if (l == 0 && wm) atomicOr(&smask, wm);
+  __syncthreads();
+  unsigned long long rem = smask;
+  const int nph = __popcll(rem);
+  const double* opsL = ops + (int64_t)lt.d[li] * opsz;
+  d4 acc[PD / 16];
+  zero_acc<PD>(acc);
+  auto next = [&]() -> Phase {
+    const int i = __builtin_ctzll(rem);
+    rem &= rem - 1ull;
+    const int s = c_valid.slot[c][i];
     const int src = active ? lst[s] : -1;
-    __syncthreads();
-    lds_load_A<PD>(A, opsL + (int64_t)(OP_M2L + s) * PD * PD);
-    __syncthreads();
-    if (__ballot(src >= 0) == 0) continue;   // wave-uniform
-    double b[KS];
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) b[ks] = src >= 0 ? mult[(int64_t)src * PD + 4 * ks + kq] : 0.0;
-    mfma_block<PD>(acc, A, b);
-  }
-  apply_pinv<PD>(acc, A, opsL);
-  if (active) {
-#pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) le[(int64_t)g * PD + 16 * rb + 4 * v + kq] = acc[rb][v];
-  }
+    return Phase{opsL + (int64_t)(OP_M2L + s) * PD * PD,
+                 src >= 0 ? mult + (int64_t)src * PD : nullptr, false};
+  };
+  run_phases<PD>(nph, acc, buf[0], buf[1], next);
+  if (active) store_acc<PD>(acc, ple + (int64_t)g * PD);
 }
 
-__global__ void k_gather_tau(const double* __restrict__ tau, const uint32_t* __restrict__ perm,
-                             int64_t n, double* __restrict__ out) {
-  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) out[i] = tau[perm[i]];
+// ---------------------------------------------------------------------------
+// Downward, one level: PLE += L2L(parent LE) (parent width's L2LC,
+// fmm.py:142,162), LE = pinv2 * (pinv1 * PLE)  (fmm.py:156).  Tiled by child
+// position c like the M2L.
+template <int PD>
+__global__ __launch_bounds__(256) void k_local_level(int g0, int ngroups, int do_l2l,
+                                                    const int* __restrict__ flags,
+                                                    const int* __restrict__ parent,
+                                                    const double* __restrict__ ple,
+                                                    const double* __restrict__ opsL,
+                                                    const double* __restrict__ opsParent,
+                                                    double* __restrict__ le) {
+  __shared__ __attribute__((aligned(16))) double buf[2][PD * Mfma<PD>::LDA];
+  const int c = blockIdx.x & 3, tile = blockIdx.x >> 2;
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int q = tile * 64 + w * 16 + (l & 15);
+  const int g = g0 + 4 * q + c;
+  const bool active = q < ngroups && !(flags[q < ngroups ? g : g0] & NF_FAKE);
+  d4 acc[PD / 16];
+  load_acc<PD>(acc, active ? ple + (int64_t)g * PD : nullptr);
+  const int pg = (active && do_l2l) ? parent[g] : -1;
+  const int nl = do_l2l ? 1 : 0;
+  int i = 0;
+  auto next = [&]() -> Phase {
+    const int k = i++;
+    if (k < nl)
+      return Phase{opsParent + (int64_t)(OP_L2L + c) * PD * PD,
+                   pg >= 0 ? le + (int64_t)pg * PD : nullptr, false};
+    return Phase{opsL + (int64_t)(OP_PINV1 + (k - nl)) * PD * PD, nullptr, true};
+  };
+  run_phases<PD>(nl + 2, acc, buf[0], buf[1], next);
+  if (active) store_acc<PD>(acc, le + (int64_t)g * PD);
 }
 
 // ---------------------------------------------------------------------------
@@ -372,17 +508,40 @@ __global__ __launch_bounds__(256) void k_make_items(const int* __restrict__ icnt
 
 // ---------------------------------------------------------------------------
 // L2T + P2P, one wave per (box, chunk of 64 targets); lane = target.
-template <int PD, int NOUT>
+template <int NOUT, bool CHECK>
+struct Target {
+  double x, y;
+  LogAcc u;
+  double gx = 0.0, gy = 0.0;
+  bool zhit = false;
+  // one source (or one equivalent-density point) at (sx, sy) with weight w
+  __device__ __forceinline__ void pair(double sx, double sy, double w,
+                                       const double2* __restrict__ tab) {
+    const double dx = x - sx;
+    const double dy = y - sy;
+    const double d2 = __builtin_fma(dy, dy, dx * dx);
+    const bool z = d2 == 0.0;
+    if (CHECK) w = z ? 0.0 : w;   // coincident source skipped (helpers.py:214)
+    else zhit |= z;
+    u.add(w, d2, tab);
+    if (NOUT == 3) {
+      const double qi = z ? 0.0 : w / d2;
+      gx = __builtin_fma(qi, dx, gx);
+      gy = __builtin_fma(qi, dy, gy);
+    }
+  }
+};
+
+template <int PD, int NOUT, bool CHECK>
 __global__ __launch_bounds__(64) void k_eval(
     const int2* __restrict__ items, const int* __restrict__ nitems,
     const int* __restrict__ evalnode, const int* __restrict__ toff,
     const int* __restrict__ tperm, const double* __restrict__ tsx, const double* __restrict__ tsy,
     const int* __restrict__ nlevel, const double* __restrict__ xmid,
     const double* __restrict__ ymid, const int* __restrict__ coll, const int* __restrict__ bot,
-    const int* __restrict__ top, const double* __restrict__ sx, const double* __restrict__ sy,
-    const double* __restrict__ tau, const double* __restrict__ le,
-    const double* __restrict__ ops, int64_t opsz, int p, const double2* __restrict__ logtab,
-    int check, int64_t nt, double* __restrict__ out) {
+    const int* __restrict__ top, const double4* __restrict__ src,
+    const double* __restrict__ le, const double* __restrict__ ops, int64_t opsz, int p,
+    const double2* __restrict__ logtab, int64_t nt, double* __restrict__ out) {
   __shared__ double2 tab[kLogTableSize];
   if ((int)blockIdx.x >= *nitems) return;
   load_logtab(tab, logtab);
@@ -393,29 +552,22 @@ __global__ __launch_bounds__(64) void k_eval(
   const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
   const int t = t0 + threadIdx.x;
   const bool act = t < t1;
-  const double x = act ? tsx[t] : 0.0;
-  const double y = act ? tsy[t] : 0.0;
-  double u = 0.0, ux = 0.0, uy = 0.0;
+  Target<NOUT, CHECK> T;
+  T.x = act ? tsx[t] : 0.0;
+  T.y = act ? tsy[t] : 0.0;
   // L2T: helpers.py:219-235 (the equivalent density sits on the large circle)
   const int d = nlevel[g];
   if (d >= 2) {
-    const double xr = x - xmid[g];
-    const double yr = y - ymid[g];
+    Target<NOUT, false> L;
+    L.x = T.x - xmid[g];
+    L.y = T.y - ymid[g];
     const double* lx = ops + d * opsz + (int64_t)OP_NBLK * PD * PD;
     const double* ly = lx + PD;
     const double* lg = le + (int64_t)g * PD;
-    for (int k = 0; k < p; ++k) {
-      const double dx = xr - lx[k];
-      const double dy = yr - ly[k];
-      const double d2 = __builtin_fma(dy, dy, dx * dx);
-      const double q = lg[k];
-      u = __builtin_fma(q, log_tab(d2, tab), u);
-      if (NOUT == 3) {
-        const double qi = q / d2;
-        ux = __builtin_fma(qi, dx, ux);
-        uy = __builtin_fma(qi, dy, uy);
-      }
-    }
+    for (int k = 0; k < p; ++k) L.pair(lx[k], ly[k], lg[k], tab);
+    T.u = L.u;
+    T.gx = L.gx;
+    T.gy = L.gy;
   }
   // P2P over the leaf's colleagues: helpers.py:180-217
   const int* cl = coll + (int64_t)g * 9;
@@ -424,27 +576,27 @@ __global__ __launch_bounds__(64) void k_eval(
     if (c < 0) continue;
     const int b = __builtin_amdgcn_readfirstlane(bot[c]);
     const int e = __builtin_amdgcn_readfirstlane(top[c]);
-    for (int j = b; j < e; ++j) {
-      const double sxj = sx[j], syj = sy[j], tj = tau[j];
-      const double dx = x - sxj;
-      const double dy = y - syj;
-      const double d2 = __builtin_fma(dy, dy, dx * dx);
-      const bool skip = check && sxj == x && syj == y;
-      const double lg2 = log_tab(d2, tab);
-      u = skip ? u : __builtin_fma(tj, lg2, u);
-      if (NOUT == 3) {
-        const double qi = skip ? 0.0 : tj / d2;
-        ux = __builtin_fma(qi, dx, ux);
-        uy = __builtin_fma(qi, dy, uy);
-      }
+    int j = b;
+    for (; j + 4 <= e; j += 4) {
+      double4 s4[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s4[u] = src[j + u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) T.pair(s4[u].x, s4[u].y, s4[u].z, tab);
+    }
+    for (; j < e; ++j) {
+      const double4 sj = src[j];
+      T.pair(sj.x, sj.y, sj.z, tab);
     }
   }
   if (act) {
     const int64_t o = tperm[t];
-    out[o] = -kInv4Pi * u;
+    double u = -kInv4Pi * T.u.value();
+    if (!CHECK && T.zhit) u = __builtin_inf();   // log(0): the reference's -log(0) = +inf
+    out[o] = u;
     if (NOUT == 3) {
-      out[nt + o] = -kInv2Pi * ux;
-      out[2 * nt + o] = -kInv2Pi * uy;
+      out[nt + o] = -kInv2Pi * T.gx;
+      out[2 * nt + o] = -kInv2Pi * T.gy;
     }
   }
 }
@@ -515,8 +667,10 @@ static void upward(Plan& P, const double* tau, hipStream_t s) {
   const int PD = P.PD;
   const int64_t opsz = ops_per_level(PD);
   P.tau_s.ensure(t.n * sizeof(double));
-  hipLaunchKernelGGL(k_gather_tau, dim3(grid256(t.n)), dim3(256), 0, s, tau, t.perm.as<uint32_t>(),
-                     t.n, P.tau_s.as<double>());
+  P.src.ensure(t.n * sizeof(double4));
+  hipLaunchKernelGGL(k_pack_sources, dim3(grid256(t.n)), dim3(256), 0, s, tau,
+                     t.perm.as<uint32_t>(), t.sx.as<double>(), t.sy.as<double>(), t.n,
+                     P.tau_s.as<double>(), P.src.as<double4>());
   FMM_LAUNCH_CHECK();
   const int64_t nr = t.n_rows;
   P.check.ensure((nr + 1) * PD * sizeof(double));
@@ -530,8 +684,7 @@ static void upward(Plan& P, const double* tau, hipStream_t s) {
     auto kern = PD == 32 ? k_s2m<32> : k_s2m<64>;
     hipLaunchKernelGGL(kern, dim3(t.n_up), dim3(64), 0, s, t.upnodes.as<int>(), t.nlevel.as<int>(),
                        t.bot.as<int>(), t.top.as<int>(), t.row.as<int>(), t.xmid.as<double>(),
-                       t.ymid.as<double>(), t.sx.as<double>(), t.sy.as<double>(),
-                       P.tau_s.as<double>(), P.ops.as<double>(), opsz, P.p,
+                       t.ymid.as<double>(), P.src.as<double4>(), P.ops.as<double>(), opsz, P.p,
                        P.logtab.as<double2>(), P.check.as<double>());
     FMM_LAUNCH_CHECK();
   }
@@ -541,7 +694,7 @@ static void upward(Plan& P, const double* tau, hipStream_t s) {
     const int nrows = (int)(t.row_off[d + 1] - r0);
     if (nrows == 0) continue;
     const double* opsL = P.ops.as<double>() + d * opsz;
-    auto kern = PD == 32 ? k_upward<32> : k_upward<64>;
+    auto kern = PD == 32 ? k_mult_level<32> : k_mult_level<64>;
     hipLaunchKernelGGL(kern, dim3(ceil_div(nrows, 64)), dim3(256), 0, s, r0, nrows,
                        d < t.L - 1 ? 1 : 0, t.rownode.as<int>(), t.flags.as<int>(),
                        t.child.as<int>(), t.row.as<int>(), P.check.as<double>(), opsL,
@@ -557,16 +710,39 @@ static void downward(Plan& P, hipStream_t s) {
   const int PD = P.PD;
   const int64_t opsz = ops_per_level(PD);
   P.le.ensure((t.n_nodes + 1) * PD * sizeof(double));
+  P.ple.ensure((t.n_nodes + 1) * PD * sizeof(double));
   FMM_HIP(hipMemsetAsync(P.le.ptr, 0, (t.n_nodes + 1) * PD * sizeof(double), s));
+  if (t.L <= 2) {
+    P.have_exp = true;
+    return;
+  }
+  LevelTable lt{};
+  int nb = 0;
+  for (int d = 2; d < t.L; ++d) {
+    const int ng = (int)(t.lev[d].n_nodes / 4);
+    lt.d[lt.nlev] = d;
+    lt.g0[lt.nlev] = (int)t.lev[d].node_off;
+    lt.ngroups[lt.nlev] = ng;
+    lt.boff[lt.nlev] = nb;
+    nb += 4 * (int)ceil_div(ng, 64);
+    ++lt.nlev;
+  }
+  lt.boff[lt.nlev] = nb;
+  {
+    auto kern = PD == 32 ? k_m2l_all<32> : k_m2l_all<64>;
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, lt, t.flags.as<int>(), t.m2l.as<int>(),
+                       P.mult.as<double>(), P.ops.as<double>(), opsz, P.ple.as<double>());
+    FMM_LAUNCH_CHECK();
+  }
   for (int d = 2; d < t.L; ++d) {
     const int ngroups = (int)(t.lev[d].n_nodes / 4);
     if (ngroups == 0) continue;
     const double* opsL = P.ops.as<double>() + d * opsz;
-    auto kern = PD == 32 ? k_downward<32> : k_downward<64>;
+    auto kern = PD == 32 ? k_local_level<32> : k_local_level<64>;
     hipLaunchKernelGGL(kern, dim3(4 * ceil_div(ngroups, 64)), dim3(256), 0, s,
                        (int)t.lev[d].node_off, ngroups, d >= 3 ? 1 : 0, t.flags.as<int>(),
-                       t.parent.as<int>(), t.m2l.as<int>(), P.mult.as<double>(), opsL,
-                       opsL - opsz, P.le.as<double>());
+                       t.parent.as<int>(), P.ple.as<double>(), opsL, opsL - opsz,
+                       P.le.as<double>());
     FMM_LAUNCH_CHECK();
   }
   P.have_exp = true;
@@ -613,16 +789,20 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   FMM_LAUNCH_CHECK();
   const int PD = P.PD;
   const int64_t opsz = ops_per_level(PD);
-  auto kern = ev == EVAL_POT ? (PD == 32 ? k_eval<32, 1> : k_eval<64, 1>)
-                             : (PD == 32 ? k_eval<32, 3> : k_eval<64, 3>);
+  decltype(&k_eval<32, 1, false>) kern;
+  if (ev == EVAL_POT)
+    kern = check ? (PD == 32 ? k_eval<32, 1, true> : k_eval<64, 1, true>)
+                 : (PD == 32 ? k_eval<32, 1, false> : k_eval<64, 1, false>);
+  else
+    kern = check ? (PD == 32 ? k_eval<32, 3, true> : k_eval<64, 3, true>)
+                 : (PD == 32 ? k_eval<32, 3, false> : k_eval<64, 3, false>);
   P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
   hipLaunchKernelGGL(kern, dim3(icap), dim3(64), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
                      t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(), P.tsx.as<double>(),
                      P.tsy.as<double>(), t.nlevel.as<int>(), t.xmid.as<double>(),
                      t.ymid.as<double>(), t.coll.as<int>(), t.bot.as<int>(), t.top.as<int>(),
-                     t.sx.as<double>(), t.sy.as<double>(), P.tau_s.as<double>(),
-                     P.le.as<double>(), P.ops.as<double>(), opsz, P.p, P.logtab.as<double2>(),
-                     check, nt, out);
+                     P.src.as<double4>(), P.le.as<double>(), P.ops.as<double>(), opsz, P.p,
+                     P.logtab.as<double2>(), nt, out);
   FMM_LAUNCH_CHECK();
   P.mark(FMM2D_PHASE_EVAL_KERNEL, true, s);
 }
@@ -684,6 +864,15 @@ int fmm2d_plan_create(int kernel_id, const double* kparams, int nkparams, int p,
     fmm::SlotTable st;
     for (int s = 0; s < fmm::kNumM2L; ++s) fmm::m2l_slot_offset(s, &st.ox[s], &st.oy[s]);
     FMM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fmm::c_slots), &st, sizeof st));
+    fmm::ValidSlots vs{};
+    for (int c = 0; c < 4; ++c) {
+      const int cx = c >> 1, cy = c & 1;
+      for (int s2 = 0; s2 < fmm::kNumM2L; ++s2)
+        if (st.ox[s2] >= -2 - cx && st.ox[s2] <= 3 - cx && st.oy[s2] >= -2 - cy &&
+            st.oy[s2] <= 3 - cy)
+          vs.slot[c][vs.n[c]++] = s2;
+    }
+    FMM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fmm::c_valid), &vs, sizeof vs));
     double2 tab[fmm::kLogTableSize];
     fmm::LogTableHost::fill(tab);
     P->logtab.ensure(sizeof tab);
