@@ -201,39 +201,42 @@ struct Mfma {
   static constexpr int KS = PD / 4;      // k-steps of 4
   static constexpr int LDA = PD + 4;     // = 4 (mod 32): the 16 rows x 4 columns of an A
                                          // fragment fall into distinct LDS banks
-  static constexpr int NA2 = PD * PD / 2 / 256;   // double2 per thread for one A block
 };
 
 typedef double dv2 __attribute__((ext_vector_type(2)));
 
-template <int PD>
+// Register stage of one PD x PD operator block, loaded by all NT threads of
+// the block with 16-byte loads.
+template <int PD, int NT = 256>
 struct AStage {
-  dv2 v[Mfma<PD>::NA2];
+  static constexpr int N2 = PD * PD / 2 / NT;
+  dv2 v[N2];
   __device__ __forceinline__ void load(const double* __restrict__ G) {
     const dv2* G2 = reinterpret_cast<const dv2*>(G);
 #pragma unroll
-    for (int i = 0; i < Mfma<PD>::NA2; ++i) v[i] = G2[threadIdx.x + i * 256];
+    for (int i = 0; i < N2; ++i) v[i] = G2[threadIdx.x + i * NT];
   }
   __device__ __forceinline__ void store(double* __restrict__ A) const {
 #pragma unroll
-    for (int i = 0; i < Mfma<PD>::NA2; ++i) {
-      const int idx = 2 * (threadIdx.x + i * 256);
+    for (int i = 0; i < N2; ++i) {
+      const int idx = 2 * (threadIdx.x + i * NT);
       const int r = idx / PD, c = idx - r * PD;
       *reinterpret_cast<dv2*>(A + r * Mfma<PD>::LDA + c) = v[i];
     }
   }
 };
 
-template <int PD>
-__device__ __forceinline__ void mfma_block(d4 (&acc)[PD / 16], const double* __restrict__ A,
-                                           const double (&b)[PD / 4]) {
-  constexpr int RB = Mfma<PD>::RB, KS = Mfma<PD>::KS, LDA = Mfma<PD>::LDA;
+// acc[rb] += A[rows of row block rb0 + rb] * B, rb < NRB
+template <int PD, int NRB = PD / 16>
+__device__ __forceinline__ void mfma_block(d4 (&acc)[NRB], const double* __restrict__ A,
+                                           const double (&b)[PD / 4], int rb0 = 0) {
+  constexpr int KS = Mfma<PD>::KS, LDA = Mfma<PD>::LDA;
   const int l = threadIdx.x & 63;
-  const double* a0 = A + (l & 15) * LDA + (l >> 4);
+  const double* a0 = A + (rb0 * 16 + (l & 15)) * LDA + (l >> 4);
 #pragma unroll
   for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
+    for (int rb = 0; rb < NRB; ++rb)
       acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[rb * 16 * LDA + ks * 4], b[ks], acc[rb], 0,
                                                      0, 0);
 }
@@ -372,13 +375,16 @@ struct ValidSlots {
 };
 __constant__ ValidSlots c_valid;
 
+// 512 threads: wave w handles columns 16 (w & 3) .. +16 and row half w >> 2.
 template <int PD>
-__global__ __launch_bounds__(256) void k_m2l_all(LevelTable lt, const int* __restrict__ flags,
+__global__ __launch_bounds__(512) void k_m2l_all(LevelTable lt, const int* __restrict__ flags,
                                                 const int* __restrict__ m2l,
                                                 const double* __restrict__ mult,
                                                 const double* __restrict__ ops, int64_t opsz,
                                                 double* __restrict__ ple) {
+  constexpr int NT = 512, KS = Mfma<PD>::KS, HRB = Mfma<PD>::RB / 2;
   __shared__ __attribute__((aligned(16))) double buf[2][PD * Mfma<PD>::LDA];
+  __shared__ int srcs[27][64];   // interaction-list entries of the block's 64 columns
   __shared__ unsigned long long smask;
   int li = 0;
   while (li + 1 < lt.nlev && (int)blockIdx.x >= lt.boff[li + 1]) ++li;
@@ -386,35 +392,67 @@ __global__ __launch_bounds__(256) void k_m2l_all(LevelTable lt, const int* __res
   const int c = lb & 3, tile = lb >> 2;
   const int ngroups = lt.ngroups[li];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int q = tile * 64 + w * 16 + (l & 15);
+  const int half = w >> 2;
+  const int col = (w & 3) * 16 + (l & 15);
+  const int q = tile * 64 + col;
   const int g = lt.g0[li] + 4 * q + c;
   const bool active = q < ngroups && !(flags[q < ngroups ? g : lt.g0[li]] & NF_FAKE);
   const int* lst = m2l + (int64_t)g * kNumM2L;
   const int nv = c_valid.n[c];
+  const int kq = l >> 4;
   if (threadIdx.x == 0) smask = 0ull;
   __syncthreads();
-  unsigned long long wm = 0ull;
-  for (int i = 0; i < nv; ++i) {
-    const int src = active ? lst[c_valid.slot[c][i]] : -1;
-    if (__ballot(src >= 0)) wm |= 1ull << i;
+  if (half == 0) {
+    unsigned long long wm = 0ull;
+    for (int i = 0; i < nv; ++i) {
+      const int src = active ? lst[c_valid.slot[c][i]] : -1;
+      if (kq == 0) srcs[i][col] = src;
+      if (__ballot(src >= 0)) wm |= 1ull << i;
+    }
+    if (l == 0 && wm) atomicOr(&smask, wm);
   }
-  if (l == 0 && wm) atomicOr(&smask, wm);
   __syncthreads();
   unsigned long long rem = smask;
   const int nph = __popcll(rem);
   const double* opsL = ops + (int64_t)lt.d[li] * opsz;
-  d4 acc[PD / 16];
-  zero_acc<PD>(acc);
-  auto next = [&]() -> Phase {
-    const int i = __builtin_ctzll(rem);
-    rem &= rem - 1ull;
-    const int s = c_valid.slot[c][i];
-    const int src = active ? lst[s] : -1;
-    return Phase{opsL + (int64_t)(OP_M2L + s) * PD * PD,
-                 src >= 0 ? mult + (int64_t)src * PD : nullptr, false};
-  };
-  run_phases<PD>(nph, acc, buf[0], buf[1], next);
-  if (active) store_acc<PD>(acc, ple + (int64_t)g * PD);
+  d4 acc[HRB];
+#pragma unroll
+  for (int rb = 0; rb < HRB; ++rb) acc[rb] = d4{0.0, 0.0, 0.0, 0.0};
+  if (nph > 0) {
+    AStage<PD, NT> st;
+    double bn[KS];
+    auto fetch = [&](bool& any) {
+      const int i = __builtin_ctzll(rem);
+      rem &= rem - 1ull;
+      const int s = c_valid.slot[c][i];
+      st.load(opsL + (int64_t)(OP_M2L + s) * PD * PD);
+      const int src = srcs[i][col];
+      const double* brow = src >= 0 ? mult + (int64_t)src * PD : nullptr;
+      any = __ballot(brow != nullptr) != 0;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) bn[ks] = brow ? brow[4 * ks + kq] : 0.0;
+    };
+    bool any_n;
+    fetch(any_n);
+    for (int i = 0; i < nph; ++i) {
+      double* A = (i & 1) ? buf[1] : buf[0];
+      st.store(A);
+      const bool any = any_n;
+      double bc[KS];
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) bc[ks] = bn[ks];
+      __syncthreads();
+      if (i + 1 < nph) fetch(any_n);
+      if (any) mfma_block<PD, HRB>(acc, A, bc, half * HRB);
+    }
+  }
+  if (active) {
+    double* rowp = ple + (int64_t)g * PD;
+#pragma unroll
+    for (int rb = 0; rb < HRB; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) rowp[16 * (half * HRB + rb) + 4 * v + kq] = acc[rb][v];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -453,6 +491,9 @@ __global__ __launch_bounds__(256) void k_local_level(int g0, int ngroups, int do
 
 // ---------------------------------------------------------------------------
 // Targets: locate, count per evaluation box, scatter into box order.
+// Each target gets its box and its rank inside the box.  Lanes of a wave that
+// land in the same box share one atomic (targets arrive spatially coherent, so
+// a wave touches few boxes).
 __global__ __launch_bounds__(256) void k_locate_count(const double* __restrict__ tx,
                                                      const double* __restrict__ ty, int64_t nt,
                                                      const int* __restrict__ flags,
@@ -460,30 +501,49 @@ __global__ __launch_bounds__(256) void k_locate_count(const double* __restrict__
                                                      const double* __restrict__ xmid,
                                                      const double* __restrict__ ymid,
                                                      const int* __restrict__ evalid,
-                                                     int* __restrict__ tbox, int* __restrict__ tcnt) {
-  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= nt) return;
-  const double x = tx[i], y = ty[i];
-  int g = 0;
-  for (int it = 0; it < kMaxLevels && !(flags[g] & NF_LEAF); ++it)
-    g = child[g] + 2 * (x > xmid[g]) + (y > ymid[g]);   // tree.py:720-725
-  const int e = evalid[g];
-  tbox[i] = e;
-  atomicAdd(&tcnt[e], 1);
+                                                     int* __restrict__ tbox, int* __restrict__ trank,
+                                                     int* __restrict__ tcnt) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool act = i < nt;
+  int e = -1;
+  if (act) {
+    const double x = tx[i], y = ty[i];
+    int g = 0;
+    for (int it = 0; it < kMaxLevels && !(flags[g] & NF_LEAF); ++it)
+      g = child[g] + 2 * (x > xmid[g]) + (y > ymid[g]);   // tree.py:720-725
+    e = evalid[g];
+  }
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  unsigned long long pending = __ballot(act);
+  int rank = 0;
+  while (pending) {
+    const int leader = __builtin_ctzll(pending);
+    const int key = __shfl(e, leader, 64);
+    const unsigned long long same = __ballot(act && e == key) & pending;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&tcnt[key], __popcll(same));
+    base = __shfl(base, leader, 64);
+    if (same >> lane & 1ull) rank = base + __popcll(same & below);
+    pending &= ~same;
+  }
+  if (act) {
+    tbox[i] = e;
+    trank[i] = rank;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_target_scatter(const double* __restrict__ tx,
                                                        const double* __restrict__ ty, int64_t nt,
                                                        const int* __restrict__ tbox,
+                                                       const int* __restrict__ trank,
                                                        const int* __restrict__ toff,
-                                                       int* __restrict__ tcur,
                                                        int* __restrict__ tperm,
                                                        double* __restrict__ tsx,
                                                        double* __restrict__ tsy) {
   int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= nt) return;
-  const int e = tbox[i];
-  const int pos = toff[e] + atomicAdd(&tcur[e], 1);
+  const int pos = toff[tbox[i]] + trank[i];
   tperm[pos] = (int)i;
   tsx[pos] = tx[i];
   tsy[pos] = ty[i];
@@ -730,7 +790,7 @@ static void downward(Plan& P, hipStream_t s) {
   lt.boff[lt.nlev] = nb;
   {
     auto kern = PD == 32 ? k_m2l_all<32> : k_m2l_all<64>;
-    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s, lt, t.flags.as<int>(), t.m2l.as<int>(),
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(512), 0, s, lt, t.flags.as<int>(), t.m2l.as<int>(),
                        P.mult.as<double>(), P.ops.as<double>(), opsz, P.ple.as<double>());
     FMM_LAUNCH_CHECK();
   }
@@ -759,7 +819,7 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   P.tnode.ensure(nt * sizeof(int));
   P.tcnt.ensure((ne + 1) * sizeof(int));
   P.toff.ensure((ne + 1) * sizeof(int));
-  P.tcur.ensure((ne + 1) * sizeof(int));
+  P.tcur.ensure((nt + 1) * sizeof(int));   // per-target rank in its box
   P.tperm.ensure(nt * sizeof(int));
   P.tsx.ensure(nt * sizeof(double));
   P.tsy.ensure(nt * sizeof(double));
@@ -769,15 +829,14 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   P.items.ensure(icap * sizeof(int2));
   P.ntot.ensure(4 * sizeof(int));
   FMM_HIP(hipMemsetAsync(P.tcnt.ptr, 0, (ne + 1) * sizeof(int), s));
-  FMM_HIP(hipMemsetAsync(P.tcur.ptr, 0, (ne + 1) * sizeof(int), s));
   hipLaunchKernelGGL(k_locate_count, dim3(grid256(nt)), dim3(256), 0, s, tx, ty, nt,
                      t.flags.as<int>(), t.child.as<int>(), t.xmid.as<double>(), t.ymid.as<double>(),
-                     t.evalid.as<int>(), P.tnode.as<int>(), P.tcnt.as<int>());
+                     t.evalid.as<int>(), P.tnode.as<int>(), P.tcur.as<int>(), P.tcnt.as<int>());
   FMM_LAUNCH_CHECK();
   // toff has ne + 1 entries: scanning ne + 1 counts (the last is zero) gives toff[ne] = nt.
   exclusive_scan_i32(P.tcnt.as<int>(), P.toff.as<int>(), ne + 1, nullptr, P.pw, s);
   hipLaunchKernelGGL(k_target_scatter, dim3(grid256(nt)), dim3(256), 0, s, tx, ty, nt,
-                     P.tnode.as<int>(), P.toff.as<int>(), P.tcur.as<int>(), P.tperm.as<int>(),
+                     P.tnode.as<int>(), P.tcur.as<int>(), P.toff.as<int>(), P.tperm.as<int>(),
                      P.tsx.as<double>(), P.tsy.as<double>());
   FMM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_item_counts, dim3(grid256(ne)), dim3(256), 0, s, P.tcnt.as<int>(), (int)ne,
@@ -873,6 +932,7 @@ int fmm2d_plan_create(int kernel_id, const double* kparams, int nkparams, int p,
           vs.slot[c][vs.n[c]++] = s2;
     }
     FMM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fmm::c_valid), &vs, sizeof vs));
+    fmm::tree_init_device();
     double2 tab[fmm::kLogTableSize];
     fmm::LogTableHost::fill(tab);
     P->logtab.ensure(sizeof tab);

@@ -88,7 +88,8 @@ namespace {
 constexpr int CHUNK = 4096;
 constexpr int SORT_BS = 256;
 
-enum { C_NEXT = 0, C_FINAL = 1, C_CHUNKS = 2, C_ACTIVE = 3, C_COUNT = 4 };
+enum { C_NEXT = 0, C_FINAL = 1, C_CHUNKS = 2, C_ACTIVE = 3, C_FINAL_BIG = 4, C_COUNT = 5 };
+constexpr int kWaveSortMax = 64;
 
 __global__ __launch_bounds__(1024) void k_make_chunks(const int2* __restrict__ seg, int nseg,
                                                      int4* __restrict__ chunks,
@@ -139,29 +140,54 @@ __global__ __launch_bounds__(SORT_BS) void k_chunk_hist(const uint64_t* __restri
 __global__ __launch_bounds__(SORT_BS) void k_seg_offsets(
     const int2* __restrict__ seg, const int* __restrict__ segChunk0, int* __restrict__ chunkHist,
     int threshold, int last_round, int2* __restrict__ segNext, int2* __restrict__ fin,
-    int* __restrict__ counters) {
+    int finBigOff, int* __restrict__ counters) {
   __shared__ int lds[SORT_BS / 64];
   const int si = blockIdx.x;
   const int d = threadIdx.x;
   int2 sg = seg[si];
   int c0 = segChunk0[si];
   int nc = (int)ceil_div(sg.y, CHUNK);
+  // per-digit exclusive prefix over the segment's chunks, 16 chunks per batch
+  // so that the loads of a batch are in flight together
+  constexpr int B = 16;
   int run = 0;
-  for (int c = c0; c < c0 + nc; ++c) {
+  int c = c0;
+  for (; c + B <= c0 + nc; c += B) {
+    int h[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) h[u] = chunkHist[(int64_t)(c + u) * 256 + d];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      chunkHist[(int64_t)(c + u) * 256 + d] = run;
+      run += h[u];
+    }
+  }
+  for (; c < c0 + nc; ++c) {
     int h = chunkHist[(int64_t)c * 256 + d];
     chunkHist[(int64_t)c * 256 + d] = run;
     run += h;
   }
   int base = block_exclusive_scan<SORT_BS>(run, lds, nullptr);
   int start = sg.x + base;
-  for (int c = c0; c < c0 + nc; ++c) chunkHist[(int64_t)c * 256 + d] += start;
+  c = c0;
+  for (; c + B <= c0 + nc; c += B) {
+    int h[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) h[u] = chunkHist[(int64_t)(c + u) * 256 + d];
+#pragma unroll
+    for (int u = 0; u < B; ++u) chunkHist[(int64_t)(c + u) * 256 + d] = h[u] + start;
+  }
+  for (; c < c0 + nc; ++c) chunkHist[(int64_t)c * 256 + d] += start;
   if (run > threshold && !last_round) {
     int idx = atomicAdd(&counters[C_NEXT], 1);
     segNext[idx] = make_int2(start, run);
     atomicAdd(&counters[C_ACTIVE], run);
-  } else if (run >= 2 && run <= kLocalSortMax) {
+  } else if (run >= 2 && run <= kWaveSortMax) {
     int idx = atomicAdd(&counters[C_FINAL], 1);
     fin[idx] = make_int2(start, run);
+  } else if (run > kWaveSortMax && run <= kLocalSortMax) {
+    int idx = atomicAdd(&counters[C_FINAL_BIG], 1);
+    fin[finBigOff - 1 - idx] = make_int2(start, run);   // big segments fill fin from the end
   }
 }
 
@@ -199,15 +225,61 @@ __global__ __launch_bounds__(SORT_BS) void k_chunk_copy(const uint64_t* __restri
   }
 }
 
-__global__ void k_sort_init(int2* __restrict__ seg0, int2* __restrict__ fin,
+__global__ void k_sort_init(int2* __restrict__ seg0, int2* __restrict__ fin, int finBigOff,
                             int* __restrict__ counters, int n, int T) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   for (int i = 0; i < C_COUNT; ++i) counters[i] = 0;
   if (n > T) {
     seg0[0] = make_int2(0, n);
-  } else {
+  } else if (n <= kWaveSortMax) {
     fin[0] = make_int2(0, n);
     counters[C_FINAL] = 1;
+  } else {
+    fin[finBigOff - 1] = make_int2(0, n);
+    counters[C_FINAL_BIG] = 1;
+  }
+}
+
+// Wave-level bitonic sort of segments of at most 64 (key, value) pairs held
+// one per lane; 4 segments per block.
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
+  const int lo = __shfl((int)(uint32_t)v, src, 64);
+  const int hi = __shfl((int)(uint32_t)(v >> 32), src, 64);
+  return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+__global__ __launch_bounds__(256) void k_wave_sort(uint64_t* __restrict__ keys,
+                                                  uint32_t* __restrict__ vals,
+                                                  const int2* __restrict__ fin, int nfin) {
+  const int si = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (si >= nfin) return;   // wave-uniform
+  const int lane = threadIdx.x & 63;
+  const int2 sg = fin[si];
+  uint64_t k = ~0ull;
+  uint32_t v = 0xffffffffu;
+  if (lane < sg.y) {
+    k = keys[sg.x + lane];
+    v = vals[sg.x + lane];
+  }
+  for (int kk = 2; kk <= 64; kk <<= 1) {
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      const int partner = lane ^ j;
+      const uint64_t pk = shfl_u64(k, partner);
+      const uint32_t pv = (uint32_t)__shfl((int)v, partner, 64);
+      const bool up = (lane & kk) == 0;
+      const bool lower = (lane & j) == 0;
+      const bool gt = k > pk || (k == pk && v > pv);
+      const bool lt = k < pk || (k == pk && v < pv);
+      const bool take = (up == lower) ? gt : lt;
+      if (take) {
+        k = pk;
+        v = pv;
+      }
+    }
+  }
+  if (lane < sg.y) {
+    keys[sg.x + lane] = k;
+    vals[sg.x + lane] = v;
   }
 }
 
@@ -277,19 +349,22 @@ void sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
   w.hcounters.ensure(C_COUNT * sizeof(int));
   int* cnt = w.counters.as<int>();
   int* hc = w.hcounters.as<int>();
+  const int finBigOff = (int)segcap;
   hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(64), 0, s, w.seg[0].as<int2>(), w.fin.as<int2>(),
-                     cnt, (int)n, T);
+                     finBigOff, cnt, (int)n, T);
   FMM_LAUNCH_CHECK();
 
   int nseg = 0;
   int64_t active = 0;
-  int nfinal = 0;
+  int nfinal = 0, nbig = 0;
   int cur = 0;
   if (n > T) {
     nseg = 1;
     active = n;
-  } else {
+  } else if (n <= kWaveSortMax) {
     nfinal = 1;
+  } else {
+    nbig = 1;
   }
   for (int r = 0; r < 8 && nseg > 0; ++r) {
     const int shift = 56 - 8 * r;
@@ -309,7 +384,7 @@ void sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
     FMM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_seg_offsets, dim3(nseg), dim3(SORT_BS), 0, s, segCur,
                        w.segChunk0.as<int>(), w.chunkHist.as<int>(), T, r == 7 ? 1 : 0, segNext,
-                       w.fin.as<int2>(), cnt);
+                       w.fin.as<int2>(), finBigOff, cnt);
     FMM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_chunk_scatter, dim3(cbound), dim3(SORT_BS), 0, s, keys, vals,
                        w.chunks.as<int4>(), cnt, shift, w.chunkHist.as<int>(),
@@ -323,11 +398,18 @@ void sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
     nseg = hc[C_NEXT];
     active = hc[C_ACTIVE];
     nfinal = hc[C_FINAL];
+    nbig = hc[C_FINAL_BIG];
     cur ^= 1;
   }
+  FMM_REQUIRE(nfinal + nbig <= segcap, FMM_E_INTERNAL, "sort: segment list overflow");
   if (nfinal > 0) {
-    hipLaunchKernelGGL(k_local_sort, dim3(nfinal), dim3(SORT_BS), 0, s, keys, vals,
-                       w.fin.as<int2>());
+    hipLaunchKernelGGL(k_wave_sort, dim3(ceil_div(nfinal, 4)), dim3(256), 0, s, keys, vals,
+                       w.fin.as<int2>(), nfinal);
+    FMM_LAUNCH_CHECK();
+  }
+  if (nbig > 0) {
+    hipLaunchKernelGGL(k_local_sort, dim3(nbig), dim3(SORT_BS), 0, s, keys, vals,
+                       w.fin.as<int2>() + (finBigOff - nbig));
     FMM_LAUNCH_CHECK();
   }
 }

@@ -76,6 +76,10 @@ struct Tree {
 void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmin, double ymin,
                 double w0, int ncut, hipStream_t s);
 
+// Upload the constant tables of the tree kernels to the current device (once
+// per device, before the first tree_build on it).
+void tree_init_device();
+
 // Locate targets: out_node[i] = global index of the leaf (NF_LEAF) reached by
 // the reference descent (x > xmid -> +2, y > ymid -> +1).
 void tree_locate(const Tree& t, const double* tx, const double* ty, int64_t nt, int* out_node,

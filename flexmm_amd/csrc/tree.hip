@@ -131,16 +131,22 @@ __global__ __launch_bounds__(BS) void k_gather_sorted(const double* __restrict__
 __global__ __launch_bounds__(BS) void k_split_pos(const uint64_t* __restrict__ keys, int n,
                                                   int ncut, int* __restrict__ flag,
                                                   int2* __restrict__ sc, int* __restrict__ dcount) {
+  __shared__ int lc[kMaxLevels];
+  if (threadIdx.x < kMaxLevels) lc[threadIdx.x] = 0;
+  __syncthreads();
   int i = blockIdx.x * BS + threadIdx.x;
-  if (i >= n) return;
-  uint64_t k = keys[i];
-  int s = (i == 0) ? -1 : min(clz64(k ^ keys[i - 1]) >> 1, kMaxDepth);
-  int c = ((int64_t)i + ncut < n) ? min(clz64(k ^ keys[i + ncut]) >> 1, kMaxDepth - 1) : -1;
-  int f = s < c;
-  flag[i] = f;
-  sc[i] = make_int2(s, c);
-  if (f)
-    for (int d = s + 1; d <= c; ++d) atomicAdd(&dcount[d], 1);
+  if (i < n) {
+    uint64_t k = keys[i];
+    int s = (i == 0) ? -1 : min(clz64(k ^ keys[i - 1]) >> 1, kMaxDepth);
+    int c = ((int64_t)i + ncut < n) ? min(clz64(k ^ keys[i + ncut]) >> 1, kMaxDepth - 1) : -1;
+    int f = s < c;
+    flag[i] = f;
+    sc[i] = make_int2(s, c);
+    if (f)
+      for (int d = s + 1; d <= c; ++d) atomicAdd(&lc[d], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < kMaxLevels && lc[threadIdx.x]) atomicAdd(&dcount[threadIdx.x], lc[threadIdx.x]);
 }
 
 __global__ __launch_bounds__(BS) void k_compact_pos(const int* __restrict__ flag,
@@ -260,14 +266,11 @@ __global__ __launch_bounds__(BS) void k_unique_scatter(const uint64_t* __restric
 
 // --- nodes with children (split or X-list) -----------------------------------
 // Real nodes of level d are the children of S_{d-1} (the root for d = 0).
-__global__ __launch_bounds__(BS) void k_real_flags(const uint64_t* __restrict__ s_all,
-                                                   const int64_t* __restrict__ soff,
-                                                   const int64_t* __restrict__ roff, int L,
-                                                   uint64_t* __restrict__ rcode,
-                                                   int* __restrict__ hasc, int* __restrict__ kind,
-                                                   int* __restrict__ pcount) {
-  int64_t r = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (r >= roff[L]) return;
+__device__ __forceinline__ void real_flags_one(const uint64_t* __restrict__ s_all,
+                                               const int64_t* __restrict__ soff,
+                                               const int64_t* __restrict__ roff, int L, int64_t r,
+                                               uint64_t* __restrict__ rcode, int* __restrict__ hasc,
+                                               int* __restrict__ kind, int* lc) {
   int d = 0;
   while (r >= roff[d + 1]) ++d;
   int64_t k = r - roff[d];
@@ -292,7 +295,22 @@ __global__ __launch_bounds__(BS) void k_real_flags(const uint64_t* __restrict__ 
   rcode[r] = code;
   hasc[r] = inS || isX;
   kind[r] = isX;
-  if (inS || isX) atomicAdd(&pcount[d], 1);
+  if (inS || isX) atomicAdd(&lc[d], 1);
+}
+
+__global__ __launch_bounds__(BS) void k_real_flags(const uint64_t* __restrict__ s_all,
+                                                   const int64_t* __restrict__ soff,
+                                                   const int64_t* __restrict__ roff, int L,
+                                                   uint64_t* __restrict__ rcode,
+                                                   int* __restrict__ hasc, int* __restrict__ kind,
+                                                   int* __restrict__ pcount) {
+  __shared__ int lc[kMaxLevels];
+  if (threadIdx.x < kMaxLevels) lc[threadIdx.x] = 0;
+  __syncthreads();
+  int64_t r = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (r < roff[L]) real_flags_one(s_all, soff, roff, L, r, rcode, hasc, kind, lc);
+  __syncthreads();
+  if (threadIdx.x < kMaxLevels && lc[threadIdx.x]) atomicAdd(&pcount[threadIdx.x], lc[threadIdx.x]);
 }
 
 __global__ __launch_bounds__(BS) void k_p_scatter(const uint64_t* __restrict__ rcode,
@@ -331,14 +349,14 @@ __global__ __launch_bounds__(BS) void k_nodes_a(const uint64_t* __restrict__ p_c
   flags[g] = fake ? NF_FAKE : 0;
 }
 
-__global__ __launch_bounds__(BS) void k_nodes_b(
+__global__ __launch_bounds__(64) void k_nodes_b(
     const uint64_t* __restrict__ p_code, const int* __restrict__ p_kind,
     const int64_t* __restrict__ poff, const int* __restrict__ levoff, int L,
     const uint64_t* __restrict__ keys, int npts, double xmin0, double ymin0, double w0,
     const uint64_t* __restrict__ code, int* __restrict__ flags, int* __restrict__ bot,
     int* __restrict__ top, int* __restrict__ parent, int* __restrict__ child,
     int* __restrict__ coll, double* __restrict__ xmid, double* __restrict__ ymid) {
-  int g = blockIdx.x * BS + threadIdx.x;
+  int g = blockIdx.x * 64 + threadIdx.x;
   if (g >= levoff[L]) return;
   const int d = find_level(levoff, L, g);
   const uint64_t c = code[g];
@@ -413,13 +431,19 @@ __global__ __launch_bounds__(BS) void k_group_flags(const int* __restrict__ flag
                                                     const int* __restrict__ levoff, int L,
                                                     int* __restrict__ gflag,
                                                     int* __restrict__ lcount) {
+  __shared__ int lc[kMaxLevels];
+  if (threadIdx.x < kMaxLevels) lc[threadIdx.x] = 0;
+  __syncthreads();
   int q = blockIdx.x * BS + threadIdx.x;
-  if (q >= ngroups) return;
-  int g = g0 + 4 * q;
-  int any = 0;
-  for (int j = 0; j < 4; ++j) any |= flags[g + j] & NF_SOURCE;
-  gflag[q] = any ? 1 : 0;
-  if (any) atomicAdd(&lcount[find_level(levoff, L, g)], 4);
+  if (q < ngroups) {
+    int g = g0 + 4 * q;
+    int any = 0;
+    for (int j = 0; j < 4; ++j) any |= flags[g + j] & NF_SOURCE;
+    gflag[q] = any ? 1 : 0;
+    if (any) atomicAdd(&lc[find_level(levoff, L, g)], 4);
+  }
+  __syncthreads();
+  if (threadIdx.x < kMaxLevels && lc[threadIdx.x]) atomicAdd(&lcount[threadIdx.x], lc[threadIdx.x]);
 }
 
 __global__ __launch_bounds__(BS) void k_rows(const int* __restrict__ gflag,
@@ -439,38 +463,38 @@ __global__ __launch_bounds__(BS) void k_rows(const int* __restrict__ gflag,
 // --- M2L partners: children of the parent's colleagues, outside the 3x3 --------
 // (tree.py:399-435).  For child position (cx, cy) the parent's 3x3 covers
 // offsets [-2-cx, 3-cx] x [-2-cy, 3-cy].
+struct SlotOffsets {
+  int ox[kNumM2L], oy[kNumM2L];
+};
+__constant__ SlotOffsets c_slot_off;
+
 __global__ __launch_bounds__(BS) void k_m2l_lists(const uint64_t* __restrict__ code,
                                                   const int* __restrict__ flags,
                                                   const int* __restrict__ row,
                                                   const int* __restrict__ levoff, int L,
                                                   int* __restrict__ m2l) {
-  int g = levoff[2] + blockIdx.x * BS + threadIdx.x;
+  const int64_t tid = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const int g = levoff[2] + (int)(tid / kNumM2L);
+  const int s = (int)(tid % kNumM2L);
   if (g >= levoff[L]) return;
-  const int d = find_level(levoff, L, g);
-  int* out = m2l + (int64_t)g * kNumM2L;
-  if (flags[g] & NF_FAKE) {
-    for (int s = 0; s < kNumM2L; ++s) out[s] = -1;
-    return;
-  }
-  uint32_t ix, iy;
-  demorton(code[g], &ix, &iy);
-  const int cx = ix & 1, cy = iy & 1;
-  const int64_t lim = (int64_t)1 << d;
-  const uint64_t* lst = code + levoff[d];
-  const int64_t nl = levoff[d + 1] - levoff[d];
-  for (int s = 0; s < kNumM2L; ++s) {
-    int ox, oy;
-    m2l_slot_offset(s, &ox, &oy);
-    int r = -1;
+  int r = -1;
+  if (!(flags[g] & NF_FAKE)) {
+    const int d = find_level(levoff, L, g);
+    uint32_t ix, iy;
+    demorton(code[g], &ix, &iy);
+    const int cx = ix & 1, cy = iy & 1;
+    const int ox = c_slot_off.ox[s], oy = c_slot_off.oy[s];
     if (ox >= -2 - cx && ox <= 3 - cx && oy >= -2 - cy && oy <= 3 - cy) {
-      int64_t nx = (int64_t)ix + ox, ny = (int64_t)iy + oy;
+      const int64_t lim = (int64_t)1 << d;
+      const int64_t nx = (int64_t)ix + ox, ny = (int64_t)iy + oy;
       if (nx >= 0 && ny >= 0 && nx < lim && ny < lim) {
-        int64_t pos = bsearch_u64(lst, nl, morton((uint32_t)nx, (uint32_t)ny));
+        const int64_t pos = bsearch_u64(code + levoff[d], levoff[d + 1] - levoff[d],
+                                        morton((uint32_t)nx, (uint32_t)ny));
         if (pos >= 0) r = row[levoff[d] + pos];
       }
     }
-    out[s] = r;
   }
+  m2l[(int64_t)g * kNumM2L + s] = r;
 }
 
 __global__ __launch_bounds__(BS) void k_flag_bit(const int* __restrict__ flags, int n, int bit,
@@ -763,7 +787,7 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
                      t.p_kind.as<int>(), d_poff, d_levoff, L, t.code.as<uint64_t>(),
                      t.nlevel.as<int>(), t.flags.as<int>());
   FMM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_nodes_b, dim3(grid_for(off)), dim3(BS), 0, s, t.p_code.as<uint64_t>(),
+  hipLaunchKernelGGL(k_nodes_b, dim3(ceil_div(off, 64)), dim3(64), 0, s, t.p_code.as<uint64_t>(),
                      t.p_kind.as<int>(), d_poff, d_levoff, L, keys, nn, xmin, ymin, w0,
                      t.code.as<uint64_t>(), t.flags.as<int>(), t.bot.as<int>(), t.top.as<int>(),
                      t.parent.as<int>(), t.child.as<int>(), t.coll.as<int>(), t.xmid.as<double>(),
@@ -788,7 +812,7 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
                      t.row.as<int>(), t.rownode.as<int>());
   FMM_LAUNCH_CHECK();
   if (L > 2) {
-    hipLaunchKernelGGL(k_m2l_lists, dim3(grid_for(NN - g0)), dim3(BS), 0, s,
+    hipLaunchKernelGGL(k_m2l_lists, dim3(grid_for((int64_t)(NN - g0) * kNumM2L)), dim3(BS), 0, s,
                        t.code.as<uint64_t>(), t.flags.as<int>(), t.row.as<int>(), d_levoff, L,
                        t.m2l.as<int>());
     FMM_LAUNCH_CHECK();
@@ -824,6 +848,12 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
                      t.upnodes.as<int>(), (int*)nullptr);
   FMM_LAUNCH_CHECK();
   t.built = true;
+}
+
+void tree_init_device() {
+  SlotOffsets so;
+  for (int q = 0; q < kNumM2L; ++q) m2l_slot_offset(q, &so.ox[q], &so.oy[q]);
+  FMM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_slot_off), &so, sizeof so));
 }
 
 void tree_locate(const Tree& t, const double* tx, const double* ty, int64_t nt, int* out_node,

@@ -1,0 +1,50 @@
+"""The product's host precomputation (flexmm_amd.kifmm2d.precompute) against
+the oracle's restatement of helpers.py:250-317, and the packed device layout."""
+import numpy as np
+import pytest
+
+from flexmm_amd import kernels
+from flexmm_amd.kifmm2d import precompute as pc
+from oracle import kernels as okern
+from oracle import kifmm2d as okifmm
+
+
+@pytest.mark.parametrize("p,w", [(32, 1.0), (60, 1.0 / 128), (64, 0.3)])
+def test_operators_match_oracle(p, w):
+    ops = pc.LevelOperators(kernels.laplace, p, w)
+    ora = okifmm.Precomputation(w, okern.laplace, p)
+    np.testing.assert_allclose(ops.M2MC, ora.M2MC, rtol=1e-15, atol=0)
+    slots = pc.m2l_slots()
+    assert len(slots) == 40
+    for s, (ox, oy) in enumerate(slots):
+        np.testing.assert_allclose(ops.m2l[s], ora.M2L[ox + 3, oy + 3], rtol=1e-15, atol=0)
+    # the pseudo-inverse factors apply the same operator
+    b = np.random.default_rng(p).standard_normal((p, 5))
+    np.testing.assert_allclose(ops.pinv2 @ (ops.pinv1 @ b), ora.S2L_Solver(b), rtol=1e-9, atol=1e-9 * np.abs(ora.S2L_Solver(b)).max())
+    np.testing.assert_allclose(ops.large[0], ora.large_x, rtol=0, atol=0)
+
+
+def test_pack_layout():
+    p = 20
+    ops = pc.LevelOperators(kernels.laplace, p, 0.5)
+    flat = ops.pack()
+    PD = pc.padded_dim(p)
+    assert PD == 32
+    blocks = flat[:pc.OP_NBLK * PD * PD].reshape(pc.OP_NBLK, PD, PD)
+    circ = flat[pc.OP_NBLK * PD * PD:].reshape(2, PD)
+    np.testing.assert_array_equal(blocks[0, :p, :p], ops.pinv1)
+    np.testing.assert_array_equal(blocks[1, :p, :p], ops.pinv2)
+    for c in range(4):
+        np.testing.assert_array_equal(blocks[2 + c, :p, :p], ops.M2MC[:, c * p:(c + 1) * p])
+        np.testing.assert_array_equal(blocks[6 + c, :p, :p], ops.M2MC[:, c * p:(c + 1) * p].T)
+    np.testing.assert_array_equal(blocks[10 + 7, :p, :p], ops.m2l[7])
+    assert np.all(blocks[:, p:, :] == 0) and np.all(blocks[:, :, p:] == 0)
+    np.testing.assert_array_equal(circ[0, :p], ops.large[0])
+    assert np.all(circ[:, p:] == 0)
+    assert flat.size == 50 * PD * PD + 2 * PD
+
+
+def test_padded_dim_limits():
+    assert pc.padded_dim(4) == 32 and pc.padded_dim(33) == 64 and pc.padded_dim(64) == 64
+    with pytest.raises(ValueError):
+        pc.padded_dim(65)
