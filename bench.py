@@ -124,45 +124,54 @@ def main():
     tyd = torch.from_numpy(ty).to(dev)
     bbox = [0.0, 1.0, 0.0, 1.0]
 
-    def step(profile=False):
-        f = FMM(xd, yd, kernels.laplace, NCUT, P, bbox=bbox)
-        if profile:
-            f.set_profiling(True)
-            f.build_tree()
-        f.build_expansions(taud)
-        u = f.target_evaluation(txd, tyd)
-        return f, u
-
-    for _ in range(args.warmup):
-        f, u = step()
-        del f
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        f, u = step()
-        del f
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1000.0 / args.steps
-    value = tx.size / (ms / 1000.0) / 1e6
-
-    # Phase and kernel durations (HIP events on the launch stream), averaged
-    # over separate profiled steps.
-    acc = {}
-    nprof = max(3, min(args.steps, 10))
-    for _ in range(nprof):
-        f, u = step(profile=True)
-        for k, v in f.phase_times().items():
-            acc[k] = acc.get(k, 0.0) + v / nprof
-        work = f.work_counts()
-        del f
-    f, u = step()
-    # accuracy: max-norm error relative to max|u| (examples/pure_kifmm_laplace.py:149-151)
     rng = np.random.default_rng(7)
     sel = torch.from_numpy(np.sort(rng.choice(tx.size, 2000, replace=False))).to(dev)
     ref = direct_sample(torch, xd, yd, taud, txd[sel], tyd[sel])
-    usel = u[0][sel]
-    err = float((usel - ref).abs().max() / ref.abs().max())
+
+    def measure(precision):
+        def step(profile=False):
+            f = FMM(xd, yd, kernels.laplace, NCUT, P, bbox=bbox, precision=precision)
+            if profile:
+                f.set_profiling(True)
+                f.build_tree()
+            f.build_expansions(taud)
+            u = f.target_evaluation(txd, tyd)
+            return f, u
+
+        for _ in range(args.warmup):
+            f, u = step()
+            del f
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            f, u = step()
+            del f
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ms = (t1 - t0) * 1000.0 / args.steps
+        # Phase and kernel durations (HIP events on the launch stream),
+        # averaged over separate profiled steps.
+        acc = {}
+        nprof = max(3, min(args.steps, 10))
+        for _ in range(nprof):
+            f, u = step(profile=True)
+            for k, v in f.phase_times().items():
+                acc[k] = acc.get(k, 0.0) + v / nprof
+            work = f.work_counts()
+            del f
+        f, u = step()
+        # max-norm error relative to max|u| (examples/pure_kifmm_laplace.py:149-151)
+        err = float((u[0][sel] - ref).abs().max() / ref.abs().max())
+        return dict(ms=ms, value=tx.size / (ms / 1000.0) / 1e6, phases=acc, work=work, err=err,
+                    u=u[0])
+
+    r64 = measure("fp64")
+    u64 = r64.pop("u")
+    rmx = measure("mixed")
+    umx = rmx.pop("u")
+    err_mx64 = float((umx - u64).abs().max() / u64.abs().max())
+    del u64, umx
+    ms, value, acc, work, err = r64["ms"], r64["value"], r64["phases"], r64["work"], r64["err"]
 
     # Algorithmic work (flop conventions: a Laplace pair is 20 flops, a GEMM 2mnk)
     pairs = work["p2p_pairs"]
@@ -189,12 +198,23 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded uniform sources, N(0,1) charges, 1000x1000 grid targets)",
-        "config": {"workload": "C2: Laplace KIFMM FP64, 1M uniform sources -> 1M grid targets, "
-                               "Nequiv 60, Ncutoff 100, bbox [0,1]^2, seed %d" % SEED,
+        "config": {"workload": "Laplace KIFMM, 1M uniform sources -> 1M grid targets, "
+                               "Nequiv 60, Ncutoff 100, bbox [0,1]^2, seed %d; value = FP64 form, "
+                               "'mixed' = mixed-precision form on the same workload" % SEED,
                    "n_sources": N_SRC, "n_targets": int(tx.size), "Nequiv": P, "Ncutoff": NCUT,
                    "precision": "fp64"},
         "error_vs_direct": err,
         "error_sample": "max|u-u_direct|/max|u_direct| over 2000 random grid targets",
+        "mixed": {
+            "value": round(rmx["value"], 3),
+            "unit": "Mpts/s",
+            "ms_per_step": round(rmx["ms"], 4),
+            "speedup_vs_fp64": round(rmx["value"] / value, 3),
+            "error_vs_direct": rmx["err"],
+            "error_vs_fp64": err_mx64,
+            "dtype": "f32 M2L (MFMA) + f32 leaf near field, f64 elsewhere",
+            "phase_ms": {k: round(v, 4) for k, v in rmx["phases"].items()},
+        },
         "phase_ms": {k: round(v, 4) for k, v in acc.items()},
         "work": work,
         "algorithmic_gflop": round(f_total / 1e9, 3),

@@ -17,6 +17,8 @@ E_ARG, E_STATE, E_LIMIT, E_INTERNAL = -1, -2, -3, -4
 KERNEL_LAPLACE = 0
 EVAL_POTENTIAL, EVAL_GRADIENT = 0, 1
 PREC_FP64 = 0
+PREC_MIXED = 1
+PREC_MIXED_M2L = 2
 TREE_ARRAYS = dict(keys=0, perm=1, sx=2, sy=3, code=4, level=5, bot=6, top=7, parent=8, child=9,
                    flags=10, colleagues=11, xmid=12, ymid=13, row=14, m2l=15, evalid=16)
 EXP_ARRAYS = dict(check=0, multipole=1, local=2, tau_sorted=3)

@@ -159,7 +159,8 @@ enum EvalId : int {
 };
 enum Precision : int {
   PREC_FP64 = 0,
-  PREC_MIXED = 1,
+  PREC_MIXED = 1,       // FP32 M2L + FP32 leaf near field
+  PREC_MIXED_M2L = 2,   // FP32 M2L only
 };
 
 }  // namespace fmm

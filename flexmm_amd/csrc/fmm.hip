@@ -55,7 +55,12 @@ enum {
   OP_M2L = 10,    // 40 blocks in slot order
   OP_NBLK = 50,
 };
-inline int64_t ops_per_level(int PD) { return (int64_t)OP_NBLK * PD * PD + 2 * PD; }
+// ... followed, for the mixed-precision M2L, by the 40 row-mean vectors c_s
+// (FP64 [40][PD]) and the FP32 residuals R_s = M2L_s - c_s 1^T ([40][PD][PD]).
+__host__ __device__ inline int64_t ops_off_circles(int PD) { return (int64_t)OP_NBLK * PD * PD; }
+__host__ __device__ inline int64_t ops_off_means(int PD) { return ops_off_circles(PD) + 2 * PD; }
+__host__ __device__ inline int64_t ops_off_r32(int PD) { return ops_off_means(PD) + 40 * PD; }
+inline int64_t ops_per_level(int PD) { return ops_off_r32(PD) + 20 * (int64_t)PD * PD; }
 
 constexpr double kInv4Pi = 0.07957747154594767;   // 1/(4 pi)
 constexpr double kInv2Pi = 0.15915494309189535;   // 1/(2 pi)
@@ -78,7 +83,9 @@ struct Plan {
   int ops_levels = 0;
   std::vector<char> ops_set;
   DevBuf logtab;
-  DevBuf tau_s, src, check, mult, le, ple;
+  DevBuf tau_s, src, check, mult, le, ple, qs;
+  DevBuf srcf, tsum;   // mixed precision: leaf-relative FP32 sources, per-leaf charge sums
+  bool have_srcf = false;
   bool have_exp = false;
   // targets
   DevBuf tnode, tcnt, toff, tcur, tperm, tsx, tsy, icnt, ioff, items, ntot;
@@ -331,7 +338,8 @@ __global__ __launch_bounds__(256) void k_mult_level(int64_t row0, int nrows, int
                                                    const int* __restrict__ row,
                                                    const double* __restrict__ check,
                                                    const double* __restrict__ opsL,
-                                                   double* __restrict__ mult) {
+                                                   double* __restrict__ mult,
+                                                   double* __restrict__ qs) {
   __shared__ __attribute__((aligned(16))) double buf[2][PD * Mfma<PD>::LDA];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int rl = blockIdx.x * 64 + w * 16 + (l & 15);
@@ -352,7 +360,18 @@ __global__ __launch_bounds__(256) void k_mult_level(int64_t row0, int nrows, int
     return Phase{opsL + (int64_t)(OP_PINV1 + (k - nm)) * PD * PD, nullptr, true};
   };
   run_phases<PD>(nm + 2, acc, buf[0], buf[1], next);
-  if (active) store_acc<PD>(acc, mult + r * PD);
+  // total equivalent charge of the box (used by the mixed-precision M2L)
+  double part = 0.0;
+#pragma unroll
+  for (int rb = 0; rb < PD / 16; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) part += acc[rb][v];
+  part += __shfl_xor(part, 16, 64);
+  part += __shfl_xor(part, 32, 64);
+  if (active) {
+    store_acc<PD>(acc, mult + r * PD);
+    if ((l >> 4) == 0) qs[r] = part;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -452,6 +471,155 @@ __global__ __launch_bounds__(512) void k_m2l_all(LevelTable lt, const int* __res
     for (int rb = 0; rb < HRB; ++rb)
 #pragma unroll
       for (int v = 0; v < 4; ++v) rowp[16 * (half * HRB + rb) + 4 * v + kq] = acc[rb][v];
+  }
+}
+
+// Mixed-precision M2L: PLE = sum_s [ R_s q_src (FP32 MFMA, FP32 accumulation
+//                                                within the slot)
+//                               + c_s * qs_src  (FP64) ], slots summed in FP64.
+// v_mfma_f32_16x16x4f32: lane l holds A[l%16][l/16], B[l/16][l%16] and
+// D[4(l/16) + v][l%16].
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <int PD, int NT>
+struct AStageF {
+  static constexpr int NV = PD * PD / 4;            // float4 per block
+  static constexpr int N4 = (NV + NT - 1) / NT;
+  f4 v[N4];
+  __device__ __forceinline__ void load(const float* __restrict__ G) {
+    const f4* G4 = reinterpret_cast<const f4*>(G);
+#pragma unroll
+    for (int i = 0; i < N4; ++i) {
+      const int idx = threadIdx.x + i * NT;
+      if (NV % NT == 0 || idx < NV) v[i] = G4[idx];
+    }
+  }
+  __device__ __forceinline__ void store(float* __restrict__ A) const {
+    constexpr int LDAF = PD + 4;
+#pragma unroll
+    for (int i = 0; i < N4; ++i) {
+      const int idx = threadIdx.x + i * NT;
+      if (NV % NT == 0 || idx < NV) {
+        const int e = 4 * idx;
+        const int r = e / PD, c = e - r * PD;
+        *reinterpret_cast<f4*>(A + r * LDAF + c) = v[i];
+      }
+    }
+  }
+};
+
+template <int PD>
+__global__ __launch_bounds__(512) void k_m2l_all_f32(LevelTable lt, const int* __restrict__ flags,
+                                                    const int* __restrict__ m2l,
+                                                    const double* __restrict__ mult,
+                                                    const double* __restrict__ qs,
+                                                    const double* __restrict__ ops, int64_t opsz,
+                                                    double* __restrict__ ple) {
+  constexpr int NT = 512, KS = PD / 4, HRB = PD / 32, LDAF = PD + 4;
+  __shared__ __attribute__((aligned(16))) float buf[2][PD * LDAF];
+  __shared__ int srcs[27][64];
+  __shared__ unsigned long long smask;
+  int li = 0;
+  while (li + 1 < lt.nlev && (int)blockIdx.x >= lt.boff[li + 1]) ++li;
+  const int lb = blockIdx.x - lt.boff[li];
+  const int c = lb & 3, tile = lb >> 2;
+  const int ngroups = lt.ngroups[li];
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int half = w >> 2;
+  const int col = (w & 3) * 16 + (l & 15);
+  const int q = tile * 64 + col;
+  const int g = lt.g0[li] + 4 * q + c;
+  const bool active = q < ngroups && !(flags[q < ngroups ? g : lt.g0[li]] & NF_FAKE);
+  const int* lst = m2l + (int64_t)g * kNumM2L;
+  const int nv = c_valid.n[c];
+  const int kq = l >> 4;
+  if (threadIdx.x == 0) smask = 0ull;
+  __syncthreads();
+  if (half == 0) {
+    unsigned long long wm = 0ull;
+    for (int i = 0; i < nv; ++i) {
+      const int src = active ? lst[c_valid.slot[c][i]] : -1;
+      if (kq == 0) srcs[i][col] = src;
+      if (__ballot(src >= 0)) wm |= 1ull << i;
+    }
+    if (l == 0 && wm) atomicOr(&smask, wm);
+  }
+  __syncthreads();
+  unsigned long long rem = smask;
+  const int nph = __popcll(rem);
+  const double* opsL = ops + (int64_t)lt.d[li] * opsz;
+  const double* means = opsL + ops_off_means(PD);
+  const float* r32 = reinterpret_cast<const float*>(opsL + ops_off_r32(PD));
+  double acc64[HRB][4];
+#pragma unroll
+  for (int rb = 0; rb < HRB; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc64[rb][v] = 0.0;
+  const int rb0 = half * HRB;
+  if (nph > 0) {
+    AStageF<PD, NT> st;
+    float bn[KS];
+    double cn[HRB][4];
+    double qn = 0.0;
+    auto fetch = [&](bool& any) {
+      const int i = __builtin_ctzll(rem);
+      rem &= rem - 1ull;
+      const int s = c_valid.slot[c][i];
+      st.load(r32 + (int64_t)s * PD * PD);
+      const int src = srcs[i][col];
+      const double* brow = src >= 0 ? mult + (int64_t)src * PD : nullptr;
+      any = __ballot(brow != nullptr) != 0;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) bn[ks] = brow ? (float)brow[4 * ks + kq] : 0.f;
+      qn = src >= 0 ? qs[src] : 0.0;
+#pragma unroll
+      for (int rb = 0; rb < HRB; ++rb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) cn[rb][v] = means[s * PD + 16 * (rb0 + rb) + 4 * kq + v];
+    };
+    bool any_n;
+    fetch(any_n);
+    for (int i = 0; i < nph; ++i) {
+      float* A = (i & 1) ? buf[1] : buf[0];
+      st.store(A);
+      const bool any = any_n;
+      float bc[KS];
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) bc[ks] = bn[ks];
+      const double qc = qn;
+#pragma unroll
+      for (int rb = 0; rb < HRB; ++rb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc64[rb][v] = __builtin_fma(cn[rb][v], qc, acc64[rb][v]);
+      __syncthreads();
+      if (i + 1 < nph) fetch(any_n);
+      if (any) {
+        // one FP32 accumulation per slot (p terms), slots summed in FP64:
+        // a single FP32 chain over all 27 slots loses about 4x more
+        const float* a0 = A + (rb0 * 16 + (l & 15)) * LDAF + kq;
+        f4 acc[HRB];
+#pragma unroll
+        for (int rb = 0; rb < HRB; ++rb) acc[rb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+          for (int rb = 0; rb < HRB; ++rb)
+            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rb * 16 * LDAF + ks * 4], bc[ks],
+                                                           acc[rb], 0, 0, 0);
+#pragma unroll
+        for (int rb = 0; rb < HRB; ++rb)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc64[rb][v] += (double)acc[rb][v];
+      }
+    }
+  }
+  if (active) {
+    double* rowp = ple + (int64_t)g * PD;
+#pragma unroll
+    for (int rb = 0; rb < HRB; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        rowp[16 * (rb0 + rb) + 4 * kq + v] = acc64[rb][v];
   }
 }
 
@@ -661,6 +829,151 @@ __global__ __launch_bounds__(64) void k_eval(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Mixed precision near field.  Every source is stored in FP32 relative to the
+// centre of its leaf in units of the leaf width, (x - c) * (1/w) computed in
+// FP64 and rounded once; a target evaluated against a leaf colleague uses the
+// same FP64 expression with the colleague's centre, so an exactly coincident
+// pair gives exactly zero and is excluded as in FP64.  The log is the hardware
+// log2 of d^2 / w^2 (O(1) values keep its absolute error small);
+// sum_j t_j log d_j^2 = ln2 * sum_j t_j log2(d_j^2/w^2) + log(w^2) * sum_j t_j.
+constexpr float kNearF32 = 1.0f / 256.0f;   // (w / 16)^2 in units of w^2
+
+struct LevelConsts {
+  double invw[kMaxLevels];
+  double logw2[kMaxLevels];
+};
+
+__global__ __launch_bounds__(64) void k_prep_mixed(const int* __restrict__ evalnode,
+                                                  const int* __restrict__ nlevel,
+                                                  const int* __restrict__ bot,
+                                                  const int* __restrict__ top,
+                                                  const double* __restrict__ xmid,
+                                                  const double* __restrict__ ymid,
+                                                  const double4* __restrict__ src,
+                                                  LevelConsts lc, f4* __restrict__ srcf,
+                                                  double* __restrict__ tsum) {
+  const int g = evalnode[blockIdx.x];
+  const double iw = lc.invw[nlevel[g]];
+  const double cx = xmid[g], cy = ymid[g];
+  const int b = bot[g], e = top[g];
+  double ts = 0.0;
+  for (int j = b + threadIdx.x; j < e; j += 64) {
+    const double4 sj = src[j];
+    srcf[j] = f4{(float)((sj.x - cx) * iw), (float)((sj.y - cy) * iw), (float)sj.z, 0.f};
+    ts += sj.z;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) ts += __shfl_xor(ts, off, 64);
+  if (threadIdx.x == 0) tsum[g] = ts;
+}
+
+template <int PD, bool CHECK>
+__global__ __launch_bounds__(64) void k_eval_mixed(
+    const int2* __restrict__ items, const int* __restrict__ nitems,
+    const int* __restrict__ evalnode, const int* __restrict__ toff,
+    const int* __restrict__ tperm, const double* __restrict__ tsx, const double* __restrict__ tsy,
+    const int* __restrict__ nlevel, const int* __restrict__ flags,
+    const double* __restrict__ xmid, const double* __restrict__ ymid,
+    const int* __restrict__ coll, const int* __restrict__ bot, const int* __restrict__ top,
+    const double4* __restrict__ src, const f4* __restrict__ srcf,
+    const double* __restrict__ tsum, LevelConsts lc, const double* __restrict__ le,
+    const double* __restrict__ ops, int64_t opsz, int p, const double2* __restrict__ logtab,
+    double* __restrict__ out) {
+  __shared__ double2 tab[kLogTableSize];
+  if ((int)blockIdx.x >= *nitems) return;
+  load_logtab(tab, logtab);
+  __syncthreads();
+  const int2 it = items[blockIdx.x];
+  const int g = evalnode[it.x];
+  const int t0 = toff[it.x] + it.y * kTgtChunk;
+  const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
+  const int t = t0 + threadIdx.x;
+  const bool act = t < t1;
+  Target<1, CHECK> T;
+  T.x = act ? tsx[t] : 0.0;
+  T.y = act ? tsy[t] : 0.0;
+  const int d = nlevel[g];
+  if (d >= 2) {   // L2T in FP64 (the local densities carry large cancelling modes)
+    Target<1, false> L;
+    L.x = T.x - xmid[g];
+    L.y = T.y - ymid[g];
+    const double* lx = ops + d * opsz + ops_off_circles(PD);
+    const double* ly = lx + PD;
+    const double* lg = le + (int64_t)g * PD;
+    for (int k = 0; k < p; ++k) L.pair(lx[k], ly[k], lg[k], tab);
+    T.u = L.u;
+  }
+  const double iw = lc.invw[d];
+  const double lw2 = lc.logw2[d];
+  double u32 = 0.0;      // ln2-scaled part from the FP32 near field
+  const int* cl = coll + (int64_t)g * 9;
+  for (int qn = 0; qn < 9; ++qn) {
+    const int c = cl[qn];
+    if (c < 0) continue;
+    const int b = __builtin_amdgcn_readfirstlane(bot[c]);
+    const int e = __builtin_amdgcn_readfirstlane(top[c]);
+    if (e <= b) continue;
+    if (flags[c] & NF_LEAF) {
+      const float xr = (float)((T.x - xmid[c]) * iw);
+      const float yr = (float)((T.y - ymid[c]) * iw);
+      float acc = 0.f;
+      double excl = 0.0;   // charges of the pairs taken in FP64
+      // pairs closer than w/16 (and coincident ones) go to FP64: the rounding
+      // of the FP32 relative coordinates costs ~1e-7 w / d in log d
+      auto near64 = [&](int jj) {
+        const double4 sd = src[jj];
+        excl += sd.z;
+        T.pair(sd.x, sd.y, sd.z, tab);
+      };
+      int j = b;
+      for (; j + 4 <= e; j += 4) {
+        f4 s4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s4[u] = srcf[j + u];
+        bool nr[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float dx = xr - s4[u].x, dy = yr - s4[u].y;
+          const float d2 = __builtin_fmaf(dy, dy, dx * dx);
+          nr[u] = d2 < kNearF32;
+          const float lg2 = __builtin_amdgcn_logf(nr[u] ? 1.0f : d2);
+          acc = __builtin_fmaf(nr[u] ? 0.f : s4[u].z, lg2, acc);
+        }
+        if (nr[0] | nr[1] | nr[2] | nr[3]) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (nr[u]) near64(j + u);
+        }
+      }
+      for (; j < e; ++j) {
+        const f4 sj = srcf[j];
+        const float dx = xr - sj.x, dy = yr - sj.y;
+        const float d2 = __builtin_fmaf(dy, dy, dx * dx);
+        const bool nr = d2 < kNearF32;
+        const float lg2 = __builtin_amdgcn_logf(nr ? 1.0f : d2);
+        acc = __builtin_fmaf(nr ? 0.f : sj.z, lg2, acc);
+        if (nr) near64(j);
+      }
+      u32 += (double)acc;
+      T.u.at = __builtin_fma(lw2, tsum[c] - excl, T.u.at);
+    } else {
+      // colleague with children (adaptive trees): its sources are not stored
+      // relative to it, so this range stays FP64
+      int j = b;
+      for (; j < e; ++j) {
+        const double4 sj = src[j];
+        T.pair(sj.x, sj.y, sj.z, tab);
+      }
+    }
+  }
+  if (act) {
+    double u = -kInv4Pi * (__builtin_fma(u32, kLn2, T.u.value()));
+    if (!CHECK && T.zhit) u = __builtin_inf();
+    out[tperm[t]] = u;
+  }
+}
+
 // Algorithmic work counts (for the roofline accounting).
 __global__ __launch_bounds__(256) void k_eval_stats(const int* __restrict__ evalnode, int ne,
                                                    const int* __restrict__ tcnt,
@@ -728,6 +1041,7 @@ static void upward(Plan& P, const double* tau, hipStream_t s) {
   const int64_t opsz = ops_per_level(PD);
   P.tau_s.ensure(t.n * sizeof(double));
   P.src.ensure(t.n * sizeof(double4));
+  P.have_srcf = false;
   hipLaunchKernelGGL(k_pack_sources, dim3(grid256(t.n)), dim3(256), 0, s, tau,
                      t.perm.as<uint32_t>(), t.sx.as<double>(), t.sy.as<double>(), t.n,
                      P.tau_s.as<double>(), P.src.as<double4>());
@@ -735,6 +1049,7 @@ static void upward(Plan& P, const double* tau, hipStream_t s) {
   const int64_t nr = t.n_rows;
   P.check.ensure((nr + 1) * PD * sizeof(double));
   P.mult.ensure((nr + 1) * PD * sizeof(double));
+  P.qs.ensure((nr + 1) * sizeof(double));
   if (t.L <= 2 || nr == 0) {
     P.have_exp = true;
     return;
@@ -758,7 +1073,7 @@ static void upward(Plan& P, const double* tau, hipStream_t s) {
     hipLaunchKernelGGL(kern, dim3(ceil_div(nrows, 64)), dim3(256), 0, s, r0, nrows,
                        d < t.L - 1 ? 1 : 0, t.rownode.as<int>(), t.flags.as<int>(),
                        t.child.as<int>(), t.row.as<int>(), P.check.as<double>(), opsL,
-                       P.mult.as<double>());
+                       P.mult.as<double>(), P.qs.as<double>());
     FMM_LAUNCH_CHECK();
   }
   P.have_exp = false;
@@ -788,7 +1103,13 @@ static void downward(Plan& P, hipStream_t s) {
     ++lt.nlev;
   }
   lt.boff[lt.nlev] = nb;
-  {
+  if (P.precision != PREC_FP64) {
+    auto kern = PD == 32 ? k_m2l_all_f32<32> : k_m2l_all_f32<64>;
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(512), 0, s, lt, t.flags.as<int>(), t.m2l.as<int>(),
+                       P.mult.as<double>(), P.qs.as<double>(), P.ops.as<double>(), opsz,
+                       P.ple.as<double>());
+    FMM_LAUNCH_CHECK();
+  } else {
     auto kern = PD == 32 ? k_m2l_all<32> : k_m2l_all<64>;
     hipLaunchKernelGGL(kern, dim3(nb), dim3(512), 0, s, lt, t.flags.as<int>(), t.m2l.as<int>(),
                        P.mult.as<double>(), P.ops.as<double>(), opsz, P.ple.as<double>());
@@ -848,6 +1169,37 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   FMM_LAUNCH_CHECK();
   const int PD = P.PD;
   const int64_t opsz = ops_per_level(PD);
+  if (P.precision == PREC_MIXED && ev == EVAL_POT) {
+    LevelConsts lc{};
+    for (int d = 0; d < t.L; ++d) {
+      lc.invw[d] = 1.0 / t.lev[d].width;
+      lc.logw2[d] = std::log(t.lev[d].width * t.lev[d].width);
+    }
+    if (!P.have_srcf) {
+      P.srcf.ensure((t.n + 1) * sizeof(f4));
+      P.tsum.ensure((t.n_nodes + 1) * sizeof(double));
+      if (t.n_eval > 0)
+        hipLaunchKernelGGL(k_prep_mixed, dim3(t.n_eval), dim3(64), 0, s, t.evalnode.as<int>(),
+                           t.nlevel.as<int>(), t.bot.as<int>(), t.top.as<int>(),
+                           t.xmid.as<double>(), t.ymid.as<double>(), P.src.as<double4>(), lc,
+                           P.srcf.as<f4>(), P.tsum.as<double>());
+      FMM_LAUNCH_CHECK();
+      P.have_srcf = true;
+    }
+    auto mk = check ? (PD == 32 ? k_eval_mixed<32, true> : k_eval_mixed<64, true>)
+                    : (PD == 32 ? k_eval_mixed<32, false> : k_eval_mixed<64, false>);
+    P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
+    hipLaunchKernelGGL(mk, dim3(icap), dim3(64), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
+                       t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(),
+                       P.tsx.as<double>(), P.tsy.as<double>(), t.nlevel.as<int>(),
+                       t.flags.as<int>(), t.xmid.as<double>(), t.ymid.as<double>(),
+                       t.coll.as<int>(), t.bot.as<int>(), t.top.as<int>(), P.src.as<double4>(),
+                       P.srcf.as<f4>(), P.tsum.as<double>(), lc, P.le.as<double>(),
+                       P.ops.as<double>(), opsz, P.p, P.logtab.as<double2>(), out);
+    FMM_LAUNCH_CHECK();
+    P.mark(FMM2D_PHASE_EVAL_KERNEL, true, s);
+    return;
+  }
   decltype(&k_eval<32, 1, false>) kern;
   if (ev == EVAL_POT)
     kern = check ? (PD == 32 ? k_eval<32, 1, true> : k_eval<64, 1, true>)
@@ -910,7 +1262,9 @@ int fmm2d_plan_create(int kernel_id, const double* kparams, int nkparams, int p,
   *out_plan = nullptr;
   FMM_REQUIRE(kernel_id == fmm::KER_LAPLACE, fmm::FMM_E_ARG, "unknown kernel id %d", kernel_id);
   FMM_REQUIRE(p >= 4 && p <= 64, fmm::FMM_E_LIMIT, "Nequiv = %d outside [4, 64]", p);
-  FMM_REQUIRE(precision == fmm::PREC_FP64, fmm::FMM_E_ARG, "unknown precision %d", precision);
+  FMM_REQUIRE(precision == fmm::PREC_FP64 || precision == fmm::PREC_MIXED ||
+                  precision == fmm::PREC_MIXED_M2L, fmm::FMM_E_ARG,
+              "unknown precision %d", precision);
   FMM_REQUIRE(nkparams >= 0 && (nkparams == 0 || kparams), fmm::FMM_E_ARG, "bad kernel params");
   Plan* P = new Plan();
   try {

@@ -133,7 +133,8 @@ def _release_plan(plan, kernel, p, precision, device):
     _plan_pool.setdefault(key, []).append(plan)
 
 
-_PRECISIONS = {"fp64": _lib.PREC_FP64}
+_PRECISIONS = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED,
+               "mixed_m2l": _lib.PREC_MIXED_M2L}
 
 
 def _stream():

@@ -41,7 +41,11 @@ extern "C" {
                                   (examples/pure_kifmm_laplace_gradient.py:32-55) */
 
 /* precision */
-#define FMM2D_PREC_FP64 0
+#define FMM2D_PREC_FP64 0  /* every step in FP64 */
+#define FMM2D_PREC_MIXED 1 /* FP32 MFMA M2L (constant mode in FP64) and FP32 near field
+                              for leaf colleagues at separations >= w/16 (potential
+                              evaluator); FP64 elsewhere */
+#define FMM2D_PREC_MIXED_M2L 2 /* FP32 MFMA M2L only; near field in FP64 */
 
 /* tree arrays for fmm2d_tree_export */
 #define FMM2D_TREE_KEYS 0       /* u64[n]   Morton keys of the sorted sources            */

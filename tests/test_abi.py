@@ -38,8 +38,8 @@ def test_abi_version(lib):
 
 
 def test_operator_size(lib):
-    assert lib.fmm2d_operator_size(60) == 50 * 64 * 64 + 2 * 64
-    assert lib.fmm2d_operator_size(32) == 50 * 32 * 32 + 2 * 32
+    for p, PD in ((60, 64), (32, 32)):
+        assert lib.fmm2d_operator_size(p) == 50 * PD * PD + 2 * PD + 40 * PD + 20 * PD * PD
     assert lib.fmm2d_operator_size(3) == -1
     assert lib.fmm2d_operator_size(65) == -1
 

@@ -30,8 +30,16 @@ def test_pack_layout():
     flat = ops.pack()
     PD = pc.padded_dim(p)
     assert PD == 32
-    blocks = flat[:pc.OP_NBLK * PD * PD].reshape(pc.OP_NBLK, PD, PD)
-    circ = flat[pc.OP_NBLK * PD * PD:].reshape(2, PD)
+    nb = pc.OP_NBLK * PD * PD
+    blocks = flat[:nb].reshape(pc.OP_NBLK, PD, PD)
+    circ = flat[nb:nb + 2 * PD].reshape(2, PD)
+    means = flat[nb + 2 * PD:nb + 42 * PD].reshape(40, PD)
+    resid = flat[nb + 42 * PD:].view(np.float32).reshape(40, PD, PD)
+    for s in (0, 13, 39):
+        m = ops.m2l[s]
+        np.testing.assert_allclose(means[s, :p], m.mean(axis=1), rtol=1e-15)
+        np.testing.assert_allclose(resid[s, :p, :p] + means[s, :p, None], m, rtol=1e-6)
+        assert np.all(resid[s, p:, :] == 0) and np.all(resid[s, :, p:] == 0)
     np.testing.assert_array_equal(blocks[0, :p, :p], ops.pinv1)
     np.testing.assert_array_equal(blocks[1, :p, :p], ops.pinv2)
     for c in range(4):
@@ -41,7 +49,7 @@ def test_pack_layout():
     assert np.all(blocks[:, p:, :] == 0) and np.all(blocks[:, :, p:] == 0)
     np.testing.assert_array_equal(circ[0, :p], ops.large[0])
     assert np.all(circ[:, p:] == 0)
-    assert flat.size == 50 * PD * PD + 2 * PD
+    assert flat.size == 50 * PD * PD + 2 * PD + 40 * PD + 20 * PD * PD
 
 
 def test_padded_dim_limits():
