@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-targets", type=int, default=200_000)
+    ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
+                    help="time one form only and print its phase times (for profiling runs)")
     return ap.parse_args()
 
 
@@ -165,6 +167,13 @@ def main():
         return dict(ms=ms, value=tx.size / (ms / 1000.0) / 1e6, phases=acc, work=work, err=err,
                     u=u[0])
 
+    if args.only:
+        r = measure(args.only)
+        r.pop("u")
+        print(json.dumps(dict(precision=args.only, ms_per_step=round(r["ms"], 4),
+                              value=round(r["value"], 3), error_vs_direct=r["err"],
+                              phase_ms={k: round(v, 4) for k, v in r["phases"].items()})))
+        return
     r64 = measure("fp64")
     u64 = r64.pop("u")
     rmx = measure("mixed")

@@ -114,8 +114,10 @@ int64_t fmm2d_operator_size(int p);
  * fmm2d_operator_size(p) doubles (copied; the caller may free it afterwards):
  * 50 blocks of [PD][PD] -- diag(1/s)U^T, V, the 4 M2M column blocks of M2MC,
  * their 4 transposes (L2L), the 40 M2L matrices in slot order -- then the
- * large-circle x and y (PD each).  PD = 32 for p <= 32, else 64.
- * Replaces Precomputation (helpers.py:250-317). */
+ * large-circle x and y (PD each), then for the mixed-precision M2L the row
+ * means c_s of each M2L matrix ([40][PD] doubles) and the FP32 residuals
+ * M_s - c_s 1^T ([40][PD][PD] floats, i.e. 20 PD^2 doubles of storage).
+ * PD = 32 for p <= 32, else 64.  Replaces Precomputation (helpers.py:250-317). */
 int fmm2d_set_operators(void* plan, int level, const double* dev_ops, int64_t n, void* stream);
 int fmm2d_clear_operators(void* plan);
 
