@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-targets", type=int, default=200_000)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the differentiable modified-Helmholtz and Stokeslet workloads")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
                     help="time one form only and print its phase times (for profiling runs)")
     return ap.parse_args()
@@ -106,6 +108,129 @@ print(json.dumps(dict(tree=t1 - t0, expansions=t2 - t1, eval_sample=t3 - t2, sam
         seconds=dict(tree=round(r["tree"], 3), expansions=round(r["expansions"], 3),
                      eval_sample=round(r["eval_sample"], 3)))
 
+
+
+def _timed(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1000.0 / steps
+
+
+def adjoint_workload(torch, steps, warmup):
+    """Differentiable modified Helmholtz (k = 20): 500,000 sources uniform on
+    the circle of radius 0.4 about (0.5, 0.5), N(0,1) charges, 200,000 uniform
+    targets, loss = ||u(t) - u_true||^2 with u_true from a second charge
+    vector; forward vs forward + adjoint (d/dcharges, d/dsource positions)."""
+    from flexmm_amd import kernels
+    from flexmm_amd.kifmm2d import fmm_evaluate
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(4)
+    n, nt = 500_000, 200_000
+    th = rng.uniform(0, 2 * np.pi, n)
+    T = lambda a: torch.tensor(a, dtype=torch.float64, device=dev)
+    X, Y = T(0.5 + 0.4 * np.cos(th)), T(0.5 + 0.4 * np.sin(th))
+    TAU, TAU2 = T(rng.standard_normal(n)), T(rng.standard_normal(n))
+    tg = rng.random((nt, 2))
+    TX, TY = T(tg[:, 0]), T(tg[:, 1])
+    ker, _, _ = kernels.modified_helmholtz(20.0)
+    box = [0.0, 1.0, 0.0, 1.0]
+    with torch.no_grad():
+        utrue = fmm_evaluate(X, Y, TAU2, TX, TY, ker, 100, 60, bbox=box)
+
+    def fwd():
+        with torch.no_grad():
+            u = fmm_evaluate(X, Y, TAU, TX, TY, ker, 100, 60, bbox=box)
+            return ((u - utrue) ** 2).sum()
+
+    grads = {}
+
+    def fwd_bwd():
+        Xg, Yg, Tg = X.clone().requires_grad_(True), Y.clone().requires_grad_(True), TAU.clone().requires_grad_(True)
+        u = fmm_evaluate(Xg, Yg, Tg, TX, TY, ker, 100, 60, bbox=box)
+        loss = ((u - utrue) ** 2).sum()
+        loss.backward()
+        grads.update(u=u.detach(), tau=Tg.grad, x=Xg.grad)
+
+    ms_f = _timed(torch, fwd, steps, warmup)
+    ms_fb = _timed(torch, fwd_bwd, steps, warmup)
+    # FP64 direct-sum check of d/dcharges on 2,000 sampled sources
+    sel = torch.from_numpy(np.sort(rng.choice(n, 2000, replace=False))).to(dev)
+    g = 2.0 * (grads["u"] - utrue)
+    ref = []
+    for s0 in range(0, sel.numel(), 8):
+        ss = sel[s0:s0 + 8]
+        r = torch.sqrt((X[ss, None] - TX[None, :]) ** 2 + (Y[ss, None] - TY[None, :]) ** 2)
+        ref.append((torch.special.modified_bessel_k0(20.0 * r) * g[None, :]).sum(1) / (2 * np.pi))
+    ref = torch.cat(ref)
+    err = float((grads["tau"][sel] - ref).abs().max() / ref.abs().max())
+    return {"workload": "modified Helmholtz K0(20 r)/(2 pi), 500K sources on a circle, 200K "
+                        "uniform targets, Nequiv 60, Ncutoff 100, loss ||u - u_true||^2",
+            "forward_ms": round(ms_f, 4), "forward_adjoint_ms": round(ms_fb, 4),
+            "adjoint_over_forward": round(ms_fb / ms_f, 3),
+            "forward_Mpts_s": round(nt / ms_f / 1e3, 3),
+            "dcharges_error_vs_direct": err,
+            "error_sample": "d loss/d charges at 2000 sampled sources vs FP64 direct sum"}
+
+
+def stokes_workload(torch, steps, warmup):
+    """Stokeslet: 2,000,000 sources in 100 Gaussian clusters (sigma 0.01,
+    clipped to the unit square), N(0,1) forces, self-evaluation, Nequiv 64,
+    Ncutoff 100, mixed-precision M2L; forward and forward + adjoint w.r.t.
+    the forces; error vs FP64 direct sums at 4,000 sampled sources."""
+    from flexmm_amd.kifmm2d import stokes_evaluate
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(5)
+    nc, per = 100, 20_000
+    c = rng.uniform(0.1, 0.9, (nc, 2))
+    pts = np.clip(c[np.repeat(np.arange(nc), per)] + 0.01 * rng.standard_normal((nc * per, 2)), 0, 1)
+    n = pts.shape[0]
+    T = lambda a: torch.tensor(a, dtype=torch.float64, device=dev)
+    X, Y = T(pts[:, 0]), T(pts[:, 1])
+    FX, FY = T(rng.standard_normal(n)), T(rng.standard_normal(n))
+    box = [0.0, 1.0, 0.0, 1.0]
+    out = {}
+
+    def fwd():
+        with torch.no_grad():
+            out["uv"] = stokes_evaluate(X, Y, FX, FY, X, Y, 64, 100, bbox=box,
+                                        precision="mixed", check=True)
+
+    def fwd_bwd():
+        fx, fy = FX.clone().requires_grad_(True), FY.clone().requires_grad_(True)
+        u, v = stokes_evaluate(X, Y, fx, fy, X, Y, 64, 100, bbox=box, precision="mixed",
+                               check=True)
+        (u.sum() + v.sum()).backward()
+
+    ms_f = _timed(torch, fwd, steps, warmup)
+    ms_fb = _timed(torch, fwd_bwd, steps, warmup)
+    u, v = out["uv"]
+    sel = torch.from_numpy(np.sort(rng.choice(n, 4000, replace=False))).to(dev)
+    ru, rv = [], []
+    for s0 in range(0, sel.numel(), 16):
+        ss = sel[s0:s0 + 16]
+        dx = X[ss, None] - X[None, :]
+        dy = Y[ss, None] - Y[None, :]
+        d2 = dx * dx + dy * dy
+        keep = d2 > 0
+        id2 = torch.where(keep, 1.0 / torch.where(keep, d2, 1.0), 0.0)
+        lg = torch.where(keep, -0.5 * torch.log(torch.where(keep, d2, 1.0)), 0.0)
+        ru.append(((lg + dx * dx * id2) * FX[None, :] + dx * dy * id2 * FY[None, :]).sum(1))
+        rv.append((dx * dy * id2 * FX[None, :] + (lg + dy * dy * id2) * FY[None, :]).sum(1))
+    ru = torch.cat(ru) / (4 * np.pi)
+    rv = torch.cat(rv) / (4 * np.pi)
+    err = max(float((u[sel] - ru).abs().max() / ru.abs().max()),
+              float((v[sel] - rv).abs().max() / rv.abs().max()))
+    return {"workload": "Stokeslet, 2M sources in 100 Gaussian clusters (sigma 0.01), N(0,1) "
+                        "forces, self-evaluation, Nequiv 64, Ncutoff 100, mixed-precision M2L",
+            "forward_ms": round(ms_f, 4), "forward_adjoint_ms": round(ms_fb, 4),
+            "forward_Mpts_s": round(n / ms_f / 1e3, 3),
+            "velocity_error_vs_direct": err,
+            "error_sample": "max over u, v of max|err|/max|ref| at 4000 sampled sources"}
 
 def main():
     args = parse()
@@ -239,6 +364,9 @@ def main():
             "traffic": None,
         },
     }
+    if not args.no_extra:
+        out["adjoint"] = adjoint_workload(torch, 5, 2)
+        out["stokes"] = stokes_workload(torch, 3, 1)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample_targets)
     print(json.dumps(out))

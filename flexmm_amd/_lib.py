@@ -15,7 +15,9 @@ ABI_VERSION = 1
 # mirrors of the #defines in include/flexmm_amd.h
 E_ARG, E_STATE, E_LIMIT, E_INTERNAL = -1, -2, -3, -4
 KERNEL_LAPLACE = 0
-EVAL_POTENTIAL, EVAL_GRADIENT = 0, 1
+KERNEL_MODIFIED_HELMHOLTZ = 1
+KERNEL_STOKESLET = 2
+EVAL_POTENTIAL, EVAL_GRADIENT, EVAL_STOKES = 0, 1, 2
 PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_MIXED_M2L = 2

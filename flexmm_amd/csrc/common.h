@@ -152,10 +152,12 @@ __device__ __forceinline__ double log_tab(double x, const double2* __restrict__ 
 enum KernelId : int {
   KER_LAPLACE = 0,   // -log(d^2)/(4 pi)
   KER_MODHELM = 1,   // K0(k r)/(2 pi)
+  KER_STOKES = 2,    // 2D Stokeslet (velocity and pressure) via 6 Laplace densities
 };
 enum EvalId : int {
   EVAL_POT = 0,      // 1 output: u
   EVAL_GRAD = 1,     // 3 outputs: u, du/dtx, du/dty
+  EVAL_STOKES = 2,   // 3 outputs: velocity u, v and pressure p
 };
 enum Precision : int {
   PREC_FP64 = 0,

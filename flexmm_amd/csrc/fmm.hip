@@ -12,6 +12,7 @@
 // Dense contractions run on FP64 MFMA (v_mfma_f64_16x16x4f64) with the
 // operator block staged in LDS; expansions are rows of PD doubles (PD = 32 or
 // 64, p padded with zeros).
+#include "bessel.h"
 #include "tree.h"
 
 #include <cstdarg>
@@ -87,6 +88,7 @@ struct Plan {
   DevBuf srcf, tsum;   // mixed precision: leaf-relative FP32 sources, per-leaf charge sums
   bool have_srcf = false;
   bool have_exp = false;
+  bool up_done = false;
   // targets
   DevBuf tnode, tcnt, toff, tcur, tperm, tsx, tsy, icnt, ioff, items, ntot;
   PrimWork pw;
@@ -141,9 +143,33 @@ struct LogAcc {
   __device__ __forceinline__ double value() const { return __builtin_fma(ae, kLn2, at); }
 };
 
+// Potential accumulators of the supported kernels: sum_j w_j K(d_j^2).
+//   Laplace           K = -log(d^2)/(4 pi)   (examples/pure_kifmm_laplace.py:24-29)
+//   modified Helmholtz K = K0(k d)/(2 pi)     (pure_kifmm_modified_helmholtz_gradient.py:36-42)
+template <int KER>
+struct PotAcc;
+template <>
+struct PotAcc<KER_LAPLACE> {
+  LogAcc a;
+  __device__ __forceinline__ void add(double w, double d2, const double2* __restrict__ tab,
+                                      double) {
+    a.add(w, d2, tab);
+  }
+  __device__ __forceinline__ double result() const { return -kInv4Pi * a.value(); }
+};
+template <>
+struct PotAcc<KER_MODHELM> {
+  double s = 0.0;
+  __device__ __forceinline__ void add(double w, double d2, const double2* __restrict__,
+                                      double k) {
+    s = __builtin_fma(w, bessel_k0(k * __builtin_sqrt(d2)), s);
+  }
+  __device__ __forceinline__ double result() const { return kInv2Pi * s; }
+};
+
 // S2M: one wave per leaf, lane k = check point k on the large circle
 // (helpers.py:159-173; the source is taken relative to the box centre).
-template <int PD>
+template <int PD, int KER>
 __global__ __launch_bounds__(64) void k_s2m(const int* __restrict__ upnodes,
                                             const int* __restrict__ nlevel,
                                             const int* __restrict__ bot, const int* __restrict__ top,
@@ -152,7 +178,7 @@ __global__ __launch_bounds__(64) void k_s2m(const int* __restrict__ upnodes,
                                             const double* __restrict__ ymid,
                                             const double4* __restrict__ src,
                                             const double* __restrict__ ops, int64_t opsz, int p,
-                                            const double2* __restrict__ logtab,
+                                            const double2* __restrict__ logtab, double kp,
                                             double* __restrict__ check) {
   __shared__ double2 tab[kLogTableSize];
   load_logtab(tab, logtab);
@@ -169,7 +195,7 @@ __global__ __launch_bounds__(64) void k_s2m(const int* __restrict__ upnodes,
   const double cx = xmid[g], cy = ymid[g];
   const int b = __builtin_amdgcn_readfirstlane(bot[g]);
   const int t = __builtin_amdgcn_readfirstlane(top[g]);
-  LogAcc acc;
+  PotAcc<KER> acc;
   int j = b;
   for (; j + 4 <= t; j += 4) {
     double4 s4[4];
@@ -179,16 +205,16 @@ __global__ __launch_bounds__(64) void k_s2m(const int* __restrict__ upnodes,
     for (int u = 0; u < 4; ++u) {
       const double dx = ax - (s4[u].x - cx);
       const double dy = ay - (s4[u].y - cy);
-      acc.add(s4[u].z, __builtin_fma(dy, dy, dx * dx), tab);
+      acc.add(s4[u].z, __builtin_fma(dy, dy, dx * dx), tab, kp);
     }
   }
   for (; j < t; ++j) {
     const double4 sj = src[j];
     const double dx = ax - (sj.x - cx);
     const double dy = ay - (sj.y - cy);
-    acc.add(sj.z, __builtin_fma(dy, dy, dx * dx), tab);
+    acc.add(sj.z, __builtin_fma(dy, dy, dx * dx), tab, kp);
   }
-  if (k < PD) check[(int64_t)r * PD + k] = k < p ? -kInv4Pi * acc.value() : 0.0;
+  if (k < PD) check[(int64_t)r * PD + k] = k < p ? acc.result() : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -339,8 +365,12 @@ __global__ __launch_bounds__(256) void k_mult_level(int64_t row0, int nrows, int
                                                    const double* __restrict__ check,
                                                    const double* __restrict__ opsL,
                                                    double* __restrict__ mult,
-                                                   double* __restrict__ qs) {
+                                                   double* __restrict__ qs, int64_t dstride) {
   __shared__ __attribute__((aligned(16))) double buf[2][PD * Mfma<PD>::LDA];
+  // density blockIdx.y: its rows sit dstride rows further in check, mult and qs
+  check += (int64_t)blockIdx.y * dstride * PD;
+  mult += (int64_t)blockIdx.y * dstride * PD;
+  qs += (int64_t)blockIdx.y * dstride;
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int rl = blockIdx.x * 64 + w * 16 + (l & 15);
   const bool active = rl < nrows;
@@ -400,7 +430,11 @@ __global__ __launch_bounds__(512) void k_m2l_all(LevelTable lt, const int* __res
                                                 const int* __restrict__ m2l,
                                                 const double* __restrict__ mult,
                                                 const double* __restrict__ ops, int64_t opsz,
-                                                double* __restrict__ ple) {
+                                                double* __restrict__ ple, int64_t rstride,
+                                                int64_t nstride) {
+  // density blockIdx.y: multipole rows rstride rows further, local rows nstride nodes further
+  mult += (int64_t)blockIdx.y * rstride * PD;
+  ple += (int64_t)blockIdx.y * nstride * PD;
   constexpr int NT = 512, KS = Mfma<PD>::KS, HRB = Mfma<PD>::RB / 2;
   __shared__ __attribute__((aligned(16))) double buf[2][PD * Mfma<PD>::LDA];
   __shared__ int srcs[27][64];   // interaction-list entries of the block's 64 columns
@@ -514,7 +548,11 @@ __global__ __launch_bounds__(512) void k_m2l_all_f32(LevelTable lt, const int* _
                                                     const double* __restrict__ mult,
                                                     const double* __restrict__ qs,
                                                     const double* __restrict__ ops, int64_t opsz,
-                                                    double* __restrict__ ple) {
+                                                    double* __restrict__ ple, int64_t rstride,
+                                                    int64_t nstride) {
+  mult += (int64_t)blockIdx.y * rstride * PD;
+  qs += (int64_t)blockIdx.y * rstride;
+  ple += (int64_t)blockIdx.y * nstride * PD;
   constexpr int NT = 512, KS = PD / 4, HRB = PD / 32, LDAF = PD + 4;
   __shared__ __attribute__((aligned(16))) float buf[2][PD * LDAF];
   __shared__ int srcs[27][64];
@@ -634,7 +672,9 @@ __global__ __launch_bounds__(256) void k_local_level(int g0, int ngroups, int do
                                                     const double* __restrict__ ple,
                                                     const double* __restrict__ opsL,
                                                     const double* __restrict__ opsParent,
-                                                    double* __restrict__ le) {
+                                                    double* __restrict__ le, int64_t nstride) {
+  ple += (int64_t)blockIdx.y * nstride * PD;
+  le += (int64_t)blockIdx.y * nstride * PD;
   __shared__ __attribute__((aligned(16))) double buf[2][PD * Mfma<PD>::LDA];
   const int c = blockIdx.x & 3, tile = blockIdx.x >> 2;
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -736,10 +776,11 @@ __global__ __launch_bounds__(256) void k_make_items(const int* __restrict__ icnt
 
 // ---------------------------------------------------------------------------
 // L2T + P2P, one wave per (box, chunk of 64 targets); lane = target.
-template <int NOUT, bool CHECK>
+template <int KER, int NOUT, bool CHECK>
 struct Target {
   double x, y;
-  LogAcc u;
+  double kp;        // kernel parameter (Helmholtz k)
+  PotAcc<KER> u;
   double gx = 0.0, gy = 0.0;
   bool zhit = false;
   // one source (or one equivalent-density point) at (sx, sy) with weight w
@@ -747,20 +788,39 @@ struct Target {
                                        const double2* __restrict__ tab) {
     const double dx = x - sx;
     const double dy = y - sy;
-    const double d2 = __builtin_fma(dy, dy, dx * dx);
+    double d2 = __builtin_fma(dy, dy, dx * dx);
     const bool z = d2 == 0.0;
     if (CHECK) w = z ? 0.0 : w;   // coincident source skipped (helpers.py:214)
     else zhit |= z;
-    u.add(w, d2, tab);
-    if (NOUT == 3) {
-      const double qi = z ? 0.0 : w / d2;
-      gx = __builtin_fma(qi, dx, gx);
-      gy = __builtin_fma(qi, dy, gy);
+    if constexpr (KER == KER_LAPLACE) {
+      u.add(w, d2, tab, kp);
+      if (NOUT == 3) {
+        const double qi = z ? 0.0 : w / d2;
+        gx = __builtin_fma(qi, dx, gx);
+        gy = __builtin_fma(qi, dy, gy);
+      }
+    } else {
+      d2 = z ? 1.0 : d2;    // weight is 0 (or the result +inf) there
+      const double r = __builtin_sqrt(d2);
+      if constexpr (NOUT == 1) {
+        u.add(w, d2, tab, kp);
+      } else {
+        double k0, k1;
+        bessel_k01(kp * r, k0, k1);
+        u.s = __builtin_fma(w, k0, u.s);
+        const double qi = z ? 0.0 : -kp * k1 / r * w;
+        gx = __builtin_fma(qi, dx, gx);
+        gy = __builtin_fma(qi, dy, gy);
+      }
     }
+  }
+  // gradient scale: Laplace -1/(2 pi) sum w (t-s)/d^2; Helmholtz 1/(2 pi) sum w (-k K1/r)(t-s)
+  __device__ __forceinline__ double gscale() const {
+    return KER == KER_LAPLACE ? -kInv2Pi : kInv2Pi;
   }
 };
 
-template <int PD, int NOUT, bool CHECK>
+template <int PD, int KER, int NOUT, bool CHECK>
 __global__ __launch_bounds__(64) void k_eval(
     const int2* __restrict__ items, const int* __restrict__ nitems,
     const int* __restrict__ evalnode, const int* __restrict__ toff,
@@ -769,7 +829,7 @@ __global__ __launch_bounds__(64) void k_eval(
     const double* __restrict__ ymid, const int* __restrict__ coll, const int* __restrict__ bot,
     const int* __restrict__ top, const double4* __restrict__ src,
     const double* __restrict__ le, const double* __restrict__ ops, int64_t opsz, int p,
-    const double2* __restrict__ logtab, int64_t nt, double* __restrict__ out) {
+    const double2* __restrict__ logtab, double kp, int64_t nt, double* __restrict__ out) {
   __shared__ double2 tab[kLogTableSize];
   if ((int)blockIdx.x >= *nitems) return;
   load_logtab(tab, logtab);
@@ -780,13 +840,15 @@ __global__ __launch_bounds__(64) void k_eval(
   const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
   const int t = t0 + threadIdx.x;
   const bool act = t < t1;
-  Target<NOUT, CHECK> T;
+  Target<KER, NOUT, CHECK> T;
+  T.kp = kp;
   T.x = act ? tsx[t] : 0.0;
   T.y = act ? tsy[t] : 0.0;
   // L2T: helpers.py:219-235 (the equivalent density sits on the large circle)
   const int d = nlevel[g];
   if (d >= 2) {
-    Target<NOUT, false> L;
+    Target<KER, NOUT, false> L;
+    L.kp = kp;
     L.x = T.x - xmid[g];
     L.y = T.y - ymid[g];
     const double* lx = ops + d * opsz + (int64_t)OP_NBLK * PD * PD;
@@ -819,12 +881,12 @@ __global__ __launch_bounds__(64) void k_eval(
   }
   if (act) {
     const int64_t o = tperm[t];
-    double u = -kInv4Pi * T.u.value();
+    double u = T.u.result();
     if (!CHECK && T.zhit) u = __builtin_inf();   // log(0): the reference's -log(0) = +inf
     out[o] = u;
     if (NOUT == 3) {
-      out[nt + o] = -kInv2Pi * T.gx;
-      out[2 * nt + o] = -kInv2Pi * T.gy;
+      out[nt + o] = T.gscale() * T.gx;
+      out[2 * nt + o] = T.gscale() * T.gy;
     }
   }
 }
@@ -890,12 +952,14 @@ __global__ __launch_bounds__(64) void k_eval_mixed(
   const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
   const int t = t0 + threadIdx.x;
   const bool act = t < t1;
-  Target<1, CHECK> T;
+  Target<KER_LAPLACE, 1, CHECK> T;
+  T.kp = 0.0;
   T.x = act ? tsx[t] : 0.0;
   T.y = act ? tsy[t] : 0.0;
   const int d = nlevel[g];
   if (d >= 2) {   // L2T in FP64 (the local densities carry large cancelling modes)
-    Target<1, false> L;
+    Target<KER_LAPLACE, 1, false> L;
+    L.kp = 0.0;
     L.x = T.x - xmid[g];
     L.y = T.y - ymid[g];
     const double* lx = ops + d * opsz + ops_off_circles(PD);
@@ -956,7 +1020,7 @@ __global__ __launch_bounds__(64) void k_eval_mixed(
         if (nr) near64(j);
       }
       u32 += (double)acc;
-      T.u.at = __builtin_fma(lw2, tsum[c] - excl, T.u.at);
+      T.u.a.at = __builtin_fma(lw2, tsum[c] - excl, T.u.a.at);
     } else {
       // colleague with children (adaptive trees): its sources are not stored
       // relative to it, so this range stays FP64
@@ -968,10 +1032,193 @@ __global__ __launch_bounds__(64) void k_eval_mixed(
     }
   }
   if (act) {
-    double u = -kInv4Pi * (__builtin_fma(u32, kLn2, T.u.value()));
+    double u = -kInv4Pi * (__builtin_fma(u32, kLn2, T.u.a.value()));
     if (!CHECK && T.zhit) u = __builtin_inf();
     out[tperm[t]] = u;
   }
+}
+
+// ---------------------------------------------------------------------------
+// 2D Stokeslet (stokes_fmm.py:33-61, 102-184): for a force f at s and r = t - s,
+//   u = (1/4 pi) [ -log|r| f + (r.f) r / |r|^2 ],   p = (1/2 pi) (r.f) / |r|^2.
+// With L[g](t) = sum_s g_s log|t - s| and coordinates X = x - c0 relative to
+// the root centre c0, since r_i r_j / |r|^2 f_j = (X_i(t) - X_i(s)) f_j d_j log|r|:
+//   u_i = (1/4 pi) [ -L[f_i] + X_i(t) B - C_i ],  p = B / (2 pi),
+//   B = d_x L[f_x] + d_y L[f_y],  C_i = d_x L[X_i f_x] + d_y L[X_i f_y].
+// The far field therefore needs six Laplace densities per source
+//   (f_x, f_y, X f_x, X f_y, Y f_x, Y f_y),
+// run through the scalar passes as one density dimension; the near field is
+// the Stokeslet itself.  The Laplace passes carry -log(d^2)/(4 pi), so
+// L[g] = -2 pi Phi[g] and grad L[g] = sum_k a_k (t - e_k) / |t - e_k|^2 for a
+// local density a on the points e_k.
+constexpr int kStokesDens = 6;
+
+// Sources in tree order as (x, y, f_x, f_y); tau holds f_x[n] then f_y[n].
+__global__ void k_pack_stokes(const double* __restrict__ tau, const uint32_t* __restrict__ perm,
+                              const double* __restrict__ sx, const double* __restrict__ sy,
+                              int64_t n, double4* __restrict__ packed) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t j = perm[i];
+  packed[i] = make_double4(sx[i], sy[i], tau[j], tau[n + j]);
+}
+
+// S2M of the six densities: one log per (source, check point) serves all six.
+template <int PD>
+__global__ __launch_bounds__(64) void k_s2m_stokes(const int* __restrict__ upnodes,
+                                                   const int* __restrict__ nlevel,
+                                                   const int* __restrict__ bot,
+                                                   const int* __restrict__ top,
+                                                   const int* __restrict__ row,
+                                                   const double* __restrict__ xmid,
+                                                   const double* __restrict__ ymid,
+                                                   const double4* __restrict__ src,
+                                                   const double* __restrict__ ops, int64_t opsz,
+                                                   int p, const double2* __restrict__ logtab,
+                                                   double c0x, double c0y, int64_t rstride,
+                                                   double* __restrict__ check) {
+  __shared__ double2 tab[kLogTableSize];
+  load_logtab(tab, logtab);
+  __syncthreads();
+  const int g = upnodes[blockIdx.x];
+  const int d = nlevel[g];
+  const int k = threadIdx.x;
+  const int r = row[g];
+  if (r < 0 || d < 2) return;   // block-uniform
+  const double* lx = ops + d * opsz + ops_off_circles(PD);
+  const double* ly = lx + PD;
+  const double ax = k < p ? lx[k] : 1.0;
+  const double ay = k < p ? ly[k] : 1.0;
+  const double cx = xmid[g], cy = ymid[g];
+  const int b = __builtin_amdgcn_readfirstlane(bot[g]);
+  const int t = __builtin_amdgcn_readfirstlane(top[g]);
+  double acc[kStokesDens] = {};
+  for (int j = b; j < t; ++j) {
+    const double4 sj = src[j];
+    const double dx = ax - (sj.x - cx);
+    const double dy = ay - (sj.y - cy);
+    const double lg = log_tab(__builtin_fma(dy, dy, dx * dx), tab);
+    const double X = sj.x - c0x, Y = sj.y - c0y;
+    const double w[kStokesDens] = {sj.z, sj.w, X * sj.z, X * sj.w, Y * sj.z, Y * sj.w};
+#pragma unroll
+    for (int q = 0; q < kStokesDens; ++q) acc[q] = __builtin_fma(w[q], lg, acc[q]);
+  }
+  if (k < PD) {
+#pragma unroll
+    for (int q = 0; q < kStokesDens; ++q)
+      check[(q * rstride + r) * PD + k] = k < p ? -kInv4Pi * acc[q] : 0.0;
+  }
+}
+
+// L2T of the six local densities + Stokeslet P2P, outputs (u, v, p).
+template <int PD, bool CHECK>
+__global__ __launch_bounds__(64) void k_eval_stokes(
+    const int2* __restrict__ items, const int* __restrict__ nitems,
+    const int* __restrict__ evalnode, const int* __restrict__ toff,
+    const int* __restrict__ tperm, const double* __restrict__ tsx, const double* __restrict__ tsy,
+    const int* __restrict__ nlevel, const double* __restrict__ xmid,
+    const double* __restrict__ ymid, const int* __restrict__ coll, const int* __restrict__ bot,
+    const int* __restrict__ top, const double4* __restrict__ src,
+    const double* __restrict__ le, int64_t nstride, const double* __restrict__ ops, int64_t opsz,
+    int p, const double2* __restrict__ logtab, double c0x, double c0y, int64_t nt,
+    double* __restrict__ out) {
+  __shared__ double2 tab[kLogTableSize];
+  if ((int)blockIdx.x >= *nitems) return;
+  load_logtab(tab, logtab);
+  __syncthreads();
+  const int2 it = items[blockIdx.x];
+  const int g = evalnode[it.x];
+  const int t0 = toff[it.x] + it.y * kTgtChunk;
+  const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
+  const int t = t0 + threadIdx.x;
+  const bool act = t < t1;
+  const double x = act ? tsx[t] : 0.0;
+  const double y = act ? tsy[t] : 0.0;
+  double u = 0.0, v = 0.0, pr = 0.0;
+  const int d = nlevel[g];
+  if (d >= 2) {
+    const double* lx = ops + d * opsz + ops_off_circles(PD);
+    const double* ly = lx + PD;
+    const double* a = le + (int64_t)g * PD;
+    const double xr = x - xmid[g], yr = y - ymid[g];
+    LogAcc l0, l1;
+    double B = 0.0, Cx = 0.0, Cy = 0.0;
+    for (int k = 0; k < p; ++k) {
+      const double dx = xr - lx[k], dy = yr - ly[k];
+      const double d2 = __builtin_fma(dy, dy, dx * dx);
+      const double id2 = 1.0 / d2;
+      const double a0 = a[k], a1 = a[nstride * PD + k], a2 = a[2 * nstride * PD + k],
+                   a3 = a[3 * nstride * PD + k], a4 = a[4 * nstride * PD + k],
+                   a5 = a[5 * nstride * PD + k];
+      l0.add(a0, d2, tab);
+      l1.add(a1, d2, tab);
+      B = __builtin_fma(__builtin_fma(a0, dx, a1 * dy), id2, B);
+      Cx = __builtin_fma(__builtin_fma(a2, dx, a3 * dy), id2, Cx);
+      Cy = __builtin_fma(__builtin_fma(a4, dx, a5 * dy), id2, Cy);
+    }
+    // -L[f] = -sum a (1/2) log d^2
+    const double X = x - c0x, Y = y - c0y;
+    u = kInv4Pi * (-0.5 * l0.value() + X * B - Cx);
+    v = kInv4Pi * (-0.5 * l1.value() + Y * B - Cy);
+    pr = kInv2Pi * B;
+  }
+  // near field: the Stokeslet itself (stokes_fmm.py:102-184)
+  LogAcc lx_, ly_;
+  double su = 0.0, sv = 0.0, sp = 0.0;
+  bool zhit = false;
+  const int* cl = coll + (int64_t)g * 9;
+  for (int qn = 0; qn < 9; ++qn) {
+    const int c = cl[qn];
+    if (c < 0) continue;
+    const int b = __builtin_amdgcn_readfirstlane(bot[c]);
+    const int e = __builtin_amdgcn_readfirstlane(top[c]);
+    for (int j = b; j < e; ++j) {
+      const double4 sj = src[j];
+      const double dx = x - sj.x, dy = y - sj.y;
+      double d2 = __builtin_fma(dy, dy, dx * dx);
+      const bool z = d2 == 0.0;
+      double fx = sj.z, fy = sj.w;
+      if (CHECK) {
+        fx = z ? 0.0 : fx;
+        fy = z ? 0.0 : fy;
+      } else {
+        zhit |= z;
+      }
+      d2 = z ? 1.0 : d2;
+      const double id2 = 1.0 / d2;
+      lx_.add(fx, d2, tab);
+      ly_.add(fy, d2, tab);
+      const double rf = __builtin_fma(dx, fx, dy * fy) * id2;
+      su = __builtin_fma(dx, rf, su);
+      sv = __builtin_fma(dy, rf, sv);
+      sp += rf;
+    }
+  }
+  if (act) {
+    const int64_t o = tperm[t];
+    u += kInv4Pi * (-0.5 * lx_.value() + su);
+    v += kInv4Pi * (-0.5 * ly_.value() + sv);
+    pr += kInv2Pi * sp;
+    if (!CHECK && zhit) u = v = pr = __builtin_nan("");
+    out[o] = u;
+    out[nt + o] = v;
+    out[2 * nt + o] = pr;
+  }
+}
+
+// Selects the k_eval instance of (PD, kernel, outputs, self-exclusion).
+typedef decltype(&k_eval<32, KER_LAPLACE, 1, false>) EvalKernelFn;
+template <int PD, int KER>
+EvalKernelFn eval_kernel_k(int nout, bool check) {
+  if (nout == 1) return check ? k_eval<PD, KER, 1, true> : k_eval<PD, KER, 1, false>;
+  return check ? k_eval<PD, KER, 3, true> : k_eval<PD, KER, 3, false>;
+}
+EvalKernelFn eval_kernel(int PD, int ker, int nout, bool check) {
+  if (ker == KER_LAPLACE)
+    return PD == 32 ? eval_kernel_k<32, KER_LAPLACE>(nout, check)
+                    : eval_kernel_k<64, KER_LAPLACE>(nout, check);
+  return PD == 32 ? eval_kernel_k<32, KER_MODHELM>(nout, check)
+                  : eval_kernel_k<64, KER_MODHELM>(nout, check);
 }
 
 // Algorithmic work counts (for the roofline accounting).
@@ -1039,28 +1286,47 @@ static void upward(Plan& P, const double* tau, hipStream_t s) {
   check_ops(P);
   const int PD = P.PD;
   const int64_t opsz = ops_per_level(PD);
+  const bool stokes = P.kernel_id == KER_STOKES;
+  const int nd = stokes ? kStokesDens : 1;
   P.tau_s.ensure(t.n * sizeof(double));
   P.src.ensure(t.n * sizeof(double4));
   P.have_srcf = false;
-  hipLaunchKernelGGL(k_pack_sources, dim3(grid256(t.n)), dim3(256), 0, s, tau,
-                     t.perm.as<uint32_t>(), t.sx.as<double>(), t.sy.as<double>(), t.n,
-                     P.tau_s.as<double>(), P.src.as<double4>());
+  if (stokes) {
+    hipLaunchKernelGGL(k_pack_stokes, dim3(grid256(t.n)), dim3(256), 0, s, tau,
+                       t.perm.as<uint32_t>(), t.sx.as<double>(), t.sy.as<double>(), t.n,
+                       P.src.as<double4>());
+  } else {
+    hipLaunchKernelGGL(k_pack_sources, dim3(grid256(t.n)), dim3(256), 0, s, tau,
+                       t.perm.as<uint32_t>(), t.sx.as<double>(), t.sy.as<double>(), t.n,
+                       P.tau_s.as<double>(), P.src.as<double4>());
+  }
   FMM_LAUNCH_CHECK();
+  P.up_done = true;
   const int64_t nr = t.n_rows;
-  P.check.ensure((nr + 1) * PD * sizeof(double));
-  P.mult.ensure((nr + 1) * PD * sizeof(double));
-  P.qs.ensure((nr + 1) * sizeof(double));
+  const int64_t rstride = nr + 1;   // rows per density
+  P.check.ensure(rstride * PD * nd * sizeof(double));
+  P.mult.ensure(rstride * PD * nd * sizeof(double));
+  P.qs.ensure(rstride * nd * sizeof(double));
   if (t.L <= 2 || nr == 0) {
     P.have_exp = true;
     return;
   }
   P.mark(FMM2D_PHASE_S2M, false, s);
-  if (t.n_up > 0) {
-    auto kern = PD == 32 ? k_s2m<32> : k_s2m<64>;
+  if (t.n_up > 0 && stokes) {
+    auto kern = PD == 32 ? k_s2m_stokes<32> : k_s2m_stokes<64>;
     hipLaunchKernelGGL(kern, dim3(t.n_up), dim3(64), 0, s, t.upnodes.as<int>(), t.nlevel.as<int>(),
                        t.bot.as<int>(), t.top.as<int>(), t.row.as<int>(), t.xmid.as<double>(),
                        t.ymid.as<double>(), P.src.as<double4>(), P.ops.as<double>(), opsz, P.p,
-                       P.logtab.as<double2>(), P.check.as<double>());
+                       P.logtab.as<double2>(), t.xmin + 0.5 * t.w0, t.ymin + 0.5 * t.w0,
+                       rstride, P.check.as<double>());
+    FMM_LAUNCH_CHECK();
+  } else if (t.n_up > 0) {
+    auto kern = P.kernel_id == KER_LAPLACE ? (PD == 32 ? k_s2m<32, KER_LAPLACE> : k_s2m<64, KER_LAPLACE>)
+                                           : (PD == 32 ? k_s2m<32, KER_MODHELM> : k_s2m<64, KER_MODHELM>);
+    hipLaunchKernelGGL(kern, dim3(t.n_up), dim3(64), 0, s, t.upnodes.as<int>(), t.nlevel.as<int>(),
+                       t.bot.as<int>(), t.top.as<int>(), t.row.as<int>(), t.xmid.as<double>(),
+                       t.ymid.as<double>(), P.src.as<double4>(), P.ops.as<double>(), opsz, P.p,
+                       P.logtab.as<double2>(), P.kparam, P.check.as<double>());
     FMM_LAUNCH_CHECK();
   }
   P.mark(FMM2D_PHASE_S2M, true, s);
@@ -1070,10 +1336,10 @@ static void upward(Plan& P, const double* tau, hipStream_t s) {
     if (nrows == 0) continue;
     const double* opsL = P.ops.as<double>() + d * opsz;
     auto kern = PD == 32 ? k_mult_level<32> : k_mult_level<64>;
-    hipLaunchKernelGGL(kern, dim3(ceil_div(nrows, 64)), dim3(256), 0, s, r0, nrows,
+    hipLaunchKernelGGL(kern, dim3(ceil_div(nrows, 64), nd), dim3(256), 0, s, r0, nrows,
                        d < t.L - 1 ? 1 : 0, t.rownode.as<int>(), t.flags.as<int>(),
                        t.child.as<int>(), t.row.as<int>(), P.check.as<double>(), opsL,
-                       P.mult.as<double>(), P.qs.as<double>());
+                       P.mult.as<double>(), P.qs.as<double>(), rstride);
     FMM_LAUNCH_CHECK();
   }
   P.have_exp = false;
@@ -1084,9 +1350,11 @@ static void downward(Plan& P, hipStream_t s) {
   check_ops(P);
   const int PD = P.PD;
   const int64_t opsz = ops_per_level(PD);
-  P.le.ensure((t.n_nodes + 1) * PD * sizeof(double));
-  P.ple.ensure((t.n_nodes + 1) * PD * sizeof(double));
-  FMM_HIP(hipMemsetAsync(P.le.ptr, 0, (t.n_nodes + 1) * PD * sizeof(double), s));
+  const int nd = P.kernel_id == KER_STOKES ? kStokesDens : 1;
+  const int64_t rstride = t.n_rows + 1, nstride = t.n_nodes + 1;
+  P.le.ensure(nstride * PD * nd * sizeof(double));
+  P.ple.ensure(nstride * PD * nd * sizeof(double));
+  FMM_HIP(hipMemsetAsync(P.le.ptr, 0, nstride * PD * nd * sizeof(double), s));
   if (t.L <= 2) {
     P.have_exp = true;
     return;
@@ -1105,14 +1373,15 @@ static void downward(Plan& P, hipStream_t s) {
   lt.boff[lt.nlev] = nb;
   if (P.precision != PREC_FP64) {
     auto kern = PD == 32 ? k_m2l_all_f32<32> : k_m2l_all_f32<64>;
-    hipLaunchKernelGGL(kern, dim3(nb), dim3(512), 0, s, lt, t.flags.as<int>(), t.m2l.as<int>(),
-                       P.mult.as<double>(), P.qs.as<double>(), P.ops.as<double>(), opsz,
-                       P.ple.as<double>());
+    hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(512), 0, s, lt, t.flags.as<int>(),
+                       t.m2l.as<int>(), P.mult.as<double>(), P.qs.as<double>(),
+                       P.ops.as<double>(), opsz, P.ple.as<double>(), rstride, nstride);
     FMM_LAUNCH_CHECK();
   } else {
     auto kern = PD == 32 ? k_m2l_all<32> : k_m2l_all<64>;
-    hipLaunchKernelGGL(kern, dim3(nb), dim3(512), 0, s, lt, t.flags.as<int>(), t.m2l.as<int>(),
-                       P.mult.as<double>(), P.ops.as<double>(), opsz, P.ple.as<double>());
+    hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(512), 0, s, lt, t.flags.as<int>(),
+                       t.m2l.as<int>(), P.mult.as<double>(), P.ops.as<double>(), opsz,
+                       P.ple.as<double>(), rstride, nstride);
     FMM_LAUNCH_CHECK();
   }
   for (int d = 2; d < t.L; ++d) {
@@ -1120,10 +1389,10 @@ static void downward(Plan& P, hipStream_t s) {
     if (ngroups == 0) continue;
     const double* opsL = P.ops.as<double>() + d * opsz;
     auto kern = PD == 32 ? k_local_level<32> : k_local_level<64>;
-    hipLaunchKernelGGL(kern, dim3(4 * ceil_div(ngroups, 64)), dim3(256), 0, s,
+    hipLaunchKernelGGL(kern, dim3(4 * ceil_div(ngroups, 64), nd), dim3(256), 0, s,
                        (int)t.lev[d].node_off, ngroups, d >= 3 ? 1 : 0, t.flags.as<int>(),
                        t.parent.as<int>(), P.ple.as<double>(), opsL, opsL - opsz,
-                       P.le.as<double>());
+                       P.le.as<double>(), nstride);
     FMM_LAUNCH_CHECK();
   }
   P.have_exp = true;
@@ -1133,7 +1402,9 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
                      double* out, hipStream_t s) {
   Tree& t = P.tree;
   FMM_REQUIRE(P.have_exp, FMM_E_STATE, "no expansions: call fmm2d_upward and fmm2d_downward first");
-  FMM_REQUIRE(ev == EVAL_POT || ev == EVAL_GRAD, FMM_E_ARG, "unknown evaluator %d", ev);
+  const bool stokes = P.kernel_id == KER_STOKES;
+  FMM_REQUIRE(stokes ? ev == EVAL_STOKES : (ev == EVAL_POT || ev == EVAL_GRAD), FMM_E_ARG,
+              "evaluator %d does not belong to kernel %d", ev, P.kernel_id);
   FMM_REQUIRE(nt >= 0 && nt < ((int64_t)1 << 31), FMM_E_LIMIT, "too many targets");
   if (nt == 0) return;
   const int64_t ne = t.n_eval;
@@ -1169,7 +1440,22 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   FMM_LAUNCH_CHECK();
   const int PD = P.PD;
   const int64_t opsz = ops_per_level(PD);
-  if (P.precision == PREC_MIXED && ev == EVAL_POT) {
+  if (stokes) {
+    auto sk = PD == 32 ? (check ? k_eval_stokes<32, true> : k_eval_stokes<32, false>)
+                       : (check ? k_eval_stokes<64, true> : k_eval_stokes<64, false>);
+    P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
+    hipLaunchKernelGGL(sk, dim3(icap), dim3(64), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
+                       t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(),
+                       P.tsx.as<double>(), P.tsy.as<double>(), t.nlevel.as<int>(),
+                       t.xmid.as<double>(), t.ymid.as<double>(), t.coll.as<int>(),
+                       t.bot.as<int>(), t.top.as<int>(), P.src.as<double4>(), P.le.as<double>(),
+                       t.n_nodes + 1, P.ops.as<double>(), opsz, P.p, P.logtab.as<double2>(),
+                       t.xmin + 0.5 * t.w0, t.ymin + 0.5 * t.w0, nt, out);
+    FMM_LAUNCH_CHECK();
+    P.mark(FMM2D_PHASE_EVAL_KERNEL, true, s);
+    return;
+  }
+  if (P.precision == PREC_MIXED && ev == EVAL_POT && P.kernel_id == KER_LAPLACE) {
     LevelConsts lc{};
     for (int d = 0; d < t.L; ++d) {
       lc.invw[d] = 1.0 / t.lev[d].width;
@@ -1200,20 +1486,14 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
     P.mark(FMM2D_PHASE_EVAL_KERNEL, true, s);
     return;
   }
-  decltype(&k_eval<32, 1, false>) kern;
-  if (ev == EVAL_POT)
-    kern = check ? (PD == 32 ? k_eval<32, 1, true> : k_eval<64, 1, true>)
-                 : (PD == 32 ? k_eval<32, 1, false> : k_eval<64, 1, false>);
-  else
-    kern = check ? (PD == 32 ? k_eval<32, 3, true> : k_eval<64, 3, true>)
-                 : (PD == 32 ? k_eval<32, 3, false> : k_eval<64, 3, false>);
+  auto kern = eval_kernel(PD, P.kernel_id, ev == EVAL_POT ? 1 : 3, check);
   P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
   hipLaunchKernelGGL(kern, dim3(icap), dim3(64), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
                      t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(), P.tsx.as<double>(),
                      P.tsy.as<double>(), t.nlevel.as<int>(), t.xmid.as<double>(),
                      t.ymid.as<double>(), t.coll.as<int>(), t.bot.as<int>(), t.top.as<int>(),
                      P.src.as<double4>(), P.le.as<double>(), P.ops.as<double>(), opsz, P.p,
-                     P.logtab.as<double2>(), nt, out);
+                     P.logtab.as<double2>(), P.kparam, nt, out);
   FMM_LAUNCH_CHECK();
   P.mark(FMM2D_PHASE_EVAL_KERNEL, true, s);
 }
@@ -1260,7 +1540,10 @@ int fmm2d_plan_create(int kernel_id, const double* kparams, int nkparams, int p,
   ABI_BEGIN
   NEED(out_plan);
   *out_plan = nullptr;
-  FMM_REQUIRE(kernel_id == fmm::KER_LAPLACE, fmm::FMM_E_ARG, "unknown kernel id %d", kernel_id);
+  FMM_REQUIRE(kernel_id == fmm::KER_LAPLACE || kernel_id == fmm::KER_MODHELM ||
+                  kernel_id == fmm::KER_STOKES, fmm::FMM_E_ARG, "unknown kernel id %d", kernel_id);
+  FMM_REQUIRE(kernel_id != fmm::KER_MODHELM || (nkparams == 1 && kparams && kparams[0] > 0.0),
+              fmm::FMM_E_ARG, "the modified Helmholtz kernel needs one parameter k > 0");
   FMM_REQUIRE(p >= 4 && p <= 64, fmm::FMM_E_LIMIT, "Nequiv = %d outside [4, 64]", p);
   FMM_REQUIRE(precision == fmm::PREC_FP64 || precision == fmm::PREC_MIXED ||
                   precision == fmm::PREC_MIXED_M2L, fmm::FMM_E_ARG,
@@ -1319,6 +1602,7 @@ int fmm2d_tree_build(void* plan, const double* x, const double* y, int64_t n, co
               (long long)ncut);
   const double w0 = bbox[1] - bbox[0];   // tree.py:484 (the x extent sets the width)
   P->have_exp = false;
+  P->up_done = false;
   P->mark(FMM2D_PHASE_TREE, false, (hipStream_t)stream);
   fmm::tree_build(P->tree, x, y, n, bbox[0], bbox[2], w0, (int)ncut, (hipStream_t)stream);
   P->mark(FMM2D_PHASE_TREE, true, (hipStream_t)stream);
@@ -1444,7 +1728,7 @@ int fmm2d_upward(void* plan, const double* tau, int64_t n, void* stream) {
 int fmm2d_downward(void* plan, void* stream) {
   ABI_BEGIN
   Plan* P = as_plan(plan);
-  FMM_REQUIRE(P->tree.built && P->tau_s.ptr, fmm::FMM_E_STATE, "call fmm2d_upward first");
+  FMM_REQUIRE(P->tree.built && P->up_done, fmm::FMM_E_STATE, "call fmm2d_upward first");
   P->mark(FMM2D_PHASE_DOWNWARD, false, (hipStream_t)stream);
   fmm::downward(*P, (hipStream_t)stream);
   P->mark(FMM2D_PHASE_DOWNWARD, true, (hipStream_t)stream);

@@ -84,8 +84,60 @@ laplace_gradient = Evaluator(
     "laplace_gradient", laplace, _lib.EVAL_GRADIENT, 3, _laplace_grad,
     "(u, du/dx_t, du/dy_t) (examples/pure_kifmm_laplace_gradient.py:32-55).")
 
+def _stokeslet_u(sx, sy, tx, ty):
+    """First row of the Stokeslet for a unit x-force (stokes_fmm.py:33-61)."""
+    dx = tx - sx
+    dy = ty - sy
+    d2 = dx * dx + dy * dy
+    return (-0.5 * np.log(d2) + dx * dx / d2) / FOUR_PI
+
+
+stokeslet = Kernel(
+    "stokeslet", _lib.KERNEL_STOKESLET, (), _stokeslet_u,
+    "2D Stokeslet (-log r I + r r^T / r^2) / (4 pi) and pressure r / (2 pi r^2) "
+    "(kifmm2d/vector/stokes_fmm.py:33-61); used through flexmm_amd.kifmm2d.StokesFMM.")
+
 KERNELS = {k.name: k for k in (laplace,)}
 EVALUATORS = {e.name: e for e in (laplace_potential, laplace_gradient)}
+
+_MH = {}
+
+
+def modified_helmholtz(k):
+    """Modified-Helmholtz kernel K0(k |t-s|)/(2 pi), k > 0, with its potential
+    and gradient evaluators (examples/pure_kifmm_modified_helmholtz_gradient.py:
+    36-68; the device evaluates K0/K1 in FP64, the example through a 1e-8
+    interpolant).  Returns (kernel, potential_evaluator, gradient_evaluator);
+    the same k returns the same objects."""
+    k = float(k)
+    if not k > 0.0:
+        raise ValueError("modified Helmholtz needs k > 0, got %r" % k)
+    if k in _MH:
+        return _MH[k]
+    from scipy import special
+
+    def mh(sx, sy, tx, ty):
+        dx = tx - sx
+        dy = ty - sy
+        return special.k0(k * np.sqrt(dx * dx + dy * dy)) / TWO_PI
+
+    def mh_grad(sx, sy, tx, ty):
+        dx = tx - sx
+        dy = ty - sy
+        r = np.sqrt(dx * dx + dy * dy)
+        g = -k * special.k1(k * r) / (TWO_PI * r)
+        return (special.k0(k * r) / TWO_PI, g * dx, g * dy)
+
+    ker = Kernel("modified_helmholtz", _lib.KERNEL_MODIFIED_HELMHOLTZ, (k,), mh,
+                 "K0(%g |t-s|)/(2 pi)" % k)
+    pot = Evaluator("modified_helmholtz_potential", ker, _lib.EVAL_POTENTIAL, 1,
+                    lambda *a: (mh(*a),), "Potential u.")
+    grad = Evaluator("modified_helmholtz_gradient", ker, _lib.EVAL_GRADIENT, 3, mh_grad,
+                     "(u, du/dx_t, du/dy_t).")
+    _MH[k] = (ker, pot, grad)
+    EVALUATORS[pot.name + "(%g)" % k] = pot
+    EVALUATORS[grad.name + "(%g)" % k] = grad
+    return _MH[k]
 
 
 def resolve_kernel(kernel_eval):
@@ -96,6 +148,14 @@ def resolve_kernel(kernel_eval):
     raise NotImplementedError(
         "kernel_eval=%r has no device implementation; pass one of flexmm_amd.kernels (%s)"
         % (kernel_eval, ", ".join(sorted(KERNELS))))
+
+
+def gradient_evaluator(kernel):
+    """The (u, du/dx_t, du/dy_t) evaluator of a kernel."""
+    for e in EVALUATORS.values():
+        if e.kernel is kernel and e.evaluator_id == _lib.EVAL_GRADIENT:
+            return e
+    raise NotImplementedError("no gradient evaluator for %r" % (kernel,))
 
 
 def default_evaluator(kernel):

@@ -34,17 +34,26 @@ extern "C" {
 
 /* kernels (the reference's user kernel_eval, fmm.py:38; examples/pure_kifmm_laplace.py:24-29) */
 #define FMM2D_KERNEL_LAPLACE 0 /* -log(|x-y|^2)/(4 pi) */
+#define FMM2D_KERNEL_MODIFIED_HELMHOLTZ 1 /* K0(k |x-y|)/(2 pi), kparams = {k}, k > 0
+                                             (examples/pure_kifmm_modified_helmholtz_gradient.py:36-68) */
+#define FMM2D_KERNEL_STOKESLET 2 /* 2D Stokeslet: velocity (-log r I + r r^T / r^2) f / (4 pi) and
+                                    pressure (r . f) / (2 pi r^2) (kifmm2d/vector/stokes_fmm.py:33-61);
+                                    computed with six Laplace densities per source, so the plan
+                                    takes the Laplace operators */
 
 /* evaluators (register_evaluator, helpers.py:176-238) */
 #define FMM2D_EVAL_POTENTIAL 0 /* 1 output: u */
+#define FMM2D_EVAL_STOKES 2    /* 3 outputs: velocity u, v and pressure p (Stokeslet kernel
+                                  only; stokes_fmm.py:827-840) */
 #define FMM2D_EVAL_GRADIENT 1  /* 3 outputs: u, du/dx_t, du/dy_t
-                                  (examples/pure_kifmm_laplace_gradient.py:32-55) */
+                                  (examples/pure_kifmm_laplace_gradient.py:32-55,
+                                  pure_kifmm_modified_helmholtz_gradient.py:45-68) */
 
 /* precision */
 #define FMM2D_PREC_FP64 0  /* every step in FP64 */
-#define FMM2D_PREC_MIXED 1 /* FP32 MFMA M2L (constant mode in FP64) and FP32 near field
-                              for leaf colleagues at separations >= w/16 (potential
-                              evaluator); FP64 elsewhere */
+#define FMM2D_PREC_MIXED 1 /* FP32 MFMA M2L (constant mode in FP64) and, for the Laplace
+                              potential, FP32 near field for leaf colleagues at
+                              separations >= w/16; FP64 elsewhere */
 #define FMM2D_PREC_MIXED_M2L 2 /* FP32 MFMA M2L only; near field in FP64 */
 
 /* tree arrays for fmm2d_tree_export */
@@ -121,8 +130,9 @@ int64_t fmm2d_operator_size(int p);
 int fmm2d_set_operators(void* plan, int level, const double* dev_ops, int64_t n, void* stream);
 int fmm2d_clear_operators(void* plan);
 
-/* Upward pass from charges tau (input order, n values).
- * Replaces FMM.upwards_pass (fmm.py:73-124). */
+/* Upward pass from charges tau (input order, n values; for the Stokeslet
+ * kernel 2n values: the forces f_x[n] then f_y[n]).
+ * Replaces FMM.upwards_pass (fmm.py:73-124; stokes_fmm.py:689-718). */
 int fmm2d_upward(void* plan, const double* tau, int64_t n, void* stream);
 
 /* Downward pass.  Replaces FMM.downwards_pass (fmm.py:125-166). */

@@ -39,3 +39,35 @@ def direct(Kv, sx, sy, tau, tx, ty, check=False, chunk_pairs=20_000_000):
     if out is None:
         out = np.zeros((1, tx.size))
     return out
+
+
+def direct_stokes(sx, sy, fx, fy, tx, ty, check=False, chunk_pairs=20_000_000):
+    """(u, v, p) of the 2D Stokeslet summed over all sources, as the reference's
+    kernel_apply_single(_check) (flexmm/kifmm2d/vector/stokes_fmm.py:102-184):
+    u = (1/4 pi) sum [(-log r + dx^2/r^2) f_x + dx dy/r^2 f_y], v likewise,
+    p = (1/2 pi) sum (dx f_x + dy f_y)/r^2; check skips coincident points.
+
+    Pinned by: the Stokes equations (div u = 0, Laplacian u = grad p away from
+    the sources) by finite differences, and a tiny explicit-loop case
+    (tests/test_oracle_kernels.py)."""
+    sx, sy, fx, fy, tx, ty = [np.asarray(a, dtype=np.float64).ravel()
+                              for a in (sx, sy, fx, fy, tx, ty)]
+    out = np.zeros((3, tx.size))
+    step = max(1, chunk_pairs // max(1, sx.size))
+    for s in range(0, tx.size, step):
+        dx = tx[s:s + step, None] - sx[None, :]
+        dy = ty[s:s + step, None] - sy[None, :]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            d2 = dx * dx + dy * dy
+            id2 = 1.0 / d2
+            logid = 0.5 * np.log(id2)
+            uu = (logid + dx * dx * id2) * fx + dx * dy * id2 * fy
+            vv = dx * dy * id2 * fx + (logid + dy * dy * id2) * fy
+            pp = (dx * fx + dy * fy) * id2
+        if check:
+            keep = ~((dx == 0) & (dy == 0))
+            uu, vv, pp = [np.where(keep, a, 0.0) for a in (uu, vv, pp)]
+        out[0, s:s + step] = uu.sum(axis=1) * (0.25 / np.pi)
+        out[1, s:s + step] = vv.sum(axis=1) * (0.25 / np.pi)
+        out[2, s:s + step] = pp.sum(axis=1) * (0.5 / np.pi)
+    return out

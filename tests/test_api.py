@@ -43,3 +43,42 @@ def test_helper_bbox_matches_reference_rule(bbox):
         h.register_width(w)
     got = h.get_bbox(x.min(), x.max(), y.min(), y.max(), bbox)
     assert got == okifmm.get_bbox(x, y, bbox, widths=[1.0, 0.5, 0.25, 0.125])
+
+
+def test_modified_helmholtz_descriptors():
+    from scipy import special
+    ker, pot, grad = kernels.modified_helmholtz(20.0)
+    assert kernels.modified_helmholtz(20.0)[0] is ker
+    assert ker.params == (20.0,) and ker.kernel_id == 1
+    assert kernels.default_evaluator(ker) is pot
+    assert kernels.gradient_evaluator(ker) is grad
+    assert kernels.gradient_evaluator(kernels.laplace) is kernels.laplace_gradient
+    rng = np.random.default_rng(2)
+    a = rng.random((4, 40))
+    r = np.hypot(a[2] - a[0], a[3] - a[1])
+    np.testing.assert_allclose(ker(*a), special.k0(20 * r) / (2 * np.pi), rtol=1e-14)
+    u, ux, uy = grad(*a)
+    h = 1e-6
+    fd = (ker(a[0], a[1], a[2] + h, a[3]) - ker(a[0], a[1], a[2] - h, a[3])) / (2 * h)
+    np.testing.assert_allclose(ux, fd, rtol=1e-6, atol=1e-9)
+    with pytest.raises(ValueError):
+        kernels.modified_helmholtz(0.0)
+
+
+def test_fmm_evaluate_rejects_host_arrays():
+    import torch
+    from flexmm_amd.kifmm2d import fmm_evaluate
+    z = torch.zeros(4, dtype=torch.float64)
+    with pytest.raises(TypeError):
+        fmm_evaluate(z, z, z, z, z, kernels.laplace, 10, 16)
+
+
+def test_square_bbox_holds_both_sets():
+    import torch
+    from flexmm_amd.kifmm2d.adjoint import _square_bbox
+    f64 = dict(dtype=torch.float64)
+    x = torch.tensor([0.2, 0.4], **f64); y = torch.tensor([0.3, 0.35], **f64)
+    tx = torch.tensor([0.1, 0.9], **f64); ty = torch.tensor([0.5, 0.6], **f64)
+    b = _square_bbox(x, y, tx, ty)
+    assert b[0] <= 0.1 and b[1] >= 0.9 and b[2] <= 0.3 and b[3] >= 0.6
+    assert abs((b[1] - b[0]) - (b[3] - b[2])) < 1e-15

@@ -178,3 +178,20 @@ def test_get_bbox_rules():
     # warm helper: the width snaps to a power-of-two multiple of the smallest cached width
     b2 = okifmm.get_bbox(x, y, [0.1, 0.7, 0.2, 0.5], widths=[1.0, 0.5, 0.25, 0.125])
     assert b2[1] - b2[0] == 0.125 * 2 ** np.ceil(np.log2(0.6 / 0.125))
+
+
+def test_modified_helmholtz_vs_direct():
+    """The oracle with the K0(k r)/(2 pi) kernel, k = 20, sources on a circle."""
+    rng = np.random.default_rng(77)
+    th = rng.uniform(0, 2 * np.pi, 3000)
+    x, y = 0.5 + 0.4 * np.cos(th), 0.5 + 0.4 * np.sin(th)
+    tau = rng.standard_normal(3000)
+    tx, ty = rng.random(500), rng.random(500)
+    mh, mhg = okern.make_modified_helmholtz(20.0)
+    f = okifmm.FMM(x, y, mh, 40, 60, bbox=[0, 1, 0, 1])   # the targets fill the unit square
+    f.register_evaluator(mhg, "gradient", 3)
+    f.build_expansions(tau)
+    ref = odirect.direct(mhg, x, y, tau, tx, ty)
+    g = f.evaluate_to_points(tx, ty, "gradient", False)
+    for k in range(3):
+        assert _err(g[k], ref[k]) < 1e-11

@@ -63,3 +63,65 @@ def test_direct_sum_self_exclusion():
         ref = sum(-math.log((sx[i] - sx[j]) ** 2 + (sy[i] - sy[j]) ** 2) / (4 * math.pi) * tau[j]
                   for j in range(30) if j != i)
         assert abs(got[i] - ref) < 1e-13 * max(1.0, abs(ref))
+
+
+def test_stokeslet_solves_stokes_equations():
+    """div u = 0 and Laplacian(u) = grad p away from the sources (unit
+    viscosity), by central differences; a tiny case by explicit loops."""
+    from oracle import direct as odirect
+    rng = np.random.default_rng(11)
+    sx, sy = rng.random(20) * 0.2 + 0.4, rng.random(20) * 0.2 + 0.4
+    fx, fy = rng.standard_normal(20), rng.standard_normal(20)
+    P = np.array([[0.1, 0.15], [0.9, 0.2], [0.2, 0.85]])
+    h = 1e-3
+    def f(x, y):
+        return odirect.direct_stokes(sx, sy, fx, fy, np.array([x]), np.array([y]))[:, 0]
+    for x, y in P:
+        c = f(x, y)
+        xp, xm, yp, ym = f(x + h, y), f(x - h, y), f(x, y + h), f(x, y - h)
+        div = (xp[0] - xm[0]) / (2 * h) + (yp[1] - ym[1]) / (2 * h)
+        lap = (xp + xm + yp + ym - 4 * c) / h ** 2
+        gp = np.array([(xp[2] - xm[2]) / (2 * h), (yp[2] - ym[2]) / (2 * h)])
+        scale = np.abs(gp).max()
+        assert abs(div) < 1e-6 * scale
+        assert np.abs(lap[:2] - gp).max() < 1e-4 * scale
+    # explicit loop
+    u = np.zeros(3)
+    x, y = 0.3, 0.7
+    for j in range(3):
+        dx, dy = x - sx[j], y - sy[j]
+        r2 = dx * dx + dy * dy
+        u[0] += ((-0.5 * np.log(r2) + dx * dx / r2) * fx[j] + dx * dy / r2 * fy[j]) / (4 * np.pi)
+        u[1] += (dx * dy / r2 * fx[j] + (-0.5 * np.log(r2) + dy * dy / r2) * fy[j]) / (4 * np.pi)
+        u[2] += (dx * fx[j] + dy * fy[j]) / r2 / (2 * np.pi)
+    got = odirect.direct_stokes(sx[:3], sy[:3], fx[:3], fy[:3], np.array([x]), np.array([y]))[:, 0]
+    np.testing.assert_allclose(got, u, rtol=1e-13)
+
+
+def test_stokeslet_laplace_decomposition():
+    """The six-Laplace-density form of the Stokeslet that the device far field
+    uses reproduces the direct Stokeslet sum."""
+    from oracle import direct as odirect
+    rng = np.random.default_rng(12)
+    sx, sy = rng.random(400), rng.random(400)
+    fx, fy = rng.standard_normal(400), rng.standard_normal(400)
+    tx, ty = rng.random(60), rng.random(60)
+    c0 = 0.5
+    X, Y = sx - c0, sy - c0
+
+    def L(g):   # sum g log|t - s| and its gradient
+        dx = tx[:, None] - sx[None, :]
+        dy = ty[:, None] - sy[None, :]
+        d2 = dx * dx + dy * dy
+        return ((0.5 * np.log(d2) * g).sum(1), (dx / d2 * g).sum(1), (dy / d2 * g).sum(1))
+
+    Lx, Ly = L(fx), L(fy)
+    B = Lx[1] + Ly[2]
+    Cx = L(X * fx)[1] + L(X * fy)[2]
+    Cy = L(Y * fx)[1] + L(Y * fy)[2]
+    u = (-Lx[0] + (tx - c0) * B - Cx) / (4 * np.pi)
+    v = (-Ly[0] + (ty - c0) * B - Cy) / (4 * np.pi)
+    p = B / (2 * np.pi)
+    ref = odirect.direct_stokes(sx, sy, fx, fy, tx, ty)
+    for got, r in zip((u, v, p), ref):
+        assert np.abs(got - r).max() < 1e-13 * np.abs(r).max()
