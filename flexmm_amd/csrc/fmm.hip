@@ -424,87 +424,132 @@ struct ValidSlots {
 };
 __constant__ ValidSlots c_valid;
 
-// 512 threads: wave w handles columns 16 (w & 3) .. +16 and row half w >> 2.
-template <int PD>
-__global__ __launch_bounds__(512) void k_m2l_all(LevelTable lt, const int* __restrict__ flags,
-                                                const int* __restrict__ m2l,
-                                                const double* __restrict__ mult,
-                                                const double* __restrict__ ops, int64_t opsz,
-                                                double* __restrict__ ple, int64_t rstride,
-                                                int64_t nstride) {
-  // density blockIdx.y: multipole rows rstride rows further, local rows nstride nodes further
-  mult += (int64_t)blockIdx.y * rstride * PD;
-  ple += (int64_t)blockIdx.y * nstride * PD;
-  constexpr int NT = 512, KS = Mfma<PD>::KS, HRB = Mfma<PD>::RB / 2;
-  __shared__ __attribute__((aligned(16))) double buf[2][PD * Mfma<PD>::LDA];
-  __shared__ int srcs[27][64];   // interaction-list entries of the block's 64 columns
-  __shared__ unsigned long long smask;
+// M2L block: 256 threads, 64 target boxes of one level and one child position
+// c (4 waves x 16 columns, every wave all PD rows), phases = the slots that
+// any of the 64 boxes uses.  Per phase the operator block is staged in LDS
+// (double buffered) and each lane gathers its box's source multipole with
+// 16-byte loads; the k index of an MFMA step is permuted so that one 16-byte
+// load feeds two steps: step 2m + h of lane group kq multiplies k = 8m + 2kq
+// + h (A is read with the same permutation, which leaves the product
+// unchanged).  Loads of phase i + 1 are issued before the barrier of phase i.
+constexpr int kM2LThreads = 256;
+constexpr int kM2LCols = 64;
+
+struct M2LSetup {
+  int li, c, col, kq, g, nph;
+  bool active;
+};
+
+// Block prologue shared by the FP64 and FP32 M2L: decode the block, collect
+// the 64 boxes' interaction lists in LDS and the list of used slots.
+__device__ __forceinline__ M2LSetup m2l_setup(const LevelTable& lt, const int* __restrict__ flags,
+                                              const int* __restrict__ m2l,
+                                              int (&srcs)[27][kM2LCols], int (&pslot)[28],
+                                              unsigned long long& smask) {
+  M2LSetup S;
   int li = 0;
   while (li + 1 < lt.nlev && (int)blockIdx.x >= lt.boff[li + 1]) ++li;
   const int lb = blockIdx.x - lt.boff[li];
-  const int c = lb & 3, tile = lb >> 2;
+  S.li = li;
+  S.c = lb & 3;
+  const int tile = lb >> 2;
   const int ngroups = lt.ngroups[li];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int half = w >> 2;
-  const int col = (w & 3) * 16 + (l & 15);
-  const int q = tile * 64 + col;
-  const int g = lt.g0[li] + 4 * q + c;
-  const bool active = q < ngroups && !(flags[q < ngroups ? g : lt.g0[li]] & NF_FAKE);
-  const int* lst = m2l + (int64_t)g * kNumM2L;
-  const int nv = c_valid.n[c];
-  const int kq = l >> 4;
+  S.col = w * 16 + (l & 15);
+  S.kq = l >> 4;
+  const int q = tile * kM2LCols + S.col;
+  S.g = lt.g0[li] + 4 * q + S.c;
+  S.active = q < ngroups && !(flags[q < ngroups ? S.g : lt.g0[li]] & NF_FAKE);
+  const int* lst = m2l + (int64_t)S.g * kNumM2L;
+  const int nv = c_valid.n[S.c];
   if (threadIdx.x == 0) smask = 0ull;
   __syncthreads();
-  if (half == 0) {
-    unsigned long long wm = 0ull;
-    for (int i = 0; i < nv; ++i) {
-      const int src = active ? lst[c_valid.slot[c][i]] : -1;
-      if (kq == 0) srcs[i][col] = src;
-      if (__ballot(src >= 0)) wm |= 1ull << i;
+  unsigned long long wm = 0ull;
+  for (int i = 0; i < nv; ++i) {
+    const int src = S.active ? lst[c_valid.slot[S.c][i]] : -1;
+    if (S.kq == 0) srcs[i][S.col] = src;
+    if (__ballot(src >= 0)) wm |= 1ull << i;
+  }
+  if (l == 0 && wm) atomicOr(&smask, wm);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = smask;
+    int k = 0;
+    while (m) {
+      pslot[k++] = __builtin_ctzll(m);
+      m &= m - 1ull;
     }
-    if (l == 0 && wm) atomicOr(&smask, wm);
+    pslot[27] = k;
   }
   __syncthreads();
-  unsigned long long rem = smask;
-  const int nph = __popcll(rem);
-  const double* opsL = ops + (int64_t)lt.d[li] * opsz;
-  d4 acc[HRB];
+  S.nph = pslot[27];
+  return S;
+}
+
+template <int PD>
+__global__ __launch_bounds__(kM2LThreads) void k_m2l_all(LevelTable lt,
+                                                        const int* __restrict__ flags,
+                                                        const int* __restrict__ m2l,
+                                                        const double* __restrict__ mult,
+                                                        const double* __restrict__ ops,
+                                                        int64_t opsz, double* __restrict__ ple,
+                                                        int64_t rstride, int64_t nstride) {
+  // density blockIdx.y: multipole rows rstride rows further, local rows nstride nodes further
+  mult += (int64_t)blockIdx.y * rstride * PD;
+  ple += (int64_t)blockIdx.y * nstride * PD;
+  constexpr int KS = Mfma<PD>::KS, RB = Mfma<PD>::RB, LDA = Mfma<PD>::LDA;
+  __shared__ __attribute__((aligned(16))) double buf[2][PD * LDA];
+  __shared__ int srcs[27][kM2LCols];
+  __shared__ int pslot[28];
+  __shared__ unsigned long long smask;
+  const M2LSetup S = m2l_setup(lt, flags, m2l, srcs, pslot, smask);
+  const int l = threadIdx.x & 63;
+  const double* opsL = ops + (int64_t)lt.d[S.li] * opsz;
+  d4 acc[RB];
 #pragma unroll
-  for (int rb = 0; rb < HRB; ++rb) acc[rb] = d4{0.0, 0.0, 0.0, 0.0};
-  if (nph > 0) {
-    AStage<PD, NT> st;
-    double bn[KS];
-    auto fetch = [&](bool& any) {
-      const int i = __builtin_ctzll(rem);
-      rem &= rem - 1ull;
-      const int s = c_valid.slot[c][i];
-      st.load(opsL + (int64_t)(OP_M2L + s) * PD * PD);
-      const int src = srcs[i][col];
-      const double* brow = src >= 0 ? mult + (int64_t)src * PD : nullptr;
-      any = __ballot(brow != nullptr) != 0;
+  for (int rb = 0; rb < RB; ++rb) acc[rb] = d4{0.0, 0.0, 0.0, 0.0};
+  if (S.nph > 0) {
+    AStage<PD, kM2LThreads> st;
+    double bn[KS], bc[KS];
+    auto fetch = [&](int k) {
+      const int i = pslot[k];
+      st.load(opsL + (int64_t)(OP_M2L + c_valid.slot[S.c][i]) * PD * PD);
+      const int src = srcs[i][S.col];
+      const double* brow = src >= 0 ? mult + (int64_t)src * PD + 2 * S.kq : nullptr;
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) bn[ks] = brow ? brow[4 * ks + kq] : 0.0;
+      for (int m = 0; m < KS / 2; ++m) {
+        const dv2 v = brow ? *reinterpret_cast<const dv2*>(brow + 8 * m) : dv2{0.0, 0.0};
+        bn[2 * m] = v.x;
+        bn[2 * m + 1] = v.y;
+      }
     };
-    bool any_n;
-    fetch(any_n);
-    for (int i = 0; i < nph; ++i) {
+    fetch(0);
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) bc[ks] = bn[ks];
+    for (int i = 0; i < S.nph; ++i) {
       double* A = (i & 1) ? buf[1] : buf[0];
       st.store(A);
-      const bool any = any_n;
-      double bc[KS];
+      if (i + 1 < S.nph) fetch(i + 1);
+      __syncthreads();
+      const double* a0 = A + (l & 15) * LDA + 2 * S.kq;
+#pragma unroll
+      for (int m = 0; m < KS / 2; ++m)
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+          const dv2 a = *reinterpret_cast<const dv2*>(a0 + rb * 16 * LDA + 8 * m);
+          acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bc[2 * m], acc[rb], 0, 0, 0);
+          acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bc[2 * m + 1], acc[rb], 0, 0, 0);
+        }
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) bc[ks] = bn[ks];
-      __syncthreads();
-      if (i + 1 < nph) fetch(any_n);
-      if (any) mfma_block<PD, HRB>(acc, A, bc, half * HRB);
     }
   }
-  if (active) {
-    double* rowp = ple + (int64_t)g * PD;
+  if (S.active) {
+    double* rowp = ple + (int64_t)S.g * PD;
 #pragma unroll
-    for (int rb = 0; rb < HRB; ++rb)
+    for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) rowp[16 * (half * HRB + rb) + 4 * v + kq] = acc[rb][v];
+      for (int v = 0; v < 4; ++v) rowp[16 * rb + 4 * v + S.kq] = acc[rb][v];
   }
 }
 
@@ -543,121 +588,93 @@ struct AStageF {
 };
 
 template <int PD>
-__global__ __launch_bounds__(512) void k_m2l_all_f32(LevelTable lt, const int* __restrict__ flags,
-                                                    const int* __restrict__ m2l,
-                                                    const double* __restrict__ mult,
-                                                    const double* __restrict__ qs,
-                                                    const double* __restrict__ ops, int64_t opsz,
-                                                    double* __restrict__ ple, int64_t rstride,
-                                                    int64_t nstride) {
+__global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
+    LevelTable lt, const int* __restrict__ flags, const int* __restrict__ m2l,
+    const double* __restrict__ mult, const double* __restrict__ qs,
+    const double* __restrict__ ops, int64_t opsz, double* __restrict__ ple, int64_t rstride,
+    int64_t nstride) {
   mult += (int64_t)blockIdx.y * rstride * PD;
   qs += (int64_t)blockIdx.y * rstride;
   ple += (int64_t)blockIdx.y * nstride * PD;
-  constexpr int NT = 512, KS = PD / 4, HRB = PD / 32, LDAF = PD + 4;
+  constexpr int KS = PD / 4, RB = PD / 16, LDAF = PD + 4;
   __shared__ __attribute__((aligned(16))) float buf[2][PD * LDAF];
-  __shared__ int srcs[27][64];
+  __shared__ __attribute__((aligned(16))) double cmean[27][PD];   // row means of the used slots
+  __shared__ int srcs[27][kM2LCols];
+  __shared__ int pslot[28];
   __shared__ unsigned long long smask;
-  int li = 0;
-  while (li + 1 < lt.nlev && (int)blockIdx.x >= lt.boff[li + 1]) ++li;
-  const int lb = blockIdx.x - lt.boff[li];
-  const int c = lb & 3, tile = lb >> 2;
-  const int ngroups = lt.ngroups[li];
-  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int half = w >> 2;
-  const int col = (w & 3) * 16 + (l & 15);
-  const int q = tile * 64 + col;
-  const int g = lt.g0[li] + 4 * q + c;
-  const bool active = q < ngroups && !(flags[q < ngroups ? g : lt.g0[li]] & NF_FAKE);
-  const int* lst = m2l + (int64_t)g * kNumM2L;
-  const int nv = c_valid.n[c];
-  const int kq = l >> 4;
-  if (threadIdx.x == 0) smask = 0ull;
-  __syncthreads();
-  if (half == 0) {
-    unsigned long long wm = 0ull;
-    for (int i = 0; i < nv; ++i) {
-      const int src = active ? lst[c_valid.slot[c][i]] : -1;
-      if (kq == 0) srcs[i][col] = src;
-      if (__ballot(src >= 0)) wm |= 1ull << i;
-    }
-    if (l == 0 && wm) atomicOr(&smask, wm);
-  }
-  __syncthreads();
-  unsigned long long rem = smask;
-  const int nph = __popcll(rem);
-  const double* opsL = ops + (int64_t)lt.d[li] * opsz;
+  const M2LSetup S = m2l_setup(lt, flags, m2l, srcs, pslot, smask);
+  const int l = threadIdx.x & 63;
+  const double* opsL = ops + (int64_t)lt.d[S.li] * opsz;
   const double* means = opsL + ops_off_means(PD);
   const float* r32 = reinterpret_cast<const float*>(opsL + ops_off_r32(PD));
-  double acc64[HRB][4];
+  for (int e = threadIdx.x; e < S.nph * PD; e += kM2LThreads) {
+    const int k = e / PD, r = e - k * PD;
+    cmean[k][r] = means[c_valid.slot[S.c][pslot[k]] * PD + r];
+  }
+  double acc64[RB][4];
 #pragma unroll
-  for (int rb = 0; rb < HRB; ++rb)
+  for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
     for (int v = 0; v < 4; ++v) acc64[rb][v] = 0.0;
-  const int rb0 = half * HRB;
-  if (nph > 0) {
-    AStageF<PD, NT> st;
-    float bn[KS];
-    double cn[HRB][4];
-    double qn = 0.0;
-    auto fetch = [&](bool& any) {
-      const int i = __builtin_ctzll(rem);
-      rem &= rem - 1ull;
-      const int s = c_valid.slot[c][i];
-      st.load(r32 + (int64_t)s * PD * PD);
-      const int src = srcs[i][col];
-      const double* brow = src >= 0 ? mult + (int64_t)src * PD : nullptr;
-      any = __ballot(brow != nullptr) != 0;
+  if (S.nph > 0) {
+    AStageF<PD, kM2LThreads> st;
+    float bn[KS], bc[KS];
+    double qn = 0.0, qc;
+    auto fetch = [&](int k) {
+      const int i = pslot[k];
+      st.load(r32 + (int64_t)c_valid.slot[S.c][i] * PD * PD);
+      const int src = srcs[i][S.col];
+      const double* brow = src >= 0 ? mult + (int64_t)src * PD + 2 * S.kq : nullptr;
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) bn[ks] = brow ? (float)brow[4 * ks + kq] : 0.f;
+      for (int m = 0; m < KS / 2; ++m) {
+        const dv2 v = brow ? *reinterpret_cast<const dv2*>(brow + 8 * m) : dv2{0.0, 0.0};
+        bn[2 * m] = (float)v.x;
+        bn[2 * m + 1] = (float)v.y;
+      }
       qn = src >= 0 ? qs[src] : 0.0;
-#pragma unroll
-      for (int rb = 0; rb < HRB; ++rb)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) cn[rb][v] = means[s * PD + 16 * (rb0 + rb) + 4 * kq + v];
     };
-    bool any_n;
-    fetch(any_n);
-    for (int i = 0; i < nph; ++i) {
+    fetch(0);
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) bc[ks] = bn[ks];
+    qc = qn;
+    for (int i = 0; i < S.nph; ++i) {
       float* A = (i & 1) ? buf[1] : buf[0];
       st.store(A);
-      const bool any = any_n;
-      float bc[KS];
+      if (i + 1 < S.nph) fetch(i + 1);
+      __syncthreads();   // also orders the cmean fill before its first use
+      // one FP32 accumulation per slot (p terms), slots summed in FP64: a
+      // single FP32 chain over all 27 slots loses about 4x more
+      const float* a0 = A + (l & 15) * LDAF + 2 * S.kq;
+      f4 acc[RB];
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) acc[rb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int m = 0; m < KS / 2; ++m)
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+          typedef float fv2 __attribute__((ext_vector_type(2)));
+          const fv2 a = *reinterpret_cast<const fv2*>(a0 + rb * 16 * LDAF + 8 * m);
+          acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, bc[2 * m], acc[rb], 0, 0, 0);
+          acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, bc[2 * m + 1], acc[rb], 0, 0, 0);
+        }
+      // constant mode c_s * (sum of the source's equivalent density), FP64
+      const double* cm = cmean[i] + 4 * S.kq;
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          acc64[rb][v] += __builtin_fma(cm[16 * rb + v], qc, (double)acc[rb][v]);
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) bc[ks] = bn[ks];
-      const double qc = qn;
-#pragma unroll
-      for (int rb = 0; rb < HRB; ++rb)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc64[rb][v] = __builtin_fma(cn[rb][v], qc, acc64[rb][v]);
-      __syncthreads();
-      if (i + 1 < nph) fetch(any_n);
-      if (any) {
-        // one FP32 accumulation per slot (p terms), slots summed in FP64:
-        // a single FP32 chain over all 27 slots loses about 4x more
-        const float* a0 = A + (rb0 * 16 + (l & 15)) * LDAF + kq;
-        f4 acc[HRB];
-#pragma unroll
-        for (int rb = 0; rb < HRB; ++rb) acc[rb] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-          for (int rb = 0; rb < HRB; ++rb)
-            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rb * 16 * LDAF + ks * 4], bc[ks],
-                                                           acc[rb], 0, 0, 0);
-#pragma unroll
-        for (int rb = 0; rb < HRB; ++rb)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) acc64[rb][v] += (double)acc[rb][v];
-      }
+      qc = qn;
     }
   }
-  if (active) {
-    double* rowp = ple + (int64_t)g * PD;
+  if (S.active) {
+    double* rowp = ple + (int64_t)S.g * PD;
 #pragma unroll
-    for (int rb = 0; rb < HRB; ++rb)
+    for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
-        rowp[16 * (rb0 + rb) + 4 * kq + v] = acc64[rb][v];
+      for (int v = 0; v < 4; ++v) rowp[16 * rb + 4 * S.kq + v] = acc64[rb][v];
   }
 }
 
@@ -696,6 +713,141 @@ __global__ __launch_bounds__(256) void k_local_level(int g0, int ngroups, int do
   run_phases<PD>(nl + 2, acc, buf[0], buf[1], next);
   if (active) store_acc<PD>(acc, le + (int64_t)g * PD);
 }
+
+// ---------------------------------------------------------------------------
+// Row-split variant of the two level kernels for levels with few boxes: a
+// block holds 16 boxes and PD/16 waves, wave w computes rows 16w..16w+15, so
+// one MFMA chain per phase is PD/16 times shorter than in the kernels above
+// (which give a wave all PD rows of its 16 boxes).  A chained phase reads the
+// previous result through a double-buffered LDS tile.
+// `phase(i)` returns the i-th phase (operator, per-lane source column or
+// null, chained?) of n.
+template <int PD, class PhaseFn>
+__device__ __forceinline__ void run_small(int n, PhaseFn phase, d4& acc,
+                                          double* __restrict__ abuf0, double* __restrict__ abuf1,
+                                          double (&bbuf)[2][PD][17]) {
+  constexpr int KS = PD / 4, LDA = Mfma<PD>::LDA, NT = 64 * (PD / 16);
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int col = l & 15, kq = l >> 4;
+  AStage<PD, NT> st;
+  st.load(phase(0).A);
+  for (int i = 0; i < n; ++i) {
+    const Phase ph = phase(i);
+    double* A = (i & 1) ? abuf1 : abuf0;
+    st.store(A);
+    if (i + 1 < n) st.load(phase(i + 1).A);
+    double b[KS];
+    if (ph.chain) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) bbuf[i & 1][16 * w + 4 * v + kq][col] = acc[v];
+      acc = d4{0.0, 0.0, 0.0, 0.0};
+    }
+    __syncthreads();
+    if (ph.chain) {
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) b[ks] = bbuf[i & 1][4 * ks + kq][col];
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) b[ks] = ph.brow ? ph.brow[4 * ks + kq] : 0.0;
+    }
+    const double* a0 = A + (16 * w + col) * LDA + kq;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[4 * ks], b[ks], acc, 0, 0, 0);
+  }
+}
+
+// Upward level (as k_mult_level): rows row0 .. row0 + nrows, 16 per block.
+template <int PD>
+__global__ __launch_bounds__(64 * (PD / 16)) void k_mult_level_small(
+    int64_t row0, int nrows, int with_m2m, const int* __restrict__ rownode,
+    const int* __restrict__ flags, const int* __restrict__ child, const int* __restrict__ row,
+    const double* __restrict__ check, const double* __restrict__ opsL, double* __restrict__ mult,
+    double* __restrict__ qs, int64_t dstride) {
+  __shared__ __attribute__((aligned(16))) double abuf[2][PD * Mfma<PD>::LDA];
+  __shared__ double bbuf[2][PD][17];
+  __shared__ double qpart[PD / 16][16];
+  check += (int64_t)blockIdx.y * dstride * PD;
+  mult += (int64_t)blockIdx.y * dstride * PD;
+  qs += (int64_t)blockIdx.y * dstride;
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int col = l & 15, kq = l >> 4;
+  const int rl = blockIdx.x * 16 + col;
+  const bool active = rl < nrows;
+  const int64_t r = row0 + rl;
+  const int g = active ? rownode[r] : 0;
+  const int f = active ? flags[g] : 0;
+  const int crow = (active && (f & NF_CHILDREN)) ? row[child[g]] : -1;
+  const double* src = (active && (f & NF_UP)) ? check + r * PD : nullptr;
+  d4 acc;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) acc[v] = src ? src[16 * w + 4 * v + kq] : 0.0;
+  const int nm = with_m2m ? 4 : 0;
+  auto phase = [&](int k) -> Phase {
+    if (k < nm)
+      return Phase{opsL + (int64_t)(OP_M2M + k) * PD * PD,
+                   crow >= 0 ? mult + (int64_t)(crow + k) * PD : nullptr, false};
+    return Phase{opsL + (int64_t)(OP_PINV1 + (k - nm)) * PD * PD, nullptr, true};
+  };
+  run_small<PD>(nm + 2, phase, acc, abuf[0], abuf[1], bbuf);
+  double part = acc[0] + acc[1] + acc[2] + acc[3];
+  part += __shfl_xor(part, 16, 64);
+  part += __shfl_xor(part, 32, 64);
+  if (kq == 0) qpart[w][col] = part;
+  __syncthreads();
+  if (active) {
+    double* dst = mult + r * PD;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) dst[16 * w + 4 * v + kq] = acc[v];
+    if (threadIdx.x < 16) {
+      double q = 0.0;
+#pragma unroll
+      for (int ww = 0; ww < PD / 16; ++ww) q += qpart[ww][col];
+      qs[r] = q;
+    }
+  }
+}
+
+// Downward level (as k_local_level): 16 boxes of child position c per block.
+template <int PD>
+__global__ __launch_bounds__(64 * (PD / 16)) void k_local_level_small(
+    int g0, int ngroups, int do_l2l, const int* __restrict__ flags,
+    const int* __restrict__ parent, const double* __restrict__ ple,
+    const double* __restrict__ opsL, const double* __restrict__ opsParent,
+    double* __restrict__ le, int64_t nstride) {
+  __shared__ __attribute__((aligned(16))) double abuf[2][PD * Mfma<PD>::LDA];
+  __shared__ double bbuf[2][PD][17];
+  ple += (int64_t)blockIdx.y * nstride * PD;
+  le += (int64_t)blockIdx.y * nstride * PD;
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int col = l & 15, kq = l >> 4;
+  const int c = blockIdx.x & 3;
+  const int q = (blockIdx.x >> 2) * 16 + col;
+  const int g = g0 + 4 * q + c;
+  const bool active = q < ngroups && !(flags[q < ngroups ? g : g0] & NF_FAKE);
+  d4 acc;
+  const double* src = active ? ple + (int64_t)g * PD : nullptr;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) acc[v] = src ? src[16 * w + 4 * v + kq] : 0.0;
+  const int pg = (active && do_l2l) ? parent[g] : -1;
+  const int nl = do_l2l ? 1 : 0;
+  auto phase = [&](int k) -> Phase {
+    if (k < nl)
+      return Phase{opsParent + (int64_t)(OP_L2L + c) * PD * PD,
+                   pg >= 0 ? le + (int64_t)pg * PD : nullptr, false};
+    return Phase{opsL + (int64_t)(OP_PINV1 + (k - nl)) * PD * PD, nullptr, true};
+  };
+  run_small<PD>(nl + 2, phase, acc, abuf[0], abuf[1], bbuf);
+  if (active) {
+    double* dst = le + (int64_t)g * PD;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) dst[16 * w + 4 * v + kq] = acc[v];
+  }
+}
+
+// Levels with at most this many multipole rows (or 4 x this many boxes) use
+// the row-split kernels.
+constexpr int kSmallLevelRows = 4096;
 
 // ---------------------------------------------------------------------------
 // Targets: locate, count per evaluation box, scatter into box order.
@@ -1335,6 +1487,15 @@ static void upward(Plan& P, const double* tau, hipStream_t s) {
     const int nrows = (int)(t.row_off[d + 1] - r0);
     if (nrows == 0) continue;
     const double* opsL = P.ops.as<double>() + d * opsz;
+    if (nrows <= kSmallLevelRows) {
+      auto ks = PD == 32 ? k_mult_level_small<32> : k_mult_level_small<64>;
+      hipLaunchKernelGGL(ks, dim3(ceil_div(nrows, 16), nd), dim3(64 * (PD / 16)), 0, s, r0,
+                         nrows, d < t.L - 1 ? 1 : 0, t.rownode.as<int>(), t.flags.as<int>(),
+                         t.child.as<int>(), t.row.as<int>(), P.check.as<double>(), opsL,
+                         P.mult.as<double>(), P.qs.as<double>(), rstride);
+      FMM_LAUNCH_CHECK();
+      continue;
+    }
     auto kern = PD == 32 ? k_mult_level<32> : k_mult_level<64>;
     hipLaunchKernelGGL(kern, dim3(ceil_div(nrows, 64), nd), dim3(256), 0, s, r0, nrows,
                        d < t.L - 1 ? 1 : 0, t.rownode.as<int>(), t.flags.as<int>(),
@@ -1367,19 +1528,19 @@ static void downward(Plan& P, hipStream_t s) {
     lt.g0[lt.nlev] = (int)t.lev[d].node_off;
     lt.ngroups[lt.nlev] = ng;
     lt.boff[lt.nlev] = nb;
-    nb += 4 * (int)ceil_div(ng, 64);
+    nb += 4 * (int)ceil_div(ng, kM2LCols);
     ++lt.nlev;
   }
   lt.boff[lt.nlev] = nb;
   if (P.precision != PREC_FP64) {
     auto kern = PD == 32 ? k_m2l_all_f32<32> : k_m2l_all_f32<64>;
-    hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(512), 0, s, lt, t.flags.as<int>(),
+    hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(kM2LThreads), 0, s, lt, t.flags.as<int>(),
                        t.m2l.as<int>(), P.mult.as<double>(), P.qs.as<double>(),
                        P.ops.as<double>(), opsz, P.ple.as<double>(), rstride, nstride);
     FMM_LAUNCH_CHECK();
   } else {
     auto kern = PD == 32 ? k_m2l_all<32> : k_m2l_all<64>;
-    hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(512), 0, s, lt, t.flags.as<int>(),
+    hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(kM2LThreads), 0, s, lt, t.flags.as<int>(),
                        t.m2l.as<int>(), P.mult.as<double>(), P.ops.as<double>(), opsz,
                        P.ple.as<double>(), rstride, nstride);
     FMM_LAUNCH_CHECK();
@@ -1388,6 +1549,15 @@ static void downward(Plan& P, hipStream_t s) {
     const int ngroups = (int)(t.lev[d].n_nodes / 4);
     if (ngroups == 0) continue;
     const double* opsL = P.ops.as<double>() + d * opsz;
+    if (4 * (int64_t)ngroups <= kSmallLevelRows) {
+      auto ks = PD == 32 ? k_local_level_small<32> : k_local_level_small<64>;
+      hipLaunchKernelGGL(ks, dim3(4 * ceil_div(ngroups, 16), nd), dim3(64 * (PD / 16)), 0, s,
+                         (int)t.lev[d].node_off, ngroups, d >= 3 ? 1 : 0, t.flags.as<int>(),
+                         t.parent.as<int>(), P.ple.as<double>(), opsL, opsL - opsz,
+                         P.le.as<double>(), nstride);
+      FMM_LAUNCH_CHECK();
+      continue;
+    }
     auto kern = PD == 32 ? k_local_level<32> : k_local_level<64>;
     hipLaunchKernelGGL(kern, dim3(4 * ceil_div(ngroups, 64), nd), dim3(256), 0, s,
                        (int)t.lev[d].node_off, ngroups, d >= 3 ? 1 : 0, t.flags.as<int>(),

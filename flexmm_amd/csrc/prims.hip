@@ -248,6 +248,10 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
   return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
+// Segments of at most 64 pairs: one wave per segment, each lane computes the
+// rank of its (key, value) among the segment's pairs by broadcasting every
+// pair in turn (equal pairs are ranked by lane, so the ranks are a
+// permutation), then stores its pair at that rank.
 __global__ __launch_bounds__(256) void k_wave_sort(uint64_t* __restrict__ keys,
                                                   uint32_t* __restrict__ vals,
                                                   const int2* __restrict__ fin, int nfin) {
@@ -255,31 +259,25 @@ __global__ __launch_bounds__(256) void k_wave_sort(uint64_t* __restrict__ keys,
   if (si >= nfin) return;   // wave-uniform
   const int lane = threadIdx.x & 63;
   const int2 sg = fin[si];
+  const int cnt = __builtin_amdgcn_readfirstlane(sg.y);
   uint64_t k = ~0ull;
   uint32_t v = 0xffffffffu;
-  if (lane < sg.y) {
+  if (lane < cnt) {
     k = keys[sg.x + lane];
     v = vals[sg.x + lane];
   }
-  for (int kk = 2; kk <= 64; kk <<= 1) {
-    for (int j = kk >> 1; j > 0; j >>= 1) {
-      const int partner = lane ^ j;
-      const uint64_t pk = shfl_u64(k, partner);
-      const uint32_t pv = (uint32_t)__shfl((int)v, partner, 64);
-      const bool up = (lane & kk) == 0;
-      const bool lower = (lane & j) == 0;
-      const bool gt = k > pk || (k == pk && v > pv);
-      const bool lt = k < pk || (k == pk && v < pv);
-      const bool take = (up == lower) ? gt : lt;
-      if (take) {
-        k = pk;
-        v = pv;
-      }
-    }
+  const uint32_t klo = (uint32_t)k, khi = (uint32_t)(k >> 32);
+  int rank = 0;
+  for (int j = 0; j < cnt; ++j) {
+    const uint32_t jlo = (uint32_t)__builtin_amdgcn_readlane((int)klo, j);
+    const uint32_t jhi = (uint32_t)__builtin_amdgcn_readlane((int)khi, j);
+    const uint32_t jv = (uint32_t)__builtin_amdgcn_readlane((int)v, j);
+    const uint64_t kj = ((uint64_t)jhi << 32) | jlo;
+    rank += (kj < k || (kj == k && (jv < v || (jv == v && j < lane)))) ? 1 : 0;
   }
-  if (lane < sg.y) {
-    keys[sg.x + lane] = k;
-    vals[sg.x + lane] = v;
+  if (lane < cnt) {
+    keys[sg.x + rank] = k;
+    vals[sg.x + rank] = v;
   }
 }
 

@@ -130,23 +130,37 @@ __global__ __launch_bounds__(BS) void k_gather_sorted(const double* __restrict__
 // it, i.e. depth <= c(i) (common depth with key i+ncut).
 __global__ __launch_bounds__(BS) void k_split_pos(const uint64_t* __restrict__ keys, int n,
                                                   int ncut, int* __restrict__ flag,
-                                                  int2* __restrict__ sc, int* __restrict__ dcount) {
+                                                  int2* __restrict__ sc) {
+  int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t k = keys[i];
+  const int s = (i == 0) ? -1 : min(clz64(k ^ keys[i - 1]) >> 1, kMaxDepth);
+  const int c = ((int64_t)i + ncut < n) ? min(clz64(k ^ keys[i + ncut]) >> 1, kMaxDepth - 1) : -1;
+  flag[i] = s < c;
+  sc[i] = make_int2(s, c);
+}
+
+// Number of split cells per depth: entry j of the compacted split positions
+// starts the split cells of depths s_j+1 .. c_j.  One block; wave ballots
+// count each depth (no contended atomics).
+__global__ __launch_bounds__(1024) void k_depth_count(const int2* __restrict__ psc,
+                                                      const int* __restrict__ np_dev,
+                                                      int* __restrict__ dcount) {
   __shared__ int lc[kMaxLevels];
   if (threadIdx.x < kMaxLevels) lc[threadIdx.x] = 0;
   __syncthreads();
-  int i = blockIdx.x * BS + threadIdx.x;
-  if (i < n) {
-    uint64_t k = keys[i];
-    int s = (i == 0) ? -1 : min(clz64(k ^ keys[i - 1]) >> 1, kMaxDepth);
-    int c = ((int64_t)i + ncut < n) ? min(clz64(k ^ keys[i + ncut]) >> 1, kMaxDepth - 1) : -1;
-    int f = s < c;
-    flag[i] = f;
-    sc[i] = make_int2(s, c);
-    if (f)
-      for (int d = s + 1; d <= c; ++d) atomicAdd(&lc[d], 1);
+  const int np = *np_dev;
+  const int lane = threadIdx.x & 63;
+  for (int base = threadIdx.x - lane; base < np; base += 1024) {
+    const int j = base + lane;
+    const int2 v = j < np ? psc[j] : make_int2(0, 0);
+    for (int d = 0; d < kMaxLevels; ++d) {
+      const unsigned long long m = __ballot(v.x < d && d <= v.y);
+      if (lane == 0 && m) atomicAdd(&lc[d], (int)__popcll(m));
+    }
   }
   __syncthreads();
-  if (threadIdx.x < kMaxLevels && lc[threadIdx.x]) atomicAdd(&dcount[threadIdx.x], lc[threadIdx.x]);
+  if (threadIdx.x < kMaxLevels) dcount[threadIdx.x] = lc[threadIdx.x];
 }
 
 __global__ __launch_bounds__(BS) void k_compact_pos(const int* __restrict__ flag,
@@ -327,6 +341,48 @@ __global__ __launch_bounds__(BS) void k_p_scatter(const uint64_t* __restrict__ r
 }
 
 // --- final node arrays ---------------------------------------------------------
+// Node index of the level-d cell with Morton code c, or -1: direct on a full
+// level, else a binary search in the level's sorted codes.
+__device__ __forceinline__ int node_of(const uint64_t* __restrict__ code,
+                                       const int* __restrict__ levoff, int d, uint64_t c) {
+  const int64_t nl = levoff[d + 1] - levoff[d];
+  if (d < 16 && nl == ((int64_t)1 << (2 * d))) return levoff[d] + (int)c;
+  const int64_t pos = bsearch_u64(code + levoff[d], nl, c);
+  return pos >= 0 ? levoff[d] + (int)pos : -1;
+}
+
+// Source ranges of all nodes from the sorted keys: point i starts its cells
+// of depths s(i)+1.. and ends those of depths s(i+1)+1.. (s = common depth
+// with the previous key).  Cells without points keep -1 (fixed below).
+__global__ __launch_bounds__(BS) void k_cell_ranges(const uint64_t* __restrict__ keys,
+                                                    const int2* __restrict__ sc, int n,
+                                                    const uint64_t* __restrict__ code,
+                                                    const int* __restrict__ levoff, int L,
+                                                    int* __restrict__ bot, int* __restrict__ top) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  const int s0 = sc[i].x;
+  const int s1 = i + 1 < n ? sc[i + 1].x : -1;
+  if (s0 >= L - 1 && s1 >= L - 1) return;
+  const uint64_t k = keys[i];
+  for (int d = max(1, s0 + 1); d <= L - 1; ++d) {
+    const int g = node_of(code, levoff, d, k >> (64 - 2 * d));
+    if (g >= 0) bot[g] = i;
+  }
+  for (int d = max(1, s1 + 1); d <= L - 1; ++d) {
+    const int g = node_of(code, levoff, d, k >> (64 - 2 * d));
+    if (g >= 0) top[g] = i + 1;
+  }
+}
+
+__global__ __launch_bounds__(BS) void k_cell_ranges_init(int nn, int npts, int* __restrict__ bot,
+                                                         int* __restrict__ top) {
+  const int g = blockIdx.x * BS + threadIdx.x;
+  if (g >= nn) return;
+  bot[g] = g == 0 ? 0 : -1;
+  top[g] = g == 0 ? npts : -1;
+}
+
 __global__ __launch_bounds__(BS) void k_nodes_a(const uint64_t* __restrict__ p_code,
                                                 const int* __restrict__ p_kind,
                                                 const int64_t* __restrict__ poff,
@@ -371,13 +427,12 @@ __global__ __launch_bounds__(64) void k_nodes_b(
     }
   }
   const int leaf = !fake && (ch < 0 || xl);
-  // point range of the cell
-  int b = 0, t = npts;
-  if (d > 0) {
-    const int sh = 64 - 2 * d;
-    b = (int)lower_bound_u64(keys, npts, c << sh);
-    const uint64_t last = ((uint64_t)1 << (2 * d)) - 1;
-    t = (c == last) ? npts : (int)lower_bound_u64(keys, npts, (c + 1) << sh);
+  // point range of the cell (k_cell_ranges); a cell without points sits at
+  // the position where its keys would be
+  int b = bot[g], t = top[g];
+  if (b < 0) {
+    b = (int)lower_bound_u64(keys, npts, c << (64 - 2 * d));
+    t = b;
   }
   const int has_src = t > b;
   int f = fake ? NF_FAKE : 0;
@@ -407,22 +462,15 @@ __global__ __launch_bounds__(64) void k_nodes_b(
   uint32_t ix, iy;
   demorton(c, &ix, &iy);
   const int64_t lim = (int64_t)1 << d;
-  const uint64_t* lst = code + levoff[d];
-  const int64_t nl = levoff[d + 1] - levoff[d];
   for (int q = 0; q < 9; ++q) {
     int64_t nx = (int64_t)ix + q / 3 - 1, ny = (int64_t)iy + q % 3 - 1;
     int r = -1;
     if (nx >= 0 && ny >= 0 && nx < lim && ny < lim) {
-      int64_t pos = (q == 4) ? (g - levoff[d]) : bsearch_u64(lst, nl, morton((uint32_t)nx, (uint32_t)ny));
-      if (pos >= 0) r = levoff[d] + (int)pos;
+      r = (q == 4) ? g : node_of(code, levoff, d, morton((uint32_t)nx, (uint32_t)ny));
     }
     coll[(int64_t)g * 9 + q] = r;
   }
-  int par = -1;
-  if (d > 0) {
-    int64_t pos = bsearch_u64(code + levoff[d - 1], levoff[d] - levoff[d - 1], c >> 2);
-    if (pos >= 0) par = levoff[d - 1] + (int)pos;
-  }
+  const int par = d > 0 ? node_of(code, levoff, d - 1, c >> 2) : -1;
   parent[g] = par;
 }
 
@@ -608,10 +656,19 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   int* flag = t.w_flag.as<int>();
   int* rank = t.w_rank.as<int>();
   int2* sc = t.w_a.as<int2>();
-  hipLaunchKernelGGL(k_split_pos, dim3(grid_for(n)), dim3(BS), 0, s, keys, nn, ncut, flag, sc,
-                     cnt + CNT_D);
+  hipLaunchKernelGGL(k_split_pos, dim3(grid_for(n)), dim3(BS), 0, s, keys, nn, ncut, flag, sc);
   FMM_LAUNCH_CHECK();
   exclusive_scan_i32(flag, rank, n, cnt + CNT_P, t.pw, s);
+  // compacted split positions (sized for the worst case, so that no count is
+  // needed on the host first) and the split cells per depth
+  t.w_c.ensure((int64_t)(n + 2) * (sizeof(int) + sizeof(int2)) + 16);
+  int* ppos = t.w_c.as<int>();
+  int2* psc = reinterpret_cast<int2*>(ppos + n + (n & 1));
+  hipLaunchKernelGGL(k_compact_pos, dim3(grid_for(n)), dim3(BS), 0, s, flag, rank, sc, nn, ppos,
+                     psc);
+  FMM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_depth_count, dim3(1), dim3(1024), 0, s, psc, cnt + CNT_P, cnt + CNT_D);
+  FMM_LAUNCH_CHECK();
   FMM_HIP(hipMemcpyAsync(hc, cnt, CNT_N * sizeof(int), hipMemcpyDeviceToHost, s));
   FMM_HIP(hipStreamSynchronize(s));
   const int np = hc[CNT_P];
@@ -633,12 +690,6 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   int64_t* d_soff = t.w_b.as<int64_t>();
   t.sref_all.ensure((ns_total + 1) * sizeof(uint64_t));
   if (np > 0) {
-    t.w_c.ensure((int64_t)(np + 2) * (sizeof(int) + sizeof(int2)) + 16);
-    int* ppos = t.w_c.as<int>();
-    int2* psc = reinterpret_cast<int2*>(ppos + np + (np & 1));
-    hipLaunchKernelGGL(k_compact_pos, dim3(grid_for(n)), dim3(BS), 0, s, flag, rank, sc, nn, ppos,
-                       psc);
-    FMM_LAUNCH_CHECK();
     upload(d_soff, t.s_off.data(), (L + 1) * sizeof(int64_t));
     hipLaunchKernelGGL(k_depth_lists, dim3(L - 1), dim3(1024), 0, s, ppos, psc, np, keys, d_soff,
                        t.sref_all.as<uint64_t>());
@@ -786,6 +837,12 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   hipLaunchKernelGGL(k_nodes_a, dim3(grid_for(off)), dim3(BS), 0, s, t.p_code.as<uint64_t>(),
                      t.p_kind.as<int>(), d_poff, d_levoff, L, t.code.as<uint64_t>(),
                      t.nlevel.as<int>(), t.flags.as<int>());
+  FMM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_cell_ranges_init, dim3(grid_for(off)), dim3(BS), 0, s, NN, nn,
+                     t.bot.as<int>(), t.top.as<int>());
+  FMM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_cell_ranges, dim3(grid_for(n)), dim3(BS), 0, s, keys, sc, nn,
+                     t.code.as<uint64_t>(), d_levoff, L, t.bot.as<int>(), t.top.as<int>());
   FMM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_nodes_b, dim3(ceil_div(off, 64)), dim3(64), 0, s, t.p_code.as<uint64_t>(),
                      t.p_kind.as<int>(), d_poff, d_levoff, L, keys, nn, xmin, ymin, w0,

@@ -123,8 +123,12 @@ _plan_pool = {}
 def _acquire_plan(lib, kernel, p, precision, device):
     key = (kernel.key, p, precision, device)
     pool = _plan_pool.setdefault(key, [])
-    if pool:
-        return pool.pop()
+    while pool:
+        plan = pool.pop()
+        # a plan finalised by the cyclic garbage collector together with the
+        # FMM that returned it to the pool is no longer usable
+        if plan.h:
+            return plan
     return _Plan(lib, kernel, p, precision)
 
 
