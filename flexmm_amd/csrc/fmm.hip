@@ -934,7 +934,9 @@ struct Target {
   double kp;        // kernel parameter (Helmholtz k)
   PotAcc<KER> u;
   double gx = 0.0, gy = 0.0;
-  bool zhit = false;
+  // without self-exclusion: the smallest high word of d^2 seen; 0 means a
+  // coincident pair (or one closer than 1e-154, below the normal range of d^2)
+  uint32_t dmin = 1u;
   // one source (or one equivalent-density point) at (sx, sy) with weight w
   __device__ __forceinline__ void pair(double sx, double sy, double w,
                                        const double2* __restrict__ tab) {
@@ -943,7 +945,7 @@ struct Target {
     double d2 = __builtin_fma(dy, dy, dx * dx);
     const bool z = d2 == 0.0;
     if (CHECK) w = z ? 0.0 : w;   // coincident source skipped (helpers.py:214)
-    else zhit |= z;
+    else dmin = min(dmin, (uint32_t)(__builtin_bit_cast(uint64_t, d2) >> 32));
     if constexpr (KER == KER_LAPLACE) {
       u.add(w, d2, tab, kp);
       if (NOUT == 3) {
@@ -970,6 +972,7 @@ struct Target {
   __device__ __forceinline__ double gscale() const {
     return KER == KER_LAPLACE ? -kInv2Pi : kInv2Pi;
   }
+  __device__ __forceinline__ bool zhit() const { return dmin == 0u; }
 };
 
 template <int PD, int KER, int NOUT, bool CHECK>
@@ -1034,7 +1037,7 @@ __global__ __launch_bounds__(64) void k_eval(
   if (act) {
     const int64_t o = tperm[t];
     double u = T.u.result();
-    if (!CHECK && T.zhit) u = __builtin_inf();   // log(0): the reference's -log(0) = +inf
+    if (!CHECK && T.zhit()) u = __builtin_inf();   // log(0): the reference's -log(0) = +inf
     out[o] = u;
     if (NOUT == 3) {
       out[nt + o] = T.gscale() * T.gx;
@@ -1185,7 +1188,7 @@ __global__ __launch_bounds__(64) void k_eval_mixed(
   }
   if (act) {
     double u = -kInv4Pi * (__builtin_fma(u32, kLn2, T.u.a.value()));
-    if (!CHECK && T.zhit) u = __builtin_inf();
+    if (!CHECK && T.zhit()) u = __builtin_inf();
     out[tperm[t]] = u;
   }
 }
@@ -1317,7 +1320,7 @@ __global__ __launch_bounds__(64) void k_eval_stokes(
   // near field: the Stokeslet itself (stokes_fmm.py:102-184)
   LogAcc lx_, ly_;
   double su = 0.0, sv = 0.0, sp = 0.0;
-  bool zhit = false;
+  double dmin = 1.0;
   const int* cl = coll + (int64_t)g * 9;
   for (int qn = 0; qn < 9; ++qn) {
     const int c = cl[qn];
@@ -1334,7 +1337,7 @@ __global__ __launch_bounds__(64) void k_eval_stokes(
         fx = z ? 0.0 : fx;
         fy = z ? 0.0 : fy;
       } else {
-        zhit |= z;
+        dmin = __builtin_fmin(dmin, d2);
       }
       d2 = z ? 1.0 : d2;
       const double id2 = 1.0 / d2;
@@ -1351,7 +1354,7 @@ __global__ __launch_bounds__(64) void k_eval_stokes(
     u += kInv4Pi * (-0.5 * lx_.value() + su);
     v += kInv4Pi * (-0.5 * ly_.value() + sv);
     pr += kInv2Pi * sp;
-    if (!CHECK && zhit) u = v = pr = __builtin_nan("");
+    if (!CHECK && dmin == 0.0) u = v = pr = __builtin_nan("");
     out[o] = u;
     out[nt + o] = v;
     out[2 * nt + o] = pr;

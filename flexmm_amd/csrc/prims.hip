@@ -62,12 +62,66 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_apply(const int* in, int* out,
   }
 }
 
+// n <= kScanSmallMax: one block of 1024 threads; element j * 1024 + t is
+// loaded by thread t (coalesced, all loads in flight together), each of the
+// J = ceil(n / 1024) tiles is scanned by wave scans, and the wave and tile
+// totals are combined through LDS with one barrier.
+constexpr int kScanSmallTiles = 32;
+constexpr int kScanSmallMax = kScanSmallTiles * 1024;
+__global__ __launch_bounds__(1024) void k_scan_small(const int* in, int* out, int n,
+                                                    int* __restrict__ total) {
+  __shared__ int wsum[kScanSmallTiles][16];
+  __shared__ int woff[kScanSmallTiles][16];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int J = (n + 1023) >> 10;
+  int v[kScanSmallTiles], inc[kScanSmallTiles];
+#pragma unroll
+  for (int j = 0; j < kScanSmallTiles; ++j) {
+    const int idx = j * 1024 + t;
+    v[j] = (j < J && idx < n) ? in[idx] : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < kScanSmallTiles; ++j) {
+    if (j < J) {
+      inc[j] = wave_inclusive_scan(v[j]);
+      if (lane == 63) wsum[j][wid] = inc[j];
+    }
+  }
+  __syncthreads();
+  if (wid == 0) {
+    // lane j < J: offsets of the 16 waves inside tile j, then the tile prefix
+    int tile_tot = 0;
+    if (lane < J)
+      for (int w = 0; w < 16; ++w) tile_tot += wsum[lane][w];
+    const int tile_off = wave_inclusive_scan(lane < J ? tile_tot : 0) - (lane < J ? tile_tot : 0);
+    if (lane < J) {
+      int run = tile_off;
+      for (int w = 0; w < 16; ++w) {
+        woff[lane][w] = run;
+        run += wsum[lane][w];
+      }
+      if (lane == J - 1 && total) *total = run;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kScanSmallTiles; ++j) {
+    const int idx = j * 1024 + t;
+    if (j < J && idx < n) out[idx] = woff[j][wid] + inc[j] - v[j];
+  }
+}
+
 }  // namespace
 
 void exclusive_scan_i32(const int* in, int* out, int64_t n, int* total_dev, PrimWork& w,
                         hipStream_t s) {
   if (n <= 0) {
     if (total_dev) FMM_HIP(hipMemsetAsync(total_dev, 0, sizeof(int), s));
+    return;
+  }
+  if (n <= kScanSmallMax) {
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, in, out, (int)n, total_dev);
+    FMM_LAUNCH_CHECK();
     return;
   }
   int64_t nb = ceil_div(n, SCAN_TILE);
@@ -103,18 +157,24 @@ __global__ __launch_bounds__(1024) void k_make_chunks(const int2* __restrict__ s
     int2 sg = si < nseg ? seg[si] : make_int2(0, 0);
     int cnt = si < nseg ? (int)ceil_div(sg.y, CHUNK) : 0;
     int ex = block_exclusive_scan<1024>(cnt, lds, &tot);
-    if (si < nseg) {
-      int c0 = carry + ex;
-      segChunk0[si] = c0;
-      for (int c = 0; c < cnt; ++c) {
-        int st = sg.x + c * CHUNK;
-        int ln = min(CHUNK, sg.y - c * CHUNK);
-        chunks[c0 + c] = make_int4(si, st, ln, 0);
-      }
-    }
+    if (si < nseg) segChunk0[si] = carry + ex;
     __syncthreads();
     carry += tot;
     __syncthreads();
+  }
+  __threadfence_block();
+  __syncthreads();
+  // chunk c belongs to the last segment whose first chunk is <= c
+  for (int c = threadIdx.x; c < carry; c += 1024) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (segChunk0[mid] <= c) lo = mid;
+      else hi = mid - 1;
+    }
+    const int2 sg = seg[lo];
+    const int k = c - segChunk0[lo];
+    chunks[c] = make_int4(lo, sg.x + k * CHUNK, min(CHUNK, sg.y - k * CHUNK), 0);
   }
   if (threadIdx.x == 0) counters[C_CHUNKS] = carry;
 }
@@ -137,58 +197,88 @@ __global__ __launch_bounds__(SORT_BS) void k_chunk_hist(const uint64_t* __restri
   chunkHist[(int64_t)b * 256 + threadIdx.x] = hist[threadIdx.x];
 }
 
-__global__ __launch_bounds__(SORT_BS) void k_seg_offsets(
+// One block per segment, 1024 threads: digit d = tid % 256, part = tid / 256
+// scans a quarter of the segment's chunks.  Output: chunkHist[c][d] = the
+// destination of chunk c's first element of digit d.  The digit segments of
+// the next round and the final ones are appended with one atomic per block.
+constexpr int SEG_BS = 1024;
+
+__global__ __launch_bounds__(SEG_BS) void k_seg_offsets(
     const int2* __restrict__ seg, const int* __restrict__ segChunk0, int* __restrict__ chunkHist,
     int threshold, int last_round, int2* __restrict__ segNext, int2* __restrict__ fin,
     int finBigOff, int* __restrict__ counters) {
-  __shared__ int lds[SORT_BS / 64];
+  __shared__ int lds[SEG_BS / 64];
+  __shared__ int part_sum[4][256];
+  __shared__ int bases[3];
   const int si = blockIdx.x;
-  const int d = threadIdx.x;
-  int2 sg = seg[si];
-  int c0 = segChunk0[si];
-  int nc = (int)ceil_div(sg.y, CHUNK);
-  // per-digit exclusive prefix over the segment's chunks, 16 chunks per batch
-  // so that the loads of a batch are in flight together
-  constexpr int B = 16;
-  int run = 0;
-  int c = c0;
-  for (; c + B <= c0 + nc; c += B) {
+  const int d = threadIdx.x & 255, part = threadIdx.x >> 8;
+  const int2 sg = seg[si];
+  const int c0 = segChunk0[si];
+  const int nc = (int)ceil_div(sg.y, CHUNK);
+  const int q = (nc + 3) >> 2;
+  const int cb = c0 + min(nc, part * q), ce = c0 + min(nc, (part + 1) * q);
+  constexpr int B = 8;
+  int sum = 0;
+  int c = cb;
+  for (; c + B <= ce; c += B) {
+    int h[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) h[u] = chunkHist[(int64_t)(c + u) * 256 + d];
+#pragma unroll
+    for (int u = 0; u < B; ++u) sum += h[u];
+  }
+  for (; c < ce; ++c) sum += chunkHist[(int64_t)c * 256 + d];
+  part_sum[part][d] = sum;
+  __syncthreads();
+  const int run = part_sum[0][d] + part_sum[1][d] + part_sum[2][d] + part_sum[3][d];
+  // digit starts: exclusive scan of the totals over the 256 digits
+  const int dstart = block_exclusive_scan<SEG_BS>(part == 0 ? run : 0, lds, nullptr);
+  __shared__ int dstart_s[256];
+  if (part == 0) dstart_s[d] = dstart;
+  __syncthreads();
+  int pre = 0;
+  for (int pp = 0; pp < part; ++pp) pre += part_sum[pp][d];
+  int pos = sg.x + dstart_s[d] + pre;
+  c = cb;
+  for (; c + B <= ce; c += B) {
     int h[B];
 #pragma unroll
     for (int u = 0; u < B; ++u) h[u] = chunkHist[(int64_t)(c + u) * 256 + d];
 #pragma unroll
     for (int u = 0; u < B; ++u) {
-      chunkHist[(int64_t)(c + u) * 256 + d] = run;
-      run += h[u];
+      chunkHist[(int64_t)(c + u) * 256 + d] = pos;
+      pos += h[u];
     }
   }
-  for (; c < c0 + nc; ++c) {
-    int h = chunkHist[(int64_t)c * 256 + d];
-    chunkHist[(int64_t)c * 256 + d] = run;
-    run += h;
+  for (; c < ce; ++c) {
+    const int h = chunkHist[(int64_t)c * 256 + d];
+    chunkHist[(int64_t)c * 256 + d] = pos;
+    pos += h;
   }
-  int base = block_exclusive_scan<SORT_BS>(run, lds, nullptr);
-  int start = sg.x + base;
-  c = c0;
-  for (; c + B <= c0 + nc; c += B) {
-    int h[B];
-#pragma unroll
-    for (int u = 0; u < B; ++u) h[u] = chunkHist[(int64_t)(c + u) * 256 + d];
-#pragma unroll
-    for (int u = 0; u < B; ++u) chunkHist[(int64_t)(c + u) * 256 + d] = h[u] + start;
+  // classify the digit segments and append them with one atomic per list
+  const int start = sg.x + dstart_s[d];
+  const bool nxt = part == 0 && run > threshold && !last_round;
+  const bool fsm = part == 0 && !nxt && run >= 2 && run <= kWaveSortMax;
+  const bool fbg = part == 0 && !nxt && run > kWaveSortMax && run <= kLocalSortMax;
+  // counts of the three kinds packed in one scan (each at most 256)
+  const int code = (nxt ? 1 : 0) | (fsm ? 1 << 10 : 0) | (fbg ? 1 << 20 : 0);
+  __shared__ int tot;
+  const int ex = block_exclusive_scan<SEG_BS>(code, lds, &tot);
+  if (threadIdx.x == 0) {
+    const int tn = tot & 1023, tf = (tot >> 10) & 1023, tb = (tot >> 20) & 1023;
+    bases[0] = tn ? atomicAdd(&counters[C_NEXT], tn) : 0;
+    bases[1] = tf ? atomicAdd(&counters[C_FINAL], tf) : 0;
+    bases[2] = tb ? atomicAdd(&counters[C_FINAL_BIG], tb) : 0;
   }
-  for (; c < c0 + nc; ++c) chunkHist[(int64_t)c * 256 + d] += start;
-  if (run > threshold && !last_round) {
-    int idx = atomicAdd(&counters[C_NEXT], 1);
-    segNext[idx] = make_int2(start, run);
-    atomicAdd(&counters[C_ACTIVE], run);
-  } else if (run >= 2 && run <= kWaveSortMax) {
-    int idx = atomicAdd(&counters[C_FINAL], 1);
-    fin[idx] = make_int2(start, run);
-  } else if (run > kWaveSortMax && run <= kLocalSortMax) {
-    int idx = atomicAdd(&counters[C_FINAL_BIG], 1);
-    fin[finBigOff - 1 - idx] = make_int2(start, run);   // big segments fill fin from the end
-  }
+  int act = nxt ? run : 0;
+  __syncthreads();
+  if (nxt) segNext[bases[0] + (ex & 1023)] = make_int2(start, run);
+  if (fsm) fin[bases[1] + ((ex >> 10) & 1023)] = make_int2(start, run);
+  if (fbg) fin[finBigOff - 1 - (bases[2] + ((ex >> 20) & 1023))] = make_int2(start, run);
+  // active elements of the next round: one atomic per block
+  __shared__ int act_tot;
+  (void)block_exclusive_scan<SEG_BS>(act, lds, &act_tot);
+  if (threadIdx.x == 0 && act_tot) atomicAdd(&counters[C_ACTIVE], act_tot);
 }
 
 __global__ __launch_bounds__(SORT_BS) void k_chunk_scatter(
@@ -380,7 +470,7 @@ void sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
     hipLaunchKernelGGL(k_chunk_hist, dim3(cbound), dim3(SORT_BS), 0, s, keys, w.chunks.as<int4>(),
                        cnt, shift, w.chunkHist.as<int>());
     FMM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_seg_offsets, dim3(nseg), dim3(SORT_BS), 0, s, segCur,
+    hipLaunchKernelGGL(k_seg_offsets, dim3(nseg), dim3(SEG_BS), 0, s, segCur,
                        w.segChunk0.as<int>(), w.chunkHist.as<int>(), T, r == 7 ? 1 : 0, segNext,
                        w.fin.as<int2>(), finBigOff, cnt);
     FMM_LAUNCH_CHECK();
