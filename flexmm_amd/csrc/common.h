@@ -148,6 +148,18 @@ __device__ __forceinline__ double log_tab(double x, const double2* __restrict__ 
 #endif
 
 // ---------------------------------------------------------------------------
+// XCD-aware block order.  MI355X dispatches consecutive workgroups round-robin
+// to its 8 XCDs, each with its own L2; this bijection of [0, n) gives XCD x
+// (blocks b with b % 8 == x) one contiguous range of logical blocks, so that
+// neighbouring tiles, which share source data, meet in the same L2.
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int xcd = b % kNumXcd, local = b / kNumXcd;
+  const int per = n / kNumXcd, rem = n % kNumXcd;
+  return xcd * per + min(xcd, rem) + local;
+}
+
+// ---------------------------------------------------------------------------
 // Kernel identifiers shared with the Python registry.
 enum KernelId : int {
   KER_LAPLACE = 0,   // -log(d^2)/(4 pi)

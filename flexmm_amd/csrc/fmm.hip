@@ -447,9 +447,10 @@ __device__ __forceinline__ M2LSetup m2l_setup(const LevelTable& lt, const int* _
                                               int (&srcs)[27][kM2LCols], int (&pslot)[28],
                                               unsigned long long& smask) {
   M2LSetup S;
+  const int bid = xcd_block(blockIdx.x, gridDim.x);
   int li = 0;
-  while (li + 1 < lt.nlev && (int)blockIdx.x >= lt.boff[li + 1]) ++li;
-  const int lb = blockIdx.x - lt.boff[li];
+  while (li + 1 < lt.nlev && bid >= lt.boff[li + 1]) ++li;
+  const int lb = bid - lt.boff[li];
   S.li = li;
   S.c = lb & 3;
   const int tile = lb >> 2;
@@ -928,6 +929,23 @@ __global__ __launch_bounds__(256) void k_make_items(const int* __restrict__ icnt
 
 // ---------------------------------------------------------------------------
 // L2T + P2P, one wave per (box, chunk of 64 targets); lane = target.
+// Work items of the evaluation kernels, one per block: XCD x (blocks b with
+// b % 8 == x, dispatched in order) takes the x-th contiguous eighth of the
+// items.  Items are in leaf (Morton) order, so the colleague sources that
+// neighbouring leaves share stay in one XCD's L2.  The grid holds at least
+// ceil(ni / 8) blocks per XCD; surplus blocks exit.
+struct ItemRange {
+  int first, last, step;
+};
+__device__ __forceinline__ ItemRange xcd_items(int ni) {
+  const int x = blockIdx.x % kNumXcd;
+  ItemRange r;
+  r.first = (int)((int64_t)ni * x / kNumXcd) + (int)(blockIdx.x / kNumXcd);
+  r.last = min(r.first + 1, (int)((int64_t)ni * (x + 1) / kNumXcd));
+  r.step = 1;
+  return r;
+}
+
 template <int KER, int NOUT, bool CHECK>
 struct Target {
   double x, y;
@@ -986,62 +1004,65 @@ __global__ __launch_bounds__(64) void k_eval(
     const double* __restrict__ le, const double* __restrict__ ops, int64_t opsz, int p,
     const double2* __restrict__ logtab, double kp, int64_t nt, double* __restrict__ out) {
   __shared__ double2 tab[kLogTableSize];
-  if ((int)blockIdx.x >= *nitems) return;
+  const int ni = *nitems;
   load_logtab(tab, logtab);
   __syncthreads();
-  const int2 it = items[blockIdx.x];
-  const int g = evalnode[it.x];
-  const int t0 = toff[it.x] + it.y * kTgtChunk;
-  const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
-  const int t = t0 + threadIdx.x;
-  const bool act = t < t1;
-  Target<KER, NOUT, CHECK> T;
-  T.kp = kp;
-  T.x = act ? tsx[t] : 0.0;
-  T.y = act ? tsy[t] : 0.0;
-  // L2T: helpers.py:219-235 (the equivalent density sits on the large circle)
-  const int d = nlevel[g];
-  if (d >= 2) {
-    Target<KER, NOUT, false> L;
-    L.kp = kp;
-    L.x = T.x - xmid[g];
-    L.y = T.y - ymid[g];
-    const double* lx = ops + d * opsz + (int64_t)OP_NBLK * PD * PD;
-    const double* ly = lx + PD;
-    const double* lg = le + (int64_t)g * PD;
-    for (int k = 0; k < p; ++k) L.pair(lx[k], ly[k], lg[k], tab);
-    T.u = L.u;
-    T.gx = L.gx;
-    T.gy = L.gy;
-  }
-  // P2P over the leaf's colleagues: helpers.py:180-217
-  const int* cl = coll + (int64_t)g * 9;
-  for (int qn = 0; qn < 9; ++qn) {
-    const int c = cl[qn];
-    if (c < 0) continue;
-    const int b = __builtin_amdgcn_readfirstlane(bot[c]);
-    const int e = __builtin_amdgcn_readfirstlane(top[c]);
-    int j = b;
-    for (; j + 4 <= e; j += 4) {
-      double4 s4[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) s4[u] = src[j + u];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) T.pair(s4[u].x, s4[u].y, s4[u].z, tab);
+  const ItemRange R = xcd_items(ni);
+  for (int ii = R.first; ii < R.last; ii += R.step) {
+    const int2 it = items[ii];
+    const int g = evalnode[it.x];
+    const int t0 = toff[it.x] + it.y * kTgtChunk;
+    const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
+    const int t = t0 + threadIdx.x;
+    const bool act = t < t1;
+    Target<KER, NOUT, CHECK> T;
+    T.kp = kp;
+    T.x = act ? tsx[t] : 0.0;
+    T.y = act ? tsy[t] : 0.0;
+    // L2T: helpers.py:219-235 (the equivalent density sits on the large circle)
+    const int d = nlevel[g];
+    if (d >= 2) {
+      Target<KER, NOUT, false> L;
+      L.kp = kp;
+      L.x = T.x - xmid[g];
+      L.y = T.y - ymid[g];
+      const double* lx = ops + d * opsz + (int64_t)OP_NBLK * PD * PD;
+      const double* ly = lx + PD;
+      const double* lg = le + (int64_t)g * PD;
+      for (int k = 0; k < p; ++k) L.pair(lx[k], ly[k], lg[k], tab);
+      T.u = L.u;
+      T.gx = L.gx;
+      T.gy = L.gy;
     }
-    for (; j < e; ++j) {
-      const double4 sj = src[j];
-      T.pair(sj.x, sj.y, sj.z, tab);
+    // P2P over the leaf's colleagues: helpers.py:180-217
+    const int* cl = coll + (int64_t)g * 9;
+    for (int qn = 0; qn < 9; ++qn) {
+      const int c = cl[qn];
+      if (c < 0) continue;
+      const int b = __builtin_amdgcn_readfirstlane(bot[c]);
+      const int e = __builtin_amdgcn_readfirstlane(top[c]);
+      int j = b;
+      for (; j + 4 <= e; j += 4) {
+        double4 s4[4];
+  #pragma unroll
+        for (int u = 0; u < 4; ++u) s4[u] = src[j + u];
+  #pragma unroll
+        for (int u = 0; u < 4; ++u) T.pair(s4[u].x, s4[u].y, s4[u].z, tab);
+      }
+      for (; j < e; ++j) {
+        const double4 sj = src[j];
+        T.pair(sj.x, sj.y, sj.z, tab);
+      }
     }
-  }
-  if (act) {
-    const int64_t o = tperm[t];
-    double u = T.u.result();
-    if (!CHECK && T.zhit()) u = __builtin_inf();   // log(0): the reference's -log(0) = +inf
-    out[o] = u;
-    if (NOUT == 3) {
-      out[nt + o] = T.gscale() * T.gx;
-      out[2 * nt + o] = T.gscale() * T.gy;
+    if (act) {
+      const int64_t o = tperm[t];
+      double u = T.u.result();
+      if (!CHECK && T.zhit()) u = __builtin_inf();   // log(0): the reference's -log(0) = +inf
+      out[o] = u;
+      if (NOUT == 3) {
+        out[nt + o] = T.gscale() * T.gx;
+        out[2 * nt + o] = T.gscale() * T.gy;
+      }
     }
   }
 }
@@ -1098,98 +1119,101 @@ __global__ __launch_bounds__(64) void k_eval_mixed(
     const double* __restrict__ ops, int64_t opsz, int p, const double2* __restrict__ logtab,
     double* __restrict__ out) {
   __shared__ double2 tab[kLogTableSize];
-  if ((int)blockIdx.x >= *nitems) return;
+  const int ni = *nitems;
   load_logtab(tab, logtab);
   __syncthreads();
-  const int2 it = items[blockIdx.x];
-  const int g = evalnode[it.x];
-  const int t0 = toff[it.x] + it.y * kTgtChunk;
-  const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
-  const int t = t0 + threadIdx.x;
-  const bool act = t < t1;
-  Target<KER_LAPLACE, 1, CHECK> T;
-  T.kp = 0.0;
-  T.x = act ? tsx[t] : 0.0;
-  T.y = act ? tsy[t] : 0.0;
-  const int d = nlevel[g];
-  if (d >= 2) {   // L2T in FP64 (the local densities carry large cancelling modes)
-    Target<KER_LAPLACE, 1, false> L;
-    L.kp = 0.0;
-    L.x = T.x - xmid[g];
-    L.y = T.y - ymid[g];
-    const double* lx = ops + d * opsz + ops_off_circles(PD);
-    const double* ly = lx + PD;
-    const double* lg = le + (int64_t)g * PD;
-    for (int k = 0; k < p; ++k) L.pair(lx[k], ly[k], lg[k], tab);
-    T.u = L.u;
-  }
-  const double iw = lc.invw[d];
-  const double lw2 = lc.logw2[d];
-  double u32 = 0.0;      // ln2-scaled part from the FP32 near field
-  const int* cl = coll + (int64_t)g * 9;
-  for (int qn = 0; qn < 9; ++qn) {
-    const int c = cl[qn];
-    if (c < 0) continue;
-    const int b = __builtin_amdgcn_readfirstlane(bot[c]);
-    const int e = __builtin_amdgcn_readfirstlane(top[c]);
-    if (e <= b) continue;
-    if (flags[c] & NF_LEAF) {
-      const float xr = (float)((T.x - xmid[c]) * iw);
-      const float yr = (float)((T.y - ymid[c]) * iw);
-      float acc = 0.f;
-      double excl = 0.0;   // charges of the pairs taken in FP64
-      // pairs closer than w/16 (and coincident ones) go to FP64: the rounding
-      // of the FP32 relative coordinates costs ~1e-7 w / d in log d
-      auto near64 = [&](int jj) {
-        const double4 sd = src[jj];
-        excl += sd.z;
-        T.pair(sd.x, sd.y, sd.z, tab);
-      };
-      int j = b;
-      for (; j + 4 <= e; j += 4) {
-        f4 s4[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) s4[u] = srcf[j + u];
-        bool nr[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const float dx = xr - s4[u].x, dy = yr - s4[u].y;
+  const ItemRange R = xcd_items(ni);
+  for (int ii = R.first; ii < R.last; ii += R.step) {
+    const int2 it = items[ii];
+    const int g = evalnode[it.x];
+    const int t0 = toff[it.x] + it.y * kTgtChunk;
+    const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
+    const int t = t0 + threadIdx.x;
+    const bool act = t < t1;
+    Target<KER_LAPLACE, 1, CHECK> T;
+    T.kp = 0.0;
+    T.x = act ? tsx[t] : 0.0;
+    T.y = act ? tsy[t] : 0.0;
+    const int d = nlevel[g];
+    if (d >= 2) {   // L2T in FP64 (the local densities carry large cancelling modes)
+      Target<KER_LAPLACE, 1, false> L;
+      L.kp = 0.0;
+      L.x = T.x - xmid[g];
+      L.y = T.y - ymid[g];
+      const double* lx = ops + d * opsz + ops_off_circles(PD);
+      const double* ly = lx + PD;
+      const double* lg = le + (int64_t)g * PD;
+      for (int k = 0; k < p; ++k) L.pair(lx[k], ly[k], lg[k], tab);
+      T.u = L.u;
+    }
+    const double iw = lc.invw[d];
+    const double lw2 = lc.logw2[d];
+    double u32 = 0.0;      // ln2-scaled part from the FP32 near field
+    const int* cl = coll + (int64_t)g * 9;
+    for (int qn = 0; qn < 9; ++qn) {
+      const int c = cl[qn];
+      if (c < 0) continue;
+      const int b = __builtin_amdgcn_readfirstlane(bot[c]);
+      const int e = __builtin_amdgcn_readfirstlane(top[c]);
+      if (e <= b) continue;
+      if (flags[c] & NF_LEAF) {
+        const float xr = (float)((T.x - xmid[c]) * iw);
+        const float yr = (float)((T.y - ymid[c]) * iw);
+        float acc = 0.f;
+        double excl = 0.0;   // charges of the pairs taken in FP64
+        // pairs closer than w/16 (and coincident ones) go to FP64: the rounding
+        // of the FP32 relative coordinates costs ~1e-7 w / d in log d
+        auto near64 = [&](int jj) {
+          const double4 sd = src[jj];
+          excl += sd.z;
+          T.pair(sd.x, sd.y, sd.z, tab);
+        };
+        int j = b;
+        for (; j + 4 <= e; j += 4) {
+          f4 s4[4];
+  #pragma unroll
+          for (int u = 0; u < 4; ++u) s4[u] = srcf[j + u];
+          bool nr[4];
+  #pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const float dx = xr - s4[u].x, dy = yr - s4[u].y;
+            const float d2 = __builtin_fmaf(dy, dy, dx * dx);
+            nr[u] = d2 < kNearF32;
+            const float lg2 = __builtin_amdgcn_logf(nr[u] ? 1.0f : d2);
+            acc = __builtin_fmaf(nr[u] ? 0.f : s4[u].z, lg2, acc);
+          }
+          if (nr[0] | nr[1] | nr[2] | nr[3]) {
+  #pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (nr[u]) near64(j + u);
+          }
+        }
+        for (; j < e; ++j) {
+          const f4 sj = srcf[j];
+          const float dx = xr - sj.x, dy = yr - sj.y;
           const float d2 = __builtin_fmaf(dy, dy, dx * dx);
-          nr[u] = d2 < kNearF32;
-          const float lg2 = __builtin_amdgcn_logf(nr[u] ? 1.0f : d2);
-          acc = __builtin_fmaf(nr[u] ? 0.f : s4[u].z, lg2, acc);
+          const bool nr = d2 < kNearF32;
+          const float lg2 = __builtin_amdgcn_logf(nr ? 1.0f : d2);
+          acc = __builtin_fmaf(nr ? 0.f : sj.z, lg2, acc);
+          if (nr) near64(j);
         }
-        if (nr[0] | nr[1] | nr[2] | nr[3]) {
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-            if (nr[u]) near64(j + u);
+        u32 += (double)acc;
+        T.u.a.at = __builtin_fma(lw2, tsum[c] - excl, T.u.a.at);
+      } else {
+        // colleague with children (adaptive trees): its sources are not stored
+        // relative to it, so this range stays FP64
+        int j = b;
+        for (; j < e; ++j) {
+          const double4 sj = src[j];
+          T.pair(sj.x, sj.y, sj.z, tab);
         }
-      }
-      for (; j < e; ++j) {
-        const f4 sj = srcf[j];
-        const float dx = xr - sj.x, dy = yr - sj.y;
-        const float d2 = __builtin_fmaf(dy, dy, dx * dx);
-        const bool nr = d2 < kNearF32;
-        const float lg2 = __builtin_amdgcn_logf(nr ? 1.0f : d2);
-        acc = __builtin_fmaf(nr ? 0.f : sj.z, lg2, acc);
-        if (nr) near64(j);
-      }
-      u32 += (double)acc;
-      T.u.a.at = __builtin_fma(lw2, tsum[c] - excl, T.u.a.at);
-    } else {
-      // colleague with children (adaptive trees): its sources are not stored
-      // relative to it, so this range stays FP64
-      int j = b;
-      for (; j < e; ++j) {
-        const double4 sj = src[j];
-        T.pair(sj.x, sj.y, sj.z, tab);
       }
     }
-  }
-  if (act) {
-    double u = -kInv4Pi * (__builtin_fma(u32, kLn2, T.u.a.value()));
-    if (!CHECK && T.zhit()) u = __builtin_inf();
-    out[tperm[t]] = u;
+    if (act) {
+      double u = -kInv4Pi * (__builtin_fma(u32, kLn2, T.u.a.value()));
+      if (!CHECK && T.zhit()) u = __builtin_inf();
+      out[tperm[t]] = u;
+    }
   }
 }
 
@@ -1278,88 +1302,93 @@ __global__ __launch_bounds__(64) void k_eval_stokes(
     int p, const double2* __restrict__ logtab, double c0x, double c0y, int64_t nt,
     double* __restrict__ out) {
   __shared__ double2 tab[kLogTableSize];
-  if ((int)blockIdx.x >= *nitems) return;
+  const int ni = *nitems;
   load_logtab(tab, logtab);
   __syncthreads();
-  const int2 it = items[blockIdx.x];
-  const int g = evalnode[it.x];
-  const int t0 = toff[it.x] + it.y * kTgtChunk;
-  const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
-  const int t = t0 + threadIdx.x;
-  const bool act = t < t1;
-  const double x = act ? tsx[t] : 0.0;
-  const double y = act ? tsy[t] : 0.0;
-  double u = 0.0, v = 0.0, pr = 0.0;
-  const int d = nlevel[g];
-  if (d >= 2) {
-    const double* lx = ops + d * opsz + ops_off_circles(PD);
-    const double* ly = lx + PD;
-    const double* a = le + (int64_t)g * PD;
-    const double xr = x - xmid[g], yr = y - ymid[g];
-    LogAcc l0, l1;
-    double B = 0.0, Cx = 0.0, Cy = 0.0;
-    for (int k = 0; k < p; ++k) {
-      const double dx = xr - lx[k], dy = yr - ly[k];
-      const double d2 = __builtin_fma(dy, dy, dx * dx);
-      const double id2 = 1.0 / d2;
-      const double a0 = a[k], a1 = a[nstride * PD + k], a2 = a[2 * nstride * PD + k],
-                   a3 = a[3 * nstride * PD + k], a4 = a[4 * nstride * PD + k],
-                   a5 = a[5 * nstride * PD + k];
-      l0.add(a0, d2, tab);
-      l1.add(a1, d2, tab);
-      B = __builtin_fma(__builtin_fma(a0, dx, a1 * dy), id2, B);
-      Cx = __builtin_fma(__builtin_fma(a2, dx, a3 * dy), id2, Cx);
-      Cy = __builtin_fma(__builtin_fma(a4, dx, a5 * dy), id2, Cy);
-    }
-    // -L[f] = -sum a (1/2) log d^2
-    const double X = x - c0x, Y = y - c0y;
-    u = kInv4Pi * (-0.5 * l0.value() + X * B - Cx);
-    v = kInv4Pi * (-0.5 * l1.value() + Y * B - Cy);
-    pr = kInv2Pi * B;
-  }
-  // near field: the Stokeslet itself (stokes_fmm.py:102-184)
-  LogAcc lx_, ly_;
-  double su = 0.0, sv = 0.0, sp = 0.0;
-  double dmin = 1.0;
-  const int* cl = coll + (int64_t)g * 9;
-  for (int qn = 0; qn < 9; ++qn) {
-    const int c = cl[qn];
-    if (c < 0) continue;
-    const int b = __builtin_amdgcn_readfirstlane(bot[c]);
-    const int e = __builtin_amdgcn_readfirstlane(top[c]);
-    for (int j = b; j < e; ++j) {
-      const double4 sj = src[j];
-      const double dx = x - sj.x, dy = y - sj.y;
-      double d2 = __builtin_fma(dy, dy, dx * dx);
-      const bool z = d2 == 0.0;
-      double fx = sj.z, fy = sj.w;
-      if (CHECK) {
-        fx = z ? 0.0 : fx;
-        fy = z ? 0.0 : fy;
-      } else {
-        dmin = __builtin_fmin(dmin, d2);
+  const ItemRange R = xcd_items(ni);
+  for (int ii = R.first; ii < R.last; ii += R.step) {
+    const int2 it = items[ii];
+    const int g = evalnode[it.x];
+    const int t0 = toff[it.x] + it.y * kTgtChunk;
+    const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
+    const int t = t0 + threadIdx.x;
+    const bool act = t < t1;
+    const double x = act ? tsx[t] : 0.0;
+    const double y = act ? tsy[t] : 0.0;
+    double u = 0.0, v = 0.0, pr = 0.0;
+    const int d = nlevel[g];
+    if (d >= 2) {
+      const double* lx = ops + d * opsz + ops_off_circles(PD);
+      const double* ly = lx + PD;
+      const double* a = le + (int64_t)g * PD;
+      const double xr = x - xmid[g], yr = y - ymid[g];
+      LogAcc l0, l1;
+      double B = 0.0, Cx = 0.0, Cy = 0.0;
+      for (int k = 0; k < p; ++k) {
+        const double dx = xr - lx[k], dy = yr - ly[k];
+        const double d2 = __builtin_fma(dy, dy, dx * dx);
+        const double id2 = 1.0 / d2;
+        const double a0 = a[k], a1 = a[nstride * PD + k], a2 = a[2 * nstride * PD + k],
+                     a3 = a[3 * nstride * PD + k], a4 = a[4 * nstride * PD + k],
+                     a5 = a[5 * nstride * PD + k];
+        l0.add(a0, d2, tab);
+        l1.add(a1, d2, tab);
+        B = __builtin_fma(__builtin_fma(a0, dx, a1 * dy), id2, B);
+        Cx = __builtin_fma(__builtin_fma(a2, dx, a3 * dy), id2, Cx);
+        Cy = __builtin_fma(__builtin_fma(a4, dx, a5 * dy), id2, Cy);
       }
-      d2 = z ? 1.0 : d2;
-      const double id2 = 1.0 / d2;
-      lx_.add(fx, d2, tab);
-      ly_.add(fy, d2, tab);
-      const double rf = __builtin_fma(dx, fx, dy * fy) * id2;
-      su = __builtin_fma(dx, rf, su);
-      sv = __builtin_fma(dy, rf, sv);
-      sp += rf;
+      // -L[f] = -sum a (1/2) log d^2
+      const double X = x - c0x, Y = y - c0y;
+      u = kInv4Pi * (-0.5 * l0.value() + X * B - Cx);
+      v = kInv4Pi * (-0.5 * l1.value() + Y * B - Cy);
+      pr = kInv2Pi * B;
     }
-  }
-  if (act) {
-    const int64_t o = tperm[t];
-    u += kInv4Pi * (-0.5 * lx_.value() + su);
-    v += kInv4Pi * (-0.5 * ly_.value() + sv);
-    pr += kInv2Pi * sp;
-    if (!CHECK && dmin == 0.0) u = v = pr = __builtin_nan("");
-    out[o] = u;
-    out[nt + o] = v;
-    out[2 * nt + o] = pr;
+    // near field: the Stokeslet itself (stokes_fmm.py:102-184)
+    LogAcc lx_, ly_;
+    double su = 0.0, sv = 0.0, sp = 0.0;
+    double dmin = 1.0;
+    const int* cl = coll + (int64_t)g * 9;
+    for (int qn = 0; qn < 9; ++qn) {
+      const int c = cl[qn];
+      if (c < 0) continue;
+      const int b = __builtin_amdgcn_readfirstlane(bot[c]);
+      const int e = __builtin_amdgcn_readfirstlane(top[c]);
+      for (int j = b; j < e; ++j) {
+        const double4 sj = src[j];
+        const double dx = x - sj.x, dy = y - sj.y;
+        double d2 = __builtin_fma(dy, dy, dx * dx);
+        const bool z = d2 == 0.0;
+        double fx = sj.z, fy = sj.w;
+        if (CHECK) {
+          fx = z ? 0.0 : fx;
+          fy = z ? 0.0 : fy;
+        } else {
+          dmin = __builtin_fmin(dmin, d2);
+        }
+        d2 = z ? 1.0 : d2;
+        const double id2 = 1.0 / d2;
+        lx_.add(fx, d2, tab);
+        ly_.add(fy, d2, tab);
+        const double rf = __builtin_fma(dx, fx, dy * fy) * id2;
+        su = __builtin_fma(dx, rf, su);
+        sv = __builtin_fma(dy, rf, sv);
+        sp += rf;
+      }
+    }
+    if (act) {
+      const int64_t o = tperm[t];
+      u += kInv4Pi * (-0.5 * lx_.value() + su);
+      v += kInv4Pi * (-0.5 * ly_.value() + sv);
+      pr += kInv2Pi * sp;
+      if (!CHECK && dmin == 0.0) u = v = pr = __builtin_nan("");
+      out[o] = u;
+      out[nt + o] = v;
+      out[2 * nt + o] = pr;
+    }
   }
 }
+
+inline int eval_grid(int64_t icap) { return (int)(ceil_div(icap + kNumXcd, kNumXcd) * kNumXcd); }
 
 // Selects the k_eval instance of (PD, kernel, outputs, self-exclusion).
 typedef decltype(&k_eval<32, KER_LAPLACE, 1, false>) EvalKernelFn;
@@ -1617,7 +1646,7 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
     auto sk = PD == 32 ? (check ? k_eval_stokes<32, true> : k_eval_stokes<32, false>)
                        : (check ? k_eval_stokes<64, true> : k_eval_stokes<64, false>);
     P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
-    hipLaunchKernelGGL(sk, dim3(icap), dim3(64), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
+    hipLaunchKernelGGL(sk, dim3(eval_grid(icap)), dim3(64), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
                        t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(),
                        P.tsx.as<double>(), P.tsy.as<double>(), t.nlevel.as<int>(),
                        t.xmid.as<double>(), t.ymid.as<double>(), t.coll.as<int>(),
@@ -1648,7 +1677,7 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
     auto mk = check ? (PD == 32 ? k_eval_mixed<32, true> : k_eval_mixed<64, true>)
                     : (PD == 32 ? k_eval_mixed<32, false> : k_eval_mixed<64, false>);
     P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
-    hipLaunchKernelGGL(mk, dim3(icap), dim3(64), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
+    hipLaunchKernelGGL(mk, dim3(eval_grid(icap)), dim3(64), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
                        t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(),
                        P.tsx.as<double>(), P.tsy.as<double>(), t.nlevel.as<int>(),
                        t.flags.as<int>(), t.xmid.as<double>(), t.ymid.as<double>(),
@@ -1661,7 +1690,7 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   }
   auto kern = eval_kernel(PD, P.kernel_id, ev == EVAL_POT ? 1 : 3, check);
   P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
-  hipLaunchKernelGGL(kern, dim3(icap), dim3(64), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
+  hipLaunchKernelGGL(kern, dim3(eval_grid(icap)), dim3(64), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
                      t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(), P.tsx.as<double>(),
                      P.tsy.as<double>(), t.nlevel.as<int>(), t.xmid.as<double>(),
                      t.ymid.as<double>(), t.coll.as<int>(), t.bot.as<int>(), t.top.as<int>(),
