@@ -358,10 +358,7 @@ __device__ __forceinline__ void run_phases(int nph, d4 (&acc)[PD / 16], double* 
 // multipole = pinv2 * (pinv1 * check)  (fmm.py:98-122).
 template <int PD>
 __global__ __launch_bounds__(256) void k_mult_level(int64_t row0, int nrows, int with_m2m,
-                                                   const int* __restrict__ rownode,
-                                                   const int* __restrict__ flags,
-                                                   const int* __restrict__ child,
-                                                   const int* __restrict__ row,
+                                                   const int2* __restrict__ rowinfo,
                                                    const double* __restrict__ check,
                                                    const double* __restrict__ opsL,
                                                    double* __restrict__ mult,
@@ -375,9 +372,9 @@ __global__ __launch_bounds__(256) void k_mult_level(int64_t row0, int nrows, int
   const int rl = blockIdx.x * 64 + w * 16 + (l & 15);
   const bool active = rl < nrows;
   const int64_t r = row0 + rl;
-  const int g = active ? rownode[r] : 0;
-  const int f = active ? flags[g] : 0;
-  const int crow = (active && (f & NF_CHILDREN)) ? row[child[g]] : -1;
+  const int2 ri = active ? rowinfo[r] : make_int2(-1, 0);
+  const int f = ri.y;
+  const int crow = ri.x;
   d4 acc[PD / 16];
   load_acc<PD>(acc, (active && (f & NF_UP)) ? check + r * PD : nullptr);
   const int nm = with_m2m ? 4 : 0;
@@ -721,35 +718,61 @@ __global__ __launch_bounds__(256) void k_local_level(int g0, int ngroups, int do
 // one MFMA chain per phase is PD/16 times shorter than in the kernels above
 // (which give a wave all PD rows of its 16 boxes).  A chained phase reads the
 // previous result through a double-buffered LDS tile.
-// `phase(i)` returns the i-th phase (operator, per-lane source column or
-// null, chained?) of n.
-template <int PD, class PhaseFn>
-__device__ __forceinline__ void run_small(int n, PhaseFn phase, d4& acc,
+constexpr int kMaxSmallPhases = 6;   // 4 M2M + 2 pseudo-inverse factors
+
+// Phases 0 .. nb-1 take their B column from row brow + i of `bbase`, where
+// brow is the lane's box's source row (-1: none); these rows are loaded once
+// per block, cooperatively and with 16-byte loads, into the LDS tile `btile`.
+// Later phases are chained.  `opA(i)` is the operator of phase i.
+template <int PD, int NB, class OpFn>
+__device__ __forceinline__ void run_small(int n, int nb, OpFn opA,
+                                          const double* __restrict__ bbase, int brow, d4& acc,
                                           double* __restrict__ abuf0, double* __restrict__ abuf1,
-                                          double (&bbuf)[2][PD][17]) {
-  constexpr int KS = PD / 4, LDA = Mfma<PD>::LDA, NT = 64 * (PD / 16);
+                                          double (&bbuf)[2][PD][17],
+                                          double (&btile)[NB][PD][17], int (&srow)[16]) {
+  constexpr int KS = PD / 4, LDA = Mfma<PD>::LDA, NT = 64 * (PD / 16), H = PD / 2;
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int col = l & 15, kq = l >> 4;
-  AStage<PD, NT> st;
-  st.load(phase(0).A);
-  for (int i = 0; i < n; ++i) {
-    const Phase ph = phase(i);
+  // operator blocks three phases ahead in a register ring (n <= kMaxSmallPhases)
+  AStage<PD, NT> st[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (i < n) st[i].load(opA(i));
+  if (w == 0 && kq == 0) srow[col] = brow;
+  __syncthreads();
+  for (int e = threadIdx.x; e < NB * 16 * H; e += NT) {
+    const int i = e / (16 * H), rem = e - i * 16 * H;
+    const int box = rem / H, k2 = rem - box * H;
+    const int rr = srow[box];
+    dv2 v = dv2{0.0, 0.0};
+    if (rr >= 0 && i < nb) v = *reinterpret_cast<const dv2*>(bbase + (int64_t)(rr + i) * PD + 2 * k2);
+    btile[i][2 * k2][box] = v.x;
+    btile[i][2 * k2 + 1][box] = v.y;
+  }
+#pragma unroll
+  for (int i = 0; i < kMaxSmallPhases; ++i) {
+    if (i >= n) break;
+    const bool chain = i >= nb;
     double* A = (i & 1) ? abuf1 : abuf0;
-    st.store(A);
-    if (i + 1 < n) st.load(phase(i + 1).A);
+    st[i % 3].store(A);
+    if (i + 3 < n) st[i % 3].load(opA(i + 3));
     double b[KS];
-    if (ph.chain) {
+    if (chain) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) bbuf[i & 1][16 * w + 4 * v + kq][col] = acc[v];
       acc = d4{0.0, 0.0, 0.0, 0.0};
     }
-    __syncthreads();
-    if (ph.chain) {
+    __syncthreads();   // also orders the btile fill before its first use
+    if (chain) {
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) b[ks] = bbuf[i & 1][4 * ks + kq][col];
     } else {
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) b[ks] = ph.brow ? ph.brow[4 * ks + kq] : 0.0;
+      for (int j = 0; j < NB; ++j)
+        if (j == i) {
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) b[ks] = btile[j][4 * ks + kq][col];
+        }
     }
     const double* a0 = A + (16 * w + col) * LDA + kq;
 #pragma unroll
@@ -761,8 +784,7 @@ __device__ __forceinline__ void run_small(int n, PhaseFn phase, d4& acc,
 // Upward level (as k_mult_level): rows row0 .. row0 + nrows, 16 per block.
 template <int PD>
 __global__ __launch_bounds__(64 * (PD / 16)) void k_mult_level_small(
-    int64_t row0, int nrows, int with_m2m, const int* __restrict__ rownode,
-    const int* __restrict__ flags, const int* __restrict__ child, const int* __restrict__ row,
+    int64_t row0, int nrows, int with_m2m, const int2* __restrict__ rowinfo,
     const double* __restrict__ check, const double* __restrict__ opsL, double* __restrict__ mult,
     double* __restrict__ qs, int64_t dstride) {
   __shared__ __attribute__((aligned(16))) double abuf[2][PD * Mfma<PD>::LDA];
@@ -776,21 +798,20 @@ __global__ __launch_bounds__(64 * (PD / 16)) void k_mult_level_small(
   const int rl = blockIdx.x * 16 + col;
   const bool active = rl < nrows;
   const int64_t r = row0 + rl;
-  const int g = active ? rownode[r] : 0;
-  const int f = active ? flags[g] : 0;
-  const int crow = (active && (f & NF_CHILDREN)) ? row[child[g]] : -1;
+  const int2 ri = active ? rowinfo[r] : make_int2(-1, 0);
+  const int f = ri.y;
+  const int crow = ri.x;
   const double* src = (active && (f & NF_UP)) ? check + r * PD : nullptr;
   d4 acc;
 #pragma unroll
   for (int v = 0; v < 4; ++v) acc[v] = src ? src[16 * w + 4 * v + kq] : 0.0;
   const int nm = with_m2m ? 4 : 0;
-  auto phase = [&](int k) -> Phase {
-    if (k < nm)
-      return Phase{opsL + (int64_t)(OP_M2M + k) * PD * PD,
-                   crow >= 0 ? mult + (int64_t)(crow + k) * PD : nullptr, false};
-    return Phase{opsL + (int64_t)(OP_PINV1 + (k - nm)) * PD * PD, nullptr, true};
+  auto opA = [&](int k) -> const double* {
+    return opsL + (int64_t)(k < nm ? OP_M2M + k : OP_PINV1 + (k - nm)) * PD * PD;
   };
-  run_small<PD>(nm + 2, phase, acc, abuf[0], abuf[1], bbuf);
+  __shared__ double btile[4][PD][17];
+  __shared__ int srow[16];
+  run_small<PD, 4>(nm + 2, nm, opA, mult, crow, acc, abuf[0], abuf[1], bbuf, btile, srow);
   double part = acc[0] + acc[1] + acc[2] + acc[3];
   part += __shfl_xor(part, 16, 64);
   part += __shfl_xor(part, 32, 64);
@@ -825,20 +846,25 @@ __global__ __launch_bounds__(64 * (PD / 16)) void k_local_level_small(
   const int c = blockIdx.x & 3;
   const int q = (blockIdx.x >> 2) * 16 + col;
   const int g = g0 + 4 * q + c;
-  const bool active = q < ngroups && !(flags[q < ngroups ? g : g0] & NF_FAKE);
+  const int gs = q < ngroups ? g : g0;   // a valid node for the independent loads
+  const int fl = flags[gs];
+  const int par = parent[gs];
+  const bool active = q < ngroups && !(fl & NF_FAKE);
   d4 acc;
-  const double* src = active ? ple + (int64_t)g * PD : nullptr;
+  const double* src = ple + (int64_t)gs * PD;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) acc[v] = src ? src[16 * w + 4 * v + kq] : 0.0;
-  const int pg = (active && do_l2l) ? parent[g] : -1;
+  for (int v = 0; v < 4; ++v) acc[v] = src[16 * w + 4 * v + kq];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) acc[v] = active ? acc[v] : 0.0;
+  const int pg = (active && do_l2l) ? par : -1;
   const int nl = do_l2l ? 1 : 0;
-  auto phase = [&](int k) -> Phase {
-    if (k < nl)
-      return Phase{opsParent + (int64_t)(OP_L2L + c) * PD * PD,
-                   pg >= 0 ? le + (int64_t)pg * PD : nullptr, false};
-    return Phase{opsL + (int64_t)(OP_PINV1 + (k - nl)) * PD * PD, nullptr, true};
+  auto opA = [&](int k) -> const double* {
+    return k < nl ? opsParent + (int64_t)(OP_L2L + c) * PD * PD
+                  : opsL + (int64_t)(OP_PINV1 + (k - nl)) * PD * PD;
   };
-  run_small<PD>(nl + 2, phase, acc, abuf[0], abuf[1], bbuf);
+  __shared__ double btile[1][PD][17];
+  __shared__ int srow[16];
+  run_small<PD, 1>(nl + 2, nl, opA, le, pg, acc, abuf[0], abuf[1], bbuf, btile, srow);
   if (active) {
     double* dst = le + (int64_t)g * PD;
 #pragma unroll
@@ -1522,16 +1548,14 @@ static void upward(Plan& P, const double* tau, hipStream_t s) {
     if (nrows <= kSmallLevelRows) {
       auto ks = PD == 32 ? k_mult_level_small<32> : k_mult_level_small<64>;
       hipLaunchKernelGGL(ks, dim3(ceil_div(nrows, 16), nd), dim3(64 * (PD / 16)), 0, s, r0,
-                         nrows, d < t.L - 1 ? 1 : 0, t.rownode.as<int>(), t.flags.as<int>(),
-                         t.child.as<int>(), t.row.as<int>(), P.check.as<double>(), opsL,
+                         nrows, d < t.L - 1 ? 1 : 0, t.rowinfo.as<int2>(), P.check.as<double>(), opsL,
                          P.mult.as<double>(), P.qs.as<double>(), rstride);
       FMM_LAUNCH_CHECK();
       continue;
     }
     auto kern = PD == 32 ? k_mult_level<32> : k_mult_level<64>;
     hipLaunchKernelGGL(kern, dim3(ceil_div(nrows, 64), nd), dim3(256), 0, s, r0, nrows,
-                       d < t.L - 1 ? 1 : 0, t.rownode.as<int>(), t.flags.as<int>(),
-                       t.child.as<int>(), t.row.as<int>(), P.check.as<double>(), opsL,
+                       d < t.L - 1 ? 1 : 0, t.rowinfo.as<int2>(), P.check.as<double>(), opsL,
                        P.mult.as<double>(), P.qs.as<double>(), rstride);
     FMM_LAUNCH_CHECK();
   }

@@ -8,13 +8,19 @@ namespace fmm {
 
 // ---------------------------------------------------------------------------
 // Wave / block helpers (wave64).
+// Inclusive scan with DPP moves (no LDS traffic): Hillis-Steele within each
+// row of 16 lanes (row_shr 1, 2, 4, 8), then row_bcast:15 adds the last lane
+// of rows 0 and 2 to rows 1 and 3, and row_bcast:31 adds the last lane of
+// row 1 to rows 2 and 3.  Lanes without a source (or outside the row mask)
+// receive the "old" operand 0.
 __device__ __forceinline__ int wave_inclusive_scan(int v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    int t = __shfl_up(v, off, 64);
-    if (lane >= off) v += t;
-  }
+  constexpr int kRowShr = 0x110, kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+  v += __builtin_amdgcn_update_dpp(0, v, kRowShr + 1, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kRowShr + 2, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kRowShr + 4, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kRowShr + 8, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kRowBcast15, 0xa, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kRowBcast31, 0xc, 0xf, false);
   return v;
 }
 

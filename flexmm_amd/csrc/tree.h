@@ -51,6 +51,7 @@ struct Tree {
   DevBuf xmid, ymid;
   DevBuf row;      // i32 multipole row (levels >= 2, sibling groups holding sources) or -1
   DevBuf rownode;  // i32[n_rows] node of each row
+  DevBuf rowinfo;  // int2[n_rows]: row of the node's first child (or -1), node flags
   std::vector<int64_t> row_off;  // first row of each level (size L+1)
   DevBuf m2l;      // i32[40] multipole rows of the M2L partners or -1 (real nodes, level >= 2)
   DevBuf evalid;   // i32 index among evaluation boxes (NF_LEAF) or -1

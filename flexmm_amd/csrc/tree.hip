@@ -508,6 +508,20 @@ __global__ __launch_bounds__(BS) void k_rows(const int* __restrict__ gflag,
   if (r >= 0) rownode[r] = g;
 }
 
+// Per multipole row: the row of its node's first child (-1 without children)
+// and the node's flags, so that the upward pass reads one record per row.
+__global__ __launch_bounds__(BS) void k_row_info(const int* __restrict__ rownode, int nrows,
+                                                 const int* __restrict__ flags,
+                                                 const int* __restrict__ child,
+                                                 const int* __restrict__ row,
+                                                 int2* __restrict__ info) {
+  const int r = blockIdx.x * BS + threadIdx.x;
+  if (r >= nrows) return;
+  const int g = rownode[r];
+  const int f = flags[g];
+  info[r] = make_int2((f & NF_CHILDREN) ? row[child[g]] : -1, f);
+}
+
 // --- M2L partners: children of the parent's colleagues, outside the 3x3 --------
 // (tree.py:399-435).  For child position (cx, cy) the parent's 3x3 covers
 // offsets [-2-cx, 3-cx] x [-2-cy, 3-cy].
@@ -896,6 +910,13 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   FMM_REQUIRE(t.row_off[L] == t.n_rows, FMM_E_INTERNAL, "tree: row count mismatch");
   t.n_eval = hc[CNT_EVAL];
   t.n_up = hc[CNT_UP];
+  t.rowinfo.ensure((t.n_rows + 1) * sizeof(int2));
+  if (t.n_rows > 0) {
+    hipLaunchKernelGGL(k_row_info, dim3(grid_for(t.n_rows)), dim3(BS), 0, s, t.rownode.as<int>(),
+                       (int)t.n_rows, t.flags.as<int>(), t.child.as<int>(), t.row.as<int>(),
+                       t.rowinfo.as<int2>());
+    FMM_LAUNCH_CHECK();
+  }
   t.evalnode.ensure((t.n_eval + 1) * sizeof(int));
   t.upnodes.ensure((t.n_up + 1) * sizeof(int));
   hipLaunchKernelGGL(k_list_scatter, dim3(grid_for(off)), dim3(BS), 0, s, flag, rank, NN,
