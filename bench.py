@@ -33,6 +33,20 @@ FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector = matrix)
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic():
+    """HBM bytes per launch of the FP64 evaluation kernel, from the rocprofv3
+    FETCH_SIZE/WRITE_SIZE runs summarised in profiles/ (scaled by the same
+    counters' readings for a copy kernel of known size); None if absent."""
+    path = os.path.join(HERE, "profiles", "laplace_1M_fp64_traffic.txt")
+    try:
+        for line in open(path):
+            if "k_eval traffic_bytes=" in line:
+                return float(line.split("traffic_bytes=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -361,7 +375,10 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(),
+            "traffic_note": "HBM bytes per launch of k_eval from rocprofv3 FETCH_SIZE/WRITE_SIZE "
+                            "(profiles/laplace_1M_fp64_traffic.txt); algorithmic bytes ~6.9e7 "
+                            "(targets, sources, local expansions, output)",
         },
     }
     if not args.no_extra:
