@@ -47,6 +47,10 @@ def measured_traffic():
     return None
 
 
+def progress(what):
+    print("bench: %s" % what, file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,13 +186,72 @@ def adjoint_workload(torch, steps, warmup):
         ref.append((torch.special.modified_bessel_k0(20.0 * r) * g[None, :]).sum(1) / (2 * np.pi))
     ref = torch.cat(ref)
     err = float((grads["tau"][sel] - ref).abs().max() / ref.abs().max())
+    # d/dx_s = tau_s sum_t g_t dK/dx_s, dK/dx_s = -k K1(k r)/(2 pi) (x_s - t_x)/r
+    refx = []
+    for s0 in range(0, sel.numel(), 8):
+        ss = sel[s0:s0 + 8]
+        dx = X[ss, None] - TX[None, :]
+        r = torch.sqrt(dx ** 2 + (Y[ss, None] - TY[None, :]) ** 2)
+        dk = -20.0 * torch.special.modified_bessel_k1(20.0 * r) * dx / r / (2 * np.pi)
+        refx.append(TAU[ss] * (dk * g[None, :]).sum(1))
+    refx = torch.cat(refx)
+    errx = float((grads["x"][sel] - refx).abs().max() / refx.abs().max())
     return {"workload": "modified Helmholtz K0(20 r)/(2 pi), 500K sources on a circle, 200K "
                         "uniform targets, Nequiv 60, Ncutoff 100, loss ||u - u_true||^2",
             "forward_ms": round(ms_f, 4), "forward_adjoint_ms": round(ms_fb, 4),
             "adjoint_over_forward": round(ms_fb / ms_f, 3),
             "forward_Mpts_s": round(nt / ms_f / 1e3, 3),
             "dcharges_error_vs_direct": err,
-            "error_sample": "d loss/d charges at 2000 sampled sources vs FP64 direct sum"}
+            "dpositions_error_vs_direct": errx,
+            "error_sample": "d loss/d charges and d loss/d x at 2000 sampled sources vs FP64 "
+                            "direct sums"}
+
+
+def clustered_workload(torch, steps, warmup):
+    """Adaptive tree: 1,000,000 sources in 50 isotropic Gaussian clusters
+    (centres uniform in [0.1, 0.9]^2, sigma 0.005, clipped to the unit
+    square), uniform(-1, 1) charges, the 1000 x 1000 grid targets, Nequiv 60,
+    Ncutoff 100; FP64 and mixed forms, errors vs FP64 direct sums at 2,000
+    grid targets."""
+    from flexmm_amd import kernels
+    from flexmm_amd.kifmm2d import FMM
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(3)
+    c = rng.uniform(0.1, 0.9, (50, 2))
+    pts = np.clip(c[np.repeat(np.arange(50), 20_000)]
+                  + 0.005 * rng.standard_normal((1_000_000, 2)), 0.0, 1.0)
+    tau = rng.uniform(-1.0, 1.0, 1_000_000)
+    g = np.linspace(0.0, 1.0, GRID)
+    tx, ty = np.meshgrid(g, g, indexing="ij")
+    T = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
+    X, Y, TAU, TX, TY = T(pts[:, 0]), T(pts[:, 1]), T(tau), T(tx.ravel()), T(ty.ravel())
+    box = [0.0, 1.0, 0.0, 1.0]
+    sel = torch.from_numpy(np.sort(rng.choice(TX.numel(), 2000, replace=False))).to(dev)
+    ref = direct_sample(torch, X, Y, TAU, TX[sel], TY[sel])
+    res = {"workload": "Laplace KIFMM, 1M sources in 50 Gaussian clusters (sigma 0.005), "
+                       "uniform(-1,1) charges, 1M grid targets, Nequiv 60, Ncutoff 100"}
+    us = {}
+    for prec in ("fp64", "mixed"):
+        out = {}
+
+        def step():
+            f = FMM(X, Y, kernels.laplace, NCUT, P, bbox=box, precision=prec)
+            f.build_expansions(TAU)
+            out["u"] = f.target_evaluation(TX, TY)[0]
+            out["levels"] = int(f.tree.levels)
+            f.close()
+
+        ms = _timed(torch, step, steps, warmup)
+        u = out["u"]
+        us[prec] = u
+        res["tree_levels"] = out["levels"]
+        res[prec] = {"ms_per_step": round(ms, 4),
+                     "value": round(TX.numel() / ms / 1e3, 3), "unit": "Mpts/s",
+                     "error_vs_direct": float((u[sel] - ref).abs().max() / ref.abs().max())}
+    res["mixed"]["error_vs_fp64"] = float((us["mixed"] - us["fp64"]).abs().max()
+                                          / us["fp64"].abs().max())
+    res["mixed"]["speedup_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
+    return res
 
 
 def stokes_workload(torch, steps, warmup):
@@ -313,8 +376,10 @@ def main():
                               value=round(r["value"], 3), error_vs_direct=r["err"],
                               phase_ms={k: round(v, 4) for k, v in r["phases"].items()})))
         return
+    progress("fp64 form")
     r64 = measure("fp64")
     u64 = r64.pop("u")
+    progress("mixed form")
     rmx = measure("mixed")
     umx = rmx.pop("u")
     err_mx64 = float((umx - u64).abs().max() / u64.abs().max())
@@ -382,9 +447,14 @@ def main():
         },
     }
     if not args.no_extra:
+        progress("clustered sources")
+        out["clustered"] = clustered_workload(torch, 5, 2)
+        progress("modified Helmholtz adjoint")
         out["adjoint"] = adjoint_workload(torch, 5, 2)
+        progress("Stokeslet")
         out["stokes"] = stokes_workload(torch, 3, 1)
     if not args.no_cpu_baseline:
+        progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample_targets)
     print(json.dumps(out))
 

@@ -1019,6 +1019,97 @@ struct Target {
   __device__ __forceinline__ bool zhit() const { return dmin == 0u; }
 };
 
+// Items with few targets (a leaf holding at most kFewTargets of them, or the
+// tail chunk of a larger one): instead of one lane per target, which leaves
+// most lanes idle while the wave walks every near source, lane l works on
+// target slot l mod TG (TG = the next power of two >= the item's targets) and
+// takes every (64/TG)-th source and L2T term; the partial sums of a slot are
+// then added by XOR butterflies over the lanes that share it, in a fixed
+// order.  All results are linear in the partial sums.
+constexpr int kFewTargets = 32;
+
+struct FewLanes {
+  int lg, ng, slot, sg;
+  bool act;
+};
+__device__ __forceinline__ FewLanes few_lanes(int na) {
+  FewLanes f;
+  f.lg = na <= 1 ? 0 : 32 - __builtin_clz(na - 1);
+  f.ng = 64 >> f.lg;
+  f.slot = threadIdx.x & ((1 << f.lg) - 1);
+  f.sg = threadIdx.x >> f.lg;
+  f.act = f.slot < na;
+  return f;
+}
+// sum over the lanes that share this lane's target slot
+__device__ __forceinline__ double slot_sum(double v, int lg) {
+  for (int off = 1 << lg; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ uint32_t slot_min(uint32_t v, int lg) {
+  for (int off = 1 << lg; off < 64; off <<= 1)
+    v = min(v, (uint32_t)__shfl_xor((int)v, off, 64));
+  return v;
+}
+
+template <int PD, int KER, int NOUT, bool CHECK>
+__device__ __forceinline__ void eval_few(
+    int g, int t0, int na, const int* __restrict__ tperm, const double* __restrict__ tsx,
+    const double* __restrict__ tsy, const int* __restrict__ nlevel,
+    const double* __restrict__ xmid, const double* __restrict__ ymid,
+    const int* __restrict__ coll, const int* __restrict__ bot, const int* __restrict__ top,
+    const double4* __restrict__ src, const double* __restrict__ le,
+    const double* __restrict__ ops, int64_t opsz, int p, const double2* __restrict__ tab,
+    double kp, int64_t nt, double* __restrict__ out) {
+  const FewLanes F = few_lanes(na);
+  const int t = t0 + F.slot;
+  Target<KER, NOUT, CHECK> T;
+  T.kp = kp;
+  T.x = F.act ? tsx[t] : 0.0;
+  T.y = F.act ? tsy[t] : 0.0;
+  const int d = nlevel[g];
+  if (d >= 2) {
+    Target<KER, NOUT, false> L;
+    L.kp = kp;
+    L.x = T.x - xmid[g];
+    L.y = T.y - ymid[g];
+    const double* lx = ops + d * opsz + (int64_t)OP_NBLK * PD * PD;
+    const double* ly = lx + PD;
+    const double* lg = le + (int64_t)g * PD;
+    for (int k = F.sg; k < p; k += F.ng) L.pair(lx[k], ly[k], lg[k], tab);
+    T.u = L.u;
+    T.gx = L.gx;
+    T.gy = L.gy;
+  }
+  const int* cl = coll + (int64_t)g * 9;
+  for (int qn = 0; qn < 9; ++qn) {
+    const int c = cl[qn];
+    if (c < 0) continue;
+    const int b = __builtin_amdgcn_readfirstlane(bot[c]);
+    const int e = __builtin_amdgcn_readfirstlane(top[c]);
+    for (int j = b + F.sg; j < e; j += F.ng) {
+      const double4 sj = src[j];
+      T.pair(sj.x, sj.y, sj.z, tab);
+    }
+  }
+  double u = slot_sum(T.u.result(), F.lg);
+  double gx = 0.0, gy = 0.0;
+  if (NOUT == 3) {
+    gx = slot_sum(T.gx, F.lg);
+    gy = slot_sum(T.gy, F.lg);
+  }
+  const bool zh = CHECK ? false : slot_min(T.dmin, F.lg) == 0u;
+  if (F.act && F.sg == 0) {
+    const int64_t o = tperm[t];
+    if (zh) u = __builtin_inf();
+    out[o] = u;
+    if (NOUT == 3) {
+      out[nt + o] = T.gscale() * gx;
+      out[2 * nt + o] = T.gscale() * gy;
+    }
+  }
+}
+
 template <int PD, int KER, int NOUT, bool CHECK>
 __global__ __launch_bounds__(64) void k_eval(
     const int2* __restrict__ items, const int* __restrict__ nitems,
@@ -1039,6 +1130,11 @@ __global__ __launch_bounds__(64) void k_eval(
     const int g = evalnode[it.x];
     const int t0 = toff[it.x] + it.y * kTgtChunk;
     const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
+    if (t1 - t0 <= kFewTargets) {
+      eval_few<PD, KER, NOUT, CHECK>(g, t0, t1 - t0, tperm, tsx, tsy, nlevel, xmid, ymid, coll,
+                                     bot, top, src, le, ops, opsz, p, tab, kp, nt, out);
+      continue;
+    }
     const int t = t0 + threadIdx.x;
     const bool act = t < t1;
     Target<KER, NOUT, CHECK> T;
@@ -1132,6 +1228,78 @@ __global__ __launch_bounds__(64) void k_prep_mixed(const int* __restrict__ evaln
   if (threadIdx.x == 0) tsum[g] = ts;
 }
 
+// Few-target items of k_eval_mixed (lane layout of eval_few).  The
+// log(w^2) * sum(tau) term of a leaf colleague is added once per target, by
+// the lanes of source group 0; every lane removes the charges it took in FP64.
+template <int PD, bool CHECK>
+__device__ __forceinline__ void eval_few_mixed(
+    int g, int t0, int na, const int* __restrict__ tperm, const double* __restrict__ tsx,
+    const double* __restrict__ tsy, const int* __restrict__ nlevel, const int* __restrict__ flags,
+    const double* __restrict__ xmid, const double* __restrict__ ymid,
+    const int* __restrict__ coll, const int* __restrict__ bot, const int* __restrict__ top,
+    const double4* __restrict__ src, const f4* __restrict__ srcf, const double* __restrict__ tsum,
+    const LevelConsts& lc, const double* __restrict__ le, const double* __restrict__ ops,
+    int64_t opsz, int p, const double2* __restrict__ tab, double* __restrict__ out) {
+  const FewLanes F = few_lanes(na);
+  const int t = t0 + F.slot;
+  Target<KER_LAPLACE, 1, CHECK> T;
+  T.kp = 0.0;
+  T.x = F.act ? tsx[t] : 0.0;
+  T.y = F.act ? tsy[t] : 0.0;
+  const int d = nlevel[g];
+  if (d >= 2) {
+    Target<KER_LAPLACE, 1, false> L;
+    L.kp = 0.0;
+    L.x = T.x - xmid[g];
+    L.y = T.y - ymid[g];
+    const double* lx = ops + d * opsz + ops_off_circles(PD);
+    const double* ly = lx + PD;
+    const double* lg = le + (int64_t)g * PD;
+    for (int k = F.sg; k < p; k += F.ng) L.pair(lx[k], ly[k], lg[k], tab);
+    T.u = L.u;
+  }
+  const double iw = lc.invw[d];
+  const double lw2 = lc.logw2[d];
+  double u32 = 0.0;
+  const int* cl = coll + (int64_t)g * 9;
+  for (int qn = 0; qn < 9; ++qn) {
+    const int c = cl[qn];
+    if (c < 0) continue;
+    const int b = __builtin_amdgcn_readfirstlane(bot[c]);
+    const int e = __builtin_amdgcn_readfirstlane(top[c]);
+    if (e <= b) continue;
+    if (flags[c] & NF_LEAF) {
+      const float xr = (float)((T.x - xmid[c]) * iw);
+      const float yr = (float)((T.y - ymid[c]) * iw);
+      float acc = 0.f;
+      double excl = 0.0;
+      for (int j = b + F.sg; j < e; j += F.ng) {
+        const f4 sj = srcf[j];
+        const float dx = xr - sj.x, dy = yr - sj.y;
+        const float d2 = __builtin_fmaf(dy, dy, dx * dx);
+        const bool nr = d2 < kNearF32;
+        const float lg2 = __builtin_amdgcn_logf(nr ? 1.0f : d2);
+        acc = __builtin_fmaf(nr ? 0.f : sj.z, lg2, acc);
+        if (nr) {
+          const double4 sd = src[j];
+          excl += sd.z;
+          T.pair(sd.x, sd.y, sd.z, tab);
+        }
+      }
+      u32 += (double)acc;
+      T.u.a.at = __builtin_fma(lw2, (F.sg == 0 ? tsum[c] : 0.0) - excl, T.u.a.at);
+    } else {
+      for (int j = b + F.sg; j < e; j += F.ng) {
+        const double4 sj = src[j];
+        T.pair(sj.x, sj.y, sj.z, tab);
+      }
+    }
+  }
+  double u = slot_sum(-kInv4Pi * (__builtin_fma(u32, kLn2, T.u.a.value())), F.lg);
+  const bool zh = CHECK ? false : slot_min(T.dmin, F.lg) == 0u;
+  if (F.act && F.sg == 0) out[tperm[t]] = zh ? __builtin_inf() : u;
+}
+
 template <int PD, bool CHECK>
 __global__ __launch_bounds__(64) void k_eval_mixed(
     const int2* __restrict__ items, const int* __restrict__ nitems,
@@ -1154,6 +1322,11 @@ __global__ __launch_bounds__(64) void k_eval_mixed(
     const int g = evalnode[it.x];
     const int t0 = toff[it.x] + it.y * kTgtChunk;
     const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
+    if (t1 - t0 <= kFewTargets) {
+      eval_few_mixed<PD, CHECK>(g, t0, t1 - t0, tperm, tsx, tsy, nlevel, flags, xmid, ymid, coll,
+                                bot, top, src, srcf, tsum, lc, le, ops, opsz, p, tab, out);
+      continue;
+    }
     const int t = t0 + threadIdx.x;
     const bool act = t < t1;
     Target<KER_LAPLACE, 1, CHECK> T;
@@ -1337,8 +1510,13 @@ __global__ __launch_bounds__(64) void k_eval_stokes(
     const int g = evalnode[it.x];
     const int t0 = toff[it.x] + it.y * kTgtChunk;
     const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
-    const int t = t0 + threadIdx.x;
-    const bool act = t < t1;
+    // lane layout: one lane per target, or (target slot, source group) for
+    // items with few targets (see eval_few)
+    const int na = t1 - t0;
+    const FewLanes F = na <= kFewTargets ? few_lanes(na)
+                                         : FewLanes{6, 1, (int)threadIdx.x, 0, t0 + (int)threadIdx.x < t1};
+    const int t = t0 + F.slot;
+    const bool act = F.act;
     const double x = act ? tsx[t] : 0.0;
     const double y = act ? tsy[t] : 0.0;
     double u = 0.0, v = 0.0, pr = 0.0;
@@ -1350,7 +1528,7 @@ __global__ __launch_bounds__(64) void k_eval_stokes(
       const double xr = x - xmid[g], yr = y - ymid[g];
       LogAcc l0, l1;
       double B = 0.0, Cx = 0.0, Cy = 0.0;
-      for (int k = 0; k < p; ++k) {
+      for (int k = F.sg; k < p; k += F.ng) {
         const double dx = xr - lx[k], dy = yr - ly[k];
         const double d2 = __builtin_fma(dy, dy, dx * dx);
         const double id2 = 1.0 / d2;
@@ -1379,7 +1557,7 @@ __global__ __launch_bounds__(64) void k_eval_stokes(
       if (c < 0) continue;
       const int b = __builtin_amdgcn_readfirstlane(bot[c]);
       const int e = __builtin_amdgcn_readfirstlane(top[c]);
-      for (int j = b; j < e; ++j) {
+      for (int j = b + F.sg; j < e; j += F.ng) {
         const double4 sj = src[j];
         const double dx = x - sj.x, dy = y - sj.y;
         double d2 = __builtin_fma(dy, dy, dx * dx);
@@ -1401,11 +1579,19 @@ __global__ __launch_bounds__(64) void k_eval_stokes(
         sp += rf;
       }
     }
-    if (act) {
+    u += kInv4Pi * (-0.5 * lx_.value() + su);
+    v += kInv4Pi * (-0.5 * ly_.value() + sv);
+    pr += kInv2Pi * sp;
+    if (F.ng > 1) {   // wave-uniform
+      u = slot_sum(u, F.lg);
+      v = slot_sum(v, F.lg);
+      pr = slot_sum(pr, F.lg);
+      if (!CHECK)
+        for (int off = 1 << F.lg; off < 64; off <<= 1)
+          dmin = __builtin_fmin(dmin, __shfl_xor(dmin, off, 64));
+    }
+    if (act && F.sg == 0) {
       const int64_t o = tperm[t];
-      u += kInv4Pi * (-0.5 * lx_.value() + su);
-      v += kInv4Pi * (-0.5 * ly_.value() + sv);
-      pr += kInv2Pi * sp;
       if (!CHECK && dmin == 0.0) u = v = pr = __builtin_nan("");
       out[o] = u;
       out[nt + o] = v;

@@ -65,12 +65,13 @@ void exclusive_scan_i32(const int* in, int* out, int64_t n, int* total_dev, Prim
                         hipStream_t s);
 
 // Sort (keys, vals) of length n by (key, val) ascending, in place.  Segments
-// whose length drops to <= threshold are finished by an LDS bitonic sort; the
-// radix part looks at 8 key bits per round, most significant first.  Keys
-// that agree in all 64 bits keep an unspecified order when more than
-// kLocalSortMax of them coincide.
+// whose length drops to <= threshold are finished by an LDS sort; the radix
+// part looks at 8 key bits per round, most significant first, over the top
+// key_bits bits (the others must be equal in all keys).  Keys that agree in
+// all their bits keep an unspecified order when more than kLocalSortMax of
+// them coincide.
 constexpr int kLocalSortMax = 2048;
 void sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, PrimWork& w,
-                    hipStream_t s);
+                    hipStream_t s, int key_bits = 64);
 
 }  // namespace fmm

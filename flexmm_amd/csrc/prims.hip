@@ -422,7 +422,7 @@ __global__ __launch_bounds__(SORT_BS) void k_local_sort(uint64_t* __restrict__ k
 }  // namespace
 
 void sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, PrimWork& w,
-                    hipStream_t s) {
+                    hipStream_t s, int key_bits) {
   if (n <= 1) return;
   FMM_REQUIRE(n < (int64_t)1 << 31, FMM_E_LIMIT, "sort: n = %lld exceeds 2^31-1", (long long)n);
   int T = threshold < 1 ? 1 : threshold;
@@ -454,7 +454,8 @@ void sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
   } else {
     nbig = 1;
   }
-  for (int r = 0; r < 8 && nseg > 0; ++r) {
+  const int rounds = key_bits < 8 ? 1 : (key_bits > 64 ? 8 : (key_bits + 7) / 8);
+  for (int r = 0; r < rounds && nseg > 0; ++r) {
     const int shift = 56 - 8 * r;
     const int64_t cbound = nseg + active / CHUNK + 1;
     w.chunks.ensure(cbound * sizeof(int4));
@@ -471,7 +472,7 @@ void sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
                        cnt, shift, w.chunkHist.as<int>());
     FMM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_seg_offsets, dim3(nseg), dim3(SEG_BS), 0, s, segCur,
-                       w.segChunk0.as<int>(), w.chunkHist.as<int>(), T, r == 7 ? 1 : 0, segNext,
+                       w.segChunk0.as<int>(), w.chunkHist.as<int>(), T, r == rounds - 1 ? 1 : 0, segNext,
                        w.fin.as<int2>(), finBigOff, cnt);
     FMM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_chunk_scatter, dim3(cbound), dim3(SORT_BS), 0, s, keys, vals,

@@ -66,6 +66,9 @@ struct Tree {
   HostBuf h_up;    // pinned staging for small host->device uploads
   std::vector<int64_t> s_off;   // host offsets of the split-cell lists per depth
   DevBuf s_all, sref_all;       // u64 split-cell codes per depth (concatenated)
+  DevBuf bal_bm, bal_pc;        // balance: cell bitmaps per depth, popcount scan
+  DevBuf bal_k, bal_v, bal_f;   // balance below the bitmap depths: candidates
+  std::vector<DevBuf> bal_lists;
   DevBuf p_code;                // u64 codes of nodes with children, per level (concatenated)
   DevBuf p_kind;                // i32 0 = split, 1 = X-list
   std::vector<int64_t> p_off;

@@ -278,6 +278,100 @@ __global__ __launch_bounds__(BS) void k_unique_scatter(const uint64_t* __restric
   out[rank[i]] = cand[i] >> (64 - 2 * d);
 }
 
+// Balance on bitmaps: bit c of depth d's bitmap is cell c (the depth-d Morton
+// code); each depth's words start at a multiple of BS, so a block of BS words
+// lies in one depth.  Marking is an idempotent OR, so the union needs neither
+// sort nor unique, and the bitmap order is the code order.
+constexpr int kBmDepth = 13;   // 4^13 bits = 8 MB at the deepest bitmap depth
+struct BmOffsets {
+  int64_t wo[kMaxLevels + 1];  // first word of depth d (depths 1 .. dmax)
+};
+__device__ __forceinline__ int bm_depth(const BmOffsets& o, int dmax, int64_t w) {
+  int d = 1;
+  while (d < dmax && w >= o.wo[d + 1]) ++d;
+  return d;
+}
+__device__ __forceinline__ void bm_set(uint32_t* bm, uint64_t c) {
+  atomicOr(bm + (c >> 5), 1u << (c & 31));
+}
+// the 3x3 neighbourhood (clipped to the domain) of parent cell pc at depth d
+__device__ __forceinline__ void bm_mark_nbrs(uint32_t* bm, uint64_t pc, int d) {
+  uint32_t ix, iy;
+  demorton(pc, &ix, &iy);
+  const int64_t lim = (int64_t)1 << d;
+  for (int k = 0; k < 9; ++k) {
+    const int64_t nx = (int64_t)ix + k / 3 - 1, ny = (int64_t)iy + k % 3 - 1;
+    if (nx < 0 || ny < 0 || nx >= lim || ny >= lim) continue;
+    bm_set(bm, morton((uint32_t)nx, (uint32_t)ny));
+  }
+}
+// Sref_d for depths 1 .. dmax into their bitmaps
+__global__ __launch_bounds__(BS) void k_bm_mark(const uint64_t* __restrict__ sref,
+                                                const int64_t* __restrict__ soff, int dmax,
+                                                BmOffsets o, uint32_t* __restrict__ bm) {
+  const int64_t e = soff[1] + (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (e >= soff[dmax + 1]) return;
+  int d = 1;
+  while (e >= soff[d + 1]) ++d;
+  bm_set(bm + o.wo[d], sref[e]);
+}
+// N(parents) of the depth-(d+1) cells in bitmap bm1 into depth d's bitmap bm0:
+// the 4 children of parent w*8+q are bits 4q..4q+3 of word w
+__global__ __launch_bounds__(BS) void k_bm_spread(const uint32_t* __restrict__ bm1, int64_t nw1,
+                                                  int d, uint32_t* __restrict__ bm0) {
+  const int64_t w = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (w >= nw1) return;
+  const uint32_t v = bm1[w];
+  if (!v) return;
+  for (int q = 0; q < 8; ++q)
+    if ((v >> (4 * q)) & 0xFu) bm_mark_nbrs(bm0, (uint64_t)w * 8 + q, d);
+}
+// the same from a sorted list of depth-(d+1) codes
+__global__ __launch_bounds__(BS) void k_bm_spread_list(const uint64_t* __restrict__ lst,
+                                                       int64_t n, int d,
+                                                       uint32_t* __restrict__ bm0) {
+  const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t c = lst[i];
+  if (i > 0 && (lst[i - 1] >> 2) == (c >> 2)) return;
+  bm_mark_nbrs(bm0, c >> 2, d);
+}
+// popcount per word (for the scan) and per depth
+__global__ __launch_bounds__(BS) void k_bm_count(const uint32_t* __restrict__ bm, BmOffsets o,
+                                                 int dmax, int* __restrict__ pc,
+                                                 int* __restrict__ dcnt) {
+  __shared__ int red[BS / 64];
+  const int64_t w = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const int c = __popc(bm[w]);
+  pc[w] = c;
+  int v = c;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int i = 0; i < BS / 64; ++i) t += red[i];
+    if (t) atomicAdd(dcnt + bm_depth(o, dmax, (int64_t)blockIdx.x * BS), t);
+  }
+}
+// set bits in word order -> codes, at out[pos[w] ...]
+__global__ __launch_bounds__(BS) void k_bm_scatter(const uint32_t* __restrict__ bm, BmOffsets o,
+                                                   int dmax, const int* __restrict__ pos,
+                                                   uint64_t* __restrict__ out) {
+  const int64_t w = (int64_t)blockIdx.x * BS + threadIdx.x;
+  uint32_t v = bm[w];
+  if (!v) return;
+  const int d = bm_depth(o, dmax, w);
+  const uint64_t base = (uint64_t)(w - o.wo[d]) * 32;
+  int p = pos[w];
+  while (v) {
+    const int b = __builtin_ctz(v);
+    out[p++] = base + (uint64_t)b;
+    v &= v - 1;
+  }
+}
+
 // --- nodes with children (split or X-list) -----------------------------------
 // Real nodes of level d are the children of S_{d-1} (the root for d = 0).
 __device__ __forceinline__ void real_flags_one(const uint64_t* __restrict__ s_all,
@@ -353,7 +447,9 @@ __device__ __forceinline__ int node_of(const uint64_t* __restrict__ code,
 
 // Source ranges of all nodes from the sorted keys: point i starts its cells
 // of depths s(i)+1.. and ends those of depths s(i+1)+1.. (s = common depth
-// with the previous key).  Cells without points keep -1 (fixed below).
+// with the previous key).  Cells without points keep -1 (fixed below).  A
+// cell exists only below an existing parent, so each walk stops at the first
+// depth without one.
 __global__ __launch_bounds__(BS) void k_cell_ranges(const uint64_t* __restrict__ keys,
                                                     const int2* __restrict__ sc, int n,
                                                     const uint64_t* __restrict__ code,
@@ -367,11 +463,13 @@ __global__ __launch_bounds__(BS) void k_cell_ranges(const uint64_t* __restrict__
   const uint64_t k = keys[i];
   for (int d = max(1, s0 + 1); d <= L - 1; ++d) {
     const int g = node_of(code, levoff, d, k >> (64 - 2 * d));
-    if (g >= 0) bot[g] = i;
+    if (g < 0) break;
+    bot[g] = i;
   }
   for (int d = max(1, s1 + 1); d <= L - 1; ++d) {
     const int g = node_of(code, levoff, d, k >> (64 - 2 * d));
-    if (g >= 0) top[g] = i + 1;
+    if (g < 0) break;
+    top[g] = i + 1;
   }
 }
 
@@ -642,7 +740,7 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
 
   // -- counters ------------------------------------------------------------------
   enum { CNT_D = 0, CNT_P = 32, CNT_R = 33, CNT_PC = 34, CNT_ROWS = 66, CNT_EVAL = 67,
-         CNT_UP = 68, CNT_UNIQ = 69, CNT_LR = 72, CNT_N = 104 };
+         CNT_UP = 68, CNT_UNIQ = 69, CNT_LR = 72, CNT_BAL = 104, CNT_N = 136 };
   t.w_cnt.ensure(CNT_N * sizeof(int));
   t.h_cnt.ensure(CNT_N * sizeof(int));
   int* cnt = t.w_cnt.as<int>();
@@ -730,55 +828,101 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
       FMM_HIP(hipMemcpyAsync(t.s_all.ptr, t.sref_all.ptr, ns_total * sizeof(uint64_t),
                              hipMemcpyDeviceToDevice, s));
   } else {
-    // Bottom-up: S_d = sort_unique(Sref_d u N(parents(S_{d+1}))) for d = L-3 .. 1.
-    std::vector<DevBuf> lists(L);
+    // Bottom-up: S_d = Sref_d u N(parents(S_{d+1})) for d = L-3 .. 1, with
+    // S_0 = Sref_0 and S_{L-2} = Sref_{L-2}.  Depths above kBmDepth (very deep
+    // trees) build sorted candidate lists; the others are bitmaps.
+    const int DM = std::min(L - 3, kBmDepth);
     std::vector<int64_t> sz(L, 0);
     for (int d = 0; d < L; ++d) sz[d] = t.s_off[d + 1] - t.s_off[d];
-    auto copy_ref = [&](int d) {
-      lists[d].ensure((sz[d] + 1) * sizeof(uint64_t));
-      if (sz[d] > 0)
-        FMM_HIP(hipMemcpyAsync(lists[d].ptr, t.sref_all.as<uint64_t>() + t.s_off[d],
-                               sz[d] * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
-    };
-    copy_ref(0);
-    if (L - 2 >= 1) copy_ref(L - 2);
-    for (int d = L - 3; d >= 1; --d) {
-      const int64_t nref = t.s_off[d + 1] - t.s_off[d];
+    const uint64_t* sref = t.sref_all.as<uint64_t>();
+    std::vector<DevBuf>& lists = t.bal_lists;
+    if ((int)lists.size() < L) lists.resize(L);
+    for (int d = L - 3; d > DM; --d) {
+      const uint64_t* nxt = d + 1 == L - 2 ? sref + t.s_off[L - 2] : lists[d + 1].as<uint64_t>();
+      const int64_t nref = sz[d];
       const int64_t nnext = sz[d + 1];
       const int64_t nc = nref + 9 * nnext;
-      DevBuf candk, candv, fl, rk;
-      candk.ensure((nc + 1) * sizeof(uint64_t));
-      candv.ensure((nc + 1) * sizeof(uint32_t));
-      fl.ensure((nc + 1) * sizeof(int));
-      rk.ensure((nc + 1) * sizeof(int));
+      t.bal_k.ensure((nc + 1) * sizeof(uint64_t));
+      t.bal_v.ensure((nc + 1) * sizeof(uint32_t));
+      t.bal_f.ensure((nc + 1) * 2 * sizeof(int));
+      uint64_t* candk = t.bal_k.as<uint64_t>();
+      int* fl = t.bal_f.as<int>();
+      int* rk = fl + (nc + 1);
       hipLaunchKernelGGL(k_bal_candidates, dim3(grid_for(nc)), dim3(BS), 0, s,
-                         t.sref_all.as<uint64_t>() + t.s_off[d], nref,
-                         lists[d + 1].as<uint64_t>(), nnext, d, candk.as<uint64_t>(),
-                         candv.as<uint32_t>());
+                         sref + t.s_off[d], nref, nxt, nnext, d, candk, t.bal_v.as<uint32_t>());
       FMM_LAUNCH_CHECK();
-      sort_pairs_u64(candk.as<uint64_t>(), candv.as<uint32_t>(), nc, 1024, t.pw, s);
-      hipLaunchKernelGGL(k_unique_flags, dim3(grid_for(nc)), dim3(BS), 0, s, candk.as<uint64_t>(),
-                         nc, fl.as<int>());
+      sort_pairs_u64(candk, t.bal_v.as<uint32_t>(), nc, 1024, t.pw, s, 2 * d);
+      hipLaunchKernelGGL(k_unique_flags, dim3(grid_for(nc)), dim3(BS), 0, s, candk, nc, fl);
       FMM_LAUNCH_CHECK();
-      exclusive_scan_i32(fl.as<int>(), rk.as<int>(), nc, cnt + CNT_UNIQ, t.pw, s);
+      exclusive_scan_i32(fl, rk, nc, cnt + CNT_UNIQ, t.pw, s);
       FMM_HIP(hipMemcpyAsync(hc + CNT_UNIQ, cnt + CNT_UNIQ, sizeof(int), hipMemcpyDeviceToHost, s));
       FMM_HIP(hipStreamSynchronize(s));
       sz[d] = hc[CNT_UNIQ];
       lists[d].ensure((sz[d] + 1) * sizeof(uint64_t));
-      hipLaunchKernelGGL(k_unique_scatter, dim3(grid_for(nc)), dim3(BS), 0, s,
-                         candk.as<uint64_t>(), fl.as<int>(), rk.as<int>(), nc, d,
-                         lists[d].as<uint64_t>());
+      hipLaunchKernelGGL(k_unique_scatter, dim3(grid_for(nc)), dim3(BS), 0, s, candk, fl, rk, nc,
+                         d, lists[d].as<uint64_t>());
       FMM_LAUNCH_CHECK();
-      FMM_HIP(hipStreamSynchronize(s));
     }
+    BmOffsets bo{};
+    int64_t nwords = 0;
+    for (int d = 1; d <= DM; ++d) {
+      bo.wo[d] = nwords;
+      const int64_t w = std::max<int64_t>(1, ((int64_t)1 << (2 * d)) / 32);
+      nwords += ceil_div(w, BS) * BS;
+    }
+    bo.wo[DM + 1] = nwords;
+    if (DM >= 1) {
+      t.bal_bm.ensure(nwords * sizeof(uint32_t));
+      t.bal_pc.ensure(nwords * sizeof(int));
+      uint32_t* bm = t.bal_bm.as<uint32_t>();
+      FMM_HIP(hipMemsetAsync(bm, 0, nwords * sizeof(uint32_t), s));
+      const int64_t nmark = t.s_off[DM + 1] - t.s_off[1];
+      if (nmark > 0) {
+        hipLaunchKernelGGL(k_bm_mark, dim3(grid_for(nmark)), dim3(BS), 0, s, sref, d_soff, DM, bo,
+                           bm);
+        FMM_LAUNCH_CHECK();
+      }
+      for (int d = DM; d >= 1; --d) {
+        if (d + 1 <= DM) {
+          const int64_t nw1 = bo.wo[d + 2] - bo.wo[d + 1];
+          hipLaunchKernelGGL(k_bm_spread, dim3(grid_for(nw1)), dim3(BS), 0, s, bm + bo.wo[d + 1],
+                             nw1, d, bm + bo.wo[d]);
+        } else {
+          const uint64_t* lst =
+              d + 1 == L - 2 ? sref + t.s_off[L - 2] : lists[d + 1].as<uint64_t>();
+          if (sz[d + 1] == 0) continue;
+          hipLaunchKernelGGL(k_bm_spread_list, dim3(grid_for(sz[d + 1])), dim3(BS), 0, s, lst,
+                             sz[d + 1], d, bm + bo.wo[d]);
+        }
+        FMM_LAUNCH_CHECK();
+      }
+      hipLaunchKernelGGL(k_bm_count, dim3(nwords / BS), dim3(BS), 0, s, bm, bo, DM,
+                         t.bal_pc.as<int>(), cnt + CNT_BAL);
+      FMM_LAUNCH_CHECK();
+      exclusive_scan_i32(t.bal_pc.as<int>(), t.bal_pc.as<int>(), nwords, nullptr, t.pw, s);
+      FMM_HIP(hipMemcpyAsync(hc + CNT_BAL, cnt + CNT_BAL, kMaxLevels * sizeof(int),
+                             hipMemcpyDeviceToHost, s));
+    }
+    FMM_HIP(hipStreamSynchronize(s));
+    for (int d = 1; d <= DM; ++d) sz[d] = hc[CNT_BAL + d];
     soff.assign(L + 1, 0);
     for (int d = 0; d < L; ++d) soff[d + 1] = soff[d] + (d <= L - 2 ? sz[d] : 0);
     t.s_all.ensure((soff[L] + 1) * sizeof(uint64_t));
-    for (int d = 0; d <= L - 2; ++d)
+    uint64_t* sall = t.s_all.as<uint64_t>();
+    if (sz[0] > 0)
+      FMM_HIP(hipMemcpyAsync(sall, sref, sz[0] * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    if (DM >= 1 && soff[DM + 1] > soff[1]) {
+      hipLaunchKernelGGL(k_bm_scatter, dim3(nwords / BS), dim3(BS), 0, s, t.bal_bm.as<uint32_t>(),
+                         bo, DM, t.bal_pc.as<int>(), sall + soff[1]);
+      FMM_LAUNCH_CHECK();
+    }
+    for (int d = DM + 1; d <= L - 3; ++d)
       if (sz[d] > 0)
-        FMM_HIP(hipMemcpyAsync(t.s_all.as<uint64_t>() + soff[d], lists[d].ptr,
-                               sz[d] * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
-    FMM_HIP(hipStreamSynchronize(s));
+        FMM_HIP(hipMemcpyAsync(sall + soff[d], lists[d].ptr, sz[d] * sizeof(uint64_t),
+                               hipMemcpyDeviceToDevice, s));
+    if (L - 2 >= 1 && sz[L - 2] > 0)
+      FMM_HIP(hipMemcpyAsync(sall + soff[L - 2], sref + t.s_off[L - 2],
+                             sz[L - 2] * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
   }
   t.s_off = soff;
 

@@ -27,6 +27,10 @@ CASES = {
     "uniform_100k_c1": lambda: (*H.uniform(100000, 1)[:2], 64, None),
     "clustered_30k": lambda: (*H.clustered(30000, 5, 0.01, 3)[:2], 50, [0, 1, 0, 1]),
     "clustered_deep": lambda: (*H.clustered(40000, 3, 0.001, 4)[:2], 20, [0, 1, 0, 1]),
+    # 21 and 23 levels: the 2:1 balance runs both its bitmap depths (<= 13)
+    # and its sorted-candidate depths below them
+    "clustered_21_levels": lambda: (*H.clustered(30000, 3, 2e-5, 8)[:2], 20, [0, 1, 0, 1]),
+    "clustered_23_levels": lambda: (*H.clustered(24000, 4, 3e-6, 10)[:2], 25, [0, 1, 0, 1]),
     "tiny_root_leaf": lambda: (*H.uniform(50, 5)[:2], 64, None),
     "small_two_levels": lambda: (*H.uniform(100, 6)[:2], 30, None),
 }
