@@ -67,6 +67,7 @@ struct Tree {
   std::vector<int64_t> s_off;   // host offsets of the split-cell lists per depth
   DevBuf s_all, sref_all;       // u64 split-cell codes per depth (concatenated)
   DevBuf bal_bm, bal_pc;        // balance: cell bitmaps per depth, popcount scan
+  DevBuf nidx;                  // dense node index of the non-full shallow levels
   DevBuf bal_k, bal_v, bal_f;   // balance below the bitmap depths: candidates
   std::vector<DevBuf> bal_lists;
   DevBuf p_code;                // u64 codes of nodes with children, per level (concatenated)

@@ -294,15 +294,33 @@ __device__ __forceinline__ int bm_depth(const BmOffsets& o, int dmax, int64_t w)
 __device__ __forceinline__ void bm_set(uint32_t* bm, uint64_t c) {
   atomicOr(bm + (c >> 5), 1u << (c & 31));
 }
-// the 3x3 neighbourhood (clipped to the domain) of parent cell pc at depth d
+// the 3x3 neighbourhood (clipped to the domain) of parent cell pc at depth d;
+// bits that fall in one word go out in one atomic (a neighbourhood spans at
+// most four words)
 __device__ __forceinline__ void bm_mark_nbrs(uint32_t* bm, uint64_t pc, int d) {
   uint32_t ix, iy;
   demorton(pc, &ix, &iy);
   const int64_t lim = (int64_t)1 << d;
+  uint64_t wd[9];
+  uint32_t bit[9];
+#pragma unroll
   for (int k = 0; k < 9; ++k) {
     const int64_t nx = (int64_t)ix + k / 3 - 1, ny = (int64_t)iy + k % 3 - 1;
-    if (nx < 0 || ny < 0 || nx >= lim || ny >= lim) continue;
-    bm_set(bm, morton((uint32_t)nx, (uint32_t)ny));
+    const bool in = nx >= 0 && ny >= 0 && nx < lim && ny < lim;
+    const uint64_t c = in ? morton((uint32_t)nx, (uint32_t)ny) : 0;
+    wd[k] = c >> 5;
+    bit[k] = in ? 1u << (c & 31) : 0u;
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    bool first = true;
+#pragma unroll
+    for (int j = 0; j < k; ++j) first = first && !(bit[j] && wd[j] == wd[k]);
+    if (!first || !bit[k]) continue;
+    uint32_t m = bit[k];
+#pragma unroll
+    for (int j = k + 1; j < 9; ++j) m |= wd[j] == wd[k] ? bit[j] : 0u;
+    atomicOr(bm + wd[k], m);
   }
 }
 // Sref_d for depths 1 .. dmax into their bitmaps
@@ -315,12 +333,22 @@ __global__ __launch_bounds__(BS) void k_bm_mark(const uint64_t* __restrict__ sre
   while (e >= soff[d + 1]) ++d;
   bm_set(bm + o.wo[d], sref[e]);
 }
-// N(parents) of the depth-(d+1) cells in bitmap bm1 into depth d's bitmap bm0:
-// the 4 children of parent w*8+q are bits 4q..4q+3 of word w
-__global__ __launch_bounds__(BS) void k_bm_spread(const uint32_t* __restrict__ bm1, int64_t nw1,
-                                                  int d, uint32_t* __restrict__ bm0) {
+// N(parents) of the depth-(d+1) cells in bitmap bm1 into depth d's bitmap bm0,
+// one thread per depth-d parent cell pc: its 4 children are bits
+// 4(pc mod 8) .. +3 of word pc / 8
+__global__ __launch_bounds__(BS) void k_bm_spread(const uint32_t* __restrict__ bm1, int d,
+                                                  uint32_t* __restrict__ bm0) {
+  const int64_t pc = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (pc >= (int64_t)1 << (2 * d)) return;
+  if (!((bm1[pc >> 3] >> (4 * (pc & 7))) & 0xFu)) return;
+  bm_mark_nbrs(bm0, (uint64_t)pc, d);
+}
+// the same with one thread per word of bm1 (8 parents), for the deep and
+// sparse bitmaps where a thread per cell would mostly find nothing
+__global__ __launch_bounds__(BS) void k_bm_spread_w(const uint32_t* __restrict__ bm1, int d,
+                                                    uint32_t* __restrict__ bm0) {
   const int64_t w = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (w >= nw1) return;
+  if (w >= ((int64_t)1 << (2 * d)) / 8) return;
   const uint32_t v = bm1[w];
   if (!v) return;
   for (int q = 0; q < 8; ++q)
@@ -436,13 +464,30 @@ __global__ __launch_bounds__(BS) void k_p_scatter(const uint64_t* __restrict__ r
 
 // --- final node arrays ---------------------------------------------------------
 // Node index of the level-d cell with Morton code c, or -1: direct on a full
-// level, else a binary search in the level's sorted codes.
+// level, a table lookup on the other levels up to depth kIdxDepth (one int per
+// cell of the level, -1 where no node), else a binary search in the level's
+// sorted codes.
+constexpr int kIdxDepth = 11;   // tables of at most 4^12/3 ints (22 MB)
+struct NodeIdx {
+  const int* tab;
+  int64_t off[kMaxLevels];      // level d's table in tab, or -1
+};
 __device__ __forceinline__ int node_of(const uint64_t* __restrict__ code,
-                                       const int* __restrict__ levoff, int d, uint64_t c) {
+                                       const int* __restrict__ levoff, const NodeIdx& ni, int d,
+                                       uint64_t c) {
   const int64_t nl = levoff[d + 1] - levoff[d];
   if (d < 16 && nl == ((int64_t)1 << (2 * d))) return levoff[d] + (int)c;
+  if (ni.off[d] >= 0) return ni.tab[ni.off[d] + (int64_t)c];
   const int64_t pos = bsearch_u64(code + levoff[d], nl, c);
   return pos >= 0 ? levoff[d] + (int)pos : -1;
+}
+__global__ __launch_bounds__(BS) void k_node_index(const uint64_t* __restrict__ code,
+                                                   const int* __restrict__ nlev, int nn,
+                                                   NodeIdx ni, int* __restrict__ tab) {
+  const int g = blockIdx.x * BS + threadIdx.x;
+  if (g >= nn) return;
+  const int64_t o = ni.off[nlev[g]];
+  if (o >= 0) tab[o + (int64_t)code[g]] = g;
 }
 
 // Source ranges of all nodes from the sorted keys: point i starts its cells
@@ -454,7 +499,8 @@ __global__ __launch_bounds__(BS) void k_cell_ranges(const uint64_t* __restrict__
                                                     const int2* __restrict__ sc, int n,
                                                     const uint64_t* __restrict__ code,
                                                     const int* __restrict__ levoff, int L,
-                                                    int* __restrict__ bot, int* __restrict__ top) {
+                                                    NodeIdx ni, int* __restrict__ bot,
+                                                    int* __restrict__ top) {
   const int i = blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
   const int s0 = sc[i].x;
@@ -462,12 +508,12 @@ __global__ __launch_bounds__(BS) void k_cell_ranges(const uint64_t* __restrict__
   if (s0 >= L - 1 && s1 >= L - 1) return;
   const uint64_t k = keys[i];
   for (int d = max(1, s0 + 1); d <= L - 1; ++d) {
-    const int g = node_of(code, levoff, d, k >> (64 - 2 * d));
+    const int g = node_of(code, levoff, ni, d, k >> (64 - 2 * d));
     if (g < 0) break;
     bot[g] = i;
   }
   for (int d = max(1, s1 + 1); d <= L - 1; ++d) {
-    const int g = node_of(code, levoff, d, k >> (64 - 2 * d));
+    const int g = node_of(code, levoff, ni, d, k >> (64 - 2 * d));
     if (g < 0) break;
     top[g] = i + 1;
   }
@@ -507,7 +553,7 @@ __global__ __launch_bounds__(64) void k_nodes_b(
     const uint64_t* __restrict__ p_code, const int* __restrict__ p_kind,
     const int64_t* __restrict__ poff, const int* __restrict__ levoff, int L,
     const uint64_t* __restrict__ keys, int npts, double xmin0, double ymin0, double w0,
-    const uint64_t* __restrict__ code, int* __restrict__ flags, int* __restrict__ bot,
+    NodeIdx ni, const uint64_t* __restrict__ code, int* __restrict__ flags, int* __restrict__ bot,
     int* __restrict__ top, int* __restrict__ parent, int* __restrict__ child,
     int* __restrict__ coll, double* __restrict__ xmid, double* __restrict__ ymid) {
   int g = blockIdx.x * 64 + threadIdx.x;
@@ -564,11 +610,11 @@ __global__ __launch_bounds__(64) void k_nodes_b(
     int64_t nx = (int64_t)ix + q / 3 - 1, ny = (int64_t)iy + q % 3 - 1;
     int r = -1;
     if (nx >= 0 && ny >= 0 && nx < lim && ny < lim) {
-      r = (q == 4) ? g : node_of(code, levoff, d, morton((uint32_t)nx, (uint32_t)ny));
+      r = (q == 4) ? g : node_of(code, levoff, ni, d, morton((uint32_t)nx, (uint32_t)ny));
     }
     coll[(int64_t)g * 9 + q] = r;
   }
-  const int par = d > 0 ? node_of(code, levoff, d - 1, c >> 2) : -1;
+  const int par = d > 0 ? node_of(code, levoff, ni, d - 1, c >> 2) : -1;
   parent[g] = par;
 }
 
@@ -632,7 +678,7 @@ __global__ __launch_bounds__(BS) void k_m2l_lists(const uint64_t* __restrict__ c
                                                   const int* __restrict__ flags,
                                                   const int* __restrict__ row,
                                                   const int* __restrict__ levoff, int L,
-                                                  int* __restrict__ m2l) {
+                                                  NodeIdx ni, int* __restrict__ m2l) {
   const int64_t tid = (int64_t)blockIdx.x * BS + threadIdx.x;
   const int g = levoff[2] + (int)(tid / kNumM2L);
   const int s = (int)(tid % kNumM2L);
@@ -648,9 +694,8 @@ __global__ __launch_bounds__(BS) void k_m2l_lists(const uint64_t* __restrict__ c
       const int64_t lim = (int64_t)1 << d;
       const int64_t nx = (int64_t)ix + ox, ny = (int64_t)iy + oy;
       if (nx >= 0 && ny >= 0 && nx < lim && ny < lim) {
-        const int64_t pos = bsearch_u64(code + levoff[d], levoff[d + 1] - levoff[d],
-                                        morton((uint32_t)nx, (uint32_t)ny));
-        if (pos >= 0) r = row[levoff[d] + pos];
+        const int q = node_of(code, levoff, ni, d, morton((uint32_t)nx, (uint32_t)ny));
+        if (q >= 0) r = row[q];
       }
     }
   }
@@ -884,9 +929,13 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
       }
       for (int d = DM; d >= 1; --d) {
         if (d + 1 <= DM) {
-          const int64_t nw1 = bo.wo[d + 2] - bo.wo[d + 1];
-          hipLaunchKernelGGL(k_bm_spread, dim3(grid_for(nw1)), dim3(BS), 0, s, bm + bo.wo[d + 1],
-                             nw1, d, bm + bo.wo[d]);
+          const int64_t npar = (int64_t)1 << (2 * d);   // depth-d cells
+          if (npar <= ((int64_t)1 << 22))
+            hipLaunchKernelGGL(k_bm_spread, dim3(grid_for(npar)), dim3(BS), 0, s,
+                               bm + bo.wo[d + 1], d, bm + bo.wo[d]);
+          else
+            hipLaunchKernelGGL(k_bm_spread_w, dim3(grid_for(npar / 8)), dim3(BS), 0, s,
+                               bm + bo.wo[d + 1], d, bm + bo.wo[d]);
         } else {
           const uint64_t* lst =
               d + 1 == L - 2 ? sref + t.s_off[L - 2] : lists[d + 1].as<uint64_t>();
@@ -996,14 +1045,30 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
                      t.p_kind.as<int>(), d_poff, d_levoff, L, t.code.as<uint64_t>(),
                      t.nlevel.as<int>(), t.flags.as<int>());
   FMM_LAUNCH_CHECK();
+  NodeIdx ni{};
+  int64_t ntab = 0;
+  for (int d = 0; d < kMaxLevels; ++d) ni.off[d] = -1;
+  for (int d = 1; d < L && d <= kIdxDepth; ++d) {
+    if (t.lev[d].n_nodes == ((int64_t)1 << (2 * d))) continue;   // full: direct
+    ni.off[d] = ntab;
+    ntab += (int64_t)1 << (2 * d);
+  }
+  if (ntab > 0) {
+    t.nidx.ensure(ntab * sizeof(int));
+    FMM_HIP(hipMemsetAsync(t.nidx.ptr, 0xFF, ntab * sizeof(int), s));
+    ni.tab = t.nidx.as<int>();
+    hipLaunchKernelGGL(k_node_index, dim3(grid_for(off)), dim3(BS), 0, s, t.code.as<uint64_t>(),
+                       t.nlevel.as<int>(), NN, ni, t.nidx.as<int>());
+    FMM_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(k_cell_ranges_init, dim3(grid_for(off)), dim3(BS), 0, s, NN, nn,
                      t.bot.as<int>(), t.top.as<int>());
   FMM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_cell_ranges, dim3(grid_for(n)), dim3(BS), 0, s, keys, sc, nn,
-                     t.code.as<uint64_t>(), d_levoff, L, t.bot.as<int>(), t.top.as<int>());
+                     t.code.as<uint64_t>(), d_levoff, L, ni, t.bot.as<int>(), t.top.as<int>());
   FMM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_nodes_b, dim3(ceil_div(off, 64)), dim3(64), 0, s, t.p_code.as<uint64_t>(),
-                     t.p_kind.as<int>(), d_poff, d_levoff, L, keys, nn, xmin, ymin, w0,
+                     t.p_kind.as<int>(), d_poff, d_levoff, L, keys, nn, xmin, ymin, w0, ni,
                      t.code.as<uint64_t>(), t.flags.as<int>(), t.bot.as<int>(), t.top.as<int>(),
                      t.parent.as<int>(), t.child.as<int>(), t.coll.as<int>(), t.xmid.as<double>(),
                      t.ymid.as<double>());
@@ -1029,7 +1094,7 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   if (L > 2) {
     hipLaunchKernelGGL(k_m2l_lists, dim3(grid_for((int64_t)(NN - g0) * kNumM2L)), dim3(BS), 0, s,
                        t.code.as<uint64_t>(), t.flags.as<int>(), t.row.as<int>(), d_levoff, L,
-                       t.m2l.as<int>());
+                       ni, t.m2l.as<int>());
     FMM_LAUNCH_CHECK();
   }
 
