@@ -86,6 +86,8 @@ struct Plan {
   DevBuf logtab;
   DevBuf tau_s, src, check, mult, le, ple, qs;
   DevBuf srcf, tsum;   // mixed precision: leaf-relative FP32 sources, per-leaf charge sums
+  DevBuf m2lperm;      // M2L column order of an adaptive tree
+  int64_t perm_tree = -1;   // tree build the column order belongs to
   bool have_srcf = false;
   bool have_exp = false;
   bool up_done = false;
@@ -167,10 +169,12 @@ struct PotAcc<KER_MODHELM> {
   __device__ __forceinline__ double result() const { return kInv2Pi * s; }
 };
 
-// S2M: one wave per leaf, lane k = check point k on the large circle
-// (helpers.py:159-173; the source is taken relative to the box centre).
+// S2M: one wave per leaf (kS2MWaves leaves per block share the log table),
+// lane k = check point k on the large circle (helpers.py:159-173; the source
+// is taken relative to the box centre).
+constexpr int kS2MWaves = 4;
 template <int PD, int KER>
-__global__ __launch_bounds__(64) void k_s2m(const int* __restrict__ upnodes,
+__global__ __launch_bounds__(64 * kS2MWaves) void k_s2m(int nup, const int* __restrict__ upnodes,
                                             const int* __restrict__ nlevel,
                                             const int* __restrict__ bot, const int* __restrict__ top,
                                             const int* __restrict__ row,
@@ -183,11 +187,13 @@ __global__ __launch_bounds__(64) void k_s2m(const int* __restrict__ upnodes,
   __shared__ double2 tab[kLogTableSize];
   load_logtab(tab, logtab);
   __syncthreads();
-  const int g = upnodes[blockIdx.x];
+  const int gi = blockIdx.x * kS2MWaves + (int)(threadIdx.x >> 6);
+  if (gi >= nup) return;
+  const int g = upnodes[gi];
   const int d = nlevel[g];
-  const int k = threadIdx.x;
+  const int k = threadIdx.x & 63;
   const int r = row[g];
-  if (r < 0 || d < 2) return;   // block-uniform
+  if (r < 0 || d < 2) return;   // wave-uniform
   const double* lx = ops + d * opsz + (int64_t)OP_NBLK * PD * PD;
   const double* ly = lx + PD;
   const double ax = k < p ? lx[k] : 1.0;
@@ -413,6 +419,10 @@ struct LevelTable {
   int g0[kMaxLevels];
   int ngroups[kMaxLevels];
   int boff[kMaxLevels + 1];
+  // optional column order (adaptive trees): sibling group of column qq of
+  // child position c at level li is perm[poff[li] + c * ngroups[li] + qq]
+  const int* perm;
+  int poff[kMaxLevels];
 };
 
 struct ValidSlots {
@@ -455,7 +465,8 @@ __device__ __forceinline__ M2LSetup m2l_setup(const LevelTable& lt, const int* _
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   S.col = w * 16 + (l & 15);
   S.kq = l >> 4;
-  const int q = tile * kM2LCols + S.col;
+  const int qq = tile * kM2LCols + S.col;
+  const int q = (lt.perm && qq < ngroups) ? lt.perm[lt.poff[li] + S.c * ngroups + qq] : qq;
   S.g = lt.g0[li] + 4 * q + S.c;
   S.active = q < ngroups && !(flags[q < ngroups ? S.g : lt.g0[li]] & NF_FAKE);
   const int* lst = m2l + (int64_t)S.g * kNumM2L;
@@ -482,6 +493,77 @@ __device__ __forceinline__ M2LSetup m2l_setup(const LevelTable& lt, const int* _
   __syncthreads();
   S.nph = pslot[27];
   return S;
+}
+
+// Column order of the M2L blocks on adaptive trees.  There the boxes of a
+// level differ in which of their offsets carry a partner, and 64 neighbouring
+// boxes together use most offsets although each uses few, so most MFMA
+// columns would multiply absent partners.  Within chunks of kM2LOrderChunk
+// sibling groups (in Morton order) of one level and child position, the
+// boxes are sorted by their set of used offsets (ties by position); boxes
+// without any (and fake nodes) go last.  One block per chunk: bitonic sort
+// of (40-bit offset mask << 11 | index) in LDS.
+constexpr int kM2LOrderChunk = 2048;
+struct M2LOrderTable {
+  int nlev;
+  int g0[kMaxLevels];
+  int ngroups[kMaxLevels];
+  int nch[kMaxLevels];     // chunks per child position
+  int boff[kMaxLevels + 1];
+  int poff[kMaxLevels];
+};
+__global__ __launch_bounds__(256) void k_m2l_order(M2LOrderTable ot, const int* __restrict__ flags,
+                                                   const int* __restrict__ m2l,
+                                                   int* __restrict__ perm) {
+  __shared__ uint64_t key[kM2LOrderChunk];
+  const int b = blockIdx.x;
+  int li = 0;
+  while (li + 1 < ot.nlev && b >= ot.boff[li + 1]) ++li;
+  const int lb = b - ot.boff[li];
+  const int c = lb / ot.nch[li];
+  const int chunk = lb % ot.nch[li];
+  const int ng = ot.ngroups[li];
+  const int q0 = chunk * kM2LOrderChunk;
+  const int cnt = min(kM2LOrderChunk, ng - q0);
+  static_assert(kNumM2L == 40 && kM2LOrderChunk <= 2048, "key layout: 40 + 11 bits");
+#pragma unroll 2
+  for (int i = threadIdx.x; i < kM2LOrderChunk; i += 256) {
+    uint64_t k = ~0ull;
+    if (i < cnt) {
+      const int g = ot.g0[li] + 4 * (q0 + i) + c;
+      const int4* lst = reinterpret_cast<const int4*>(m2l + (int64_t)g * kNumM2L);
+      int4 e[kNumM2L / 4];
+#pragma unroll
+      for (int j = 0; j < kNumM2L / 4; ++j) e[j] = lst[j];
+      uint64_t mask = 0;
+#pragma unroll
+      for (int j = 0; j < kNumM2L / 4; ++j)
+        mask |= (uint64_t)((e[j].x >= 0) | (e[j].y >= 0) << 1 | (e[j].z >= 0) << 2 |
+                           (e[j].w >= 0) << 3)
+                << (4 * j);
+      if (flags[g] & NF_FAKE) mask = 0;
+      k = ((mask ? mask : (1ull << 52) - 1) << 11) | (uint64_t)i;
+    }
+    key[i] = k;
+  }
+  __syncthreads();
+  for (int size = 2; size <= kM2LOrderChunk; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < kM2LOrderChunk / 2; t += 256) {
+        const int lo = 2 * stride * (t / stride) + (t % stride);
+        const int hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const uint64_t a = key[lo], e = key[hi];
+        if ((a > e) == up) {
+          key[lo] = e;
+          key[hi] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < cnt; i += 256)
+    perm[ot.poff[li] + c * ng + q0 + i] = q0 + (int)(key[i] & 2047u);
 }
 
 template <int PD>
@@ -955,18 +1037,21 @@ __global__ __launch_bounds__(256) void k_make_items(const int* __restrict__ icnt
 
 // ---------------------------------------------------------------------------
 // L2T + P2P, one wave per (box, chunk of 64 targets); lane = target.
-// Work items of the evaluation kernels, one per block: XCD x (blocks b with
+// Work items of the evaluation kernels, one per wave, kEvalWaves waves per
+// block (the block shares one copy of the log table): XCD x (blocks b with
 // b % 8 == x, dispatched in order) takes the x-th contiguous eighth of the
 // items.  Items are in leaf (Morton) order, so the colleague sources that
 // neighbouring leaves share stay in one XCD's L2.  The grid holds at least
-// ceil(ni / 8) blocks per XCD; surplus blocks exit.
+// ceil(ni / 8) waves per XCD; surplus waves exit.
+constexpr int kEvalWaves = 4;
 struct ItemRange {
   int first, last, step;
 };
 __device__ __forceinline__ ItemRange xcd_items(int ni) {
   const int x = blockIdx.x % kNumXcd;
   ItemRange r;
-  r.first = (int)((int64_t)ni * x / kNumXcd) + (int)(blockIdx.x / kNumXcd);
+  r.first = (int)((int64_t)ni * x / kNumXcd) +
+            (int)(blockIdx.x / kNumXcd) * kEvalWaves + (int)(threadIdx.x >> 6);
   r.last = min(r.first + 1, (int)((int64_t)ni * (x + 1) / kNumXcd));
   r.step = 1;
   return r;
@@ -1036,8 +1121,9 @@ __device__ __forceinline__ FewLanes few_lanes(int na) {
   FewLanes f;
   f.lg = na <= 1 ? 0 : 32 - __builtin_clz(na - 1);
   f.ng = 64 >> f.lg;
-  f.slot = threadIdx.x & ((1 << f.lg) - 1);
-  f.sg = threadIdx.x >> f.lg;
+  const int lane = threadIdx.x & 63;
+  f.slot = lane & ((1 << f.lg) - 1);
+  f.sg = lane >> f.lg;
   f.act = f.slot < na;
   return f;
 }
@@ -1111,7 +1197,7 @@ __device__ __forceinline__ void eval_few(
 }
 
 template <int PD, int KER, int NOUT, bool CHECK>
-__global__ __launch_bounds__(64) void k_eval(
+__global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
     const int2* __restrict__ items, const int* __restrict__ nitems,
     const int* __restrict__ evalnode, const int* __restrict__ toff,
     const int* __restrict__ tperm, const double* __restrict__ tsx, const double* __restrict__ tsy,
@@ -1135,7 +1221,7 @@ __global__ __launch_bounds__(64) void k_eval(
                                      bot, top, src, le, ops, opsz, p, tab, kp, nt, out);
       continue;
     }
-    const int t = t0 + threadIdx.x;
+    const int t = t0 + (int)(threadIdx.x & 63);
     const bool act = t < t1;
     Target<KER, NOUT, CHECK> T;
     T.kp = kp;
@@ -1301,7 +1387,7 @@ __device__ __forceinline__ void eval_few_mixed(
 }
 
 template <int PD, bool CHECK>
-__global__ __launch_bounds__(64) void k_eval_mixed(
+__global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
     const int2* __restrict__ items, const int* __restrict__ nitems,
     const int* __restrict__ evalnode, const int* __restrict__ toff,
     const int* __restrict__ tperm, const double* __restrict__ tsx, const double* __restrict__ tsy,
@@ -1327,7 +1413,7 @@ __global__ __launch_bounds__(64) void k_eval_mixed(
                                 bot, top, src, srcf, tsum, lc, le, ops, opsz, p, tab, out);
       continue;
     }
-    const int t = t0 + threadIdx.x;
+    const int t = t0 + (int)(threadIdx.x & 63);
     const bool act = t < t1;
     Target<KER_LAPLACE, 1, CHECK> T;
     T.kp = 0.0;
@@ -1443,7 +1529,8 @@ __global__ void k_pack_stokes(const double* __restrict__ tau, const uint32_t* __
 
 // S2M of the six densities: one log per (source, check point) serves all six.
 template <int PD>
-__global__ __launch_bounds__(64) void k_s2m_stokes(const int* __restrict__ upnodes,
+__global__ __launch_bounds__(64 * kS2MWaves) void k_s2m_stokes(int nup,
+                                                               const int* __restrict__ upnodes,
                                                    const int* __restrict__ nlevel,
                                                    const int* __restrict__ bot,
                                                    const int* __restrict__ top,
@@ -1458,11 +1545,13 @@ __global__ __launch_bounds__(64) void k_s2m_stokes(const int* __restrict__ upnod
   __shared__ double2 tab[kLogTableSize];
   load_logtab(tab, logtab);
   __syncthreads();
-  const int g = upnodes[blockIdx.x];
+  const int gi = blockIdx.x * kS2MWaves + (int)(threadIdx.x >> 6);
+  if (gi >= nup) return;
+  const int g = upnodes[gi];
   const int d = nlevel[g];
-  const int k = threadIdx.x;
+  const int k = threadIdx.x & 63;
   const int r = row[g];
-  if (r < 0 || d < 2) return;   // block-uniform
+  if (r < 0 || d < 2) return;   // wave-uniform
   const double* lx = ops + d * opsz + ops_off_circles(PD);
   const double* ly = lx + PD;
   const double ax = k < p ? lx[k] : 1.0;
@@ -1488,9 +1577,132 @@ __global__ __launch_bounds__(64) void k_s2m_stokes(const int* __restrict__ upnod
   }
 }
 
+// 1/x to a few ulp: hardware reciprocal and two Newton steps (x > 0, normal)
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+// One item of k_eval_stokes: L2T of the six local densities + Stokeslet P2P,
+// outputs (u, v, p).  FEW selects the (target slot, source group) lane layout
+// of eval_few; otherwise lane = target and the source loop index is uniform.
+template <int PD, bool CHECK, bool FEW>
+__device__ __forceinline__ void stokes_item(
+    int g, int t0, int t1, const int* __restrict__ tperm, const double* __restrict__ tsx,
+    const double* __restrict__ tsy, const int* __restrict__ nlevel,
+    const double* __restrict__ xmid, const double* __restrict__ ymid,
+    const int* __restrict__ coll, const int* __restrict__ bot, const int* __restrict__ top,
+    const double4* __restrict__ src, const double* __restrict__ le, int64_t nstride,
+    const double* __restrict__ ops, int64_t opsz, int p, const double2* __restrict__ tab,
+    double c0x, double c0y, int64_t nt, double* __restrict__ out) {
+  FewLanes F;
+  if constexpr (FEW) {
+    F = few_lanes(t1 - t0);
+  } else {
+    F.lg = 6;
+    F.ng = 1;
+    F.slot = threadIdx.x & 63;
+    F.sg = 0;
+    F.act = t0 + F.slot < t1;
+  }
+  const int t = t0 + F.slot;
+  const bool act = F.act;
+  const double x = act ? tsx[t] : 0.0;
+  const double y = act ? tsy[t] : 0.0;
+  double u = 0.0, v = 0.0, pr = 0.0;
+  const int d = nlevel[g];
+  if (d >= 2) {
+    const double* lx = ops + d * opsz + ops_off_circles(PD);
+    const double* ly = lx + PD;
+    const double* a = le + (int64_t)g * PD;
+    const double xr = x - xmid[g], yr = y - ymid[g];
+    LogAcc l0, l1;
+    double B = 0.0, Cx = 0.0, Cy = 0.0;
+    for (int k = F.sg; k < p; k += F.ng) {
+      const double dx = xr - lx[k], dy = yr - ly[k];
+      const double d2 = __builtin_fma(dy, dy, dx * dx);
+      const double id2 = rcp_nr(d2);
+      const double a0 = a[k], a1 = a[nstride * PD + k], a2 = a[2 * nstride * PD + k],
+                   a3 = a[3 * nstride * PD + k], a4 = a[4 * nstride * PD + k],
+                   a5 = a[5 * nstride * PD + k];
+      double e, lt;
+      log_split(d2, tab, e, lt);
+      l0.ae = __builtin_fma(a0, e, l0.ae);
+      l0.at = __builtin_fma(a0, lt, l0.at);
+      l1.ae = __builtin_fma(a1, e, l1.ae);
+      l1.at = __builtin_fma(a1, lt, l1.at);
+      B = __builtin_fma(__builtin_fma(a0, dx, a1 * dy), id2, B);
+      Cx = __builtin_fma(__builtin_fma(a2, dx, a3 * dy), id2, Cx);
+      Cy = __builtin_fma(__builtin_fma(a4, dx, a5 * dy), id2, Cy);
+    }
+    // -L[f] = -sum a (1/2) log d^2
+    const double X = x - c0x, Y = y - c0y;
+    u = kInv4Pi * (-0.5 * l0.value() + X * B - Cx);
+    v = kInv4Pi * (-0.5 * l1.value() + Y * B - Cy);
+    pr = kInv2Pi * B;
+  }
+  // near field: the Stokeslet itself (stokes_fmm.py:102-184)
+  LogAcc lx_, ly_;
+  double su = 0.0, sv = 0.0, sp = 0.0;
+  double dmin = 1.0;
+  const int* cl = coll + (int64_t)g * 9;
+  for (int qn = 0; qn < 9; ++qn) {
+    const int c = cl[qn];
+    if (c < 0) continue;
+    const int b = __builtin_amdgcn_readfirstlane(bot[c]);
+    const int e = __builtin_amdgcn_readfirstlane(top[c]);
+    for (int j = b + F.sg; j < e; j += F.ng) {
+      const double4 sj = src[j];
+      const double dx = x - sj.x, dy = y - sj.y;
+      double d2 = __builtin_fma(dy, dy, dx * dx);
+      const bool z = d2 == 0.0;
+      double fx = sj.z, fy = sj.w;
+      if (CHECK) {
+        fx = z ? 0.0 : fx;
+        fy = z ? 0.0 : fy;
+      } else {
+        dmin = __builtin_fmin(dmin, d2);
+      }
+      d2 = z ? 1.0 : d2;
+      const double id2 = rcp_nr(d2);
+      double le_, lt_;
+      log_split(d2, tab, le_, lt_);
+      lx_.ae = __builtin_fma(fx, le_, lx_.ae);
+      lx_.at = __builtin_fma(fx, lt_, lx_.at);
+      ly_.ae = __builtin_fma(fy, le_, ly_.ae);
+      ly_.at = __builtin_fma(fy, lt_, ly_.at);
+      const double rf = __builtin_fma(dx, fx, dy * fy) * id2;
+      su = __builtin_fma(dx, rf, su);
+      sv = __builtin_fma(dy, rf, sv);
+      sp += rf;
+    }
+  }
+  u += kInv4Pi * (-0.5 * lx_.value() + su);
+  v += kInv4Pi * (-0.5 * ly_.value() + sv);
+  pr += kInv2Pi * sp;
+  if constexpr (FEW) {
+    u = slot_sum(u, F.lg);
+    v = slot_sum(v, F.lg);
+    pr = slot_sum(pr, F.lg);
+    if (!CHECK)
+      for (int off = 1 << F.lg; off < 64; off <<= 1)
+        dmin = __builtin_fmin(dmin, __shfl_xor(dmin, off, 64));
+  }
+  if (act && F.sg == 0) {
+    const int64_t o = tperm[t];
+    if (!CHECK && dmin == 0.0) u = v = pr = __builtin_nan("");
+    out[o] = u;
+    out[nt + o] = v;
+    out[2 * nt + o] = pr;
+  }
+}
+
 // L2T of the six local densities + Stokeslet P2P, outputs (u, v, p).
 template <int PD, bool CHECK>
-__global__ __launch_bounds__(64) void k_eval_stokes(
+__global__ __launch_bounds__(64 * kEvalWaves) void k_eval_stokes(
     const int2* __restrict__ items, const int* __restrict__ nitems,
     const int* __restrict__ evalnode, const int* __restrict__ toff,
     const int* __restrict__ tperm, const double* __restrict__ tsx, const double* __restrict__ tsy,
@@ -1510,97 +1722,18 @@ __global__ __launch_bounds__(64) void k_eval_stokes(
     const int g = evalnode[it.x];
     const int t0 = toff[it.x] + it.y * kTgtChunk;
     const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
-    // lane layout: one lane per target, or (target slot, source group) for
-    // items with few targets (see eval_few)
-    const int na = t1 - t0;
-    const FewLanes F = na <= kFewTargets ? few_lanes(na)
-                                         : FewLanes{6, 1, (int)threadIdx.x, 0, t0 + (int)threadIdx.x < t1};
-    const int t = t0 + F.slot;
-    const bool act = F.act;
-    const double x = act ? tsx[t] : 0.0;
-    const double y = act ? tsy[t] : 0.0;
-    double u = 0.0, v = 0.0, pr = 0.0;
-    const int d = nlevel[g];
-    if (d >= 2) {
-      const double* lx = ops + d * opsz + ops_off_circles(PD);
-      const double* ly = lx + PD;
-      const double* a = le + (int64_t)g * PD;
-      const double xr = x - xmid[g], yr = y - ymid[g];
-      LogAcc l0, l1;
-      double B = 0.0, Cx = 0.0, Cy = 0.0;
-      for (int k = F.sg; k < p; k += F.ng) {
-        const double dx = xr - lx[k], dy = yr - ly[k];
-        const double d2 = __builtin_fma(dy, dy, dx * dx);
-        const double id2 = 1.0 / d2;
-        const double a0 = a[k], a1 = a[nstride * PD + k], a2 = a[2 * nstride * PD + k],
-                     a3 = a[3 * nstride * PD + k], a4 = a[4 * nstride * PD + k],
-                     a5 = a[5 * nstride * PD + k];
-        l0.add(a0, d2, tab);
-        l1.add(a1, d2, tab);
-        B = __builtin_fma(__builtin_fma(a0, dx, a1 * dy), id2, B);
-        Cx = __builtin_fma(__builtin_fma(a2, dx, a3 * dy), id2, Cx);
-        Cy = __builtin_fma(__builtin_fma(a4, dx, a5 * dy), id2, Cy);
-      }
-      // -L[f] = -sum a (1/2) log d^2
-      const double X = x - c0x, Y = y - c0y;
-      u = kInv4Pi * (-0.5 * l0.value() + X * B - Cx);
-      v = kInv4Pi * (-0.5 * l1.value() + Y * B - Cy);
-      pr = kInv2Pi * B;
-    }
-    // near field: the Stokeslet itself (stokes_fmm.py:102-184)
-    LogAcc lx_, ly_;
-    double su = 0.0, sv = 0.0, sp = 0.0;
-    double dmin = 1.0;
-    const int* cl = coll + (int64_t)g * 9;
-    for (int qn = 0; qn < 9; ++qn) {
-      const int c = cl[qn];
-      if (c < 0) continue;
-      const int b = __builtin_amdgcn_readfirstlane(bot[c]);
-      const int e = __builtin_amdgcn_readfirstlane(top[c]);
-      for (int j = b + F.sg; j < e; j += F.ng) {
-        const double4 sj = src[j];
-        const double dx = x - sj.x, dy = y - sj.y;
-        double d2 = __builtin_fma(dy, dy, dx * dx);
-        const bool z = d2 == 0.0;
-        double fx = sj.z, fy = sj.w;
-        if (CHECK) {
-          fx = z ? 0.0 : fx;
-          fy = z ? 0.0 : fy;
-        } else {
-          dmin = __builtin_fmin(dmin, d2);
-        }
-        d2 = z ? 1.0 : d2;
-        const double id2 = 1.0 / d2;
-        lx_.add(fx, d2, tab);
-        ly_.add(fy, d2, tab);
-        const double rf = __builtin_fma(dx, fx, dy * fy) * id2;
-        su = __builtin_fma(dx, rf, su);
-        sv = __builtin_fma(dy, rf, sv);
-        sp += rf;
-      }
-    }
-    u += kInv4Pi * (-0.5 * lx_.value() + su);
-    v += kInv4Pi * (-0.5 * ly_.value() + sv);
-    pr += kInv2Pi * sp;
-    if (F.ng > 1) {   // wave-uniform
-      u = slot_sum(u, F.lg);
-      v = slot_sum(v, F.lg);
-      pr = slot_sum(pr, F.lg);
-      if (!CHECK)
-        for (int off = 1 << F.lg; off < 64; off <<= 1)
-          dmin = __builtin_fmin(dmin, __shfl_xor(dmin, off, 64));
-    }
-    if (act && F.sg == 0) {
-      const int64_t o = tperm[t];
-      if (!CHECK && dmin == 0.0) u = v = pr = __builtin_nan("");
-      out[o] = u;
-      out[nt + o] = v;
-      out[2 * nt + o] = pr;
-    }
+    if (t1 - t0 <= kFewTargets)
+      stokes_item<PD, CHECK, true>(g, t0, t1, tperm, tsx, tsy, nlevel, xmid, ymid, coll, bot,
+                                   top, src, le, nstride, ops, opsz, p, tab, c0x, c0y, nt, out);
+    else
+      stokes_item<PD, CHECK, false>(g, t0, t1, tperm, tsx, tsy, nlevel, xmid, ymid, coll, bot,
+                                    top, src, le, nstride, ops, opsz, p, tab, c0x, c0y, nt, out);
   }
 }
 
-inline int eval_grid(int64_t icap) { return (int)(ceil_div(icap + kNumXcd, kNumXcd) * kNumXcd); }
+inline int eval_grid(int64_t icap) {
+  return (int)(ceil_div(ceil_div(icap, kNumXcd) + 1, kEvalWaves) * kNumXcd);
+}
 
 // Selects the k_eval instance of (PD, kernel, outputs, self-exclusion).
 typedef decltype(&k_eval<32, KER_LAPLACE, 1, false>) EvalKernelFn;
@@ -1710,7 +1843,8 @@ static void upward(Plan& P, const double* tau, hipStream_t s) {
   P.mark(FMM2D_PHASE_S2M, false, s);
   if (t.n_up > 0 && stokes) {
     auto kern = PD == 32 ? k_s2m_stokes<32> : k_s2m_stokes<64>;
-    hipLaunchKernelGGL(kern, dim3(t.n_up), dim3(64), 0, s, t.upnodes.as<int>(), t.nlevel.as<int>(),
+    hipLaunchKernelGGL(kern, dim3(ceil_div(t.n_up, kS2MWaves)), dim3(64 * kS2MWaves), 0, s,
+                       (int)t.n_up, t.upnodes.as<int>(), t.nlevel.as<int>(),
                        t.bot.as<int>(), t.top.as<int>(), t.row.as<int>(), t.xmid.as<double>(),
                        t.ymid.as<double>(), P.src.as<double4>(), P.ops.as<double>(), opsz, P.p,
                        P.logtab.as<double2>(), t.xmin + 0.5 * t.w0, t.ymin + 0.5 * t.w0,
@@ -1719,7 +1853,8 @@ static void upward(Plan& P, const double* tau, hipStream_t s) {
   } else if (t.n_up > 0) {
     auto kern = P.kernel_id == KER_LAPLACE ? (PD == 32 ? k_s2m<32, KER_LAPLACE> : k_s2m<64, KER_LAPLACE>)
                                            : (PD == 32 ? k_s2m<32, KER_MODHELM> : k_s2m<64, KER_MODHELM>);
-    hipLaunchKernelGGL(kern, dim3(t.n_up), dim3(64), 0, s, t.upnodes.as<int>(), t.nlevel.as<int>(),
+    hipLaunchKernelGGL(kern, dim3(ceil_div(t.n_up, kS2MWaves)), dim3(64 * kS2MWaves), 0, s,
+                       (int)t.n_up, t.upnodes.as<int>(), t.nlevel.as<int>(),
                        t.bot.as<int>(), t.top.as<int>(), t.row.as<int>(), t.xmid.as<double>(),
                        t.ymid.as<double>(), P.src.as<double4>(), P.ops.as<double>(), opsz, P.p,
                        P.logtab.as<double2>(), P.kparam, P.check.as<double>());
@@ -1764,6 +1899,7 @@ static void downward(Plan& P, hipStream_t s) {
   }
   LevelTable lt{};
   int nb = 0;
+  bool adaptive = false;
   for (int d = 2; d < t.L; ++d) {
     const int ng = (int)(t.lev[d].n_nodes / 4);
     lt.d[lt.nlev] = d;
@@ -1771,9 +1907,36 @@ static void downward(Plan& P, hipStream_t s) {
     lt.ngroups[lt.nlev] = ng;
     lt.boff[lt.nlev] = nb;
     nb += 4 * (int)ceil_div(ng, kM2LCols);
+    adaptive = adaptive || t.lev[d].n_nodes != ((int64_t)1 << (2 * d));
     ++lt.nlev;
   }
   lt.boff[lt.nlev] = nb;
+  lt.perm = nullptr;
+  if (adaptive) {
+    // column order by used offsets, once per tree
+    M2LOrderTable ot{};
+    ot.nlev = lt.nlev;
+    int ob = 0, po = 0;
+    for (int li = 0; li < lt.nlev; ++li) {
+      ot.g0[li] = lt.g0[li];
+      ot.ngroups[li] = lt.ngroups[li];
+      ot.nch[li] = (int)ceil_div(lt.ngroups[li], kM2LOrderChunk);
+      ot.boff[li] = ob;
+      ot.poff[li] = po;
+      lt.poff[li] = po;
+      ob += 4 * ot.nch[li];
+      po += 4 * lt.ngroups[li];
+    }
+    ot.boff[lt.nlev] = ob;
+    if (P.perm_tree != t.build_id) {
+      P.m2lperm.ensure((po + 1) * sizeof(int));
+      hipLaunchKernelGGL(k_m2l_order, dim3(ob), dim3(256), 0, s, ot, t.flags.as<int>(),
+                         t.m2l.as<int>(), P.m2lperm.as<int>());
+      FMM_LAUNCH_CHECK();
+      P.perm_tree = t.build_id;
+    }
+    lt.perm = P.m2lperm.as<int>();
+  }
   if (P.precision != PREC_FP64) {
     auto kern = PD == 32 ? k_m2l_all_f32<32> : k_m2l_all_f32<64>;
     hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(kM2LThreads), 0, s, lt, t.flags.as<int>(),
@@ -1856,7 +2019,7 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
     auto sk = PD == 32 ? (check ? k_eval_stokes<32, true> : k_eval_stokes<32, false>)
                        : (check ? k_eval_stokes<64, true> : k_eval_stokes<64, false>);
     P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
-    hipLaunchKernelGGL(sk, dim3(eval_grid(icap)), dim3(64), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
+    hipLaunchKernelGGL(sk, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
                        t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(),
                        P.tsx.as<double>(), P.tsy.as<double>(), t.nlevel.as<int>(),
                        t.xmid.as<double>(), t.ymid.as<double>(), t.coll.as<int>(),
@@ -1887,7 +2050,7 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
     auto mk = check ? (PD == 32 ? k_eval_mixed<32, true> : k_eval_mixed<64, true>)
                     : (PD == 32 ? k_eval_mixed<32, false> : k_eval_mixed<64, false>);
     P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
-    hipLaunchKernelGGL(mk, dim3(eval_grid(icap)), dim3(64), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
+    hipLaunchKernelGGL(mk, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
                        t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(),
                        P.tsx.as<double>(), P.tsy.as<double>(), t.nlevel.as<int>(),
                        t.flags.as<int>(), t.xmid.as<double>(), t.ymid.as<double>(),
@@ -1900,7 +2063,7 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   }
   auto kern = eval_kernel(PD, P.kernel_id, ev == EVAL_POT ? 1 : 3, check);
   P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
-  hipLaunchKernelGGL(kern, dim3(eval_grid(icap)), dim3(64), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
+  hipLaunchKernelGGL(kern, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
                      t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(), P.tsx.as<double>(),
                      P.tsy.as<double>(), t.nlevel.as<int>(), t.xmid.as<double>(),
                      t.ymid.as<double>(), t.coll.as<int>(), t.bot.as<int>(), t.top.as<int>(),

@@ -28,6 +28,7 @@ struct TreeLevel {
 
 struct Tree {
   bool built = false;
+  int64_t build_id = 0;   // incremented by every build
   int64_t n = 0;           // number of sources
   int ncut = 0;
   int L = 0;               // number of levels
@@ -68,6 +69,7 @@ struct Tree {
   DevBuf s_all, sref_all;       // u64 split-cell codes per depth (concatenated)
   DevBuf bal_bm, bal_pc;        // balance: cell bitmaps per depth, popcount scan
   DevBuf nidx;                  // dense node index of the non-full shallow levels
+  DevBuf xy;                    // interleaved input coordinates (input order)
   DevBuf bal_k, bal_v, bal_f;   // balance below the bitmap depths: candidates
   std::vector<DevBuf> bal_lists;
   DevBuf p_code;                // u64 codes of nodes with children, per level (concatenated)

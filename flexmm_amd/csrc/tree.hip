@@ -87,14 +87,18 @@ __device__ __forceinline__ int find_level(const T* __restrict__ off, int L, int6
 }
 
 // --- points ----------------------------------------------------------------
+// Morton keys of the points and an interleaved copy of their coordinates
+// (the gather into tree order then reads one 16-byte record per point).
 __global__ __launch_bounds__(BS) void k_point_keys(const double* __restrict__ x,
                                                    const double* __restrict__ y, int n,
                                                    double xmin0, double ymin0, double w0,
                                                    uint64_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ vals) {
+                                                   uint32_t* __restrict__ vals,
+                                                   double2* __restrict__ xy) {
   int i = blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
   const double px = x[i], py = y[i];
+  xy[i] = make_double2(px, py);
   double xmin = xmin0, ymin = ymin0, w = w0;
   uint64_t key = 0;
   for (int l = 1; l <= kMaxDepth; ++l) {
@@ -112,16 +116,15 @@ __global__ __launch_bounds__(BS) void k_point_keys(const double* __restrict__ x,
   vals[i] = (uint32_t)i;
 }
 
-__global__ __launch_bounds__(BS) void k_gather_sorted(const double* __restrict__ x,
-                                                      const double* __restrict__ y,
+__global__ __launch_bounds__(BS) void k_gather_sorted(const double2* __restrict__ xy,
                                                       const uint32_t* __restrict__ perm, int n,
                                                       double* __restrict__ sx,
                                                       double* __restrict__ sy) {
   int i = blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
-  uint32_t j = perm[i];
-  sx[i] = x[j];
-  sy[i] = y[j];
+  const double2 v = xy[perm[i]];
+  sx[i] = v.x;
+  sy[i] = v.y;
 }
 
 // --- refinement: split cells from the sorted keys ----------------------------
@@ -775,12 +778,13 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   t.sy.ensure(n * sizeof(double));
   uint64_t* keys = t.keys.as<uint64_t>();
   uint32_t* perm = t.perm.as<uint32_t>();
+  t.xy.ensure(n * sizeof(double2));
   hipLaunchKernelGGL(k_point_keys, dim3(grid_for(n)), dim3(BS), 0, s, x, y, nn, xmin, ymin, w0,
-                     keys, perm);
+                     keys, perm, t.xy.as<double2>());
   FMM_LAUNCH_CHECK();
   sort_pairs_u64(keys, perm, n, ncut < kLocalSortMax ? ncut : kLocalSortMax, t.pw, s);
-  hipLaunchKernelGGL(k_gather_sorted, dim3(grid_for(n)), dim3(BS), 0, s, x, y, perm, nn,
-                     t.sx.as<double>(), t.sy.as<double>());
+  hipLaunchKernelGGL(k_gather_sorted, dim3(grid_for(n)), dim3(BS), 0, s, t.xy.as<double2>(), perm,
+                     nn, t.sx.as<double>(), t.sy.as<double>());
   FMM_LAUNCH_CHECK();
 
   // -- counters ------------------------------------------------------------------
@@ -857,7 +861,13 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   bool balanced = true;
   if (L >= 4) {
     int64_t ne = t.s_off[L - 1] - t.s_off[2];
-    if (ne > 0) {
+    // the check looks for neighbours of split cells' parents (depths 1 .. L-3)
+    // that are not split themselves; with every cell of those depths split
+    // there is none
+    bool full = true;
+    for (int d = 1; d <= L - 3 && full; ++d)
+      full = d < 31 && t.s_off[d + 1] - t.s_off[d] == ((int64_t)1 << (2 * d));
+    if (ne > 0 && !full) {
       hipLaunchKernelGGL(k_balance_check, dim3(grid_for(ne)), dim3(BS), 0, s,
                          t.sref_all.as<uint64_t>(), d_soff, L, cnt + CNT_R);
       FMM_LAUNCH_CHECK();
@@ -1134,6 +1144,7 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   hipLaunchKernelGGL(k_list_scatter, dim3(grid_for(off)), dim3(BS), 0, s, f2, r2, NN,
                      t.upnodes.as<int>(), (int*)nullptr);
   FMM_LAUNCH_CHECK();
+  ++t.build_id;
   t.built = true;
 }
 

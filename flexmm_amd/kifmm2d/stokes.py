@@ -93,6 +93,16 @@ class FMM:
     def handle(self):
         return self._plan.h
 
+    def set_profiling(self, enable=True):
+        """Record HIP events around each phase on the current stream."""
+        _lib.check(self.lib.fmm2d_set_profiling(self.handle, 1 if enable else 0))
+
+    def phase_times(self):
+        """Milliseconds of the last recorded run of each phase (waits for it)."""
+        out = (_lib.ctypes.c_double * len(_lib.PHASES))()
+        _lib.check(self.lib.fmm2d_phase_times(self.handle, out, len(out)))
+        return {k: float(v) for k, v in zip(_lib.PHASES, out) if v >= 0}
+
     def build_tree(self):
         """stokes_fmm.py:642-646."""
         st = time.time()
