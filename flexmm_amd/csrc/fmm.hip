@@ -1810,6 +1810,100 @@ static void check_ops(const Plan& P) {
                 "operators for level %d not set (fmm2d_set_operators)", d);
 }
 
+// M2L launch tables of the current tree: all levels >= 2 and, on adaptive
+// trees, the column order (k_m2l_order).
+struct M2LTables {
+  LevelTable lt;
+  M2LOrderTable ot;
+  int order_blocks;
+  int perm_len;
+  bool adaptive;
+};
+static M2LTables m2l_tables(const Plan& P) {
+  const Tree& t = P.tree;
+  M2LTables T{};
+  LevelTable& lt = T.lt;
+  for (int d = 2; d < t.L; ++d) {
+    lt.d[lt.nlev] = d;
+    lt.g0[lt.nlev] = (int)t.lev[d].node_off;
+    lt.ngroups[lt.nlev] = (int)(t.lev[d].n_nodes / 4);
+    T.adaptive = T.adaptive || t.lev[d].n_nodes != ((int64_t)1 << (2 * d));
+    ++lt.nlev;
+  }
+  int nb = 0;
+  for (int li = 0; li < lt.nlev; ++li) {
+    lt.boff[li] = nb;
+    nb += 4 * (int)ceil_div(lt.ngroups[li], kM2LCols);
+  }
+  lt.boff[lt.nlev] = nb;
+  lt.perm = nullptr;
+  if (T.adaptive) {
+    M2LOrderTable& ot = T.ot;
+    ot.nlev = lt.nlev;
+    int ob = 0, po = 0;
+    for (int li = 0; li < lt.nlev; ++li) {
+      ot.g0[li] = lt.g0[li];
+      ot.ngroups[li] = lt.ngroups[li];
+      ot.nch[li] = (int)ceil_div(lt.ngroups[li], kM2LOrderChunk);
+      ot.boff[li] = ob;
+      ot.poff[li] = po;
+      lt.poff[li] = po;
+      ob += 4 * ot.nch[li];
+      po += 4 * lt.ngroups[li];
+    }
+    ot.boff[lt.nlev] = ob;
+    T.order_blocks = ob;
+    T.perm_len = po;
+  }
+  return T;
+}
+// The column order, once per tree (on the stream that runs the M2L).
+static void m2l_prepare(Plan& P, const M2LTables& T, hipStream_t s) {
+  const Tree& t = P.tree;
+  const int nd = P.kernel_id == KER_STOKES ? kStokesDens : 1;
+  P.ple.ensure((t.n_nodes + 1) * P.PD * nd * sizeof(double));
+  if (!T.adaptive || P.perm_tree == t.build_id) return;
+  P.m2lperm.ensure((T.perm_len + 1) * sizeof(int));
+  hipLaunchKernelGGL(k_m2l_order, dim3(T.order_blocks), dim3(256), 0, s, T.ot, t.flags.as<int>(),
+                     t.m2l.as<int>(), P.m2lperm.as<int>());
+  FMM_LAUNCH_CHECK();
+  P.perm_tree = t.build_id;
+}
+// M2L of the table's levels [l0, l1).
+static void m2l_launch(Plan& P, const M2LTables& T, int l0, int l1, hipStream_t s) {
+  const Tree& t = P.tree;
+  const int PD = P.PD;
+  const int64_t opsz = ops_per_level(PD);
+  const int nd = P.kernel_id == KER_STOKES ? kStokesDens : 1;
+  const int64_t rstride = t.n_rows + 1, nstride = t.n_nodes + 1;
+  LevelTable lt{};
+  lt.nlev = l1 - l0;
+  if (lt.nlev <= 0) return;
+  for (int i = 0; i < lt.nlev; ++i) {
+    lt.d[i] = T.lt.d[l0 + i];
+    lt.g0[i] = T.lt.g0[l0 + i];
+    lt.ngroups[i] = T.lt.ngroups[l0 + i];
+    lt.boff[i] = T.lt.boff[l0 + i] - T.lt.boff[l0];
+    lt.poff[i] = T.lt.poff[l0 + i];
+  }
+  lt.boff[lt.nlev] = T.lt.boff[l1] - T.lt.boff[l0];
+  lt.perm = T.adaptive ? P.m2lperm.as<int>() : nullptr;
+  const int nb = lt.boff[lt.nlev];
+  if (nb == 0) return;
+  if (P.precision != PREC_FP64) {
+    auto kern = PD == 32 ? k_m2l_all_f32<32> : k_m2l_all_f32<64>;
+    hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(kM2LThreads), 0, s, lt, t.flags.as<int>(),
+                       t.m2l.as<int>(), P.mult.as<double>(), P.qs.as<double>(),
+                       P.ops.as<double>(), opsz, P.ple.as<double>(), rstride, nstride);
+  } else {
+    auto kern = PD == 32 ? k_m2l_all<32> : k_m2l_all<64>;
+    hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(kM2LThreads), 0, s, lt, t.flags.as<int>(),
+                       t.m2l.as<int>(), P.mult.as<double>(), P.ops.as<double>(), opsz,
+                       P.ple.as<double>(), rstride, nstride);
+  }
+  FMM_LAUNCH_CHECK();
+}
+
 static void upward(Plan& P, const double* tau, hipStream_t s) {
   Tree& t = P.tree;
   check_ops(P);
@@ -1897,59 +1991,9 @@ static void downward(Plan& P, hipStream_t s) {
     P.have_exp = true;
     return;
   }
-  LevelTable lt{};
-  int nb = 0;
-  bool adaptive = false;
-  for (int d = 2; d < t.L; ++d) {
-    const int ng = (int)(t.lev[d].n_nodes / 4);
-    lt.d[lt.nlev] = d;
-    lt.g0[lt.nlev] = (int)t.lev[d].node_off;
-    lt.ngroups[lt.nlev] = ng;
-    lt.boff[lt.nlev] = nb;
-    nb += 4 * (int)ceil_div(ng, kM2LCols);
-    adaptive = adaptive || t.lev[d].n_nodes != ((int64_t)1 << (2 * d));
-    ++lt.nlev;
-  }
-  lt.boff[lt.nlev] = nb;
-  lt.perm = nullptr;
-  if (adaptive) {
-    // column order by used offsets, once per tree
-    M2LOrderTable ot{};
-    ot.nlev = lt.nlev;
-    int ob = 0, po = 0;
-    for (int li = 0; li < lt.nlev; ++li) {
-      ot.g0[li] = lt.g0[li];
-      ot.ngroups[li] = lt.ngroups[li];
-      ot.nch[li] = (int)ceil_div(lt.ngroups[li], kM2LOrderChunk);
-      ot.boff[li] = ob;
-      ot.poff[li] = po;
-      lt.poff[li] = po;
-      ob += 4 * ot.nch[li];
-      po += 4 * lt.ngroups[li];
-    }
-    ot.boff[lt.nlev] = ob;
-    if (P.perm_tree != t.build_id) {
-      P.m2lperm.ensure((po + 1) * sizeof(int));
-      hipLaunchKernelGGL(k_m2l_order, dim3(ob), dim3(256), 0, s, ot, t.flags.as<int>(),
-                         t.m2l.as<int>(), P.m2lperm.as<int>());
-      FMM_LAUNCH_CHECK();
-      P.perm_tree = t.build_id;
-    }
-    lt.perm = P.m2lperm.as<int>();
-  }
-  if (P.precision != PREC_FP64) {
-    auto kern = PD == 32 ? k_m2l_all_f32<32> : k_m2l_all_f32<64>;
-    hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(kM2LThreads), 0, s, lt, t.flags.as<int>(),
-                       t.m2l.as<int>(), P.mult.as<double>(), P.qs.as<double>(),
-                       P.ops.as<double>(), opsz, P.ple.as<double>(), rstride, nstride);
-    FMM_LAUNCH_CHECK();
-  } else {
-    auto kern = PD == 32 ? k_m2l_all<32> : k_m2l_all<64>;
-    hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(kM2LThreads), 0, s, lt, t.flags.as<int>(),
-                       t.m2l.as<int>(), P.mult.as<double>(), P.ops.as<double>(), opsz,
-                       P.ple.as<double>(), rstride, nstride);
-    FMM_LAUNCH_CHECK();
-  }
+  const M2LTables T = m2l_tables(P);
+  m2l_prepare(P, T, s);
+  m2l_launch(P, T, 0, T.lt.nlev, s);
   for (int d = 2; d < t.L; ++d) {
     const int ngroups = (int)(t.lev[d].n_nodes / 4);
     if (ngroups == 0) continue;
