@@ -124,14 +124,20 @@ __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + 
 // Table-driven FP64 logarithm (P. T. P. Tang, "Table-driven implementation of
 // the logarithm function in IEEE floating-point arithmetic", ACM TOMS 16(4),
 // 1990), in an FMA variant split for accumulation:
-//   x = m 2^e, m in [0.5, 1) (frexp);  j = top 8 fraction bits of m;
-//   r = fma(m, T_j, -1) with T_j = 2/c_j, c_j = 1 + (j + 1/2)/256, |r| < 2^-9;
-//   log(x) = e ln2 + t,  t = -log(T_j) + log1p(r),
-// with log1p by its Taylor polynomial to r^5 (truncation < 1e-17).  Callers
-// accumulate sum(w e) and sum(w t) separately and multiply the first by ln 2
-// once.  The table {T_j, -log(T_j)} is filled on the host in extended precision.
-constexpr int kLogTableSize = 256;
+//   x = m 2^e, m in [0.5, 1) (frexp);  j = top 9 fraction bits of m;
+//   r = fma(m, T_j, -1) with T_j = 2/c_j, c_j = 1 + (j + 1/2)/512, |r| < 2^-10;
+//   log(x) = e ln2 + t,  t = -log(T_j) + r p(r),
+// with r p(r) = r (1 - r/2 + a3 r^2 + a4 r^3) the degree-4 polynomial closest
+// to log1p(r) in the max norm on |r| <= 2^-10 (error below 2.4e-17; a3, a4
+// fitted in extended precision), evaluated by Horner's rule with the table
+// term as the last addend: four FMAs.  Callers accumulate sum(w e) and
+// sum(w t) separately and multiply the first by ln 2 once.  The table
+// {T_j, -log(T_j)} is filled on the host in extended precision.
+constexpr int kLogTableBits = 9;
+constexpr int kLogTableSize = 1 << kLogTableBits;
 constexpr double kLn2 = 0.6931471805599453;
+constexpr double kLog1pA3 = 0.3333334990879393;
+constexpr double kLog1pA4 = -0.2500001566895226;
 
 struct LogTableHost {
   static void fill(double2* out);  // out[kLogTableSize]
@@ -144,13 +150,13 @@ __device__ __forceinline__ void log_split(double x, const double2* __restrict__ 
                                           double& t) {
   const double m = __builtin_amdgcn_frexp_mant(x);
   e = (double)__builtin_amdgcn_frexp_exp(x);
-  const int j = (__double2hiint(m) >> 12) & (kLogTableSize - 1);
+  const int j = (__double2hiint(m) >> (20 - kLogTableBits)) & (kLogTableSize - 1);
   const double2 c = tab[j];
   const double r = __builtin_fma(m, c.x, -1.0);
-  double q = __builtin_fma(r, 0.2, -0.25);
-  q = __builtin_fma(r, q, 1.0 / 3.0);
-  q = __builtin_fma(r, q, -0.5);
-  t = __builtin_fma(r * r, q, r) + c.y;
+  double q = __builtin_fma(kLog1pA4, r, kLog1pA3);
+  q = __builtin_fma(q, r, -0.5);
+  q = __builtin_fma(q, r, 1.0);
+  t = __builtin_fma(q, r, c.y);
 }
 
 __device__ __forceinline__ double log_tab(double x, const double2* __restrict__ tab) {
