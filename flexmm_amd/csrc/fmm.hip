@@ -196,9 +196,9 @@ __global__ __launch_bounds__(64 * kS2MWaves) void k_s2m(int nup, const int* __re
   if (r < 0 || d < 2) return;   // wave-uniform
   const double* lx = ops + d * opsz + (int64_t)OP_NBLK * PD * PD;
   const double* ly = lx + PD;
-  const double ax = k < p ? lx[k] : 1.0;
-  const double ay = k < p ? ly[k] : 1.0;
-  const double cx = xmid[g], cy = ymid[g];
+  // check point k in absolute coordinates (box centre + large-circle offset)
+  const double ax = (k < p ? lx[k] : 1.0) + xmid[g];
+  const double ay = (k < p ? ly[k] : 1.0) + ymid[g];
   const int b = __builtin_amdgcn_readfirstlane(bot[g]);
   const int t = __builtin_amdgcn_readfirstlane(top[g]);
   PotAcc<KER> acc;
@@ -209,15 +209,15 @@ __global__ __launch_bounds__(64 * kS2MWaves) void k_s2m(int nup, const int* __re
     for (int u = 0; u < 4; ++u) s4[u] = src[j + u];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const double dx = ax - (s4[u].x - cx);
-      const double dy = ay - (s4[u].y - cy);
+      const double dx = ax - s4[u].x;
+      const double dy = ay - s4[u].y;
       acc.add(s4[u].z, __builtin_fma(dy, dy, dx * dx), tab, kp);
     }
   }
   for (; j < t; ++j) {
     const double4 sj = src[j];
-    const double dx = ax - (sj.x - cx);
-    const double dy = ay - (sj.y - cy);
+    const double dx = ax - sj.x;
+    const double dy = ay - sj.y;
     acc.add(sj.z, __builtin_fma(dy, dy, dx * dx), tab, kp);
   }
   if (k < PD) check[(int64_t)r * PD + k] = k < p ? acc.result() : 0.0;
@@ -960,11 +960,42 @@ constexpr int kSmallLevelRows = 4096;
 
 // ---------------------------------------------------------------------------
 // Targets: locate, count per evaluation box, scatter into box order.
-// Each target gets its box and its rank inside the box.  Lanes of a wave that
-// land in the same box share one atomic (targets arrive spatially coherent, so
-// a wave touches few boxes).
+// Node lookup by (depth, cell code) for target location: direct on a full
+// level, a dense table on the other levels where the tree build made one,
+// else a binary search in the level's sorted codes.
+struct LocateTable {
+  int L;
+  double xmin, ymin, w0;
+  int levoff[kMaxLevels + 1];
+  int64_t tab_off[kMaxLevels];
+  const int* tab;
+  const uint64_t* code;
+};
+__device__ __forceinline__ int locate_node(const LocateTable& lt, int d, uint64_t c) {
+  const int64_t nl = lt.levoff[d + 1] - lt.levoff[d];
+  if (d < 16 && nl == ((int64_t)1 << (2 * d))) return lt.levoff[d] + (int)c;
+  if (lt.tab_off[d] >= 0) return lt.tab[lt.tab_off[d] + (int64_t)c];
+  const uint64_t* a = lt.code + lt.levoff[d];
+  int64_t lo = 0, hi = nl;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (a[mid] < c) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < nl && a[lo] == c) ? lt.levoff[d] + (int)lo : -1;
+}
+
+// Each target gets its box and its rank inside the box.  The box is the first
+// leaf on the target's root-to-leaf path (tree.py:712-727: descend while not
+// leaf, child 2*(x > xmid) + (y > ymid)).  The cell codes along the path come
+// from the reference's FP64 midpoint recurrence in registers (bit-identical to
+// the stored node centres), so the node lookups of four depths are issued
+// together instead of one dependent descent step per level.  Lanes of a wave
+// that land in the same box share one atomic (targets arrive spatially
+// coherent, so a wave touches few boxes).
 __global__ __launch_bounds__(256) void k_locate_count(const double* __restrict__ tx,
                                                      const double* __restrict__ ty, int64_t nt,
+                                                     LocateTable lt,
                                                      const int* __restrict__ flags,
                                                      const int* __restrict__ child,
                                                      const double* __restrict__ xmid,
@@ -978,8 +1009,47 @@ __global__ __launch_bounds__(256) void k_locate_count(const double* __restrict__
   if (act) {
     const double x = tx[i], y = ty[i];
     int g = 0;
-    for (int it = 0; it < kMaxLevels && !(flags[g] & NF_LEAF); ++it)
-      g = child[g] + 2 * (x > xmid[g]) + (y > ymid[g]);   // tree.py:720-725
+    bool found = (flags[0] & NF_LEAF) != 0;
+    if (!found) {
+      double xmin = lt.xmin, ymin = lt.ymin, w = lt.w0;
+      uint64_t c = 0;
+      for (int d0 = 1; d0 < lt.L; d0 += 4) {
+        int gg[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int d = d0 + u;
+          gg[u] = -1;
+          if (d < lt.L) {
+            const double h = 0.5 * w;
+            const double xm = xmin + h, ym = ymin + h;
+            const int bx = x > xm, by = y > ym;
+            c = (c << 2) | (uint64_t)(2 * bx + by);
+            xmin = bx ? xm : xm - h;
+            ymin = by ? ym : ym - h;
+            w = h;
+            gg[u] = locate_node(lt, d, c);
+          }
+        }
+        int f[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) f[u] = gg[u] >= 0 ? flags[gg[u]] : 0;
+        int hit = -1;
+#pragma unroll
+        for (int u = 3; u >= 0; --u)
+          if (f[u] & NF_LEAF) hit = u;
+        if (hit >= 0) {
+          g = gg[hit];
+          found = true;
+          break;
+        }
+      }
+    }
+    // the path always ends in a leaf; the pointer descent is kept as a guard
+    if (!found) {
+      g = 0;
+      for (int it = 0; it < kMaxLevels && !(flags[g] & NF_LEAF); ++it)
+        g = child[g] + 2 * (x > xmid[g]) + (y > ymid[g]);
+    }
     e = evalid[g];
   }
   const int lane = threadIdx.x & 63;
@@ -1554,16 +1624,15 @@ __global__ __launch_bounds__(64 * kS2MWaves) void k_s2m_stokes(int nup,
   if (r < 0 || d < 2) return;   // wave-uniform
   const double* lx = ops + d * opsz + ops_off_circles(PD);
   const double* ly = lx + PD;
-  const double ax = k < p ? lx[k] : 1.0;
-  const double ay = k < p ? ly[k] : 1.0;
-  const double cx = xmid[g], cy = ymid[g];
+  const double ax = (k < p ? lx[k] : 1.0) + xmid[g];   // check point, absolute
+  const double ay = (k < p ? ly[k] : 1.0) + ymid[g];
   const int b = __builtin_amdgcn_readfirstlane(bot[g]);
   const int t = __builtin_amdgcn_readfirstlane(top[g]);
   double acc[kStokesDens] = {};
   for (int j = b; j < t; ++j) {
     const double4 sj = src[j];
-    const double dx = ax - (sj.x - cx);
-    const double dy = ay - (sj.y - cy);
+    const double dx = ax - sj.x;
+    const double dy = ay - sj.y;
     const double lg = log_tab(__builtin_fma(dy, dy, dx * dx), tab);
     const double X = sj.x - c0x, Y = sj.y - c0y;
     const double w[kStokesDens] = {sj.z, sj.w, X * sj.z, X * sj.w, Y * sj.z, Y * sj.w};
@@ -2040,9 +2109,21 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   P.items.ensure(icap * sizeof(int2));
   P.ntot.ensure(4 * sizeof(int));
   FMM_HIP(hipMemsetAsync(P.tcnt.ptr, 0, (ne + 1) * sizeof(int), s));
-  hipLaunchKernelGGL(k_locate_count, dim3(grid256(nt)), dim3(256), 0, s, tx, ty, nt,
-                     t.flags.as<int>(), t.child.as<int>(), t.xmid.as<double>(), t.ymid.as<double>(),
-                     t.evalid.as<int>(), P.tnode.as<int>(), P.tcur.as<int>(), P.tcnt.as<int>());
+  LocateTable lt{};
+  lt.L = t.L;
+  lt.xmin = t.xmin;
+  lt.ymin = t.ymin;
+  lt.w0 = t.w0;
+  for (int d = 0; d <= t.L; ++d)
+    lt.levoff[d] = (int)(d < t.L ? t.lev[d].node_off : t.n_nodes);
+  for (int d = 0; d < kMaxLevels; ++d)
+    lt.tab_off[d] = d < (int)t.nidx_off.size() ? t.nidx_off[d] : -1;
+  lt.tab = t.nidx.as<int>();
+  lt.code = t.code.as<uint64_t>();
+  hipLaunchKernelGGL(k_locate_count, dim3(grid256(nt)), dim3(256), 0, s, tx, ty, nt, lt,
+                     t.flags.as<int>(), t.child.as<int>(), t.xmid.as<double>(),
+                     t.ymid.as<double>(), t.evalid.as<int>(), P.tnode.as<int>(),
+                     P.tcur.as<int>(), P.tcnt.as<int>());
   FMM_LAUNCH_CHECK();
   // toff has ne + 1 entries: scanning ne + 1 counts (the last is zero) gives toff[ne] = nt.
   exclusive_scan_i32(P.tcnt.as<int>(), P.toff.as<int>(), ne + 1, nullptr, P.pw, s);

@@ -151,6 +151,12 @@ __global__ __launch_bounds__(1024) void k_make_chunks(const int2* __restrict__ s
                                                      int* __restrict__ counters) {
   __shared__ int lds[1024 / 64];
   __shared__ int tot;
+  // counters of this round's segment classification (k_seg_offsets adds to
+  // them, after this kernel)
+  if (threadIdx.x == 0) {
+    counters[C_NEXT] = 0;
+    counters[C_ACTIVE] = 0;
+  }
   int carry = 0;
   for (int base = 0; base < nseg; base += 1024) {
     int si = base + threadIdx.x;
@@ -463,8 +469,6 @@ void sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
     w.segChunk0.ensure((int64_t)nseg * sizeof(int));
     int2* segCur = w.seg[cur].as<int2>();
     int2* segNext = w.seg[cur ^ 1].as<int2>();
-    FMM_HIP(hipMemsetAsync(cnt + C_NEXT, 0, sizeof(int), s));
-    FMM_HIP(hipMemsetAsync(cnt + C_ACTIVE, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_make_chunks, dim3(1), dim3(1024), 0, s, segCur, nseg,
                        w.chunks.as<int4>(), w.segChunk0.as<int>(), cnt);
     FMM_LAUNCH_CHECK();

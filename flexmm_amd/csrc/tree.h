@@ -69,6 +69,7 @@ struct Tree {
   DevBuf s_all, sref_all;       // u64 split-cell codes per depth (concatenated)
   DevBuf bal_bm, bal_pc;        // balance: cell bitmaps per depth, popcount scan
   DevBuf nidx;                  // dense node index of the non-full shallow levels
+  std::vector<int64_t> nidx_off; // level d's table in nidx (per cell code), or -1
   DevBuf xy;                    // interleaved input coordinates (input order)
   DevBuf bal_k, bal_v, bal_f;   // balance below the bitmap depths: candidates
   std::vector<DevBuf> bal_lists;

@@ -1063,6 +1063,7 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
     ni.off[d] = ntab;
     ntab += (int64_t)1 << (2 * d);
   }
+  t.nidx_off.assign(ni.off, ni.off + kMaxLevels);
   if (ntab > 0) {
     t.nidx.ensure(ntab * sizeof(int));
     FMM_HIP(hipMemsetAsync(t.nidx.ptr, 0xFF, ntab * sizeof(int), s));
