@@ -17,7 +17,7 @@ E_ARG, E_STATE, E_LIMIT, E_INTERNAL = -1, -2, -3, -4
 KERNEL_LAPLACE = 0
 KERNEL_MODIFIED_HELMHOLTZ = 1
 KERNEL_STOKESLET = 2
-EVAL_POTENTIAL, EVAL_GRADIENT, EVAL_STOKES = 0, 1, 2
+EVAL_POTENTIAL, EVAL_GRADIENT, EVAL_STOKES, EVAL_DIPOLE = 0, 1, 2, 3
 PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_MIXED_M2L = 2
@@ -42,6 +42,7 @@ SIGNATURES = {
     "fmm2d_set_operators": (_int, [_vp, _int, _vp, _i64, _vp]),
     "fmm2d_clear_operators": (_int, [_vp]),
     "fmm2d_upward": (_int, [_vp, _vp, _i64, _vp]),
+    "fmm2d_upward_dipole": (_int, [_vp, _vp, _vp, _i64, _vp]),
     "fmm2d_downward": (_int, [_vp, _vp]),
     "fmm2d_evaluate": (_int, [_vp, _vp, _vp, _i64, _int, _int, _vp, _vp]),
     "fmm2d_export_expansions": (_int, [_vp, _int, _vp, _i64, _vp]),

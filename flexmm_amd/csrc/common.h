@@ -190,6 +190,7 @@ enum EvalId : int {
   EVAL_POT = 0,      // 1 output: u
   EVAL_GRAD = 1,     // 3 outputs: u, du/dtx, du/dty
   EVAL_STOKES = 2,   // 3 outputs: velocity u, v and pressure p
+  EVAL_DIPOLE = 3,   // 1 output: u of Laplace single + double layer sources
 };
 enum Precision : int {
   PREC_FP64 = 0,

@@ -86,6 +86,8 @@ struct Plan {
   DevBuf logtab;
   DevBuf tau_s, src, check, mult, le, ple, qs;
   DevBuf srcf, tsum;   // mixed precision: leaf-relative FP32 sources, per-leaf charge sums
+  DevBuf dip;          // dipole moments of the sources in tree order (double layer)
+  bool have_dip = false;
   DevBuf m2lperm;      // M2L column order of an adaptive tree
   int64_t perm_tree = -1;   // tree build the column order belongs to
   bool have_srcf = false;
@@ -133,6 +135,33 @@ __global__ void k_pack_sources(const double* __restrict__ tau, const uint32_t* _
   packed[i] = make_double4(sx[i], sy[i], t, 0.0);
 }
 
+// 1/x to a few ulp: hardware reciprocal and two Newton steps (x > 0, normal)
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+// Single + double layer sources (examples/pure_kifmm_laplace_dipoles.py:32-40):
+// tau holds the single-layer charges [n] then the dipole strengths [n], dvec
+// the dipole directions, x [n] then y [n].  Packed in tree order as
+// (x, y, charge, 0) and the dipole moment strength * direction.
+__global__ void k_pack_dipole(const double* __restrict__ tau, const double* __restrict__ dvec,
+                              const uint32_t* __restrict__ perm, const double* __restrict__ sx,
+                              const double* __restrict__ sy, int64_t n,
+                              double* __restrict__ tau_s, double4* __restrict__ packed,
+                              double2* __restrict__ dip) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t j = perm[i];
+  const double t = tau[j], st = tau[n + j];
+  tau_s[i] = t;
+  packed[i] = make_double4(sx[i], sy[i], t, 0.0);
+  dip[i] = make_double2(st * dvec[j], st * dvec[n + j]);
+}
+
 // Accumulate w * log(|t - s|^2) = w * (e ln2 + lt) as (ae += w e, at += w lt).
 struct LogAcc {
   double ae = 0.0, at = 0.0;
@@ -173,7 +202,9 @@ struct PotAcc<KER_MODHELM> {
 // lane k = check point k on the large circle (helpers.py:159-173; the source
 // is taken relative to the box centre).
 constexpr int kS2MWaves = 4;
-template <int PD, int KER>
+// DIP: sources also carry a dipole moment q (the double layer of the Laplace
+// kernel, (q.d)/d^2/(2 pi) with d = check point - source).
+template <int PD, int KER, bool DIP>
 __global__ __launch_bounds__(64 * kS2MWaves) void k_s2m(int nup, const int* __restrict__ upnodes,
                                             const int* __restrict__ nlevel,
                                             const int* __restrict__ bot, const int* __restrict__ top,
@@ -183,6 +214,7 @@ __global__ __launch_bounds__(64 * kS2MWaves) void k_s2m(int nup, const int* __re
                                             const double4* __restrict__ src,
                                             const double* __restrict__ ops, int64_t opsz, int p,
                                             const double2* __restrict__ logtab, double kp,
+                                            const double2* __restrict__ dip,
                                             double* __restrict__ check) {
   __shared__ double2 tab[kLogTableSize];
   load_logtab(tab, logtab);
@@ -202,6 +234,7 @@ __global__ __launch_bounds__(64 * kS2MWaves) void k_s2m(int nup, const int* __re
   const int b = __builtin_amdgcn_readfirstlane(bot[g]);
   const int t = __builtin_amdgcn_readfirstlane(top[g]);
   PotAcc<KER> acc;
+  double dsum = 0.0;   // double layer
   int j = b;
   for (; j + 4 <= t; j += 4) {
     double4 s4[4];
@@ -211,16 +244,26 @@ __global__ __launch_bounds__(64 * kS2MWaves) void k_s2m(int nup, const int* __re
     for (int u = 0; u < 4; ++u) {
       const double dx = ax - s4[u].x;
       const double dy = ay - s4[u].y;
-      acc.add(s4[u].z, __builtin_fma(dy, dy, dx * dx), tab, kp);
+      const double d2 = __builtin_fma(dy, dy, dx * dx);
+      acc.add(s4[u].z, d2, tab, kp);
+      if constexpr (DIP) {
+        const double2 q = dip[j + u];
+        dsum = __builtin_fma(__builtin_fma(q.x, dx, q.y * dy), rcp_nr(d2), dsum);
+      }
     }
   }
   for (; j < t; ++j) {
     const double4 sj = src[j];
     const double dx = ax - sj.x;
     const double dy = ay - sj.y;
-    acc.add(sj.z, __builtin_fma(dy, dy, dx * dx), tab, kp);
+    const double d2 = __builtin_fma(dy, dy, dx * dx);
+    acc.add(sj.z, d2, tab, kp);
+    if constexpr (DIP) {
+      const double2 q = dip[j];
+      dsum = __builtin_fma(__builtin_fma(q.x, dx, q.y * dy), rcp_nr(d2), dsum);
+    }
   }
-  if (k < PD) check[(int64_t)r * PD + k] = k < p ? acc.result() : 0.0;
+  if (k < PD) check[(int64_t)r * PD + k] = k < p ? acc.result() + kInv2Pi * dsum : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1136,6 +1179,25 @@ struct Target {
   // without self-exclusion: the smallest high word of d^2 seen; 0 means a
   // coincident pair (or one closer than 1e-154, below the normal range of d^2)
   uint32_t dmin = 1u;
+  double dip = 0.0;   // double layer: sum (q.d)/d^2 (Laplace dipole sources)
+  // a Laplace source with charge w and dipole moment (qx, qy)
+  // (examples/pure_kifmm_laplace_dipoles.py:32-40)
+  __device__ __forceinline__ void pair_dip(double sx, double sy, double w, double qx, double qy,
+                                           const double2* __restrict__ tab) {
+    const double dx = x - sx;
+    const double dy = y - sy;
+    const double d2 = __builtin_fma(dy, dy, dx * dx);
+    const bool z = d2 == 0.0;
+    if (CHECK) {
+      w = z ? 0.0 : w;
+      qx = z ? 0.0 : qx;
+      qy = z ? 0.0 : qy;
+    } else {
+      dmin = min(dmin, (uint32_t)(__builtin_bit_cast(uint64_t, d2) >> 32));
+    }
+    u.add(w, d2, tab, kp);
+    dip = __builtin_fma(__builtin_fma(qx, dx, qy * dy), rcp_nr(z ? 1.0 : d2), dip);
+  }
   // one source (or one equivalent-density point) at (sx, sy) with weight w
   __device__ __forceinline__ void pair(double sx, double sy, double w,
                                        const double2* __restrict__ tab) {
@@ -1208,15 +1270,15 @@ __device__ __forceinline__ uint32_t slot_min(uint32_t v, int lg) {
   return v;
 }
 
-template <int PD, int KER, int NOUT, bool CHECK>
+template <int PD, int KER, int NOUT, bool CHECK, bool DIP>
 __device__ __forceinline__ void eval_few(
     int g, int t0, int na, const int* __restrict__ tperm, const double* __restrict__ tsx,
     const double* __restrict__ tsy, const int* __restrict__ nlevel,
     const double* __restrict__ xmid, const double* __restrict__ ymid,
     const int* __restrict__ coll, const int* __restrict__ bot, const int* __restrict__ top,
-    const double4* __restrict__ src, const double* __restrict__ le,
-    const double* __restrict__ ops, int64_t opsz, int p, const double2* __restrict__ tab,
-    double kp, int64_t nt, double* __restrict__ out) {
+    const double4* __restrict__ src, const double2* __restrict__ dip,
+    const double* __restrict__ le, const double* __restrict__ ops, int64_t opsz, int p,
+    const double2* __restrict__ tab, double kp, int64_t nt, double* __restrict__ out) {
   const FewLanes F = few_lanes(na);
   const int t = t0 + F.slot;
   Target<KER, NOUT, CHECK> T;
@@ -1245,10 +1307,15 @@ __device__ __forceinline__ void eval_few(
     const int e = __builtin_amdgcn_readfirstlane(top[c]);
     for (int j = b + F.sg; j < e; j += F.ng) {
       const double4 sj = src[j];
-      T.pair(sj.x, sj.y, sj.z, tab);
+      if constexpr (DIP) {
+        const double2 q = dip[j];
+        T.pair_dip(sj.x, sj.y, sj.z, q.x, q.y, tab);
+      } else {
+        T.pair(sj.x, sj.y, sj.z, tab);
+      }
     }
   }
-  double u = slot_sum(T.u.result(), F.lg);
+  double u = slot_sum(DIP ? __builtin_fma(kInv2Pi, T.dip, T.u.result()) : T.u.result(), F.lg);
   double gx = 0.0, gy = 0.0;
   if (NOUT == 3) {
     gx = slot_sum(T.gx, F.lg);
@@ -1257,7 +1324,7 @@ __device__ __forceinline__ void eval_few(
   const bool zh = CHECK ? false : slot_min(T.dmin, F.lg) == 0u;
   if (F.act && F.sg == 0) {
     const int64_t o = tperm[t];
-    if (zh) u = __builtin_inf();
+    if (zh) u = DIP ? __builtin_nan("") : __builtin_inf();
     out[o] = u;
     if (NOUT == 3) {
       out[nt + o] = T.gscale() * gx;
@@ -1266,7 +1333,7 @@ __device__ __forceinline__ void eval_few(
   }
 }
 
-template <int PD, int KER, int NOUT, bool CHECK>
+template <int PD, int KER, int NOUT, bool CHECK, bool DIP>
 __global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
     const int2* __restrict__ items, const int* __restrict__ nitems,
     const int* __restrict__ evalnode, const int* __restrict__ toff,
@@ -1274,6 +1341,7 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
     const int* __restrict__ nlevel, const double* __restrict__ xmid,
     const double* __restrict__ ymid, const int* __restrict__ coll, const int* __restrict__ bot,
     const int* __restrict__ top, const double4* __restrict__ src,
+    const double2* __restrict__ dip,
     const double* __restrict__ le, const double* __restrict__ ops, int64_t opsz, int p,
     const double2* __restrict__ logtab, double kp, int64_t nt, double* __restrict__ out) {
   __shared__ double2 tab[kLogTableSize];
@@ -1287,8 +1355,9 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
     const int t0 = toff[it.x] + it.y * kTgtChunk;
     const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
     if (t1 - t0 <= kFewTargets) {
-      eval_few<PD, KER, NOUT, CHECK>(g, t0, t1 - t0, tperm, tsx, tsy, nlevel, xmid, ymid, coll,
-                                     bot, top, src, le, ops, opsz, p, tab, kp, nt, out);
+      eval_few<PD, KER, NOUT, CHECK, DIP>(g, t0, t1 - t0, tperm, tsx, tsy, nlevel, xmid, ymid,
+                                          coll, bot, top, src, dip, le, ops, opsz, p, tab, kp,
+                                          nt, out);
       continue;
     }
     const int t = t0 + (int)(threadIdx.x & 63);
@@ -1320,6 +1389,14 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
       const int b = __builtin_amdgcn_readfirstlane(bot[c]);
       const int e = __builtin_amdgcn_readfirstlane(top[c]);
       int j = b;
+      if constexpr (DIP) {
+        for (; j < e; ++j) {
+          const double4 sj = src[j];
+          const double2 q = dip[j];
+          T.pair_dip(sj.x, sj.y, sj.z, q.x, q.y, tab);
+        }
+        continue;
+      }
       for (; j + 4 <= e; j += 4) {
         double4 s4[4];
   #pragma unroll
@@ -1334,8 +1411,9 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
     }
     if (act) {
       const int64_t o = tperm[t];
-      double u = T.u.result();
-      if (!CHECK && T.zhit()) u = __builtin_inf();   // log(0): the reference's -log(0) = +inf
+      double u = DIP ? __builtin_fma(kInv2Pi, T.dip, T.u.result()) : T.u.result();
+      // log(0): the reference's -log(0) = +inf; with a dipole also 0/0
+      if (!CHECK && T.zhit()) u = DIP ? __builtin_nan("") : __builtin_inf();
       out[o] = u;
       if (NOUT == 3) {
         out[nt + o] = T.gscale() * T.gx;
@@ -1646,15 +1724,6 @@ __global__ __launch_bounds__(64 * kS2MWaves) void k_s2m_stokes(int nup,
   }
 }
 
-// 1/x to a few ulp: hardware reciprocal and two Newton steps (x > 0, normal)
-__device__ __forceinline__ double rcp_nr(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = __builtin_fma(-x, r, 1.0);
-  r = __builtin_fma(r, e, r);
-  e = __builtin_fma(-x, r, 1.0);
-  return __builtin_fma(r, e, r);
-}
-
 // One item of k_eval_stokes: L2T of the six local densities + Stokeslet P2P,
 // outputs (u, v, p).  FEW selects the (target slot, source group) lane layout
 // of eval_few; otherwise lane = target and the source loop index is uniform.
@@ -1805,11 +1874,18 @@ inline int eval_grid(int64_t icap) {
 }
 
 // Selects the k_eval instance of (PD, kernel, outputs, self-exclusion).
-typedef decltype(&k_eval<32, KER_LAPLACE, 1, false>) EvalKernelFn;
+typedef decltype(&k_eval<32, KER_LAPLACE, 1, false, false>) EvalKernelFn;
 template <int PD, int KER>
 EvalKernelFn eval_kernel_k(int nout, bool check) {
-  if (nout == 1) return check ? k_eval<PD, KER, 1, true> : k_eval<PD, KER, 1, false>;
-  return check ? k_eval<PD, KER, 3, true> : k_eval<PD, KER, 3, false>;
+  if (nout == 1)
+    return check ? k_eval<PD, KER, 1, true, false> : k_eval<PD, KER, 1, false, false>;
+  return check ? k_eval<PD, KER, 3, true, false> : k_eval<PD, KER, 3, false, false>;
+}
+// Laplace single + double layer sources
+EvalKernelFn eval_kernel_dipole(int PD, bool check) {
+  if (PD == 32)
+    return check ? k_eval<32, KER_LAPLACE, 1, true, true> : k_eval<32, KER_LAPLACE, 1, false, true>;
+  return check ? k_eval<64, KER_LAPLACE, 1, true, true> : k_eval<64, KER_LAPLACE, 1, false, true>;
 }
 EvalKernelFn eval_kernel(int PD, int ker, int nout, bool check) {
   if (ker == KER_LAPLACE)
@@ -1973,7 +2049,9 @@ static void m2l_launch(Plan& P, const M2LTables& T, int l0, int l1, hipStream_t 
   FMM_LAUNCH_CHECK();
 }
 
-static void upward(Plan& P, const double* tau, hipStream_t s) {
+// dvec: dipole directions (x [n] then y [n]) when the sources carry single
+// and double layer densities (tau: charges [n] then dipole strengths [n]).
+static void upward(Plan& P, const double* tau, const double* dvec, hipStream_t s) {
   Tree& t = P.tree;
   check_ops(P);
   const int PD = P.PD;
@@ -1987,12 +2065,18 @@ static void upward(Plan& P, const double* tau, hipStream_t s) {
     hipLaunchKernelGGL(k_pack_stokes, dim3(grid256(t.n)), dim3(256), 0, s, tau,
                        t.perm.as<uint32_t>(), t.sx.as<double>(), t.sy.as<double>(), t.n,
                        P.src.as<double4>());
+  } else if (dvec) {
+    P.dip.ensure((t.n + 1) * sizeof(double2));
+    hipLaunchKernelGGL(k_pack_dipole, dim3(grid256(t.n)), dim3(256), 0, s, tau, dvec,
+                       t.perm.as<uint32_t>(), t.sx.as<double>(), t.sy.as<double>(), t.n,
+                       P.tau_s.as<double>(), P.src.as<double4>(), P.dip.as<double2>());
   } else {
     hipLaunchKernelGGL(k_pack_sources, dim3(grid256(t.n)), dim3(256), 0, s, tau,
                        t.perm.as<uint32_t>(), t.sx.as<double>(), t.sy.as<double>(), t.n,
                        P.tau_s.as<double>(), P.src.as<double4>());
   }
   FMM_LAUNCH_CHECK();
+  P.have_dip = dvec != nullptr;
   P.up_done = true;
   const int64_t nr = t.n_rows;
   const int64_t rstride = nr + 1;   // rows per density
@@ -2014,13 +2098,16 @@ static void upward(Plan& P, const double* tau, hipStream_t s) {
                        rstride, P.check.as<double>());
     FMM_LAUNCH_CHECK();
   } else if (t.n_up > 0) {
-    auto kern = P.kernel_id == KER_LAPLACE ? (PD == 32 ? k_s2m<32, KER_LAPLACE> : k_s2m<64, KER_LAPLACE>)
-                                           : (PD == 32 ? k_s2m<32, KER_MODHELM> : k_s2m<64, KER_MODHELM>);
+    auto kern = dvec ? (PD == 32 ? k_s2m<32, KER_LAPLACE, true> : k_s2m<64, KER_LAPLACE, true>)
+                : P.kernel_id == KER_LAPLACE
+                    ? (PD == 32 ? k_s2m<32, KER_LAPLACE, false> : k_s2m<64, KER_LAPLACE, false>)
+                    : (PD == 32 ? k_s2m<32, KER_MODHELM, false> : k_s2m<64, KER_MODHELM, false>);
     hipLaunchKernelGGL(kern, dim3(ceil_div(t.n_up, kS2MWaves)), dim3(64 * kS2MWaves), 0, s,
                        (int)t.n_up, t.upnodes.as<int>(), t.nlevel.as<int>(),
                        t.bot.as<int>(), t.top.as<int>(), t.row.as<int>(), t.xmid.as<double>(),
                        t.ymid.as<double>(), P.src.as<double4>(), P.ops.as<double>(), opsz, P.p,
-                       P.logtab.as<double2>(), P.kparam, P.check.as<double>());
+                       P.logtab.as<double2>(), P.kparam, dvec ? P.dip.as<double2>() : nullptr,
+                       P.check.as<double>());
     FMM_LAUNCH_CHECK();
   }
   P.mark(FMM2D_PHASE_S2M, true, s);
@@ -2091,8 +2178,12 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   Tree& t = P.tree;
   FMM_REQUIRE(P.have_exp, FMM_E_STATE, "no expansions: call fmm2d_upward and fmm2d_downward first");
   const bool stokes = P.kernel_id == KER_STOKES;
-  FMM_REQUIRE(stokes ? ev == EVAL_STOKES : (ev == EVAL_POT || ev == EVAL_GRAD), FMM_E_ARG,
-              "evaluator %d does not belong to kernel %d", ev, P.kernel_id);
+  FMM_REQUIRE(stokes ? ev == EVAL_STOKES
+                     : (ev == EVAL_POT || ev == EVAL_GRAD ||
+                        (ev == EVAL_DIPOLE && P.kernel_id == KER_LAPLACE)),
+              FMM_E_ARG, "evaluator %d does not belong to kernel %d", ev, P.kernel_id);
+  FMM_REQUIRE(ev != EVAL_DIPOLE || P.have_dip, FMM_E_STATE,
+              "the dipole evaluator needs expansions from fmm2d_upward_dipole");
   FMM_REQUIRE(nt >= 0 && nt < ((int64_t)1 << 31), FMM_E_LIMIT, "too many targets");
   if (nt == 0) return;
   const int64_t ne = t.n_eval;
@@ -2186,13 +2277,15 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
     P.mark(FMM2D_PHASE_EVAL_KERNEL, true, s);
     return;
   }
-  auto kern = eval_kernel(PD, P.kernel_id, ev == EVAL_POT ? 1 : 3, check);
+  auto kern = ev == EVAL_DIPOLE ? eval_kernel_dipole(PD, check)
+                                : eval_kernel(PD, P.kernel_id, ev == EVAL_POT ? 1 : 3, check);
   P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
   hipLaunchKernelGGL(kern, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
                      t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(), P.tsx.as<double>(),
                      P.tsy.as<double>(), t.nlevel.as<int>(), t.xmid.as<double>(),
                      t.ymid.as<double>(), t.coll.as<int>(), t.bot.as<int>(), t.top.as<int>(),
-                     P.src.as<double4>(), P.le.as<double>(), P.ops.as<double>(), opsz, P.p,
+                     P.src.as<double4>(), ev == EVAL_DIPOLE ? P.dip.as<double2>() : nullptr,
+                     P.le.as<double>(), P.ops.as<double>(), opsz, P.p,
                      P.logtab.as<double2>(), P.kparam, nt, out);
   FMM_LAUNCH_CHECK();
   P.mark(FMM2D_PHASE_EVAL_KERNEL, true, s);
@@ -2420,7 +2513,24 @@ int fmm2d_upward(void* plan, const double* tau, int64_t n, void* stream) {
   FMM_REQUIRE(n == P->tree.n, fmm::FMM_E_ARG, "tau has %lld entries, tree has %lld sources",
               (long long)n, (long long)P->tree.n);
   P->mark(FMM2D_PHASE_UPWARD, false, (hipStream_t)stream);
-  fmm::upward(*P, tau, (hipStream_t)stream);
+  fmm::upward(*P, tau, nullptr, (hipStream_t)stream);
+  P->mark(FMM2D_PHASE_UPWARD, true, (hipStream_t)stream);
+  ABI_END
+}
+
+int fmm2d_upward_dipole(void* plan, const double* tau, const double* dvec, int64_t n,
+                        void* stream) {
+  ABI_BEGIN
+  Plan* P = as_plan(plan);
+  NEED(tau);
+  NEED(dvec);
+  FMM_REQUIRE(P->kernel_id == fmm::KER_LAPLACE, fmm::FMM_E_ARG,
+              "dipole sources need the Laplace kernel (plan has kernel %d)", P->kernel_id);
+  FMM_REQUIRE(P->tree.built, fmm::FMM_E_STATE, "no tree");
+  FMM_REQUIRE(n == P->tree.n, fmm::FMM_E_ARG, "n = %lld, tree has %lld sources", (long long)n,
+              (long long)P->tree.n);
+  P->mark(FMM2D_PHASE_UPWARD, false, (hipStream_t)stream);
+  fmm::upward(*P, tau, dvec, (hipStream_t)stream);
   P->mark(FMM2D_PHASE_UPWARD, true, (hipStream_t)stream);
   ABI_END
 }

@@ -84,6 +84,50 @@ laplace_gradient = Evaluator(
     "laplace_gradient", laplace, _lib.EVAL_GRADIENT, 3, _laplace_grad,
     "(u, du/dx_t, du/dy_t) (examples/pure_kifmm_laplace_gradient.py:32-55).")
 
+class SourceKernel:
+    """An upwardor kernel KAS(sx, sy, tx, ty, tau, extras) -> (value,) whose
+    sources carry several densities and per-source extras (register_upwardor,
+    helpers.py:155-174), with a device implementation."""
+
+    def __init__(self, name, kernel, ntau, nextras, fn, doc):
+        self.name = name
+        self.kernel = kernel
+        self.ntau = ntau
+        self.nextras = nextras
+        self._fn = fn
+        self.__doc__ = doc
+
+    def __call__(self, sx, sy, tx, ty, tau, extras):
+        return self._fn(np.asarray(sx, dtype=np.float64), np.asarray(sy, dtype=np.float64),
+                        np.asarray(tx, dtype=np.float64), np.asarray(ty, dtype=np.float64),
+                        np.asarray(tau, dtype=np.float64), np.asarray(extras, dtype=np.float64))
+
+    def __repr__(self):
+        return "SourceKernel(%s)" % self.name
+
+
+def _laplace_slp_dlp(sx, sy, tx, ty, tau, extras):
+    dx = tx - sx
+    dy = ty - sy
+    d2 = dx * dx + dy * dy
+    return (-np.log(d2) / FOUR_PI * tau[0]
+            + (extras[0] * dx + extras[1] * dy) / d2 / TWO_PI * tau[1],)
+
+
+laplace_slp_dlp = SourceKernel(
+    "laplace_slp_dlp", laplace, 2, 2, _laplace_slp_dlp,
+    "Single + double layer Laplace sources, -log(d^2)/(4 pi) tau[0] + (n.d)/d^2/(2 pi) tau[1], "
+    "d = t - s, n = (extras[0], extras[1]) (Laplace_Both_Eval, "
+    "examples/pure_kifmm_laplace_dipoles.py:32-40).")
+
+laplace_slp_dlp_potential = Evaluator(
+    "laplace_slp_dlp_potential", laplace, _lib.EVAL_DIPOLE, 1,
+    lambda *a: (_laplace(*a),),
+    "Potential of single + double layer sources: L2T with the single-layer "
+    "kernel on the local density, P2P with laplace_slp_dlp (the 'slp+dlp->u' "
+    "evaluator of examples/pure_kifmm_laplace_dipoles.py:140).")
+
+
 def _stokeslet_u(sx, sy, tx, ty):
     """First row of the Stokeslet for a unit x-force (stokes_fmm.py:33-61)."""
     dx = tx - sx
@@ -98,7 +142,9 @@ stokeslet = Kernel(
     "(kifmm2d/vector/stokes_fmm.py:33-61); used through flexmm_amd.kifmm2d.StokesFMM.")
 
 KERNELS = {k.name: k for k in (laplace,)}
-EVALUATORS = {e.name: e for e in (laplace_potential, laplace_gradient)}
+EVALUATORS = {e.name: e for e in (laplace_potential, laplace_gradient,
+                                   laplace_slp_dlp_potential)}
+SOURCE_KERNELS = {k.name: k for k in (laplace_slp_dlp,)}
 
 _MH = {}
 
@@ -166,7 +212,23 @@ def default_evaluator(kernel):
 
 
 def resolve_evaluator(kernel, KAS1, KAS2, outputs):
-    """Map register_evaluator's (KAS1, KAS2, outputs) to a device evaluator."""
+    """Map register_evaluator's (KAS1, KAS2, outputs) to a device evaluator.
+    KAS2 may be a SourceKernel (P2P with the sources' own densities and
+    extras); KAS1 is then the kernel's potential evaluator (the L2T kernel)."""
+    if isinstance(KAS2, SourceKernel):
+        if KAS2.kernel is not kernel:
+            raise ValueError("source kernel %s belongs to %s, not %s" % (KAS2.name, KAS2.kernel.name, kernel.name))
+        if KAS1 is not None and KAS1 is not kernel and not (
+                isinstance(KAS1, Evaluator) and KAS1.kernel is kernel
+                and KAS1.evaluator_id == _lib.EVAL_POTENTIAL):
+            raise NotImplementedError("with a source kernel, KAS1 must be the kernel's potential "
+                                      "evaluator (the L2T kernel), got %r" % (KAS1,))
+        if outputs != 1:
+            raise ValueError("%s produces 1 output, not %d" % (KAS2.name, outputs))
+        for e in EVALUATORS.values():
+            if e.kernel is kernel and e.evaluator_id == _lib.EVAL_DIPOLE:
+                return e
+        raise NotImplementedError("no device evaluator for %r" % (KAS2,))
     cands = [e for e in (KAS1, KAS2) if e is not None]
     if not cands or not all(isinstance(e, Evaluator) for e in cands):
         raise NotImplementedError(

@@ -45,6 +45,11 @@ extern "C" {
 #define FMM2D_EVAL_POTENTIAL 0 /* 1 output: u */
 #define FMM2D_EVAL_STOKES 2    /* 3 outputs: velocity u, v and pressure p (Stokeslet kernel
                                   only; stokes_fmm.py:827-840) */
+#define FMM2D_EVAL_DIPOLE 3    /* 1 output: u of Laplace single + double layer sources
+                                  (needs fmm2d_upward_dipole; the evaluator
+                                  'slp+dlp->u' of examples/pure_kifmm_laplace_dipoles.py:
+                                  139-145); a target on a source without self-exclusion
+                                  gives NaN */
 #define FMM2D_EVAL_GRADIENT 1  /* 3 outputs: u, du/dx_t, du/dy_t
                                   (examples/pure_kifmm_laplace_gradient.py:32-55,
                                   pure_kifmm_modified_helmholtz_gradient.py:45-68) */
@@ -134,6 +139,17 @@ int fmm2d_clear_operators(void* plan);
  * kernel 2n values: the forces f_x[n] then f_y[n]).
  * Replaces FMM.upwards_pass (fmm.py:73-124; stokes_fmm.py:689-718). */
 int fmm2d_upward(void* plan, const double* tau, int64_t n, void* stream);
+
+/* Upward pass for Laplace sources carrying a single and a double layer
+ * density: tau = 2n values (charges, then dipole strengths), dvec = 2n values
+ * (dipole directions: x components, then y components), both in input order.
+ * Source kernel -log(d^2)/(4 pi) tau_0 + (n.d)/d^2/(2 pi) tau_1, d = t - s.
+ * Replaces FMM.upwards_pass with a registered upwardor and extras
+ * (fmm.py:73-124 with extras, helpers.py:155-174; the 'slp+dlp' upwardor of
+ * examples/pure_kifmm_laplace_dipoles.py:32-40,139-141).  Evaluate with
+ * FMM2D_EVAL_DIPOLE. */
+int fmm2d_upward_dipole(void* plan, const double* tau, const double* dvec, int64_t n,
+                        void* stream);
 
 /* Downward pass.  Replaces FMM.downwards_pass (fmm.py:125-166). */
 int fmm2d_downward(void* plan, void* stream);

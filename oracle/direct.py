@@ -71,3 +71,34 @@ def direct_stokes(sx, sy, fx, fy, tx, ty, check=False, chunk_pairs=20_000_000):
         out[1, s:s + step] = vv.sum(axis=1) * (0.25 / np.pi)
         out[2, s:s + step] = pp.sum(axis=1) * (0.5 / np.pi)
     return out
+
+
+def direct_src(src, sx, sy, tau_rows, extras_rows, tx, ty, check=False, chunk_pairs=20_000_000):
+    """(nout, nt) array of sum_s src(s, t, tau_s, extras_s) for a source kernel
+    ``src(sx, sy, tx, ty, tau_rows, extras_rows) -> tuple`` already weighted by
+    the densities (the upwardor/evaluator kernels of register_upwardor,
+    helpers.py:155-174); check skips exactly coincident points.
+
+    Pinned by: tiny explicit-loop cases (tests/test_oracle_kernels.py)."""
+    sx, sy, tx, ty = [np.asarray(a, dtype=np.float64).ravel() for a in (sx, sy, tx, ty)]
+    tau_rows = np.asarray(tau_rows, dtype=np.float64).reshape(-1, sx.size)
+    ex = None if extras_rows is None else np.asarray(extras_rows, dtype=np.float64).reshape(-1, sx.size)
+    out = None
+    step = max(1, chunk_pairs // max(1, sx.size))
+    for s in range(0, tx.size, step):
+        ttx = tx[s:s + step, None]
+        tty = ty[s:s + step, None]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            vals = src(sx[None, :], sy[None, :], ttx, tty, tau_rows[:, None, :],
+                       None if ex is None else ex[:, None, :])
+        if out is None:
+            out = np.zeros((len(vals), tx.size))
+        if check:
+            keep = ~((sx[None, :] == ttx) & (sy[None, :] == tty))
+        for k, v in enumerate(vals):
+            if check:
+                v = np.where(keep, v, 0.0)
+            out[k, s:s + step] = v.sum(axis=1)
+    if out is None:
+        out = np.zeros((1, tx.size))
+    return out

@@ -6,7 +6,9 @@ argument convention as the user kernels of the reference examples
 
 Pinned by: closed forms (``tests/test_oracle_kernels.py``) -- the Laplace
 kernel against ``-log|x-y|/(2 pi)``, gradients against central finite
-differences, the modified-Helmholtz kernel against ``scipy.special.k0/k1``.
+differences, the modified-Helmholtz kernel against ``scipy.special.k0/k1``,
+the double-layer term against the finite-difference derivative of the
+single layer with respect to the source position.
 """
 import numpy as np
 from scipy import special
@@ -34,6 +36,17 @@ def laplace_gradient(sx, sy, tx, ty):
     id2 = 1.0 / d2
     scale = -1.0 / TWO_PI
     return (-np.log(d2) / FOUR_PI, scale * dx * id2, scale * dy * id2)
+
+
+def laplace_slp_dlp(sx, sy, tx, ty, tau, extras):
+    """Single plus double layer: -log(d^2)/(4 pi) tau[0] + (n.d)/d^2/(2 pi) tau[1]
+    with n = (extras[0], extras[1]) and d = t - s; the upwardor/evaluator
+    kernel Laplace_Both_Eval of examples/pure_kifmm_laplace_dipoles.py:32-40.
+    Returns a 1-tuple like the reference's KAS."""
+    dx = tx - sx
+    dy = ty - sy
+    d2 = dx * dx + dy * dy
+    return (-np.log(d2) / FOUR_PI * tau[0] + (extras[0] * dx + extras[1] * dy) / d2 / TWO_PI * tau[1],)
 
 
 def make_modified_helmholtz(k):

@@ -153,17 +153,37 @@ class FMM:
         self.bbox = get_bbox(self.x, self.y, bbox, widths)
         self.tree = Tree(self.x, self.y, Ncutoff, self.bbox)
         self.precs = [get_precomputation(K, Nequiv, lev.width) for lev in self.tree.Levels]
-        self.evaluators = {"potential_evaluation": (lambda sx, sy, tx, ty: (K(sx, sy, tx, ty),), 1)}
+        self.evaluators = {"potential_evaluation": (lambda sx, sy, tx, ty: (K(sx, sy, tx, ty),), 1,
+                                                    None)}
 
-    def register_evaluator(self, Kvec, name, outputs):
-        self.evaluators[name] = (Kvec, outputs)
+    def register_evaluator(self, Kvec, name, outputs, src=None):
+        """Kvec: the L2T kernel applied to the local density (KAS1 of
+        helpers.py:176-238).  src (optional): the P2P kernel of the sources,
+        ``src(sx, sy, tx, ty, tau_rows, extras_rows) -> tuple`` already
+        weighted by the densities (KAS2, e.g. the slp+dlp kernel of
+        examples/pure_kifmm_laplace_dipoles.py:32-40); without it P2P is
+        Kvec times the single density."""
+        self.evaluators[name] = (Kvec, outputs, src)
 
     # -- upward pass: fmm.py:73-124 -------------------------------------
-    def upwards_pass(self, tau):
+    def upwards_pass(self, tau, src=None, extras=None):
+        """tau: (n,) charges, or (k, n) densities with src, the upwardor's
+        kernel ``src(sx, sy, tx, ty, tau_rows, extras_rows) -> value``
+        (register_upwardor, helpers.py:155-174; extras (m, n), fmm.py:80-90)."""
         tree = self.tree
         p = self.Nequiv
-        tau = np.asarray(tau, dtype=np.float64).reshape(-1)
-        self.tau_ordered = tau[tree.ordv]
+        tau = np.asarray(tau, dtype=np.float64)
+        self.src = src
+        if src is None:
+            tau = tau.reshape(-1)
+            self.tau_ordered = tau[tree.ordv]
+            self.tau_rows = self.tau_ordered[None, :]
+        else:
+            tau = tau.reshape(-1, tree.x.size)
+            self.tau_rows = tau[:, tree.ordv]
+            self.tau_ordered = self.tau_rows[0]
+        self.extras_rows = None if extras is None else \
+            np.asarray(extras, dtype=np.float64).reshape(-1, tree.x.size)[:, tree.ordv]
         self.multipoles = [None] * tree.levels
         self.reshaped = [None] * tree.levels
         self.check_potentials = [None] * tree.levels
@@ -202,8 +222,13 @@ class FMM:
             ob = boxes[owner[sl]]
             sx = tree.x[pp] - lev.xmid[ob]
             sy = tree.y[pp] - lev.ymid[ob]
-            vals = self.K(sx[:, None], sy[:, None], prec.large_x[None, :], prec.large_y[None, :])
-            vals *= self.tau_ordered[pp][:, None]
+            if self.src is None:
+                vals = self.K(sx[:, None], sy[:, None], prec.large_x[None, :], prec.large_y[None, :])
+                vals *= self.tau_ordered[pp][:, None]
+            else:
+                ex = None if self.extras_rows is None else self.extras_rows[:, pp][:, :, None]
+                vals = self.src(sx[:, None], sy[:, None], prec.large_x[None, :],
+                                prec.large_y[None, :], self.tau_rows[:, pp][:, :, None], ex)[0]
             _segment_add(acc, owner[sl], vals)
         partial[rows] += acc
 
@@ -239,9 +264,9 @@ class FMM:
                 sorter = np.argsort(lev.children_ind[doit])
                 self.PLE[ind + 1] = ple[sorter].reshape(tree.Levels[ind + 1].n_node, p)
 
-    def build_expansions(self, tau):
+    def build_expansions(self, tau, src=None, extras=None):
         """fmm.py:168-170."""
-        self.upwards_pass(tau)
+        self.upwards_pass(tau, src, extras)
         self.downwards_pass()
 
     def source_evaluation(self, x, y):
@@ -253,13 +278,13 @@ class FMM:
     # -- evaluation: fmm.py:182-199 -------------------------------------
     def evaluate_to_points(self, x, y, name, check):
         tree = self.tree
-        Kv, nout = self.evaluators[name]
+        Kv, nout, src = self.evaluators[name]
         x = np.asarray(x, dtype=np.float64).ravel()
         y = np.asarray(y, dtype=np.float64).ravel()
         inds, locs = tree.locate_points(x, y)
         pot = np.zeros((nout, x.size))
         self._l2t(x, y, inds, locs, Kv, pot)
-        self._p2p(x, y, inds, locs, Kv, pot, check)
+        self._p2p(x, y, inds, locs, Kv, pot, check, src)
         return pot
 
     def _l2t(self, x, y, inds, locs, Kv, pot, chunk=4_000_000):
@@ -283,7 +308,7 @@ class FMM:
                 for k, o in enumerate(outs):
                     pot[k, ss] += (o * le).sum(axis=1)
 
-    def _p2p(self, x, y, inds, locs, Kv, pot, check):
+    def _p2p(self, x, y, inds, locs, Kv, pot, check, srck=None):
         # neighbor_evaluation, helpers.py:180-217: per target, the sources of
         # the 9 colleagues of its leaf; skip exact coincidences when check.
         tree = self.tree
@@ -299,16 +324,27 @@ class FMM:
             lev = tree.Levels[ind]
             col = lev.colleagues[loc]
             col = col[col >= 0]
-            src = np.concatenate([np.arange(lev.bot_ind[c], lev.top_ind[c]) for c in col]) if col.size else np.empty(0, dtype=np.int64)
-            if src.size == 0:
+            idx = np.concatenate([np.arange(lev.bot_ind[c], lev.top_ind[c]) for c in col]) if col.size else np.empty(0, dtype=np.int64)
+            if idx.size == 0:
                 continue
             tx = x[g][:, None]
             ty = y[g][:, None]
-            sx = sx_all[src][None, :]
-            sy = sy_all[src][None, :]
+            sx = sx_all[idx][None, :]
+            sy = sy_all[idx][None, :]
+            if srck is not None:
+                # the sources' own kernel, weighted by their densities (KAS2)
+                ex = None if self.extras_rows is None else self.extras_rows[:, idx][:, None, :]
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    outs = srck(sx, sy, tx, ty, self.tau_rows[:, idx][:, None, :], ex)
+                if check:
+                    keep = ~((sx == tx) & (sy == ty))
+                    outs = [np.where(keep, o, 0.0) for o in outs]
+                for k, o in enumerate(outs):
+                    pot[k, g] += o.sum(axis=1)
+                continue
             with np.errstate(divide="ignore", invalid="ignore"):
                 outs = Kv(sx, sy, tx, ty)
-            w = np.broadcast_to(tau[src][None, :], (g.size, src.size))
+            w = np.broadcast_to(tau[idx][None, :], (g.size, idx.size))
             if check:
                 keep = ~((sx == tx) & (sy == ty))
                 w = np.where(keep, w, 0.0)

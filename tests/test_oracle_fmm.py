@@ -195,3 +195,24 @@ def test_modified_helmholtz_vs_direct():
     g = f.evaluate_to_points(tx, ty, "gradient", False)
     for k in range(3):
         assert _err(g[k], ref[k]) < 1e-11
+
+
+def test_slp_dlp_upwardor_p64():
+    """Upwardor with extras (the dipole example, examples/pure_kifmm_laplace_dipoles.py:
+    139-151): single + double layer sources, self and target evaluation, against
+    FP64 direct sums of the same source kernel."""
+    x, y, _ = H.uniform(4000, 54)
+    rng = np.random.default_rng(55)
+    tau = rng.random((2, x.size))
+    nx = rng.random(x.size)
+    ex = np.stack([nx, np.sqrt(1 - nx ** 2)])
+    tx, ty, _ = H.uniform(400, 56)
+    f = okifmm.FMM(x, y, okern.laplace, 40, 64)
+    f.register_evaluator(lambda *a: (okern.laplace(*a),), "slp+dlp->u", 1, src=okern.laplace_slp_dlp)
+    f.build_expansions(tau, okern.laplace_slp_dlp, ex)
+    u = f.evaluate_to_points(tx, ty, "slp+dlp->u", False)[0]
+    ref = odirect.direct_src(okern.laplace_slp_dlp, x, y, tau, ex, tx, ty)[0]
+    assert _err(u, ref) < 1e-12
+    us = f.evaluate_to_points(x[:300], y[:300], "slp+dlp->u", True)[0]
+    refs = odirect.direct_src(okern.laplace_slp_dlp, x, y, tau, ex, x[:300], y[:300], check=True)[0]
+    assert _err(us, refs) < 1e-12

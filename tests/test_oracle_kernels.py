@@ -125,3 +125,38 @@ def test_stokeslet_laplace_decomposition():
     ref = odirect.direct_stokes(sx, sy, fx, fy, tx, ty)
     for got, r in zip((u, v, p), ref):
         assert np.abs(got - r).max() < 1e-13 * np.abs(r).max()
+
+
+def test_double_layer_is_source_derivative_of_single_layer():
+    """(n.d)/d^2/(2 pi) with d = t - s is the derivative of the single layer
+    -log(d^2)/(4 pi) along n at the source: the dipole of
+    examples/pure_kifmm_laplace_dipoles.py:32-40."""
+    rng = np.random.default_rng(17)
+    sx, sy, tx, ty = rng.random(4)
+    th = rng.uniform(0, 2 * np.pi)
+    nx, ny = np.cos(th), np.sin(th)
+    h = 1e-6
+    fd = (okern.laplace(sx + h * nx, sy + h * ny, tx, ty)
+          - okern.laplace(sx - h * nx, sy - h * ny, tx, ty)) / (2 * h)
+    dl = okern.laplace_slp_dlp(sx, sy, tx, ty, np.array([0.0, 1.0]), np.array([nx, ny]))[0]
+    assert abs(fd - dl) < 1e-8 * abs(dl)
+    sl = okern.laplace_slp_dlp(sx, sy, tx, ty, np.array([1.0, 0.0]), np.array([nx, ny]))[0]
+    assert sl == okern.laplace(sx, sy, tx, ty)
+
+
+def test_direct_src_matches_python_loops():
+    rng = np.random.default_rng(18)
+    sx, sy = rng.random(7), rng.random(7)
+    tau = rng.standard_normal((2, 7))
+    ex = rng.standard_normal((2, 7))
+    tx, ty = np.concatenate([rng.random(3), sx[:2]]), np.concatenate([rng.random(3), sy[:2]])
+    got = odirect.direct_src(okern.laplace_slp_dlp, sx, sy, tau, ex, tx, ty, check=True)[0]
+    for i in range(tx.size):
+        acc = 0.0
+        for j in range(sx.size):
+            if sx[j] == tx[i] and sy[j] == ty[i]:
+                continue
+            dx, dy = tx[i] - sx[j], ty[i] - sy[j]
+            d2 = dx * dx + dy * dy
+            acc += -np.log(d2) / (4 * np.pi) * tau[0, j] + (ex[0, j] * dx + ex[1, j] * dy) / d2 / (2 * np.pi) * tau[1, j]
+        assert abs(got[i] - acc) < 1e-13 * max(1.0, abs(acc))
