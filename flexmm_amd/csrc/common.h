@@ -173,6 +173,7 @@ __device__ __forceinline__ double log_tab(double x, const double2* __restrict__ 
 // (blocks b with b % 8 == x) one contiguous range of logical blocks, so that
 // neighbouring tiles, which share source data, meet in the same L2.
 constexpr int kNumXcd = 8;
+constexpr int kNumCUs = 256;   // MI355X compute units (launch-shape heuristics only)
 __device__ __forceinline__ int xcd_block(int b, int n) {
   const int xcd = b % kNumXcd, local = b / kNumXcd;
   const int per = n / kNumXcd, rem = n % kNumXcd;

@@ -466,6 +466,10 @@ struct LevelTable {
   // child position c at level li is perm[poff[li] + c * ngroups[li] + qq]
   const int* perm;
   int poff[kMaxLevels];
+  // 2: each tile runs as two blocks taking the first and second half of its
+  // phases and adding into a zeroed PLE (two addends onto 0 give the same sum
+  // in either order); used when few tiles would leave CUs unevenly loaded
+  int split;
 };
 
 struct ValidSlots {
@@ -497,7 +501,9 @@ __device__ __forceinline__ M2LSetup m2l_setup(const LevelTable& lt, const int* _
                                               int (&srcs)[27][kM2LCols], int (&pslot)[28],
                                               unsigned long long& smask) {
   M2LSetup S;
-  const int bid = xcd_block(blockIdx.x, gridDim.x);
+  const int bid0 = xcd_block(blockIdx.x, gridDim.x);
+  const int half = lt.split == 2 ? (bid0 & 1) : 0;   // the two halves of a tile are neighbours
+  const int bid = lt.split == 2 ? (bid0 >> 1) : bid0;
   int li = 0;
   while (li + 1 < lt.nlev && bid >= lt.boff[li + 1]) ++li;
   const int lb = bid - lt.boff[li];
@@ -530,6 +536,12 @@ __device__ __forceinline__ M2LSetup m2l_setup(const LevelTable& lt, const int* _
     while (m) {
       pslot[k++] = __builtin_ctzll(m);
       m &= m - 1ull;
+    }
+    if (lt.split == 2) {   // this block's half of the phases
+      const int h = (k + 1) / 2;
+      const int lo = half ? h : 0, hi = half ? k : h;
+      for (int i = lo; i < hi; ++i) pslot[i - lo] = pslot[i];
+      k = hi - lo;
     }
     pslot[27] = k;
   }
@@ -669,10 +681,17 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all(LevelTable lt,
   }
   if (S.active) {
     double* rowp = ple + (int64_t)S.g * PD;
+    if (lt.split == 2) {
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
+      for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) rowp[16 * rb + 4 * v + S.kq] = acc[rb][v];
+        for (int v = 0; v < 4; ++v) unsafeAtomicAdd(rowp + 16 * rb + 4 * v + S.kq, acc[rb][v]);
+    } else {
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) rowp[16 * rb + 4 * v + S.kq] = acc[rb][v];
+    }
   }
 }
 
@@ -794,10 +813,17 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
   }
   if (S.active) {
     double* rowp = ple + (int64_t)S.g * PD;
+    if (lt.split == 2) {
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
+      for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) rowp[16 * rb + 4 * S.kq + v] = acc64[rb][v];
+        for (int v = 0; v < 4; ++v) unsafeAtomicAdd(rowp + 16 * rb + 4 * S.kq + v, acc64[rb][v]);
+    } else {
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) rowp[16 * rb + 4 * S.kq + v] = acc64[rb][v];
+    }
   }
 }
 
@@ -2033,8 +2059,16 @@ static void m2l_launch(Plan& P, const M2LTables& T, int l0, int l1, hipStream_t 
   }
   lt.boff[lt.nlev] = T.lt.boff[l1] - T.lt.boff[l0];
   lt.perm = T.adaptive ? P.m2lperm.as<int>() : nullptr;
-  const int nb = lt.boff[lt.nlev];
+  int nb = lt.boff[lt.nlev];
   if (nb == 0) return;
+  // fewer than 3 tiles per CU: split the tiles' phases over two blocks each
+  // (FP64 only: the FP32 phases are short enough that the second block
+  // setup costs more than the balance gains, as measured on 1M points)
+  lt.split = P.precision == PREC_FP64 && (int64_t)nb * nd < 3 * kNumCUs ? 2 : 1;
+  if (lt.split == 2) {
+    FMM_HIP(hipMemsetAsync(P.ple.ptr, 0, nstride * PD * nd * sizeof(double), s));
+    nb *= 2;
+  }
   if (P.precision != PREC_FP64) {
     auto kern = PD == 32 ? k_m2l_all_f32<32> : k_m2l_all_f32<64>;
     hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(kM2LThreads), 0, s, lt, t.flags.as<int>(),
