@@ -82,3 +82,31 @@ def test_square_bbox_holds_both_sets():
     b = _square_bbox(x, y, tx, ty)
     assert b[0] <= 0.1 and b[1] >= 0.9 and b[2] <= 0.3 and b[3] >= 0.6
     assert abs((b[1] - b[0]) - (b[3] - b[2])) < 1e-15
+
+
+def test_slp_dlp_source_kernel_matches_oracle_and_resolves():
+    """kernels.laplace_slp_dlp is the reference's Laplace_Both_Eval
+    (examples/pure_kifmm_laplace_dipoles.py:32-40) and register_evaluator's
+    (kernel_app, Laplace_Both_Eval) pair maps to the dipole device evaluator."""
+    from flexmm_amd import _lib, kernels
+    from oracle import kernels as okern
+    rng = np.random.default_rng(5)
+    sx, sy, tx, ty = rng.random((4, 50))
+    tau = rng.random((2, 50))
+    ex = rng.standard_normal((2, 50))
+    a = kernels.laplace_slp_dlp(sx, sy, tx, ty, tau, ex)[0]
+    b = okern.laplace_slp_dlp(sx, sy, tx, ty, tau, ex)[0]
+    assert np.allclose(a, b, rtol=1e-15, atol=0)
+    ev = kernels.resolve_evaluator(kernels.laplace, kernels.laplace_potential,
+                                   kernels.laplace_slp_dlp, 1)
+    assert ev.evaluator_id == _lib.EVAL_DIPOLE and ev.outputs == 1
+    assert kernels.resolve_evaluator(kernels.laplace, None, kernels.laplace_slp_dlp, 1) is ev
+    with pytest.raises(ValueError):
+        kernels.resolve_evaluator(kernels.laplace, kernels.laplace_potential,
+                                  kernels.laplace_slp_dlp, 3)
+    with pytest.raises(NotImplementedError):
+        kernels.resolve_evaluator(kernels.laplace, kernels.laplace_gradient,
+                                  kernels.laplace_slp_dlp, 1)
+    mh, _, _ = kernels.modified_helmholtz(3.0)
+    with pytest.raises(ValueError):
+        kernels.resolve_evaluator(mh, None, kernels.laplace_slp_dlp, 1)
