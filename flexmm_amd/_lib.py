@@ -17,7 +17,7 @@ E_ARG, E_STATE, E_LIMIT, E_INTERNAL = -1, -2, -3, -4
 KERNEL_LAPLACE = 0
 KERNEL_MODIFIED_HELMHOLTZ = 1
 KERNEL_STOKESLET = 2
-EVAL_POTENTIAL, EVAL_GRADIENT, EVAL_STOKES, EVAL_DIPOLE = 0, 1, 2, 3
+EVAL_POTENTIAL, EVAL_GRADIENT, EVAL_STOKES, EVAL_DIPOLE, EVAL_STOKES_DIPOLE = 0, 1, 2, 3, 4
 PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_MIXED_M2L = 2

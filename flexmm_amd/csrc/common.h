@@ -192,6 +192,7 @@ enum EvalId : int {
   EVAL_GRAD = 1,     // 3 outputs: u, du/dtx, du/dty
   EVAL_STOKES = 2,   // 3 outputs: velocity u, v and pressure p
   EVAL_DIPOLE = 3,   // 1 output: u of Laplace single + double layer sources
+  EVAL_STOKES_DIPOLE = 4,   // 3 outputs: u, v, p of Stokeslet + stresslet sources
 };
 enum Precision : int {
   PREC_FP64 = 0,

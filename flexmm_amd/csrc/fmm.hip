@@ -87,7 +87,9 @@ struct Plan {
   DevBuf tau_s, src, check, mult, le, ple, qs;
   DevBuf srcf, tsum;   // mixed precision: leaf-relative FP32 sources, per-leaf charge sums
   DevBuf dip;          // dipole moments of the sources in tree order (double layer)
+  DevBuf sdip;         // Stokes double layer: (n.sigma, a, b, 0) per source in tree order
   bool have_dip = false;
+  int nd = 1;          // densities of the current expansions
   DevBuf m2lperm;      // M2L column order of an adaptive tree
   int64_t perm_tree = -1;   // tree build the column order belongs to
   bool have_srcf = false;
@@ -1690,6 +1692,21 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
 // L[g] = -2 pi Phi[g] and grad L[g] = sum_k a_k (t - e_k) / |t - e_k|^2 for a
 // local density a on the points e_k.
 constexpr int kStokesDens = 6;
+// Double layer (stresslet, stokes_fmm.py:64-101,187-236): for a source with
+// strength sigma and normal n, and r = t - s,
+//   u = (1/pi) (r.n)(r.sigma) r / |r|^4,
+//   p = (1/pi) [-(n.sigma) / |r|^2 + 2 (r.n)(r.sigma) / |r|^4].
+// With g = n.sigma, a = n_x s_x - n_y s_y, b = n_x s_y + n_y s_x and the
+// quadrupole field Q[a, b](t) = sum (a d_xx + b d_xy) log|t - s| (harmonic:
+// d_yy = -d_xx), since r_i r_j r_k / |r|^4 = (1/2)(delta_jk d_i log|r|
+// - (X_i(t) - X_i(s)) d_j d_k log|r|):
+//   u_i = (1/2 pi) [ d_i L[g] - X_i(t) Q[a, b] + Q[X_i a, X_i b] ],
+//   p   = -(1/pi) Q[a, b].
+// The far field therefore needs four more densities (g as a single layer,
+// and the three quadrupole fields, whose check values are the fields
+// themselves); the near field evaluates the stresslet directly, with
+// (r.n)(r.sigma) = (1/2) [g |r|^2 + a (r_x^2 - r_y^2) + 2 b r_x r_y].
+constexpr int kStokesDipDens = 10;
 
 // Sources in tree order as (x, y, f_x, f_y); tau holds f_x[n] then f_y[n].
 __global__ void k_pack_stokes(const double* __restrict__ tau, const uint32_t* __restrict__ perm,
@@ -1701,8 +1718,29 @@ __global__ void k_pack_stokes(const double* __restrict__ tau, const uint32_t* __
   packed[i] = make_double4(sx[i], sy[i], tau[j], tau[n + j]);
 }
 
+// Forces and double-layer densities: tau holds f_x, f_y, sigma_x, sigma_y
+// ([n] each), dvec the normals n_x, n_y; (x, y, f_x, f_y) and (g, a, b, 0) in
+// tree order.
+__global__ void k_pack_stokes_dipole(const double* __restrict__ tau,
+                                     const double* __restrict__ dvec,
+                                     const uint32_t* __restrict__ perm,
+                                     const double* __restrict__ sx, const double* __restrict__ sy,
+                                     int64_t n, double4* __restrict__ packed,
+                                     double4* __restrict__ sdip) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t j = perm[i];
+  packed[i] = make_double4(sx[i], sy[i], tau[j], tau[n + j]);
+  const double gx = tau[2 * n + j], gy = tau[3 * n + j];
+  const double nx = dvec[j], ny = dvec[n + j];
+  sdip[i] = make_double4(nx * gx + ny * gy, nx * gx - ny * gy, nx * gy + ny * gx, 0.0);
+}
+
 // S2M of the six densities: one log per (source, check point) serves all six.
-template <int PD>
+// DIP: also the double layer's four (g as a single layer; Q[a, b], Q[X a, X b]
+// and Q[Y a, Y b] as fields, with d_xx log|r| = (r_y^2 - r_x^2)/|r|^4 and
+// d_xy log|r| = -2 r_x r_y/|r|^4 at the check point).
+template <int PD, bool DIP>
 __global__ __launch_bounds__(64 * kS2MWaves) void k_s2m_stokes(int nup,
                                                                const int* __restrict__ upnodes,
                                                    const int* __restrict__ nlevel,
@@ -1715,6 +1753,7 @@ __global__ __launch_bounds__(64 * kS2MWaves) void k_s2m_stokes(int nup,
                                                    const double* __restrict__ ops, int64_t opsz,
                                                    int p, const double2* __restrict__ logtab,
                                                    double c0x, double c0y, int64_t rstride,
+                                                   const double4* __restrict__ sdip,
                                                    double* __restrict__ check) {
   __shared__ double2 tab[kLogTableSize];
   load_logtab(tab, logtab);
@@ -1733,33 +1772,52 @@ __global__ __launch_bounds__(64 * kS2MWaves) void k_s2m_stokes(int nup,
   const int b = __builtin_amdgcn_readfirstlane(bot[g]);
   const int t = __builtin_amdgcn_readfirstlane(top[g]);
   double acc[kStokesDens] = {};
+  double ag = 0.0, q1 = 0.0, qx = 0.0, qy = 0.0;
   for (int j = b; j < t; ++j) {
     const double4 sj = src[j];
     const double dx = ax - sj.x;
     const double dy = ay - sj.y;
-    const double lg = log_tab(__builtin_fma(dy, dy, dx * dx), tab);
+    const double d2 = __builtin_fma(dy, dy, dx * dx);
+    const double lg = log_tab(d2, tab);
     const double X = sj.x - c0x, Y = sj.y - c0y;
     const double w[kStokesDens] = {sj.z, sj.w, X * sj.z, X * sj.w, Y * sj.z, Y * sj.w};
 #pragma unroll
     for (int q = 0; q < kStokesDens; ++q) acc[q] = __builtin_fma(w[q], lg, acc[q]);
+    if constexpr (DIP) {
+      const double4 e = sdip[j];
+      const double id2 = rcp_nr(d2);
+      const double id4 = id2 * id2;
+      const double qv = __builtin_fma(e.y, (dy - dx) * (dy + dx), -2.0 * e.z * dx * dy) * id4;
+      ag = __builtin_fma(e.x, lg, ag);
+      q1 += qv;
+      qx = __builtin_fma(X, qv, qx);
+      qy = __builtin_fma(Y, qv, qy);
+    }
   }
   if (k < PD) {
 #pragma unroll
     for (int q = 0; q < kStokesDens; ++q)
       check[(q * rstride + r) * PD + k] = k < p ? -kInv4Pi * acc[q] : 0.0;
+    if constexpr (DIP) {
+      check[(6 * rstride + r) * PD + k] = k < p ? -kInv4Pi * ag : 0.0;
+      check[(7 * rstride + r) * PD + k] = k < p ? q1 : 0.0;
+      check[(8 * rstride + r) * PD + k] = k < p ? qx : 0.0;
+      check[(9 * rstride + r) * PD + k] = k < p ? qy : 0.0;
+    }
   }
 }
 
 // One item of k_eval_stokes: L2T of the six local densities + Stokeslet P2P,
 // outputs (u, v, p).  FEW selects the (target slot, source group) lane layout
 // of eval_few; otherwise lane = target and the source loop index is uniform.
-template <int PD, bool CHECK, bool FEW>
+template <int PD, bool CHECK, bool FEW, bool DIP>
 __device__ __forceinline__ void stokes_item(
     int g, int t0, int t1, const int* __restrict__ tperm, const double* __restrict__ tsx,
     const double* __restrict__ tsy, const int* __restrict__ nlevel,
     const double* __restrict__ xmid, const double* __restrict__ ymid,
     const int* __restrict__ coll, const int* __restrict__ bot, const int* __restrict__ top,
-    const double4* __restrict__ src, const double* __restrict__ le, int64_t nstride,
+    const double4* __restrict__ src, const double4* __restrict__ sdip,
+    const double* __restrict__ le, int64_t nstride,
     const double* __restrict__ ops, int64_t opsz, int p, const double2* __restrict__ tab,
     double c0x, double c0y, int64_t nt, double* __restrict__ out) {
   FewLanes F;
@@ -1783,8 +1841,8 @@ __device__ __forceinline__ void stokes_item(
     const double* ly = lx + PD;
     const double* a = le + (int64_t)g * PD;
     const double xr = x - xmid[g], yr = y - ymid[g];
-    LogAcc l0, l1;
-    double B = 0.0, Cx = 0.0, Cy = 0.0;
+    LogAcc l0, l1, lq1, lqx, lqy;
+    double B = 0.0, Cx = 0.0, Cy = 0.0, Gx = 0.0, Gy = 0.0;
     for (int k = F.sg; k < p; k += F.ng) {
       const double dx = xr - lx[k], dy = yr - ly[k];
       const double d2 = __builtin_fma(dy, dy, dx * dx);
@@ -1801,16 +1859,37 @@ __device__ __forceinline__ void stokes_item(
       B = __builtin_fma(__builtin_fma(a0, dx, a1 * dy), id2, B);
       Cx = __builtin_fma(__builtin_fma(a2, dx, a3 * dy), id2, Cx);
       Cy = __builtin_fma(__builtin_fma(a4, dx, a5 * dy), id2, Cy);
+      if constexpr (DIP) {
+        const double a6 = a[6 * nstride * PD + k], a7 = a[7 * nstride * PD + k],
+                     a8 = a[8 * nstride * PD + k], a9 = a[9 * nstride * PD + k];
+        Gx = __builtin_fma(a6 * dx, id2, Gx);
+        Gy = __builtin_fma(a6 * dy, id2, Gy);
+        lq1.ae = __builtin_fma(a7, e, lq1.ae);
+        lq1.at = __builtin_fma(a7, lt, lq1.at);
+        lqx.ae = __builtin_fma(a8, e, lqx.ae);
+        lqx.at = __builtin_fma(a8, lt, lqx.at);
+        lqy.ae = __builtin_fma(a9, e, lqy.ae);
+        lqy.at = __builtin_fma(a9, lt, lqy.at);
+      }
     }
     // -L[f] = -sum a (1/2) log d^2
     const double X = x - c0x, Y = y - c0y;
     u = kInv4Pi * (-0.5 * l0.value() + X * B - Cx);
     v = kInv4Pi * (-0.5 * l1.value() + Y * B - Cy);
     pr = kInv2Pi * B;
+    if constexpr (DIP) {
+      // fields of the quadrupole densities (single-layer representation)
+      const double Q1 = -kInv4Pi * lq1.value(), QX = -kInv4Pi * lqx.value(),
+                   QY = -kInv4Pi * lqy.value();
+      u += kInv2Pi * (Gx - X * Q1 + QX);
+      v += kInv2Pi * (Gy - Y * Q1 + QY);
+      pr -= 2.0 * kInv2Pi * Q1;
+    }
   }
   // near field: the Stokeslet itself (stokes_fmm.py:102-184)
   LogAcc lx_, ly_;
   double su = 0.0, sv = 0.0, sp = 0.0;
+  double du = 0.0, dv = 0.0, dp = 0.0;   // double layer
   double dmin = 1.0;
   const int* cl = coll + (int64_t)g * 9;
   for (int qn = 0; qn < 9; ++qn) {
@@ -1842,11 +1921,31 @@ __device__ __forceinline__ void stokes_item(
       su = __builtin_fma(dx, rf, su);
       sv = __builtin_fma(dy, rf, sv);
       sp += rf;
+      if constexpr (DIP) {   // the stresslet itself
+        const double4 e4 = sdip[j];
+        double gg = e4.x, aa = e4.y, bb = e4.z;
+        if (CHECK) {
+          gg = z ? 0.0 : gg;
+          aa = z ? 0.0 : aa;
+          bb = z ? 0.0 : bb;
+        }
+        const double nrns =
+            0.5 * __builtin_fma(gg, d2, __builtin_fma(aa, (dx - dy) * (dx + dy), 2.0 * bb * dx * dy));
+        const double w = nrns * id2 * id2;
+        du = __builtin_fma(w, dx, du);
+        dv = __builtin_fma(w, dy, dv);
+        dp += __builtin_fma(-gg, id2, 2.0 * w);
+      }
     }
   }
   u += kInv4Pi * (-0.5 * lx_.value() + su);
   v += kInv4Pi * (-0.5 * ly_.value() + sv);
   pr += kInv2Pi * sp;
+  if constexpr (DIP) {
+    u += 2.0 * kInv2Pi * du;
+    v += 2.0 * kInv2Pi * dv;
+    pr += 2.0 * kInv2Pi * dp;
+  }
   if constexpr (FEW) {
     u = slot_sum(u, F.lg);
     v = slot_sum(v, F.lg);
@@ -1865,7 +1964,7 @@ __device__ __forceinline__ void stokes_item(
 }
 
 // L2T of the six local densities + Stokeslet P2P, outputs (u, v, p).
-template <int PD, bool CHECK>
+template <int PD, bool CHECK, bool DIP>
 __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_stokes(
     const int2* __restrict__ items, const int* __restrict__ nitems,
     const int* __restrict__ evalnode, const int* __restrict__ toff,
@@ -1873,6 +1972,7 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_stokes(
     const int* __restrict__ nlevel, const double* __restrict__ xmid,
     const double* __restrict__ ymid, const int* __restrict__ coll, const int* __restrict__ bot,
     const int* __restrict__ top, const double4* __restrict__ src,
+    const double4* __restrict__ sdip,
     const double* __restrict__ le, int64_t nstride, const double* __restrict__ ops, int64_t opsz,
     int p, const double2* __restrict__ logtab, double c0x, double c0y, int64_t nt,
     double* __restrict__ out) {
@@ -1887,11 +1987,13 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_stokes(
     const int t0 = toff[it.x] + it.y * kTgtChunk;
     const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
     if (t1 - t0 <= kFewTargets)
-      stokes_item<PD, CHECK, true>(g, t0, t1, tperm, tsx, tsy, nlevel, xmid, ymid, coll, bot,
-                                   top, src, le, nstride, ops, opsz, p, tab, c0x, c0y, nt, out);
+      stokes_item<PD, CHECK, true, DIP>(g, t0, t1, tperm, tsx, tsy, nlevel, xmid, ymid, coll,
+                                        bot, top, src, sdip, le, nstride, ops, opsz, p, tab, c0x,
+                                        c0y, nt, out);
     else
-      stokes_item<PD, CHECK, false>(g, t0, t1, tperm, tsx, tsy, nlevel, xmid, ymid, coll, bot,
-                                    top, src, le, nstride, ops, opsz, p, tab, c0x, c0y, nt, out);
+      stokes_item<PD, CHECK, false, DIP>(g, t0, t1, tperm, tsx, tsy, nlevel, xmid, ymid, coll,
+                                         bot, top, src, sdip, le, nstride, ops, opsz, p, tab,
+                                         c0x, c0y, nt, out);
   }
 }
 
@@ -2031,7 +2133,7 @@ static M2LTables m2l_tables(const Plan& P) {
 // The column order, once per tree (on the stream that runs the M2L).
 static void m2l_prepare(Plan& P, const M2LTables& T, hipStream_t s) {
   const Tree& t = P.tree;
-  const int nd = P.kernel_id == KER_STOKES ? kStokesDens : 1;
+  const int nd = P.nd;
   P.ple.ensure((t.n_nodes + 1) * P.PD * nd * sizeof(double));
   if (!T.adaptive || P.perm_tree == t.build_id) return;
   P.m2lperm.ensure((T.perm_len + 1) * sizeof(int));
@@ -2045,7 +2147,7 @@ static void m2l_launch(Plan& P, const M2LTables& T, int l0, int l1, hipStream_t 
   const Tree& t = P.tree;
   const int PD = P.PD;
   const int64_t opsz = ops_per_level(PD);
-  const int nd = P.kernel_id == KER_STOKES ? kStokesDens : 1;
+  const int nd = P.nd;
   const int64_t rstride = t.n_rows + 1, nstride = t.n_nodes + 1;
   LevelTable lt{};
   lt.nlev = l1 - l0;
@@ -2091,11 +2193,17 @@ static void upward(Plan& P, const double* tau, const double* dvec, hipStream_t s
   const int PD = P.PD;
   const int64_t opsz = ops_per_level(PD);
   const bool stokes = P.kernel_id == KER_STOKES;
-  const int nd = stokes ? kStokesDens : 1;
+  const int nd = stokes ? (dvec ? kStokesDipDens : kStokesDens) : 1;
+  P.nd = nd;
   P.tau_s.ensure(t.n * sizeof(double));
   P.src.ensure(t.n * sizeof(double4));
   P.have_srcf = false;
-  if (stokes) {
+  if (stokes && dvec) {
+    P.sdip.ensure((t.n + 1) * sizeof(double4));
+    hipLaunchKernelGGL(k_pack_stokes_dipole, dim3(grid256(t.n)), dim3(256), 0, s, tau, dvec,
+                       t.perm.as<uint32_t>(), t.sx.as<double>(), t.sy.as<double>(), t.n,
+                       P.src.as<double4>(), P.sdip.as<double4>());
+  } else if (stokes) {
     hipLaunchKernelGGL(k_pack_stokes, dim3(grid256(t.n)), dim3(256), 0, s, tau,
                        t.perm.as<uint32_t>(), t.sx.as<double>(), t.sy.as<double>(), t.n,
                        P.src.as<double4>());
@@ -2123,13 +2231,14 @@ static void upward(Plan& P, const double* tau, const double* dvec, hipStream_t s
   }
   P.mark(FMM2D_PHASE_S2M, false, s);
   if (t.n_up > 0 && stokes) {
-    auto kern = PD == 32 ? k_s2m_stokes<32> : k_s2m_stokes<64>;
+    auto kern = dvec ? (PD == 32 ? k_s2m_stokes<32, true> : k_s2m_stokes<64, true>)
+                     : (PD == 32 ? k_s2m_stokes<32, false> : k_s2m_stokes<64, false>);
     hipLaunchKernelGGL(kern, dim3(ceil_div(t.n_up, kS2MWaves)), dim3(64 * kS2MWaves), 0, s,
                        (int)t.n_up, t.upnodes.as<int>(), t.nlevel.as<int>(),
                        t.bot.as<int>(), t.top.as<int>(), t.row.as<int>(), t.xmid.as<double>(),
                        t.ymid.as<double>(), P.src.as<double4>(), P.ops.as<double>(), opsz, P.p,
                        P.logtab.as<double2>(), t.xmin + 0.5 * t.w0, t.ymin + 0.5 * t.w0,
-                       rstride, P.check.as<double>());
+                       rstride, dvec ? P.sdip.as<double4>() : nullptr, P.check.as<double>());
     FMM_LAUNCH_CHECK();
   } else if (t.n_up > 0) {
     auto kern = dvec ? (PD == 32 ? k_s2m<32, KER_LAPLACE, true> : k_s2m<64, KER_LAPLACE, true>)
@@ -2172,7 +2281,7 @@ static void downward(Plan& P, hipStream_t s) {
   check_ops(P);
   const int PD = P.PD;
   const int64_t opsz = ops_per_level(PD);
-  const int nd = P.kernel_id == KER_STOKES ? kStokesDens : 1;
+  const int nd = P.nd;
   const int64_t rstride = t.n_rows + 1, nstride = t.n_nodes + 1;
   P.le.ensure(nstride * PD * nd * sizeof(double));
   P.ple.ensure(nstride * PD * nd * sizeof(double));
@@ -2212,12 +2321,15 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   Tree& t = P.tree;
   FMM_REQUIRE(P.have_exp, FMM_E_STATE, "no expansions: call fmm2d_upward and fmm2d_downward first");
   const bool stokes = P.kernel_id == KER_STOKES;
-  FMM_REQUIRE(stokes ? ev == EVAL_STOKES
+  FMM_REQUIRE(stokes ? (ev == EVAL_STOKES || ev == EVAL_STOKES_DIPOLE)
                      : (ev == EVAL_POT || ev == EVAL_GRAD ||
                         (ev == EVAL_DIPOLE && P.kernel_id == KER_LAPLACE)),
               FMM_E_ARG, "evaluator %d does not belong to kernel %d", ev, P.kernel_id);
-  FMM_REQUIRE(ev != EVAL_DIPOLE || P.have_dip, FMM_E_STATE,
-              "the dipole evaluator needs expansions from fmm2d_upward_dipole");
+  FMM_REQUIRE((ev != EVAL_DIPOLE && ev != EVAL_STOKES_DIPOLE) || P.have_dip, FMM_E_STATE,
+              "the dipole evaluators need expansions from fmm2d_upward_dipole");
+  FMM_REQUIRE(!(stokes && ev == EVAL_STOKES && P.have_dip), FMM_E_STATE,
+              "the expansions hold double-layer densities: evaluate with the Stokes dipole "
+              "evaluator");
   FMM_REQUIRE(nt >= 0 && nt < ((int64_t)1 << 31), FMM_E_LIMIT, "too many targets");
   if (nt == 0) return;
   const int64_t ne = t.n_eval;
@@ -2266,14 +2378,18 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   const int PD = P.PD;
   const int64_t opsz = ops_per_level(PD);
   if (stokes) {
-    auto sk = PD == 32 ? (check ? k_eval_stokes<32, true> : k_eval_stokes<32, false>)
-                       : (check ? k_eval_stokes<64, true> : k_eval_stokes<64, false>);
+    const bool dl = ev == EVAL_STOKES_DIPOLE;
+    auto sk = dl ? (PD == 32 ? (check ? k_eval_stokes<32, true, true> : k_eval_stokes<32, false, true>)
+                             : (check ? k_eval_stokes<64, true, true> : k_eval_stokes<64, false, true>))
+                 : (PD == 32 ? (check ? k_eval_stokes<32, true, false> : k_eval_stokes<32, false, false>)
+                             : (check ? k_eval_stokes<64, true, false> : k_eval_stokes<64, false, false>));
     P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
     hipLaunchKernelGGL(sk, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
                        t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(),
                        P.tsx.as<double>(), P.tsy.as<double>(), t.nlevel.as<int>(),
                        t.xmid.as<double>(), t.ymid.as<double>(), t.coll.as<int>(),
-                       t.bot.as<int>(), t.top.as<int>(), P.src.as<double4>(), P.le.as<double>(),
+                       t.bot.as<int>(), t.top.as<int>(), P.src.as<double4>(),
+                       dl ? P.sdip.as<double4>() : nullptr, P.le.as<double>(),
                        t.n_nodes + 1, P.ops.as<double>(), opsz, P.p, P.logtab.as<double2>(),
                        t.xmin + 0.5 * t.w0, t.ymin + 0.5 * t.w0, nt, out);
     FMM_LAUNCH_CHECK();
@@ -2558,8 +2674,9 @@ int fmm2d_upward_dipole(void* plan, const double* tau, const double* dvec, int64
   Plan* P = as_plan(plan);
   NEED(tau);
   NEED(dvec);
-  FMM_REQUIRE(P->kernel_id == fmm::KER_LAPLACE, fmm::FMM_E_ARG,
-              "dipole sources need the Laplace kernel (plan has kernel %d)", P->kernel_id);
+  FMM_REQUIRE(P->kernel_id == fmm::KER_LAPLACE || P->kernel_id == fmm::KER_STOKES,
+              fmm::FMM_E_ARG, "dipole sources need the Laplace or Stokeslet kernel (plan has %d)",
+              P->kernel_id);
   FMM_REQUIRE(P->tree.built, fmm::FMM_E_STATE, "no tree");
   FMM_REQUIRE(n == P->tree.n, fmm::FMM_E_ARG, "n = %lld, tree has %lld sources", (long long)n,
               (long long)P->tree.n);

@@ -146,8 +146,40 @@ class FMM:
         self._tau = tau
         self.print("....Time for expansions:        {:0.2f}".format(1000 * (time.time() - st)))
 
+    def build_expansions_dipole(self, taux, tauy, sigmax, sigmay, dipx, dipy):
+        """stokes_fmm.py:738-826: forces (taux, tauy), double-layer densities
+        (sigmax, sigmay) and normals (dipx, dipy), all in input order.
+        Evaluate with evaluate_to_points_dipole."""
+        if not self._prepared:
+            self.precompute()
+        n = self.x.numel()
+        arrs = []
+        for name, a in (("taux", taux), ("tauy", tauy), ("sigmax", sigmax), ("sigmay", sigmay),
+                        ("dipx", dipx), ("dipy", dipy)):
+            t = _to_device(a, self.device, name)
+            if t.numel() != n:
+                raise ValueError("%s must have one entry per source" % name)
+            arrs.append(t)
+        self.taux, self.tauy, self.sigmax, self.sigmay, self.dipx, self.dipy = arrs
+        tau = torch.cat(arrs[:4])
+        dvec = torch.cat(arrs[4:])
+        st = time.time()
+        _lib.check(self.lib.fmm2d_upward_dipole(self.handle, tau.data_ptr(), dvec.data_ptr(), n,
+                                                _stream()))
+        _lib.check(self.lib.fmm2d_downward(self.handle, _stream()))
+        self._tau, self._dvec = tau, dvec
+        self.print("....Time for expansions:        {:0.2f}".format(1000 * (time.time() - st)))
+
+    def evaluate_to_points_dipole(self, x, y, check_self=False):
+        """stokes_fmm.py:844-858: (u, v, p) of the Stokeslets and stresslets
+        given to build_expansions_dipole."""
+        return self._evaluate(x, y, check_self, _lib.EVAL_STOKES_DIPOLE)
+
     def evaluate_to_points(self, x, y, check_self=False):
         """stokes_fmm.py:827-840: returns (u, v, p) at the points."""
+        return self._evaluate(x, y, check_self, _lib.EVAL_STOKES)
+
+    def _evaluate(self, x, y, check_self, evaluator):
         numpy_out = not isinstance(x, torch.Tensor)
         tx = _to_device(x, self.device, "x")
         ty = _to_device(y, self.device, "y")
@@ -155,7 +187,7 @@ class FMM:
             raise ValueError("x and y differ in length")
         out = torch.empty((3, tx.numel()), dtype=torch.float64, device=self.device)
         _lib.check(self.lib.fmm2d_evaluate(self.handle, tx.data_ptr(), ty.data_ptr(), tx.numel(),
-                                           _lib.EVAL_STOKES, 1 if check_self else 0,
+                                           evaluator, 1 if check_self else 0,
                                            out.data_ptr(), _stream()))
         if numpy_out:
             out = out.cpu().numpy()

@@ -50,6 +50,11 @@ extern "C" {
                                   'slp+dlp->u' of examples/pure_kifmm_laplace_dipoles.py:
                                   139-145); a target on a source without self-exclusion
                                   gives NaN */
+#define FMM2D_EVAL_STOKES_DIPOLE 4 /* 3 outputs: u, v, p of Stokeslets plus stresslets (needs
+                                      fmm2d_upward_dipole on a Stokeslet plan; replaces
+                                      evaluate_to_points_dipole, stokes_fmm.py:844-858);
+                                      a target on a source without self-exclusion
+                                      gives NaN */
 #define FMM2D_EVAL_GRADIENT 1  /* 3 outputs: u, du/dx_t, du/dy_t
                                   (examples/pure_kifmm_laplace_gradient.py:32-55,
                                   pure_kifmm_modified_helmholtz_gradient.py:45-68) */
@@ -147,7 +152,13 @@ int fmm2d_upward(void* plan, const double* tau, int64_t n, void* stream);
  * Replaces FMM.upwards_pass with a registered upwardor and extras
  * (fmm.py:73-124 with extras, helpers.py:155-174; the 'slp+dlp' upwardor of
  * examples/pure_kifmm_laplace_dipoles.py:32-40,139-141).  Evaluate with
- * FMM2D_EVAL_DIPOLE. */
+ * FMM2D_EVAL_DIPOLE.
+ * On a Stokeslet plan: tau = 4n values (forces f_x, f_y, then double-layer
+ * densities sigma_x, sigma_y), dvec = 2n values (normals n_x, then n_y);
+ * stresslet velocity (1/pi)(r.n)(r.sigma) r/|r|^4 and pressure
+ * (1/pi)[-(n.sigma)/|r|^2 + 2(r.n)(r.sigma)/|r|^4], r = t - s.  Replaces
+ * FMM.build_expansions_dipole (stokes_fmm.py:738-826; kernels :129-235).
+ * Evaluate with FMM2D_EVAL_STOKES_DIPOLE. */
 int fmm2d_upward_dipole(void* plan, const double* tau, const double* dvec, int64_t n,
                         void* stream);
 

@@ -73,6 +73,45 @@ def direct_stokes(sx, sy, fx, fy, tx, ty, check=False, chunk_pairs=20_000_000):
     return out
 
 
+def direct_stokes_dipole(sx, sy, fx, fy, sigx, sigy, nx, ny, tx, ty, check=False,
+                         chunk_pairs=10_000_000):
+    """(u, v, p) of Stokeslets (forces f) plus stresslets (double-layer density
+    sigma, normal n) summed over all sources, as the reference's
+    kernel_apply_single(_check)_with_dipole
+    (flexmm/kifmm2d/vector/stokes_fmm.py:129-235).  With r = t - s:
+    stresslet velocity (1/pi) (r.n)(r.sigma) r / |r|^4,
+    stresslet pressure (1/pi) [-(n.sigma)/|r|^2 + 2 (r.n)(r.sigma)/|r|^4];
+    check skips coincident points.
+
+    Pinned by: the stresslet velocity equals minus the stress of a Stokeslet
+    of force sigma contracted with n (stress from finite differences of
+    direct_stokes), the Stokes equations of the double layer (div u = 0,
+    Laplacian u = grad p) by finite differences, and a tiny explicit-loop case
+    (tests/test_oracle_kernels.py)."""
+    sx, sy, sigx, sigy, nx, ny, tx, ty = [np.asarray(a, dtype=np.float64).ravel()
+                                          for a in (sx, sy, sigx, sigy, nx, ny, tx, ty)]
+    out = direct_stokes(sx, sy, fx, fy, tx, ty, check=check, chunk_pairs=chunk_pairs)
+    step = max(1, chunk_pairs // max(1, sx.size))
+    for s in range(0, tx.size, step):
+        dx = tx[s:s + step, None] - sx[None, :]
+        dy = ty[s:s + step, None] - sy[None, :]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            id2 = 1.0 / (dx * dx + dy * dy)
+            rn = dx * nx + dy * ny
+            rs = dx * sigx + dy * sigy
+            w = rn * rs * id2 * id2
+            uu = w * dx
+            vv = w * dy
+            pp = -(nx * sigx + ny * sigy) * id2 + 2.0 * w
+        if check:
+            keep = ~((dx == 0) & (dy == 0))
+            uu, vv, pp = [np.where(keep, a, 0.0) for a in (uu, vv, pp)]
+        out[0, s:s + step] += uu.sum(axis=1) / np.pi
+        out[1, s:s + step] += vv.sum(axis=1) / np.pi
+        out[2, s:s + step] += pp.sum(axis=1) / np.pi
+    return out
+
+
 def direct_src(src, sx, sy, tau_rows, extras_rows, tx, ty, check=False, chunk_pairs=20_000_000):
     """(nout, nt) array of sum_s src(s, t, tau_s, extras_s) for a source kernel
     ``src(sx, sy, tx, ty, tau_rows, extras_rows) -> tuple`` already weighted by

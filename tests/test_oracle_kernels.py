@@ -127,6 +127,121 @@ def test_stokeslet_laplace_decomposition():
         assert np.abs(got - r).max() < 1e-13 * np.abs(r).max()
 
 
+def _stokes_fd(fun, x, y, h):
+    c = fun(x, y)
+    return c, fun(x + h, y), fun(x - h, y), fun(x, y + h), fun(x, y - h)
+
+
+def test_stresslet_is_minus_stokeslet_stress_times_normal():
+    """The double-layer velocity of a source with density sigma and normal n
+    equals -T n, T = -p I + grad u + grad u^T the stress of a Stokeslet of
+    force sigma at the same source (unit viscosity), by central differences
+    of direct_stokes; the double-layer pressure follows from the Stokes
+    equations (next test)."""
+    from oracle import direct as odirect
+    rng = np.random.default_rng(21)
+    for trial in range(4):
+        s = rng.random(2)
+        sig = rng.standard_normal(2)
+        nrm = rng.standard_normal(2)
+        nrm /= np.linalg.norm(nrm)
+        t = rng.random(2) + np.array([1.0, 0.0]) * (trial % 2)
+        h = 1e-5
+
+        def f(x, y):
+            return odirect.direct_stokes([s[0]], [s[1]], [sig[0]], [sig[1]],
+                                         np.array([x]), np.array([y]))[:, 0]
+        c, xp, xm, yp, ym = _stokes_fd(f, t[0], t[1], h)
+        gu = np.array([[xp[0] - xm[0], yp[0] - ym[0]], [xp[1] - xm[1], yp[1] - ym[1]]]) / (2 * h)
+        T = -c[2] * np.eye(2) + gu + gu.T
+        got = odirect.direct_stokes_dipole([s[0]], [s[1]], [0.0], [0.0], [sig[0]], [sig[1]],
+                                           [nrm[0]], [nrm[1]], t[:1], t[1:])[:, 0]
+        ref = -T @ nrm
+        assert np.abs(got[:2] - ref).max() < 1e-7 * np.abs(ref).max()
+
+
+def test_stokes_double_layer_solves_stokes_equations():
+    """div u = 0 and Laplacian(u) = grad p for Stokeslets plus stresslets away
+    from the sources, by central differences; a tiny case by explicit loops in
+    the reference's own form (flexmm/kifmm2d/vector/stokes_fmm.py:187-235)."""
+    from oracle import direct as odirect
+    rng = np.random.default_rng(22)
+    m = 20
+    sx, sy = rng.random(m) * 0.2 + 0.4, rng.random(m) * 0.2 + 0.4
+    fx, fy, gx, gy = [rng.standard_normal(m) for _ in range(4)]
+    th = rng.random(m) * 2 * np.pi
+    nx, ny = np.cos(th), np.sin(th)
+    h = 1e-4
+
+    def f(x, y):
+        return odirect.direct_stokes_dipole(sx, sy, fx, fy, gx, gy, nx, ny,
+                                            np.array([x]), np.array([y]))[:, 0]
+    for x, y in [[0.1, 0.15], [0.9, 0.2], [0.2, 0.85]]:
+        c, xp, xm, yp, ym = _stokes_fd(f, x, y, h)
+        div = (xp[0] - xm[0]) / (2 * h) + (yp[1] - ym[1]) / (2 * h)
+        lap = (xp + xm + yp + ym - 4 * c) / h ** 2
+        gp = np.array([(xp[2] - xm[2]) / (2 * h), (yp[2] - ym[2]) / (2 * h)])
+        scale = np.abs(gp).max()
+        assert abs(div) < 1e-6 * scale
+        assert np.abs(lap[:2] - gp).max() < 1e-4 * scale
+    # explicit loop over three sources
+    x, y = 0.3, 0.7
+    u = np.zeros(3)
+    for j in range(3):
+        dx, dy = x - sx[j], y - sy[j]
+        r2 = dx * dx + dy * dy
+        dn = 4 * (dx * nx[j] + dy * ny[j]) / r2 ** 2
+        u[0] += ((-0.5 * np.log(r2) + dx * dx / r2) * fx[j] + dn * dx * dx * gx[j]
+                 + dx * dy / r2 * fy[j] + dn * dx * dy * gy[j]) / (4 * np.pi)
+        u[1] += (dx * dy / r2 * fx[j] + dn * dx * dy * gx[j]
+                 + (-0.5 * np.log(r2) + dy * dy / r2) * fy[j] + dn * dy * dy * gy[j]) / (4 * np.pi)
+        u[2] += ((dx * fx[j] + dy * fy[j]) / r2 + 2 * (-nx[j] / r2 + 0.5 * dx * dn) * gx[j]
+                 + 2 * (-ny[j] / r2 + 0.5 * dy * dn) * gy[j]) / (2 * np.pi)
+    got = odirect.direct_stokes_dipole(sx[:3], sy[:3], fx[:3], fy[:3], gx[:3], gy[:3], nx[:3],
+                                       ny[:3], np.array([x]), np.array([y]))[:, 0]
+    np.testing.assert_allclose(got, u, rtol=1e-12)
+    # coincident target skipped under check
+    got = odirect.direct_stokes_dipole(sx[:3], sy[:3], fx[:3], fy[:3], gx[:3], gy[:3], nx[:3],
+                                       ny[:3], sx[:1], sy[:1], check=True)[:, 0]
+    ref = odirect.direct_stokes_dipole(sx[1:3], sy[1:3], fx[1:3], fy[1:3], gx[1:3], gy[1:3],
+                                       nx[1:3], ny[1:3], sx[:1], sy[:1])[:, 0]
+    np.testing.assert_allclose(got, ref, rtol=1e-14)
+
+
+def test_stresslet_laplace_decomposition():
+    """The four extra Laplace fields that the device far field uses for the
+    stresslet (g = n.sigma, a = n_x sigma_x - n_y sigma_y, b = n_x sigma_y +
+    n_y sigma_x; Q[a, b] = sum a d_xx log|r| + b d_xy log|r|) reproduce the
+    direct stresslet sum: u_i = (d_i L[g] - X_i Q[a, b] + Q[X_i a, X_i b])/(2 pi),
+    p = -Q[a, b]/pi, X the coordinates about a fixed centre."""
+    from oracle import direct as odirect
+    rng = np.random.default_rng(23)
+    m = 300
+    sx, sy = rng.random(m), rng.random(m)
+    gx, gy = rng.standard_normal(m), rng.standard_normal(m)
+    th = rng.random(m) * 2 * np.pi
+    nx, ny = np.cos(th), np.sin(th)
+    tx, ty = rng.random(50) + 1.5, rng.random(50)
+    c0 = np.array([0.4, 0.6])
+    dx = tx[:, None] - sx[None, :]
+    dy = ty[:, None] - sy[None, :]
+    d2 = dx * dx + dy * dy
+    g = nx * gx + ny * gy
+    a = nx * gx - ny * gy
+    b = nx * gy + ny * gx
+    X, Y = sx - c0[0], sy - c0[1]
+
+    def Q(aa, bb):
+        return ((aa * (dy * dy - dx * dx) - 2 * bb * dx * dy) / d2 ** 2).sum(1)
+    u = ((dx / d2 * g).sum(1) - (tx - c0[0]) * Q(a, b) + Q(X * a, X * b)) / (2 * np.pi)
+    v = ((dy / d2 * g).sum(1) - (ty - c0[1]) * Q(a, b) + Q(Y * a, Y * b)) / (2 * np.pi)
+    p = -Q(a, b) / np.pi
+    z = np.zeros(m)
+    ref = odirect.direct_stokes_dipole(sx, sy, z, z, gx, gy, nx, ny, tx, ty)
+    for got, r in zip((u, v, p), ref):
+        assert np.abs(got - r).max() < 1e-12 * np.abs(r).max()
+
+
 def test_double_layer_is_source_derivative_of_single_layer():
     """(n.d)/d^2/(2 pi) with d = t - s is the derivative of the single layer
     -log(d^2)/(4 pi) along n at the source: the dipole of
