@@ -34,6 +34,7 @@ SIGNATURES = {
     "fmm2d_abi_version": (_int, []),
     "fmm2d_last_error": (ctypes.c_char_p, []),
     "fmm2d_plan_create": (_int, [_int, _vp, _int, _int, _int, ctypes.POINTER(_vp)]),
+    "fmm2d_plan_create_cheb": (_int, [_int, _vp, _int, _int, _int, ctypes.POINTER(_vp)]),
     "fmm2d_plan_destroy": (_int, [_vp]),
     "fmm2d_tree_build": (_int, [_vp, _vp, _vp, _i64, ctypes.POINTER(ctypes.c_double), _i64, _vp]),
     "fmm2d_tree_info": (_int, [_vp, ctypes.POINTER(_i64), _i64]),

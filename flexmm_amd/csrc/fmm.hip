@@ -63,6 +63,13 @@ __host__ __device__ inline int64_t ops_off_means(int PD) { return ops_off_circle
 __host__ __device__ inline int64_t ops_off_r32(int PD) { return ops_off_means(PD) + 40 * PD; }
 inline int64_t ops_per_level(int PD) { return ops_off_r32(PD) + 20 * (int64_t)PD * PD; }
 
+// Chebyshev (black-box) expansions, bbfmm.py:95-231: order pc <= kMaxCheb,
+// p = pc^2 coefficients per box, index i pc + j for T_i(x) T_j(y), with
+// (x, y) = (point - box centre) * (2 / w).  The level's 2/w sits in the first
+// entry of the operator block's circle area (no circles are used), and the
+// pseudo-inverse blocks are the identity (fmm.py:58-61 of flexmm2d).
+constexpr int kMaxCheb = 8;
+
 constexpr double kInv4Pi = 0.07957747154594767;   // 1/(4 pi)
 constexpr double kInv2Pi = 0.15915494309189535;   // 1/(2 pi)
 constexpr int LDA = 68;   // LDS row stride of an A block: 68 = 4 (mod 32) spreads an MFMA A
@@ -79,6 +86,7 @@ struct Plan {
   int kernel_id = KER_LAPLACE;
   double kparam = 0.0;
   int p = 0, PD = 0, precision = PREC_FP64;
+  int cheb = 0;        // Chebyshev order of a black-box plan (p = cheb^2); 0: KIFMM
   Tree tree;
   DevBuf ops;
   int ops_levels = 0;
@@ -266,6 +274,66 @@ __global__ __launch_bounds__(64 * kS2MWaves) void k_s2m(int nup, const int* __re
     }
   }
   if (k < PD) check[(int64_t)r * PD + k] = k < p ? acc.result() + kInv2Pi * dsum : 0.0;
+}
+
+// Chebyshev moments (source2pMs, bbfmm.py:173-184; anterpolate2 :11-24):
+// m[i pc + j] = sum_s T_i(x_s) T_j(y_s) tau_s, (x, y) = (s - centre) * (2 / w).
+// One 64-lane block per box: for each chunk of 64 sources, lane l writes
+// T_i(x) tau and T_j(y) of source l to LDS (three-term recurrence), then lane
+// i pc + j sums the products over the chunk.
+template <int PD>
+__global__ __launch_bounds__(64) void k_s2m_cheb(const int* __restrict__ upnodes,
+                                                 const int* __restrict__ nlevel,
+                                                 const int* __restrict__ bot,
+                                                 const int* __restrict__ top,
+                                                 const int* __restrict__ row,
+                                                 const double* __restrict__ xmid,
+                                                 const double* __restrict__ ymid,
+                                                 const double4* __restrict__ src,
+                                                 const double* __restrict__ ops, int64_t opsz,
+                                                 int pc, double* __restrict__ check) {
+  __shared__ double tx[64][kMaxCheb + 1];
+  __shared__ double ty[64][kMaxCheb + 1];
+  const int g = upnodes[blockIdx.x];
+  const int d = nlevel[g];
+  const int r = row[g];
+  if (r < 0 || d < 2) return;   // block-uniform
+  const int k = threadIdx.x;
+  const double wadj = ops[d * opsz + ops_off_circles(PD)];
+  const double cx = xmid[g], cy = ymid[g];
+  const int b = bot[g], e = top[g];
+  const int ci = k / pc, cj = k - ci * pc;
+  const bool live = k < pc * pc;
+  double acc = 0.0;
+  for (int c0 = b; c0 < e; c0 += 64) {
+    const int sidx = c0 + k;
+    if (sidx < e) {
+      const double4 sj = src[sidx];
+      const double x = (sj.x - cx) * wadj, y = (sj.y - cy) * wadj;
+      double a0 = sj.z, a1 = x * sj.z, b0 = 1.0, b1 = y;
+      tx[k][0] = a0;
+      tx[k][1] = a1;
+      ty[k][0] = b0;
+      ty[k][1] = b1;
+#pragma unroll
+      for (int m = 2; m < kMaxCheb; ++m) {
+        const double a2 = __builtin_fma(2.0 * x, a1, -a0);
+        const double b2 = __builtin_fma(2.0 * y, b1, -b0);
+        tx[k][m] = a2;
+        ty[k][m] = b2;
+        a0 = a1;
+        a1 = a2;
+        b0 = b1;
+        b1 = b2;
+      }
+    }
+    __syncthreads();
+    const int n = min(64, e - c0);
+    if (live)
+      for (int q = 0; q < n; ++q) acc = __builtin_fma(tx[q][ci], ty[q][cj], acc);
+    __syncthreads();
+  }
+  if (k < PD) check[(int64_t)r * PD + k] = live ? acc : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1287,6 +1355,38 @@ __device__ __forceinline__ FewLanes few_lanes(int na) {
   f.act = f.slot < na;
   return f;
 }
+// L2T (chebeval2d1, bbfmm.py:27-32,185-199): sum_{i,j} c[i pc + j] T_i(x) T_j(y)
+// over the rows i = i0, i0 + istep, ... (istep a power of two; lanes of one
+// target slot split the rows).  Indices are unrolled to constants so the
+// T values stay in registers.
+__device__ __forceinline__ double cheb_l2t(double xr, double yr, double wadj,
+                                           const double* __restrict__ c, int pc, int i0,
+                                           int istep) {
+  const double x = xr * wadj, y = yr * wadj;
+  double tx[kMaxCheb], ty[kMaxCheb];
+  tx[0] = 1.0;
+  tx[1] = x;
+  ty[0] = 1.0;
+  ty[1] = y;
+#pragma unroll
+  for (int m = 2; m < kMaxCheb; ++m) {
+    tx[m] = __builtin_fma(2.0 * x, tx[m - 1], -tx[m - 2]);
+    ty[m] = __builtin_fma(2.0 * y, ty[m - 1], -ty[m - 2]);
+  }
+  double u = 0.0;
+#pragma unroll
+  for (int i = 0; i < kMaxCheb; ++i) {
+    if (i < pc && i >= i0 && ((i - i0) & (istep - 1)) == 0) {
+      double r = 0.0;
+#pragma unroll
+      for (int j = 0; j < kMaxCheb; ++j)
+        if (j < pc) r = __builtin_fma(c[i * pc + j], ty[j], r);
+      u = __builtin_fma(tx[i], r, u);
+    }
+  }
+  return u;
+}
+
 // sum over the lanes that share this lane's target slot
 __device__ __forceinline__ double slot_sum(double v, int lg) {
   for (int off = 1 << lg; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
@@ -1298,7 +1398,7 @@ __device__ __forceinline__ uint32_t slot_min(uint32_t v, int lg) {
   return v;
 }
 
-template <int PD, int KER, int NOUT, bool CHECK, bool DIP>
+template <int PD, int KER, int NOUT, bool CHECK, bool DIP, bool CHEB>
 __device__ __forceinline__ void eval_few(
     int g, int t0, int na, const int* __restrict__ tperm, const double* __restrict__ tsx,
     const double* __restrict__ tsy, const int* __restrict__ nlevel,
@@ -1314,7 +1414,11 @@ __device__ __forceinline__ void eval_few(
   T.x = F.act ? tsx[t] : 0.0;
   T.y = F.act ? tsy[t] : 0.0;
   const int d = nlevel[g];
-  if (d >= 2) {
+  double far = 0.0;   // Chebyshev L2T
+  if (CHEB && d >= 2) {
+    far = cheb_l2t(T.x - xmid[g], T.y - ymid[g], ops[d * opsz + ops_off_circles(PD)],
+                   le + (int64_t)g * PD, p, F.sg, F.ng);
+  } else if (d >= 2) {
     Target<KER, NOUT, false> L;
     L.kp = kp;
     L.x = T.x - xmid[g];
@@ -1343,7 +1447,8 @@ __device__ __forceinline__ void eval_few(
       }
     }
   }
-  double u = slot_sum(DIP ? __builtin_fma(kInv2Pi, T.dip, T.u.result()) : T.u.result(), F.lg);
+  double u = slot_sum(DIP ? __builtin_fma(kInv2Pi, T.dip, T.u.result()) : T.u.result() + far,
+                      F.lg);
   double gx = 0.0, gy = 0.0;
   if (NOUT == 3) {
     gx = slot_sum(T.gx, F.lg);
@@ -1361,7 +1466,7 @@ __device__ __forceinline__ void eval_few(
   }
 }
 
-template <int PD, int KER, int NOUT, bool CHECK, bool DIP>
+template <int PD, int KER, int NOUT, bool CHECK, bool DIP, bool CHEB>
 __global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
     const int2* __restrict__ items, const int* __restrict__ nitems,
     const int* __restrict__ evalnode, const int* __restrict__ toff,
@@ -1383,9 +1488,9 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
     const int t0 = toff[it.x] + it.y * kTgtChunk;
     const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
     if (t1 - t0 <= kFewTargets) {
-      eval_few<PD, KER, NOUT, CHECK, DIP>(g, t0, t1 - t0, tperm, tsx, tsy, nlevel, xmid, ymid,
-                                          coll, bot, top, src, dip, le, ops, opsz, p, tab, kp,
-                                          nt, out);
+      eval_few<PD, KER, NOUT, CHECK, DIP, CHEB>(g, t0, t1 - t0, tperm, tsx, tsy, nlevel, xmid,
+                                                ymid, coll, bot, top, src, dip, le, ops, opsz, p,
+                                                tab, kp, nt, out);
       continue;
     }
     const int t = t0 + (int)(threadIdx.x & 63);
@@ -1394,9 +1499,14 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
     T.kp = kp;
     T.x = act ? tsx[t] : 0.0;
     T.y = act ? tsy[t] : 0.0;
-    // L2T: helpers.py:219-235 (the equivalent density sits on the large circle)
+    // L2T: helpers.py:219-235 (the equivalent density sits on the large circle);
+    // Chebyshev expansions: bbfmm.py:185-199
     const int d = nlevel[g];
-    if (d >= 2) {
+    double far = 0.0;
+    if (CHEB && d >= 2) {
+      far = cheb_l2t(T.x - xmid[g], T.y - ymid[g], ops[d * opsz + ops_off_circles(PD)],
+                     le + (int64_t)g * PD, p, 0, 1);
+    } else if (d >= 2) {
       Target<KER, NOUT, false> L;
       L.kp = kp;
       L.x = T.x - xmid[g];
@@ -1439,7 +1549,7 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
     }
     if (act) {
       const int64_t o = tperm[t];
-      double u = DIP ? __builtin_fma(kInv2Pi, T.dip, T.u.result()) : T.u.result();
+      double u = DIP ? __builtin_fma(kInv2Pi, T.dip, T.u.result()) : T.u.result() + far;
       // log(0): the reference's -log(0) = +inf; with a dipole also 0/0
       if (!CHECK && T.zhit()) u = DIP ? __builtin_nan("") : __builtin_inf();
       out[o] = u;
@@ -1493,7 +1603,7 @@ __global__ __launch_bounds__(64) void k_prep_mixed(const int* __restrict__ evaln
 // Few-target items of k_eval_mixed (lane layout of eval_few).  The
 // log(w^2) * sum(tau) term of a leaf colleague is added once per target, by
 // the lanes of source group 0; every lane removes the charges it took in FP64.
-template <int PD, bool CHECK>
+template <int PD, bool CHECK, bool CHEB>
 __device__ __forceinline__ void eval_few_mixed(
     int g, int t0, int na, const int* __restrict__ tperm, const double* __restrict__ tsx,
     const double* __restrict__ tsy, const int* __restrict__ nlevel, const int* __restrict__ flags,
@@ -1509,7 +1619,11 @@ __device__ __forceinline__ void eval_few_mixed(
   T.x = F.act ? tsx[t] : 0.0;
   T.y = F.act ? tsy[t] : 0.0;
   const int d = nlevel[g];
-  if (d >= 2) {
+  double far = 0.0;   // Chebyshev L2T
+  if (CHEB && d >= 2) {
+    far = cheb_l2t(T.x - xmid[g], T.y - ymid[g], ops[d * opsz + ops_off_circles(PD)],
+                   le + (int64_t)g * PD, p, F.sg, F.ng);
+  } else if (d >= 2) {
     Target<KER_LAPLACE, 1, false> L;
     L.kp = 0.0;
     L.x = T.x - xmid[g];
@@ -1557,12 +1671,12 @@ __device__ __forceinline__ void eval_few_mixed(
       }
     }
   }
-  double u = slot_sum(-kInv4Pi * (__builtin_fma(u32, kLn2, T.u.a.value())), F.lg);
+  double u = slot_sum(__builtin_fma(-kInv4Pi, __builtin_fma(u32, kLn2, T.u.a.value()), far), F.lg);
   const bool zh = CHECK ? false : slot_min(T.dmin, F.lg) == 0u;
   if (F.act && F.sg == 0) out[tperm[t]] = zh ? __builtin_inf() : u;
 }
 
-template <int PD, bool CHECK>
+template <int PD, bool CHECK, bool CHEB>
 __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
     const int2* __restrict__ items, const int* __restrict__ nitems,
     const int* __restrict__ evalnode, const int* __restrict__ toff,
@@ -1585,8 +1699,9 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
     const int t0 = toff[it.x] + it.y * kTgtChunk;
     const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
     if (t1 - t0 <= kFewTargets) {
-      eval_few_mixed<PD, CHECK>(g, t0, t1 - t0, tperm, tsx, tsy, nlevel, flags, xmid, ymid, coll,
-                                bot, top, src, srcf, tsum, lc, le, ops, opsz, p, tab, out);
+      eval_few_mixed<PD, CHECK, CHEB>(g, t0, t1 - t0, tperm, tsx, tsy, nlevel, flags, xmid,
+                                      ymid, coll, bot, top, src, srcf, tsum, lc, le, ops, opsz,
+                                      p, tab, out);
       continue;
     }
     const int t = t0 + (int)(threadIdx.x & 63);
@@ -1596,7 +1711,11 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
     T.x = act ? tsx[t] : 0.0;
     T.y = act ? tsy[t] : 0.0;
     const int d = nlevel[g];
-    if (d >= 2) {   // L2T in FP64 (the local densities carry large cancelling modes)
+    double far = 0.0;   // Chebyshev L2T
+    if (CHEB && d >= 2) {
+      far = cheb_l2t(T.x - xmid[g], T.y - ymid[g], ops[d * opsz + ops_off_circles(PD)],
+                     le + (int64_t)g * PD, p, 0, 1);
+    } else if (d >= 2) {   // L2T in FP64 (the local densities carry large cancelling modes)
       Target<KER_LAPLACE, 1, false> L;
       L.kp = 0.0;
       L.x = T.x - xmid[g];
@@ -1671,7 +1790,7 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
       }
     }
     if (act) {
-      double u = -kInv4Pi * (__builtin_fma(u32, kLn2, T.u.a.value()));
+      double u = __builtin_fma(-kInv4Pi, __builtin_fma(u32, kLn2, T.u.a.value()), far);
       if (!CHECK && T.zhit()) u = __builtin_inf();
       out[tperm[t]] = u;
     }
@@ -2002,18 +2121,32 @@ inline int eval_grid(int64_t icap) {
 }
 
 // Selects the k_eval instance of (PD, kernel, outputs, self-exclusion).
-typedef decltype(&k_eval<32, KER_LAPLACE, 1, false, false>) EvalKernelFn;
+typedef decltype(&k_eval<32, KER_LAPLACE, 1, false, false, false>) EvalKernelFn;
 template <int PD, int KER>
 EvalKernelFn eval_kernel_k(int nout, bool check) {
   if (nout == 1)
-    return check ? k_eval<PD, KER, 1, true, false> : k_eval<PD, KER, 1, false, false>;
-  return check ? k_eval<PD, KER, 3, true, false> : k_eval<PD, KER, 3, false, false>;
+    return check ? k_eval<PD, KER, 1, true, false, false> : k_eval<PD, KER, 1, false, false, false>;
+  return check ? k_eval<PD, KER, 3, true, false, false> : k_eval<PD, KER, 3, false, false, false>;
 }
 // Laplace single + double layer sources
 EvalKernelFn eval_kernel_dipole(int PD, bool check) {
   if (PD == 32)
-    return check ? k_eval<32, KER_LAPLACE, 1, true, true> : k_eval<32, KER_LAPLACE, 1, false, true>;
-  return check ? k_eval<64, KER_LAPLACE, 1, true, true> : k_eval<64, KER_LAPLACE, 1, false, true>;
+    return check ? k_eval<32, KER_LAPLACE, 1, true, true, false>
+                 : k_eval<32, KER_LAPLACE, 1, false, true, false>;
+  return check ? k_eval<64, KER_LAPLACE, 1, true, true, false>
+               : k_eval<64, KER_LAPLACE, 1, false, true, false>;
+}
+// Chebyshev (black-box) expansions: potential only
+template <int PD>
+EvalKernelFn eval_kernel_cheb_pd(int ker, bool check) {
+  if (ker == KER_LAPLACE)
+    return check ? k_eval<PD, KER_LAPLACE, 1, true, false, true>
+                 : k_eval<PD, KER_LAPLACE, 1, false, false, true>;
+  return check ? k_eval<PD, KER_MODHELM, 1, true, false, true>
+               : k_eval<PD, KER_MODHELM, 1, false, false, true>;
+}
+EvalKernelFn eval_kernel_cheb(int PD, int ker, bool check) {
+  return PD == 32 ? eval_kernel_cheb_pd<32>(ker, check) : eval_kernel_cheb_pd<64>(ker, check);
 }
 EvalKernelFn eval_kernel(int PD, int ker, int nout, bool check) {
   if (ker == KER_LAPLACE)
@@ -2240,6 +2373,13 @@ static void upward(Plan& P, const double* tau, const double* dvec, hipStream_t s
                        P.logtab.as<double2>(), t.xmin + 0.5 * t.w0, t.ymin + 0.5 * t.w0,
                        rstride, dvec ? P.sdip.as<double4>() : nullptr, P.check.as<double>());
     FMM_LAUNCH_CHECK();
+  } else if (t.n_up > 0 && P.cheb) {
+    auto kern = PD == 32 ? k_s2m_cheb<32> : k_s2m_cheb<64>;
+    hipLaunchKernelGGL(kern, dim3(t.n_up), dim3(64), 0, s, t.upnodes.as<int>(),
+                       t.nlevel.as<int>(), t.bot.as<int>(), t.top.as<int>(), t.row.as<int>(),
+                       t.xmid.as<double>(), t.ymid.as<double>(), P.src.as<double4>(),
+                       P.ops.as<double>(), opsz, P.cheb, P.check.as<double>());
+    FMM_LAUNCH_CHECK();
   } else if (t.n_up > 0) {
     auto kern = dvec ? (PD == 32 ? k_s2m<32, KER_LAPLACE, true> : k_s2m<64, KER_LAPLACE, true>)
                 : P.kernel_id == KER_LAPLACE
@@ -2327,6 +2467,8 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
               FMM_E_ARG, "evaluator %d does not belong to kernel %d", ev, P.kernel_id);
   FMM_REQUIRE((ev != EVAL_DIPOLE && ev != EVAL_STOKES_DIPOLE) || P.have_dip, FMM_E_STATE,
               "the dipole evaluators need expansions from fmm2d_upward_dipole");
+  FMM_REQUIRE(!P.cheb || ev == EVAL_POT, FMM_E_ARG,
+              "a Chebyshev (black-box) plan evaluates the potential only");
   FMM_REQUIRE(!(stokes && ev == EVAL_STOKES && P.have_dip), FMM_E_STATE,
               "the expansions hold double-layer densities: evaluate with the Stokes dipole "
               "evaluator");
@@ -2413,8 +2555,10 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
       FMM_LAUNCH_CHECK();
       P.have_srcf = true;
     }
-    auto mk = check ? (PD == 32 ? k_eval_mixed<32, true> : k_eval_mixed<64, true>)
-                    : (PD == 32 ? k_eval_mixed<32, false> : k_eval_mixed<64, false>);
+    auto mk = P.cheb ? (check ? (PD == 32 ? k_eval_mixed<32, true, true> : k_eval_mixed<64, true, true>)
+                              : (PD == 32 ? k_eval_mixed<32, false, true> : k_eval_mixed<64, false, true>))
+                     : (check ? (PD == 32 ? k_eval_mixed<32, true, false> : k_eval_mixed<64, true, false>)
+                              : (PD == 32 ? k_eval_mixed<32, false, false> : k_eval_mixed<64, false, false>));
     P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
     hipLaunchKernelGGL(mk, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
                        t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(),
@@ -2422,20 +2566,22 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
                        t.flags.as<int>(), t.xmid.as<double>(), t.ymid.as<double>(),
                        t.coll.as<int>(), t.bot.as<int>(), t.top.as<int>(), P.src.as<double4>(),
                        P.srcf.as<f4>(), P.tsum.as<double>(), lc, P.le.as<double>(),
-                       P.ops.as<double>(), opsz, P.p, P.logtab.as<double2>(), out);
+                       P.ops.as<double>(), opsz, P.cheb ? P.cheb : P.p, P.logtab.as<double2>(),
+                       out);
     FMM_LAUNCH_CHECK();
     P.mark(FMM2D_PHASE_EVAL_KERNEL, true, s);
     return;
   }
-  auto kern = ev == EVAL_DIPOLE ? eval_kernel_dipole(PD, check)
-                                : eval_kernel(PD, P.kernel_id, ev == EVAL_POT ? 1 : 3, check);
+  auto kern = P.cheb ? eval_kernel_cheb(PD, P.kernel_id, check)
+              : ev == EVAL_DIPOLE ? eval_kernel_dipole(PD, check)
+                                  : eval_kernel(PD, P.kernel_id, ev == EVAL_POT ? 1 : 3, check);
   P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
   hipLaunchKernelGGL(kern, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
                      t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(), P.tsx.as<double>(),
                      P.tsy.as<double>(), t.nlevel.as<int>(), t.xmid.as<double>(),
                      t.ymid.as<double>(), t.coll.as<int>(), t.bot.as<int>(), t.top.as<int>(),
                      P.src.as<double4>(), ev == EVAL_DIPOLE ? P.dip.as<double2>() : nullptr,
-                     P.le.as<double>(), P.ops.as<double>(), opsz, P.p,
+                     P.le.as<double>(), P.ops.as<double>(), opsz, P.cheb ? P.cheb : P.p,
                      P.logtab.as<double2>(), P.kparam, nt, out);
   FMM_LAUNCH_CHECK();
   P.mark(FMM2D_PHASE_EVAL_KERNEL, true, s);
@@ -2478,9 +2624,10 @@ int fmm2d_abi_version(void) { return FMM2D_ABI_VERSION; }
 
 const char* fmm2d_last_error(void) { return fmm::last_error(); }
 
-int fmm2d_plan_create(int kernel_id, const double* kparams, int nkparams, int p, int precision,
-                      void** out_plan) {
-  ABI_BEGIN
+namespace {
+// p equivalent points per box; cheb > 0: Chebyshev order of a black-box plan (p = cheb^2)
+void create_plan(int kernel_id, const double* kparams, int nkparams, int p, int cheb,
+                 int precision, void** out_plan) {
   NEED(out_plan);
   *out_plan = nullptr;
   FMM_REQUIRE(kernel_id == fmm::KER_LAPLACE || kernel_id == fmm::KER_MODHELM ||
@@ -2497,6 +2644,7 @@ int fmm2d_plan_create(int kernel_id, const double* kparams, int nkparams, int p,
     P->kernel_id = kernel_id;
     P->kparam = nkparams > 0 ? kparams[0] : 0.0;
     P->p = p;
+    P->cheb = cheb;
     P->PD = p <= 32 ? 32 : 64;
     P->precision = precision;
     FMM_HIP(hipGetDevice(&P->device));
@@ -2522,6 +2670,29 @@ int fmm2d_plan_create(int kernel_id, const double* kparams, int nkparams, int p,
     throw;
   }
   *out_plan = P;
+}
+}  // namespace
+
+int fmm2d_plan_create(int kernel_id, const double* kparams, int nkparams, int p, int precision,
+                      void** out_plan) {
+  ABI_BEGIN
+  create_plan(kernel_id, kparams, nkparams, p, 0, precision, out_plan);
+  ABI_END
+}
+
+int fmm2d_plan_create_cheb(int kernel_id, const double* kparams, int nkparams, int order,
+                           int precision, void** out_plan) {
+  ABI_BEGIN
+  NEED(out_plan);
+  *out_plan = nullptr;
+  FMM_REQUIRE(kernel_id == fmm::KER_LAPLACE || kernel_id == fmm::KER_MODHELM, fmm::FMM_E_ARG,
+              "black-box plans take the Laplace or modified Helmholtz kernel (got %d)",
+              kernel_id);
+  FMM_REQUIRE(order >= 2 && order <= fmm::kMaxCheb, fmm::FMM_E_LIMIT,
+              "Chebyshev order %d outside [2, %d]", order, fmm::kMaxCheb);
+  FMM_REQUIRE(precision != fmm::PREC_MIXED || kernel_id == fmm::KER_LAPLACE, fmm::FMM_E_ARG,
+              "the FP32 near field is Laplace only; use FP64 or FP32 M2L");
+  create_plan(kernel_id, kparams, nkparams, order * order, order, precision, out_plan);
   ABI_END
 }
 
@@ -2677,6 +2848,7 @@ int fmm2d_upward_dipole(void* plan, const double* tau, const double* dvec, int64
   FMM_REQUIRE(P->kernel_id == fmm::KER_LAPLACE || P->kernel_id == fmm::KER_STOKES,
               fmm::FMM_E_ARG, "dipole sources need the Laplace or Stokeslet kernel (plan has %d)",
               P->kernel_id);
+  FMM_REQUIRE(!P->cheb, fmm::FMM_E_ARG, "black-box plans take single-layer charges only");
   FMM_REQUIRE(P->tree.built, fmm::FMM_E_STATE, "no tree");
   FMM_REQUIRE(n == P->tree.n, fmm::FMM_E_ARG, "n = %lld, tree has %lld sources", (long long)n,
               (long long)P->tree.n);

@@ -102,11 +102,16 @@ class Helper:
 class _Plan:
     """Owner of one libflexmm_amd plan handle."""
 
-    def __init__(self, lib, kernel, p, precision):
+    def __init__(self, lib, kernel, p, precision, cheb=0):
         h = _lib.ctypes.c_void_p()
         params = (_lib.ctypes.c_double * max(1, len(kernel.params)))(*kernel.params) if kernel.params else None
-        _lib.check(lib.fmm2d_plan_create(kernel.kernel_id, params, len(kernel.params), p, precision,
-                                         _lib.ctypes.byref(h)))
+        if cheb:
+            # black-box plan of Chebyshev order cheb (p = cheb^2 coefficients)
+            _lib.check(lib.fmm2d_plan_create_cheb(kernel.kernel_id, params, len(kernel.params),
+                                                  cheb, precision, _lib.ctypes.byref(h)))
+        else:
+            _lib.check(lib.fmm2d_plan_create(kernel.kernel_id, params, len(kernel.params), p,
+                                             precision, _lib.ctypes.byref(h)))
         self.lib = lib
         self.h = h
         self.level_width = {}    # level -> width whose operators are loaded
@@ -128,8 +133,8 @@ class _Plan:
 _plan_pool = {}
 
 
-def _acquire_plan(lib, kernel, p, precision, device):
-    key = (kernel.key, p, precision, device)
+def _acquire_plan(lib, kernel, p, precision, device, cheb=0):
+    key = (kernel.key, p, precision, device, cheb)
     pool = _plan_pool.setdefault(key, [])
     while pool:
         plan = pool.pop()
@@ -137,11 +142,11 @@ def _acquire_plan(lib, kernel, p, precision, device):
         # FMM that returned it to the pool is no longer usable
         if plan.h:
             return plan
-    return _Plan(lib, kernel, p, precision)
+    return _Plan(lib, kernel, p, precision, cheb)
 
 
-def _release_plan(plan, kernel, p, precision, device):
-    key = (kernel.key, p, precision, device)
+def _release_plan(plan, kernel, p, precision, device, cheb=0):
+    key = (kernel.key, p, precision, device, cheb)
     _plan_pool.setdefault(key, []).append(plan)
 
 

@@ -109,6 +109,19 @@ const char* fmm2d_last_error(void);
  * Replaces FMM.__init__'s helper lookup (fmm.py:38-43, 61-65). */
 int fmm2d_plan_create(int kernel_id, const double* kparams, int nkparams, int p, int precision,
                       void** plan);
+/* Create a black-box (Chebyshev interpolation) plan of the given order
+ * (2 <= order <= 8; Nequiv = order^2 Chebyshev moments / coefficients per
+ * box) for the Laplace or modified Helmholtz kernel.  Replaces
+ * BB_FMM.__init__ and BB_Helper.prepare (flexmm2d/bbfmm.py:122-209): the
+ * level operators passed to fmm2d_set_operators are M2MC / L2LC
+ * (bbfmm.py:138-157) and the 40 M2L matrices VI K VI^T (:97-117), with
+ * identity pseudo-inverse blocks and the level's 2/width as the first
+ * circle entry.  fmm2d_upward takes charges (S2M = Chebyshev moments,
+ * :173-184); fmm2d_evaluate takes FMM2D_EVAL_POTENTIAL only (L2T = the
+ * Chebyshev sum, :185-199).  FMM2D_PREC_MIXED (FP32 near field) needs the
+ * Laplace kernel. */
+int fmm2d_plan_create_cheb(int kernel_id, const double* kparams, int nkparams, int order,
+                           int precision, void** plan);
 int fmm2d_plan_destroy(void* plan);
 
 /* Build the quadtree of n sources in the square box bbox = [xmin, xmax, ymin,

@@ -56,3 +56,30 @@ def test_padded_dim_limits():
     assert pc.padded_dim(4) == 32 and pc.padded_dim(33) == 64 and pc.padded_dim(64) == 64
     with pytest.raises(ValueError):
         pc.padded_dim(65)
+
+
+@pytest.mark.parametrize("order,w", [(4, 1.0), (8, 1.0 / 64), (5, 0.3)])
+def test_chebyshev_operators_match_oracle(order, w):
+    """Black-box operators (flexmm_amd.flexmm2d.ChebOperators) against the
+    oracle's BB_Precomputation restatement, and their packed layout: identity
+    pseudo-inverse blocks and 2/w as the first circle entry."""
+    from flexmm_amd.flexmm2d import bbfmm as bb
+    from oracle import bbfmm as obb
+    K = kernels.laplace
+    ops = bb.ChebOperators(K, order, w)
+    ora = obb.BBPrecomputation(w, okern.laplace, obb.BBInfo(order))
+    n = order * order
+    assert ops.p == n
+    np.testing.assert_allclose(ops.M2MC, ora.M2MC, rtol=0, atol=1e-13)
+    for s, (ox, oy) in enumerate(pc.m2l_slots()):
+        ref = ora.M2L[ox + 3, oy + 3]
+        np.testing.assert_allclose(ops.m2l[s], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    flat = ops.pack()
+    PD = pc.padded_dim(n)
+    nb = pc.OP_NBLK * PD * PD
+    blocks = flat[:nb].reshape(pc.OP_NBLK, PD, PD)
+    np.testing.assert_array_equal(blocks[0, :n, :n], np.eye(n))
+    np.testing.assert_array_equal(blocks[1, :n, :n], np.eye(n))
+    np.testing.assert_array_equal(blocks[6, :n, :n], ops.M2MC[:, :n].T)
+    circ = flat[nb:nb + 2 * PD].reshape(2, PD)
+    assert circ[0, 0] == 2.0 / w == ora.width_adj
