@@ -309,6 +309,85 @@ def stokes_workload(torch, steps, warmup):
             "velocity_error_vs_direct": err,
             "error_sample": "max over u, v of max|err|/max|ref| at 4000 sampled sources"}
 
+def bbfmm_workload(torch, steps, warmup):
+    """Black-box (Chebyshev) FMM on the headline shape: 1,000,000 uniform
+    sources, N(0,1) charges, the 1000 x 1000 grid targets, order 8 and
+    Ncutoff 50 (the reference's BB_FMM defaults, flexmm2d/bbfmm.py:201); FP64
+    and mixed forms; errors vs FP64 direct sums at 2,000 grid targets."""
+    from flexmm_amd import kernels
+    from flexmm_amd.flexmm2d import BB_FMM
+    dev = torch.device("cuda", 0)
+    x, y, tau, tx, ty = workload()
+    T = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
+    X, Y, TAU, TX, TY = T(x), T(y), T(tau), T(tx), T(ty)
+    box = [0.0, 1.0, 0.0, 1.0]
+    rng = np.random.default_rng(7)
+    sel = torch.from_numpy(np.sort(rng.choice(TX.numel(), 2000, replace=False))).to(dev)
+    ref = direct_sample(torch, X, Y, TAU, TX[sel], TY[sel])
+    res = {"workload": "Laplace black-box FMM, 1M uniform sources, N(0,1) charges, 1M grid "
+                       "targets, Chebyshev order 8, Ncutoff 50"}
+    us = {}
+    for prec in ("fp64", "mixed"):
+        out = {}
+
+        def step():
+            f = BB_FMM(X, Y, kernels.laplace, Ncutoff=50, p=8, bbox=box, precision=prec)
+            f.build_expansions(TAU)
+            out["u"] = f.target_evaluation(TX, TY)
+            out["levels"] = int(f.tree.levels)
+            f.close()
+
+        ms = _timed(torch, step, steps, warmup)
+        us[prec] = out["u"]
+        res["tree_levels"] = out["levels"]
+        res[prec] = {"ms_per_step": round(ms, 4),
+                     "value": round(TX.numel() / ms / 1e3, 3), "unit": "Mpts/s",
+                     "error_vs_direct": float((out["u"][sel] - ref).abs().max()
+                                              / ref.abs().max())}
+    res["mixed"]["error_vs_fp64"] = float((us["mixed"] - us["fp64"]).abs().max()
+                                          / us["fp64"].abs().max())
+    return res
+
+
+def stokes_dipole_workload(torch, steps, warmup):
+    """Stokeslets plus stresslets: 2,000,000 sources in 100 Gaussian clusters
+    (sigma 0.01), N(0,1) forces and double-layer densities, random unit
+    normals, self-evaluation of velocity and pressure, Nequiv 64, Ncutoff
+    100, FP64; the single-layer forward on the same points beside it."""
+    from flexmm_amd.kifmm2d import StokesFMM
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(9)
+    nc, per = 100, 20_000
+    c = rng.uniform(0.1, 0.9, (nc, 2))
+    pts = np.clip(c[np.repeat(np.arange(nc), per)] + 0.01 * rng.standard_normal((nc * per, 2)), 0, 1)
+    n = pts.shape[0]
+    T = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
+    X, Y = T(pts[:, 0]), T(pts[:, 1])
+    FX, FY, GX, GY = [T(rng.standard_normal(n)) for _ in range(4)]
+    th = rng.uniform(0, 2 * np.pi, n)
+    NX, NY = T(np.cos(th)), T(np.sin(th))
+    box = [0.0, 1.0, 0.0, 1.0]
+
+    def single():
+        f = StokesFMM(X, Y, 64, 100, bbox=box)
+        f.build_expansions(FX, FY)
+        f.evaluate_to_points(X, Y, True)
+        f.close()
+
+    def double():
+        f = StokesFMM(X, Y, 64, 100, bbox=box)
+        f.build_expansions_dipole(FX, FY, GX, GY, NX, NY)
+        f.evaluate_to_points_dipole(X, Y, True)
+        f.close()
+
+    ms_s = _timed(torch, single, steps, warmup)
+    ms_d = _timed(torch, double, steps, warmup)
+    return {"workload": "Stokeslets + stresslets, 2M sources in 100 Gaussian clusters "
+                        "(sigma 0.01), self-evaluation of u, v, p, Nequiv 64, Ncutoff 100, FP64",
+            "single_layer_ms": round(ms_s, 4), "single_plus_double_layer_ms": round(ms_d, 4),
+            "single_plus_double_layer_Mpts_s": round(n / ms_d / 1e3, 3)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -453,6 +532,10 @@ def main():
         out["adjoint"] = adjoint_workload(torch, 5, 2)
         progress("Stokeslet")
         out["stokes"] = stokes_workload(torch, 3, 1)
+        progress("Stokes double layer")
+        out["stokes_double_layer"] = stokes_dipole_workload(torch, 3, 1)
+        progress("black-box FMM")
+        out["bbfmm"] = bbfmm_workload(torch, 5, 2)
     if not args.no_cpu_baseline:
         progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample_targets)
