@@ -1251,7 +1251,8 @@ __global__ __launch_bounds__(256) void k_make_items(const int* __restrict__ icnt
 // b % 8 == x, dispatched in order) takes the x-th contiguous eighth of the
 // items.  Items are in leaf (Morton) order, so the colleague sources that
 // neighbouring leaves share stay in one XCD's L2.  The grid holds at least
-// ceil(ni / 8) waves per XCD; surplus waves exit.
+// ceil(ni / 8) waves per XCD; surplus waves exit.  (A grid capped at 8
+// blocks per CU, waves striding over several items, measured 8-20% slower.)
 constexpr int kEvalWaves = 4;
 struct ItemRange {
   int first, last, step;
@@ -1387,6 +1388,25 @@ __device__ __forceinline__ double cheb_l2t(double xr, double yr, double wadj,
   return u;
 }
 
+// The nine colleague source ranges of box g, loaded together: lane q < 9
+// holds colleague q and its [bot, top) (an absent colleague an empty range),
+// read back with readlane, so that an item pays one load latency for all
+// colleagues instead of a dependent chain per colleague.  Call with the whole
+// wave active.
+struct CollLanes {
+  int c, b, e;
+};
+__device__ __forceinline__ CollLanes coll_lanes(int g, const int* __restrict__ coll,
+                                                const int* __restrict__ bot,
+                                                const int* __restrict__ top) {
+  const int q = threadIdx.x & 63;
+  CollLanes L;
+  L.c = q < 9 ? coll[(int64_t)g * 9 + q] : -1;
+  L.b = L.c >= 0 ? bot[L.c] : 0;
+  L.e = L.c >= 0 ? top[L.c] : 0;
+  return L;
+}
+
 // sum over the lanes that share this lane's target slot
 __device__ __forceinline__ double slot_sum(double v, int lg) {
   for (int off = 1 << lg; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
@@ -1431,13 +1451,24 @@ __device__ __forceinline__ void eval_few(
     T.gx = L.gx;
     T.gy = L.gy;
   }
-  const int* cl = coll + (int64_t)g * 9;
+  const CollLanes CL = coll_lanes(g, coll, bot, top);
   for (int qn = 0; qn < 9; ++qn) {
-    const int c = cl[qn];
-    if (c < 0) continue;
-    const int b = __builtin_amdgcn_readfirstlane(bot[c]);
-    const int e = __builtin_amdgcn_readfirstlane(top[c]);
-    for (int j = b + F.sg; j < e; j += F.ng) {
+    const int b = __builtin_amdgcn_readlane(CL.b, qn);
+    const int e = __builtin_amdgcn_readlane(CL.e, qn);
+    if (e <= b) continue;
+    int j = b + F.sg;
+    if constexpr (!DIP) {
+      // four sources per lane in flight: the loads are per lane here (lanes
+      // of different source groups read different sources)
+      for (; j + 3 * F.ng < e; j += 4 * F.ng) {
+        double4 s4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s4[u] = src[j + u * F.ng];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) T.pair(s4[u].x, s4[u].y, s4[u].z, tab);
+      }
+    }
+    for (; j < e; j += F.ng) {
       const double4 sj = src[j];
       if constexpr (DIP) {
         const double2 q = dip[j];
@@ -1520,12 +1551,11 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
       T.gy = L.gy;
     }
     // P2P over the leaf's colleagues: helpers.py:180-217
-    const int* cl = coll + (int64_t)g * 9;
+    const CollLanes CL = coll_lanes(g, coll, bot, top);
     for (int qn = 0; qn < 9; ++qn) {
-      const int c = cl[qn];
-      if (c < 0) continue;
-      const int b = __builtin_amdgcn_readfirstlane(bot[c]);
-      const int e = __builtin_amdgcn_readfirstlane(top[c]);
+      const int b = __builtin_amdgcn_readlane(CL.b, qn);
+      const int e = __builtin_amdgcn_readlane(CL.e, qn);
+      if (e <= b) continue;
       int j = b;
       if constexpr (DIP) {
         for (; j < e; ++j) {
@@ -1637,35 +1667,50 @@ __device__ __forceinline__ void eval_few_mixed(
   const double iw = lc.invw[d];
   const double lw2 = lc.logw2[d];
   double u32 = 0.0;
-  const int* cl = coll + (int64_t)g * 9;
+  const CollLanes CL = coll_lanes(g, coll, bot, top);
   for (int qn = 0; qn < 9; ++qn) {
-    const int c = cl[qn];
-    if (c < 0) continue;
-    const int b = __builtin_amdgcn_readfirstlane(bot[c]);
-    const int e = __builtin_amdgcn_readfirstlane(top[c]);
+    const int c = __builtin_amdgcn_readlane(CL.c, qn);
+    const int b = __builtin_amdgcn_readlane(CL.b, qn);
+    const int e = __builtin_amdgcn_readlane(CL.e, qn);
     if (e <= b) continue;
     if (flags[c] & NF_LEAF) {
       const float xr = (float)((T.x - xmid[c]) * iw);
       const float yr = (float)((T.y - ymid[c]) * iw);
       float acc = 0.f;
       double excl = 0.0;
-      for (int j = b + F.sg; j < e; j += F.ng) {
-        const f4 sj = srcf[j];
+      auto pair32 = [&](const f4& sj, int jj) {
         const float dx = xr - sj.x, dy = yr - sj.y;
         const float d2 = __builtin_fmaf(dy, dy, dx * dx);
         const bool nr = d2 < kNearF32;
         const float lg2 = __builtin_amdgcn_logf(nr ? 1.0f : d2);
         acc = __builtin_fmaf(nr ? 0.f : sj.z, lg2, acc);
         if (nr) {
-          const double4 sd = src[j];
+          const double4 sd = src[jj];
           excl += sd.z;
           T.pair(sd.x, sd.y, sd.z, tab);
         }
+      };
+      int j = b + F.sg;
+      for (; j + 3 * F.ng < e; j += 4 * F.ng) {   // four per-lane loads in flight
+        f4 s4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s4[u] = srcf[j + u * F.ng];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pair32(s4[u], j + u * F.ng);
       }
+      for (; j < e; j += F.ng) pair32(srcf[j], j);
       u32 += (double)acc;
       T.u.a.at = __builtin_fma(lw2, (F.sg == 0 ? tsum[c] : 0.0) - excl, T.u.a.at);
     } else {
-      for (int j = b + F.sg; j < e; j += F.ng) {
+      int j = b + F.sg;
+      for (; j + 3 * F.ng < e; j += 4 * F.ng) {
+        double4 s4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s4[u] = src[j + u * F.ng];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) T.pair(s4[u].x, s4[u].y, s4[u].z, tab);
+      }
+      for (; j < e; j += F.ng) {
         const double4 sj = src[j];
         T.pair(sj.x, sj.y, sj.z, tab);
       }
@@ -1729,12 +1774,11 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
     const double iw = lc.invw[d];
     const double lw2 = lc.logw2[d];
     double u32 = 0.0;      // ln2-scaled part from the FP32 near field
-    const int* cl = coll + (int64_t)g * 9;
+    const CollLanes CL = coll_lanes(g, coll, bot, top);
     for (int qn = 0; qn < 9; ++qn) {
-      const int c = cl[qn];
-      if (c < 0) continue;
-      const int b = __builtin_amdgcn_readfirstlane(bot[c]);
-      const int e = __builtin_amdgcn_readfirstlane(top[c]);
+      const int c = __builtin_amdgcn_readlane(CL.c, qn);
+      const int b = __builtin_amdgcn_readlane(CL.b, qn);
+      const int e = __builtin_amdgcn_readlane(CL.e, qn);
       if (e <= b) continue;
       if (flags[c] & NF_LEAF) {
         const float xr = (float)((T.x - xmid[c]) * iw);
@@ -2010,14 +2054,12 @@ __device__ __forceinline__ void stokes_item(
   double su = 0.0, sv = 0.0, sp = 0.0;
   double du = 0.0, dv = 0.0, dp = 0.0;   // double layer
   double dmin = 1.0;
-  const int* cl = coll + (int64_t)g * 9;
+  const CollLanes CL = coll_lanes(g, coll, bot, top);
   for (int qn = 0; qn < 9; ++qn) {
-    const int c = cl[qn];
-    if (c < 0) continue;
-    const int b = __builtin_amdgcn_readfirstlane(bot[c]);
-    const int e = __builtin_amdgcn_readfirstlane(top[c]);
-    for (int j = b + F.sg; j < e; j += F.ng) {
-      const double4 sj = src[j];
+    const int b = __builtin_amdgcn_readlane(CL.b, qn);
+    const int e = __builtin_amdgcn_readlane(CL.e, qn);
+    if (e <= b) continue;
+    auto pairS = [&](const double4& sj, int jj) {
       const double dx = x - sj.x, dy = y - sj.y;
       double d2 = __builtin_fma(dy, dy, dx * dx);
       const bool z = d2 == 0.0;
@@ -2041,7 +2083,7 @@ __device__ __forceinline__ void stokes_item(
       sv = __builtin_fma(dy, rf, sv);
       sp += rf;
       if constexpr (DIP) {   // the stresslet itself
-        const double4 e4 = sdip[j];
+        const double4 e4 = sdip[jj];
         double gg = e4.x, aa = e4.y, bb = e4.z;
         if (CHECK) {
           gg = z ? 0.0 : gg;
@@ -2055,7 +2097,18 @@ __device__ __forceinline__ void stokes_item(
         dv = __builtin_fma(w, dy, dv);
         dp += __builtin_fma(-gg, id2, 2.0 * w);
       }
+    };
+    int j = b + F.sg;
+    if constexpr (FEW) {   // four per-lane loads in flight
+      for (; j + 3 * F.ng < e; j += 4 * F.ng) {
+        double4 s4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s4[u] = src[j + u * F.ng];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pairS(s4[u], j + u * F.ng);
+      }
     }
+    for (; j < e; j += F.ng) pairS(src[j], j);
   }
   u += kInv4Pi * (-0.5 * lx_.value() + su);
   v += kInv4Pi * (-0.5 * ly_.value() + sv);
