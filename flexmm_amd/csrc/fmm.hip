@@ -1235,13 +1235,19 @@ __global__ __launch_bounds__(256) void k_item_counts(const int* __restrict__ tcn
   if (b < ne) icnt[b] = (tcnt[b] + kTgtChunk - 1) / kTgtChunk;
 }
 
+// Item = (evaluation node, first target, end target) of one chunk of at most
+// kTgtChunk targets, in sorted-target positions.
 __global__ __launch_bounds__(256) void k_make_items(const int* __restrict__ icnt,
                                                    const int* __restrict__ ioff, int ne,
-                                                   int2* __restrict__ items) {
+                                                   const int* __restrict__ evalnode,
+                                                   const int* __restrict__ toff,
+                                                   int4* __restrict__ items) {
   int b = blockIdx.x * 256 + threadIdx.x;
   if (b >= ne) return;
   const int n = icnt[b], o = ioff[b];
-  for (int k = 0; k < n; ++k) items[o + k] = make_int2(b, k);
+  const int g = evalnode[b], t0 = toff[b], t1 = toff[b + 1];
+  for (int k = 0; k < n; ++k)
+    items[o + k] = make_int4(g, t0 + k * kTgtChunk, min(t1, t0 + (k + 1) * kTgtChunk), 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1397,13 +1403,13 @@ struct CollLanes {
   int c, b, e;
 };
 __device__ __forceinline__ CollLanes coll_lanes(int g, const int* __restrict__ coll,
-                                                const int* __restrict__ bot,
-                                                const int* __restrict__ top) {
+                                                const int2* __restrict__ crange) {
   const int q = threadIdx.x & 63;
   CollLanes L;
   L.c = q < 9 ? coll[(int64_t)g * 9 + q] : -1;
-  L.b = L.c >= 0 ? bot[L.c] : 0;
-  L.e = L.c >= 0 ? top[L.c] : 0;
+  const int2 r = q < 9 ? crange[(int64_t)g * 9 + q] : make_int2(0, 0);
+  L.b = r.x;
+  L.e = r.y;
   return L;
 }
 
@@ -1423,7 +1429,7 @@ __device__ __forceinline__ void eval_few(
     int g, int t0, int na, const int* __restrict__ tperm, const double* __restrict__ tsx,
     const double* __restrict__ tsy, const int* __restrict__ nlevel,
     const double* __restrict__ xmid, const double* __restrict__ ymid,
-    const int* __restrict__ coll, const int* __restrict__ bot, const int* __restrict__ top,
+    const int* __restrict__ coll, const int2* __restrict__ crange, const int* __restrict__ bot, const int* __restrict__ top,
     const double4* __restrict__ src, const double2* __restrict__ dip,
     const double* __restrict__ le, const double* __restrict__ ops, int64_t opsz, int p,
     const double2* __restrict__ tab, double kp, int64_t nt, double* __restrict__ out) {
@@ -1451,7 +1457,7 @@ __device__ __forceinline__ void eval_few(
     T.gx = L.gx;
     T.gy = L.gy;
   }
-  const CollLanes CL = coll_lanes(g, coll, bot, top);
+  const CollLanes CL = coll_lanes(g, coll, crange);
   for (int qn = 0; qn < 9; ++qn) {
     const int b = __builtin_amdgcn_readlane(CL.b, qn);
     const int e = __builtin_amdgcn_readlane(CL.e, qn);
@@ -1499,11 +1505,11 @@ __device__ __forceinline__ void eval_few(
 
 template <int PD, int KER, int NOUT, bool CHECK, bool DIP, bool CHEB>
 __global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
-    const int2* __restrict__ items, const int* __restrict__ nitems,
+    const int4* __restrict__ items, const int* __restrict__ nitems,
     const int* __restrict__ evalnode, const int* __restrict__ toff,
     const int* __restrict__ tperm, const double* __restrict__ tsx, const double* __restrict__ tsy,
     const int* __restrict__ nlevel, const double* __restrict__ xmid,
-    const double* __restrict__ ymid, const int* __restrict__ coll, const int* __restrict__ bot,
+    const double* __restrict__ ymid, const int* __restrict__ coll, const int2* __restrict__ crange, const int* __restrict__ bot,
     const int* __restrict__ top, const double4* __restrict__ src,
     const double2* __restrict__ dip,
     const double* __restrict__ le, const double* __restrict__ ops, int64_t opsz, int p,
@@ -1514,13 +1520,11 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
   __syncthreads();
   const ItemRange R = xcd_items(ni);
   for (int ii = R.first; ii < R.last; ii += R.step) {
-    const int2 it = items[ii];
-    const int g = evalnode[it.x];
-    const int t0 = toff[it.x] + it.y * kTgtChunk;
-    const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
+    const int4 it = items[ii];
+    const int g = it.x, t0 = it.y, t1 = it.z;
     if (t1 - t0 <= kFewTargets) {
       eval_few<PD, KER, NOUT, CHECK, DIP, CHEB>(g, t0, t1 - t0, tperm, tsx, tsy, nlevel, xmid,
-                                                ymid, coll, bot, top, src, dip, le, ops, opsz, p,
+                                                ymid, coll, crange, bot, top, src, dip, le, ops, opsz, p,
                                                 tab, kp, nt, out);
       continue;
     }
@@ -1551,7 +1555,7 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
       T.gy = L.gy;
     }
     // P2P over the leaf's colleagues: helpers.py:180-217
-    const CollLanes CL = coll_lanes(g, coll, bot, top);
+    const CollLanes CL = coll_lanes(g, coll, crange);
     for (int qn = 0; qn < 9; ++qn) {
       const int b = __builtin_amdgcn_readlane(CL.b, qn);
       const int e = __builtin_amdgcn_readlane(CL.e, qn);
@@ -1638,7 +1642,7 @@ __device__ __forceinline__ void eval_few_mixed(
     int g, int t0, int na, const int* __restrict__ tperm, const double* __restrict__ tsx,
     const double* __restrict__ tsy, const int* __restrict__ nlevel, const int* __restrict__ flags,
     const double* __restrict__ xmid, const double* __restrict__ ymid,
-    const int* __restrict__ coll, const int* __restrict__ bot, const int* __restrict__ top,
+    const int* __restrict__ coll, const int2* __restrict__ crange, const int* __restrict__ bot, const int* __restrict__ top,
     const double4* __restrict__ src, const f4* __restrict__ srcf, const double* __restrict__ tsum,
     const LevelConsts& lc, const double* __restrict__ le, const double* __restrict__ ops,
     int64_t opsz, int p, const double2* __restrict__ tab, double* __restrict__ out) {
@@ -1667,7 +1671,7 @@ __device__ __forceinline__ void eval_few_mixed(
   const double iw = lc.invw[d];
   const double lw2 = lc.logw2[d];
   double u32 = 0.0;
-  const CollLanes CL = coll_lanes(g, coll, bot, top);
+  const CollLanes CL = coll_lanes(g, coll, crange);
   for (int qn = 0; qn < 9; ++qn) {
     const int c = __builtin_amdgcn_readlane(CL.c, qn);
     const int b = __builtin_amdgcn_readlane(CL.b, qn);
@@ -1723,12 +1727,12 @@ __device__ __forceinline__ void eval_few_mixed(
 
 template <int PD, bool CHECK, bool CHEB>
 __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
-    const int2* __restrict__ items, const int* __restrict__ nitems,
+    const int4* __restrict__ items, const int* __restrict__ nitems,
     const int* __restrict__ evalnode, const int* __restrict__ toff,
     const int* __restrict__ tperm, const double* __restrict__ tsx, const double* __restrict__ tsy,
     const int* __restrict__ nlevel, const int* __restrict__ flags,
     const double* __restrict__ xmid, const double* __restrict__ ymid,
-    const int* __restrict__ coll, const int* __restrict__ bot, const int* __restrict__ top,
+    const int* __restrict__ coll, const int2* __restrict__ crange, const int* __restrict__ bot, const int* __restrict__ top,
     const double4* __restrict__ src, const f4* __restrict__ srcf,
     const double* __restrict__ tsum, LevelConsts lc, const double* __restrict__ le,
     const double* __restrict__ ops, int64_t opsz, int p, const double2* __restrict__ logtab,
@@ -1739,13 +1743,11 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
   __syncthreads();
   const ItemRange R = xcd_items(ni);
   for (int ii = R.first; ii < R.last; ii += R.step) {
-    const int2 it = items[ii];
-    const int g = evalnode[it.x];
-    const int t0 = toff[it.x] + it.y * kTgtChunk;
-    const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
+    const int4 it = items[ii];
+    const int g = it.x, t0 = it.y, t1 = it.z;
     if (t1 - t0 <= kFewTargets) {
       eval_few_mixed<PD, CHECK, CHEB>(g, t0, t1 - t0, tperm, tsx, tsy, nlevel, flags, xmid,
-                                      ymid, coll, bot, top, src, srcf, tsum, lc, le, ops, opsz,
+                                      ymid, coll, crange, bot, top, src, srcf, tsum, lc, le, ops, opsz,
                                       p, tab, out);
       continue;
     }
@@ -1774,7 +1776,7 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
     const double iw = lc.invw[d];
     const double lw2 = lc.logw2[d];
     double u32 = 0.0;      // ln2-scaled part from the FP32 near field
-    const CollLanes CL = coll_lanes(g, coll, bot, top);
+    const CollLanes CL = coll_lanes(g, coll, crange);
     for (int qn = 0; qn < 9; ++qn) {
       const int c = __builtin_amdgcn_readlane(CL.c, qn);
       const int b = __builtin_amdgcn_readlane(CL.b, qn);
@@ -1978,7 +1980,7 @@ __device__ __forceinline__ void stokes_item(
     int g, int t0, int t1, const int* __restrict__ tperm, const double* __restrict__ tsx,
     const double* __restrict__ tsy, const int* __restrict__ nlevel,
     const double* __restrict__ xmid, const double* __restrict__ ymid,
-    const int* __restrict__ coll, const int* __restrict__ bot, const int* __restrict__ top,
+    const int* __restrict__ coll, const int2* __restrict__ crange, const int* __restrict__ bot, const int* __restrict__ top,
     const double4* __restrict__ src, const double4* __restrict__ sdip,
     const double* __restrict__ le, int64_t nstride,
     const double* __restrict__ ops, int64_t opsz, int p, const double2* __restrict__ tab,
@@ -2054,7 +2056,7 @@ __device__ __forceinline__ void stokes_item(
   double su = 0.0, sv = 0.0, sp = 0.0;
   double du = 0.0, dv = 0.0, dp = 0.0;   // double layer
   double dmin = 1.0;
-  const CollLanes CL = coll_lanes(g, coll, bot, top);
+  const CollLanes CL = coll_lanes(g, coll, crange);
   for (int qn = 0; qn < 9; ++qn) {
     const int b = __builtin_amdgcn_readlane(CL.b, qn);
     const int e = __builtin_amdgcn_readlane(CL.e, qn);
@@ -2138,11 +2140,11 @@ __device__ __forceinline__ void stokes_item(
 // L2T of the six local densities + Stokeslet P2P, outputs (u, v, p).
 template <int PD, bool CHECK, bool DIP>
 __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_stokes(
-    const int2* __restrict__ items, const int* __restrict__ nitems,
+    const int4* __restrict__ items, const int* __restrict__ nitems,
     const int* __restrict__ evalnode, const int* __restrict__ toff,
     const int* __restrict__ tperm, const double* __restrict__ tsx, const double* __restrict__ tsy,
     const int* __restrict__ nlevel, const double* __restrict__ xmid,
-    const double* __restrict__ ymid, const int* __restrict__ coll, const int* __restrict__ bot,
+    const double* __restrict__ ymid, const int* __restrict__ coll, const int2* __restrict__ crange, const int* __restrict__ bot,
     const int* __restrict__ top, const double4* __restrict__ src,
     const double4* __restrict__ sdip,
     const double* __restrict__ le, int64_t nstride, const double* __restrict__ ops, int64_t opsz,
@@ -2154,16 +2156,14 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_stokes(
   __syncthreads();
   const ItemRange R = xcd_items(ni);
   for (int ii = R.first; ii < R.last; ii += R.step) {
-    const int2 it = items[ii];
-    const int g = evalnode[it.x];
-    const int t0 = toff[it.x] + it.y * kTgtChunk;
-    const int t1 = min(toff[it.x + 1], t0 + kTgtChunk);
+    const int4 it = items[ii];
+    const int g = it.x, t0 = it.y, t1 = it.z;
     if (t1 - t0 <= kFewTargets)
-      stokes_item<PD, CHECK, true, DIP>(g, t0, t1, tperm, tsx, tsy, nlevel, xmid, ymid, coll,
+      stokes_item<PD, CHECK, true, DIP>(g, t0, t1, tperm, tsx, tsy, nlevel, xmid, ymid, coll, crange,
                                         bot, top, src, sdip, le, nstride, ops, opsz, p, tab, c0x,
                                         c0y, nt, out);
     else
-      stokes_item<PD, CHECK, false, DIP>(g, t0, t1, tperm, tsx, tsy, nlevel, xmid, ymid, coll,
+      stokes_item<PD, CHECK, false, DIP>(g, t0, t1, tperm, tsx, tsy, nlevel, xmid, ymid, coll, crange,
                                          bot, top, src, sdip, le, nstride, ops, opsz, p, tab,
                                          c0x, c0y, nt, out);
   }
@@ -2538,7 +2538,7 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   P.icnt.ensure((ne + 1) * sizeof(int));
   P.ioff.ensure((ne + 1) * sizeof(int));
   const int64_t icap = ne + nt / kTgtChunk + 1;
-  P.items.ensure(icap * sizeof(int2));
+  P.items.ensure(icap * sizeof(int4));
   P.ntot.ensure(4 * sizeof(int));
   FMM_HIP(hipMemsetAsync(P.tcnt.ptr, 0, (ne + 1) * sizeof(int), s));
   LocateTable lt{};
@@ -2568,7 +2568,8 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   FMM_LAUNCH_CHECK();
   exclusive_scan_i32(P.icnt.as<int>(), P.ioff.as<int>(), ne, P.ntot.as<int>(), P.pw, s);
   hipLaunchKernelGGL(k_make_items, dim3(grid256(ne)), dim3(256), 0, s, P.icnt.as<int>(),
-                     P.ioff.as<int>(), (int)ne, P.items.as<int2>());
+                     P.ioff.as<int>(), (int)ne, t.evalnode.as<int>(), P.toff.as<int>(),
+                     P.items.as<int4>());
   FMM_LAUNCH_CHECK();
   const int PD = P.PD;
   const int64_t opsz = ops_per_level(PD);
@@ -2579,10 +2580,10 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
                  : (PD == 32 ? (check ? k_eval_stokes<32, true, false> : k_eval_stokes<32, false, false>)
                              : (check ? k_eval_stokes<64, true, false> : k_eval_stokes<64, false, false>));
     P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
-    hipLaunchKernelGGL(sk, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
+    hipLaunchKernelGGL(sk, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, P.items.as<int4>(), P.ntot.as<int>(),
                        t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(),
                        P.tsx.as<double>(), P.tsy.as<double>(), t.nlevel.as<int>(),
-                       t.xmid.as<double>(), t.ymid.as<double>(), t.coll.as<int>(),
+                       t.xmid.as<double>(), t.ymid.as<double>(), t.coll.as<int>(), t.crange.as<int2>(),
                        t.bot.as<int>(), t.top.as<int>(), P.src.as<double4>(),
                        dl ? P.sdip.as<double4>() : nullptr, P.le.as<double>(),
                        t.n_nodes + 1, P.ops.as<double>(), opsz, P.p, P.logtab.as<double2>(),
@@ -2613,11 +2614,11 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
                      : (check ? (PD == 32 ? k_eval_mixed<32, true, false> : k_eval_mixed<64, true, false>)
                               : (PD == 32 ? k_eval_mixed<32, false, false> : k_eval_mixed<64, false, false>));
     P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
-    hipLaunchKernelGGL(mk, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
+    hipLaunchKernelGGL(mk, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, P.items.as<int4>(), P.ntot.as<int>(),
                        t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(),
                        P.tsx.as<double>(), P.tsy.as<double>(), t.nlevel.as<int>(),
                        t.flags.as<int>(), t.xmid.as<double>(), t.ymid.as<double>(),
-                       t.coll.as<int>(), t.bot.as<int>(), t.top.as<int>(), P.src.as<double4>(),
+                       t.coll.as<int>(), t.crange.as<int2>(), t.bot.as<int>(), t.top.as<int>(), P.src.as<double4>(),
                        P.srcf.as<f4>(), P.tsum.as<double>(), lc, P.le.as<double>(),
                        P.ops.as<double>(), opsz, P.cheb ? P.cheb : P.p, P.logtab.as<double2>(),
                        out);
@@ -2629,10 +2630,10 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
               : ev == EVAL_DIPOLE ? eval_kernel_dipole(PD, check)
                                   : eval_kernel(PD, P.kernel_id, ev == EVAL_POT ? 1 : 3, check);
   P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
-  hipLaunchKernelGGL(kern, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, P.items.as<int2>(), P.ntot.as<int>(),
+  hipLaunchKernelGGL(kern, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, P.items.as<int4>(), P.ntot.as<int>(),
                      t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(), P.tsx.as<double>(),
                      P.tsy.as<double>(), t.nlevel.as<int>(), t.xmid.as<double>(),
-                     t.ymid.as<double>(), t.coll.as<int>(), t.bot.as<int>(), t.top.as<int>(),
+                     t.ymid.as<double>(), t.coll.as<int>(), t.crange.as<int2>(), t.bot.as<int>(), t.top.as<int>(),
                      P.src.as<double4>(), ev == EVAL_DIPOLE ? P.dip.as<double2>() : nullptr,
                      P.le.as<double>(), P.ops.as<double>(), opsz, P.cheb ? P.cheb : P.p,
                      P.logtab.as<double2>(), P.kparam, nt, out);

@@ -49,6 +49,7 @@ struct Tree {
   DevBuf child;    // i32 global index of first child or -1
   DevBuf flags;    // i32 NodeFlag bits
   DevBuf coll;     // i32[9] global colleagues or -1; slot (dx+1)*3+(dy+1)
+  DevBuf crange;   // int2[9]: colleague q's source range [bot, top); (0, 0) when absent
   DevBuf xmid, ymid;
   DevBuf row;      // i32 multipole row (levels >= 2, sibling groups holding sources) or -1
   DevBuf rownode;  // i32[n_rows] node of each row

@@ -621,6 +621,18 @@ __global__ __launch_bounds__(64) void k_nodes_b(
   parent[g] = par;
 }
 
+// Source ranges of every node's nine colleagues, for the evaluation kernels
+// (one load per colleague instead of a colleague index and then its range).
+__global__ __launch_bounds__(BS) void k_coll_ranges(int nslots, const int* __restrict__ coll,
+                                                   const int* __restrict__ bot,
+                                                   const int* __restrict__ top,
+                                                   int2* __restrict__ crange) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= nslots) return;
+  const int c = coll[i];
+  crange[i] = c >= 0 ? make_int2(bot[c], top[c]) : make_int2(0, 0);
+}
+
 // --- density rows: sibling groups (levels >= 2) holding a source ------------------
 __global__ __launch_bounds__(BS) void k_group_flags(const int* __restrict__ flags, int g0, int ngroups,
                                                     const int* __restrict__ levoff, int L,
@@ -1083,6 +1095,11 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
                      t.code.as<uint64_t>(), t.flags.as<int>(), t.bot.as<int>(), t.top.as<int>(),
                      t.parent.as<int>(), t.child.as<int>(), t.coll.as<int>(), t.xmid.as<double>(),
                      t.ymid.as<double>());
+  FMM_LAUNCH_CHECK();
+  t.crange.ensure(((int64_t)off * 9 + 1) * sizeof(int2));
+  hipLaunchKernelGGL(k_coll_ranges, dim3(grid_for((int64_t)off * 9)), dim3(BS), 0, s,
+                     (int)(off * 9), t.coll.as<int>(), t.bot.as<int>(), t.top.as<int>(),
+                     t.crange.as<int2>());
   FMM_LAUNCH_CHECK();
 
   // -- density rows ------------------------------------------------------------------
