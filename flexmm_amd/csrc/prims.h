@@ -70,8 +70,13 @@ void exclusive_scan_i32(const int* in, int* out, int64_t n, int* total_dev, Prim
 // key_bits bits (the others must be equal in all keys).  Keys that agree in
 // all their bits keep an unspecified order when more than kLocalSortMax of
 // them coincide.
+// prefix_only: when every segment reaches the threshold in the same radix
+// round (so all of them agree on the same top 8r bits), leave each segment in
+// its radix (input) order instead of sorting it on the remaining bits, and
+// return true.  The quadtree uses this: a segment of at most Ncutoff points
+// is never split, so no cell below that depth needs its points grouped.
 constexpr int kLocalSortMax = 2048;
-void sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, PrimWork& w,
-                    hipStream_t s, int key_bits = 64);
+bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, PrimWork& w,
+                    hipStream_t s, int key_bits = 64, bool prefix_only = false);
 
 }  // namespace fmm

@@ -427,9 +427,9 @@ __global__ __launch_bounds__(SORT_BS) void k_local_sort(uint64_t* __restrict__ k
 
 }  // namespace
 
-void sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, PrimWork& w,
-                    hipStream_t s, int key_bits) {
-  if (n <= 1) return;
+bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, PrimWork& w,
+                    hipStream_t s, int key_bits, bool prefix_only) {
+  if (n <= 1) return prefix_only;
   FMM_REQUIRE(n < (int64_t)1 << 31, FMM_E_LIMIT, "sort: n = %lld exceeds 2^31-1", (long long)n);
   int T = threshold < 1 ? 1 : threshold;
   if (T > kLocalSortMax) T = kLocalSortMax;
@@ -461,7 +461,9 @@ void sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
     nbig = 1;
   }
   const int rounds = key_bits < 8 ? 1 : (key_bits > 64 ? 8 : (key_bits + 7) / 8);
+  int finished_before_last = nfinal + nbig;   // segments finished before the last round run
   for (int r = 0; r < rounds && nseg > 0; ++r) {
+    finished_before_last = nfinal + nbig;
     const int shift = 56 - 8 * r;
     const int64_t cbound = nseg + active / CHUNK + 1;
     w.chunks.ensure(cbound * sizeof(int4));
@@ -495,6 +497,7 @@ void sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
     cur ^= 1;
   }
   FMM_REQUIRE(nfinal + nbig <= segcap, FMM_E_INTERNAL, "sort: segment list overflow");
+  if (prefix_only && nseg == 0 && finished_before_last == 0) return true;
   if (nfinal > 0) {
     hipLaunchKernelGGL(k_wave_sort, dim3(ceil_div(nfinal, 4)), dim3(256), 0, s, keys, vals,
                        w.fin.as<int2>(), nfinal);
@@ -505,6 +508,7 @@ void sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
                        w.fin.as<int2>() + (finBigOff - nbig));
     FMM_LAUNCH_CHECK();
   }
+  return false;
 }
 
 }  // namespace fmm

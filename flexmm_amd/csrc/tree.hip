@@ -794,7 +794,11 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   hipLaunchKernelGGL(k_point_keys, dim3(grid_for(n)), dim3(BS), 0, s, x, y, nn, xmin, ymin, w0,
                      keys, perm, t.xy.as<double2>());
   FMM_LAUNCH_CHECK();
-  sort_pairs_u64(keys, perm, n, ncut < kLocalSortMax ? ncut : kLocalSortMax, t.pw, s);
+  // Segments of at most Ncutoff points are never split, so when they all
+  // appear at the same radix depth no cell below it needs its points grouped
+  // (2:1 balance splits lie at least two levels above the deepest leaf, X-list
+  // children one level below a leaf): the radix order within them stands.
+  sort_pairs_u64(keys, perm, n, ncut < kLocalSortMax ? ncut : kLocalSortMax, t.pw, s, 64, true);
   hipLaunchKernelGGL(k_gather_sorted, dim3(grid_for(n)), dim3(BS), 0, s, t.xy.as<double2>(), perm,
                      nn, t.sx.as<double>(), t.sy.as<double>());
   FMM_LAUNCH_CHECK();

@@ -100,3 +100,27 @@ def test_direct_sum_p64():
     err = H.rel_max_err(u[sel], ref)
     print("err vs direct %.3e" % err)
     assert err < 5e-13
+
+
+def test_reference_cpu_case_100k():
+    """The reference's own CPU-runnable case: 100,000 uniform sources, N(0,1)
+    charges, self-evaluation with self-exclusion, 32 equivalent points, leaf
+    capacity 64 -- against the oracle (parity) and FP64 direct sums (the
+    truncation error of p = 32)."""
+    rng = np.random.default_rng(100)
+    x, y = rng.random(100_000), rng.random(100_000)
+    tau = rng.standard_normal(100_000)
+    print("100k self-eval p 32 Ncutoff 64", flush=True)
+    f = _fmm(x, y, 64, 32)
+    f.build_expansions(tau)
+    u = f.source_evaluation(x, y)[0]
+    o = _oracle(x, y, 64, 32)
+    o.build_expansions(tau)
+    uo = o.source_evaluation(x, y)[0]
+    err = H.rel_max_err(u, uo)
+    sel = np.arange(0, x.size, 97)
+    ref = odirect.direct(okern.laplace, x, y, tau, x[sel], y[sel], check=True)[0]
+    errd = H.rel_max_err(u[sel], ref)
+    print("levels", f.tree.levels, "vs oracle %.3e, vs direct %.3e" % (err, errd))
+    assert err < PARITY_TOL
+    assert errd < 3e-8   # p = 32 truncation: 7.2e-9 measured
