@@ -637,7 +637,8 @@ struct M2LOrderTable {
   int boff[kMaxLevels + 1];
   int poff[kMaxLevels];
 };
-__global__ __launch_bounds__(256) void k_m2l_order(M2LOrderTable ot, const int* __restrict__ flags,
+constexpr int kM2LOrderThreads = 1024;
+__global__ __launch_bounds__(kM2LOrderThreads) void k_m2l_order(M2LOrderTable ot, const int* __restrict__ flags,
                                                    const int* __restrict__ m2l,
                                                    int* __restrict__ perm) {
   __shared__ uint64_t key[kM2LOrderChunk];
@@ -652,7 +653,7 @@ __global__ __launch_bounds__(256) void k_m2l_order(M2LOrderTable ot, const int* 
   const int cnt = min(kM2LOrderChunk, ng - q0);
   static_assert(kNumM2L == 40 && kM2LOrderChunk <= 2048, "key layout: 40 + 11 bits");
 #pragma unroll 2
-  for (int i = threadIdx.x; i < kM2LOrderChunk; i += 256) {
+  for (int i = threadIdx.x; i < kM2LOrderChunk; i += kM2LOrderThreads) {
     uint64_t k = ~0ull;
     if (i < cnt) {
       const int g = ot.g0[li] + 4 * (q0 + i) + c;
@@ -674,8 +675,9 @@ __global__ __launch_bounds__(256) void k_m2l_order(M2LOrderTable ot, const int* 
   __syncthreads();
   for (int size = 2; size <= kM2LOrderChunk; size <<= 1) {
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int t = threadIdx.x; t < kM2LOrderChunk / 2; t += 256) {
-        const int lo = 2 * stride * (t / stride) + (t % stride);
+      for (int t = threadIdx.x; t < kM2LOrderChunk / 2; t += kM2LOrderThreads) {
+        // stride is a power of two: lo = 2 stride (t / stride) + t % stride
+        const int lo = ((t & ~(stride - 1)) << 1) | (t & (stride - 1));
         const int hi = lo + stride;
         const bool up = (lo & size) == 0;
         const uint64_t a = key[lo], e = key[hi];
@@ -687,7 +689,7 @@ __global__ __launch_bounds__(256) void k_m2l_order(M2LOrderTable ot, const int* 
       __syncthreads();
     }
   }
-  for (int i = threadIdx.x; i < cnt; i += 256)
+  for (int i = threadIdx.x; i < cnt; i += kM2LOrderThreads)
     perm[ot.poff[li] + c * ng + q0 + i] = q0 + (int)(key[i] & 2047u);
 }
 
@@ -2323,7 +2325,8 @@ static void m2l_prepare(Plan& P, const M2LTables& T, hipStream_t s) {
   P.ple.ensure((t.n_nodes + 1) * P.PD * nd * sizeof(double));
   if (!T.adaptive || P.perm_tree == t.build_id) return;
   P.m2lperm.ensure((T.perm_len + 1) * sizeof(int));
-  hipLaunchKernelGGL(k_m2l_order, dim3(T.order_blocks), dim3(256), 0, s, T.ot, t.flags.as<int>(),
+  hipLaunchKernelGGL(k_m2l_order, dim3(T.order_blocks), dim3(kM2LOrderThreads), 0, s, T.ot,
+                     t.flags.as<int>(),
                      t.m2l.as<int>(), P.m2lperm.as<int>());
   FMM_LAUNCH_CHECK();
   P.perm_tree = t.build_id;

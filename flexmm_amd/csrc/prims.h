@@ -70,11 +70,12 @@ void exclusive_scan_i32(const int* in, int* out, int64_t n, int* total_dev, Prim
 // key_bits bits (the others must be equal in all keys).  Keys that agree in
 // all their bits keep an unspecified order when more than kLocalSortMax of
 // them coincide.
-// prefix_only: when every segment reaches the threshold in the same radix
-// round (so all of them agree on the same top 8r bits), leave each segment in
-// its radix (input) order instead of sorting it on the remaining bits, and
-// return true.  The quadtree uses this: a segment of at most Ncutoff points
-// is never split, so no cell below that depth needs its points grouped.
+// prefix_only: leave the segments that reach the threshold in the last radix
+// round run in their radix (input) order instead of sorting them on the
+// remaining bits, and return true if any was left so.  The quadtree uses
+// this: those segments lie at the deepest radix depth D, every cell at depth
+// D holds at most Ncutoff points, so no cell of the tree below depth D exists
+// and none needs its points grouped; segments that ended earlier are sorted.
 constexpr int kLocalSortMax = 2048;
 bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, PrimWork& w,
                     hipStream_t s, int key_bits = 64, bool prefix_only = false);

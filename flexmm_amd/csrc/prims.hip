@@ -461,9 +461,16 @@ bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
     nbig = 1;
   }
   const int rounds = key_bits < 8 ? 1 : (key_bits > 64 ? 8 : (key_bits + 7) / 8);
-  int finished_before_last = nfinal + nbig;   // segments finished before the last round run
+  // final segments (small, big) found before the last round run: the small
+  // ones fill fin from the front, the big ones from finBigOff downwards, so
+  // the segments of the last round are fin[nfin_early, nfinal) and
+  // fin[finBigOff - nbig, finBigOff - nbig_early)
+  int nfin_early = nfinal, nbig_early = nbig;
+  bool ran = false;
   for (int r = 0; r < rounds && nseg > 0; ++r) {
-    finished_before_last = nfinal + nbig;
+    nfin_early = nfinal;
+    nbig_early = nbig;
+    ran = true;
     const int shift = 56 - 8 * r;
     const int64_t cbound = nseg + active / CHUNK + 1;
     w.chunks.ensure(cbound * sizeof(int4));
@@ -497,18 +504,21 @@ bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
     cur ^= 1;
   }
   FMM_REQUIRE(nfinal + nbig <= segcap, FMM_E_INTERNAL, "sort: segment list overflow");
-  if (prefix_only && nseg == 0 && finished_before_last == 0) return true;
-  if (nfinal > 0) {
-    hipLaunchKernelGGL(k_wave_sort, dim3(ceil_div(nfinal, 4)), dim3(256), 0, s, keys, vals,
-                       w.fin.as<int2>(), nfinal);
+  // prefix_only: the segments of the last round are left in radix order
+  const bool skip_last = prefix_only && ran && nseg == 0;
+  const int nws = skip_last ? nfin_early : nfinal;
+  const int nls = skip_last ? nbig_early : nbig;
+  if (nws > 0) {
+    hipLaunchKernelGGL(k_wave_sort, dim3(ceil_div(nws, 4)), dim3(256), 0, s, keys, vals,
+                       w.fin.as<int2>(), nws);
     FMM_LAUNCH_CHECK();
   }
-  if (nbig > 0) {
-    hipLaunchKernelGGL(k_local_sort, dim3(nbig), dim3(SORT_BS), 0, s, keys, vals,
-                       w.fin.as<int2>() + (finBigOff - nbig));
+  if (nls > 0) {
+    hipLaunchKernelGGL(k_local_sort, dim3(nls), dim3(SORT_BS), 0, s, keys, vals,
+                       w.fin.as<int2>() + (finBigOff - nls));
     FMM_LAUNCH_CHECK();
   }
-  return false;
+  return skip_last && (nws < nfinal || nls < nbig);
 }
 
 }  // namespace fmm

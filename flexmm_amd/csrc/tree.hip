@@ -794,10 +794,12 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   hipLaunchKernelGGL(k_point_keys, dim3(grid_for(n)), dim3(BS), 0, s, x, y, nn, xmin, ymin, w0,
                      keys, perm, t.xy.as<double2>());
   FMM_LAUNCH_CHECK();
-  // Segments of at most Ncutoff points are never split, so when they all
-  // appear at the same radix depth no cell below it needs its points grouped
-  // (2:1 balance splits lie at least two levels above the deepest leaf, X-list
-  // children one level below a leaf): the radix order within them stands.
+  // Segments of at most Ncutoff points that end in the last radix round sit
+  // at the deepest radix depth D, which no cell of the tree goes below (2:1
+  // balance splits lie at least two levels above the deepest leaf, X-list
+  // children at most at depth D): the radix order within them stands.
+  // Segments that end earlier are sorted, since balance and X-list children
+  // can split their cells further.
   sort_pairs_u64(keys, perm, n, ncut < kLocalSortMax ? ncut : kLocalSortMax, t.pw, s, 64, true);
   hipLaunchKernelGGL(k_gather_sorted, dim3(grid_for(n)), dim3(BS), 0, s, t.xy.as<double2>(), perm,
                      nn, t.sx.as<double>(), t.sy.as<double>());
