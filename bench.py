@@ -47,6 +47,16 @@ def measured_traffic():
     return None
 
 
+# extra workloads of the default run: key -> (function, steps, warmup)
+EXTRA_WORKLOADS = {
+    "clustered": ("clustered_workload", 5, 2),
+    "adjoint": ("adjoint_workload", 5, 2),
+    "stokes": ("stokes_workload", 3, 1),
+    "stokes_double_layer": ("stokes_dipole_workload", 3, 1),
+    "bbfmm": ("bbfmm_workload", 5, 2),
+}
+
+
 def progress(what):
     print("bench: %s" % what, file=sys.stderr, flush=True)
 
@@ -59,9 +69,13 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-targets", type=int, default=200_000)
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the differentiable modified-Helmholtz and Stokeslet workloads")
-    ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
-                    help="time one form only and print its phase times (for profiling runs)")
+                    help="skip the extra workloads (clustered, adjoint, Stokeslet, Stokes "
+                         "double layer, black-box)")
+    ap.add_argument("--only", choices=["fp64", "mixed"] + sorted(EXTRA_WORKLOADS), default=None,
+                    help="time one form or one extra workload only (for profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a device-to-device copy of 16777216 doubles (the known-size "
+                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -399,6 +413,18 @@ def main():
     from flexmm_amd.kifmm2d import FMM
 
     dev = torch.device("cuda", 0)
+    if args.copy_calibration:
+        src = torch.ones(1 << 24, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": src.numel() * 8}))
+        return
+    if args.only in EXTRA_WORKLOADS:
+        fn, steps, warmup = EXTRA_WORKLOADS[args.only]
+        print(json.dumps({args.only: globals()[fn](torch, steps, warmup)}))
+        return
     x, y, tau, tx, ty = workload()
     xd = torch.from_numpy(x).to(dev)
     yd = torch.from_numpy(y).to(dev)
@@ -526,16 +552,9 @@ def main():
         },
     }
     if not args.no_extra:
-        progress("clustered sources")
-        out["clustered"] = clustered_workload(torch, 5, 2)
-        progress("modified Helmholtz adjoint")
-        out["adjoint"] = adjoint_workload(torch, 5, 2)
-        progress("Stokeslet")
-        out["stokes"] = stokes_workload(torch, 3, 1)
-        progress("Stokes double layer")
-        out["stokes_double_layer"] = stokes_dipole_workload(torch, 3, 1)
-        progress("black-box FMM")
-        out["bbfmm"] = bbfmm_workload(torch, 5, 2)
+        for key, (fn, steps, warmup) in EXTRA_WORKLOADS.items():
+            progress(key)
+            out[key] = globals()[fn](torch, steps, warmup)
     if not args.no_cpu_baseline:
         progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample_targets)
