@@ -31,13 +31,24 @@ def _clenshaw(c, u):
 
 
 def _series(x, terms=14):
+    """The ascending series as the device evaluates them: polynomials in
+    q = x^2/4 with coefficients 1/(k!)^2, H_k/(k!)^2, 1/(k!(k+1)!) and
+    (H_k + H_{k+1} - 2 gamma)/(k!(k+1)!), by Horner's rule."""
+    c = np.zeros((4, terms))
+    f = g = 1.0
+    h = 0.0
+    for k in range(terms):
+        if k > 0:
+            f /= k * k
+            g /= k * (k + 1)
+            h += 1.0 / k
+        h1 = h + 1.0 / (k + 1)
+        c[:, k] = f, h * f, g, (h + h1 - 2 * GAMMA) * g
     q = 0.25 * x * x
     lg = np.log(0.5 * x)
-    t = np.ones_like(x); i0 = np.ones_like(x); s0 = np.zeros_like(x); h = 0.0
-    v = np.ones_like(x); i1 = np.ones_like(x); s1 = np.full_like(x, 1 - 2 * GAMMA); h1 = 1.0
-    for k in range(1, terms):
-        t = t * q / (k * k); h += 1.0 / k; i0 += t; s0 += h * t
-        v = v * q / (k * (k + 1)); h1 += 1.0 / (k + 1); i1 += v; s1 += (h + h1 - 2 * GAMMA) * v
+    i0, s0, i1, s1 = [np.full_like(x, c[j, -1]) for j in range(4)]
+    for k in range(terms - 2, -1, -1):
+        i0, s0, i1, s1 = i0 * q + c[0, k], s0 * q + c[1, k], i1 * q + c[2, k], s1 * q + c[3, k]
     return s0 - (lg + GAMMA) * i0, 1.0 / x + lg * 0.5 * x * i1 - 0.25 * x * s1
 
 
