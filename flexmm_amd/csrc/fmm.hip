@@ -1193,23 +1193,31 @@ __global__ __launch_bounds__(256) void k_locate_count(const double* __restrict__
     }
     e = evalid[g];
   }
+  // Group the wave's lanes by box with ballots first (no memory traffic),
+  // then let every group's lowest lane add the group size with one atomic
+  // instruction for all groups, so that targets spread over many boxes (in
+  // random input order) cost one atomic round trip instead of one per box.
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (1ull << lane) - 1ull;
   unsigned long long pending = __ballot(act);
-  int rank = 0;
+  int leader = lane, inrank = 0, gsize = 0;
   while (pending) {
-    const int leader = __builtin_ctzll(pending);
-    const int key = __shfl(e, leader, 64);
+    const int ld = __builtin_ctzll(pending);
+    const int key = __builtin_amdgcn_readlane(e, ld);
     const unsigned long long same = __ballot(act && e == key) & pending;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&tcnt[key], __popcll(same));
-    base = __shfl(base, leader, 64);
-    if (same >> lane & 1ull) rank = base + __popcll(same & below);
+    if (same >> lane & 1ull) {
+      leader = ld;
+      inrank = __popcll(same & below);
+    }
+    if (lane == ld) gsize = __popcll(same);
     pending &= ~same;
   }
+  int base = 0;
+  if (act && lane == leader) base = atomicAdd(&tcnt[e], gsize);
+  base = __shfl(base, leader, 64);
   if (act) {
     tbox[i] = e;
-    trank[i] = rank;
+    trank[i] = base + inrank;
   }
 }
 
