@@ -812,7 +812,9 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
   ple += (int64_t)blockIdx.y * nstride * PD;
   constexpr int KS = PD / 4, RB = PD / 16, LDAF = PD + 4;
   __shared__ __attribute__((aligned(16))) float buf[2][PD * LDAF];
-  __shared__ __attribute__((aligned(16))) double cmean[27][PD];   // row means of the used slots
+  // row means of the phase's slot, staged with its residual block (keeping
+  // all 27 resident would hold the block to 2 per CU by LDS)
+  __shared__ __attribute__((aligned(16))) double cmbuf[2][PD];
   __shared__ int srcs[27][kM2LCols];
   __shared__ int pslot[28];
   __shared__ unsigned long long smask;
@@ -821,10 +823,6 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
   const double* opsL = ops + (int64_t)lt.d[S.li] * opsz;
   const double* means = opsL + ops_off_means(PD);
   const float* r32 = reinterpret_cast<const float*>(opsL + ops_off_r32(PD));
-  for (int e = threadIdx.x; e < S.nph * PD; e += kM2LThreads) {
-    const int k = e / PD, r = e - k * PD;
-    cmean[k][r] = means[c_valid.slot[S.c][pslot[k]] * PD + r];
-  }
   double acc64[RB][4];
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb)
@@ -833,10 +831,11 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
   if (S.nph > 0) {
     AStageF<PD, kM2LThreads> st;
     float bn[KS], bc[KS];
-    double qn = 0.0, qc;
+    double qn = 0.0, qc, mn = 0.0;
     auto fetch = [&](int k) {
       const int i = pslot[k];
       st.load(r32 + (int64_t)c_valid.slot[S.c][i] * PD * PD);
+      if (threadIdx.x < PD) mn = means[c_valid.slot[S.c][i] * PD + threadIdx.x];
       const int src = srcs[i][S.col];
       const double* brow = src >= 0 ? mult + (int64_t)src * PD + 2 * S.kq : nullptr;
 #pragma unroll
@@ -854,8 +853,9 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
     for (int i = 0; i < S.nph; ++i) {
       float* A = (i & 1) ? buf[1] : buf[0];
       st.store(A);
+      if (threadIdx.x < PD) cmbuf[i & 1][threadIdx.x] = mn;
       if (i + 1 < S.nph) fetch(i + 1);
-      __syncthreads();   // also orders the cmean fill before its first use
+      __syncthreads();
       // one FP32 accumulation per slot (p terms), slots summed in FP64: a
       // single FP32 chain over all 27 slots loses about 4x more
       const float* a0 = A + (l & 15) * LDAF + 2 * S.kq;
@@ -872,7 +872,7 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
           acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, bc[2 * m + 1], acc[rb], 0, 0, 0);
         }
       // constant mode c_s * (sum of the source's equivalent density), FP64
-      const double* cm = cmean[i] + 4 * S.kq;
+      const double* cm = cmbuf[i & 1] + 4 * S.kq;
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
