@@ -1270,6 +1270,7 @@ __global__ __launch_bounds__(256) void k_make_items(const int* __restrict__ icnt
 // ceil(ni / 8) waves per XCD; surplus waves exit.  (A grid capped at 8
 // blocks per CU, waves striding over several items, measured 8-20% slower.)
 constexpr int kEvalWaves = 4;
+constexpr int kStageSources = 64;   // LDS staging of few-target items: sources per wave and chunk
 struct ItemRange {
   int first, last, step;
 };
@@ -1442,7 +1443,8 @@ __device__ __forceinline__ void eval_few(
     const int* __restrict__ coll, const int2* __restrict__ crange, const int* __restrict__ bot, const int* __restrict__ top,
     const double4* __restrict__ src, const double2* __restrict__ dip,
     const double* __restrict__ le, const double* __restrict__ ops, int64_t opsz, int p,
-    const double2* __restrict__ tab, double kp, int64_t nt, double* __restrict__ out) {
+    const double2* __restrict__ tab, double kp, int64_t nt, double* __restrict__ out,
+    double4* __restrict__ stage, int scap) {
   const FewLanes F = few_lanes(na);
   const int t = t0 + F.slot;
   Target<KER, NOUT, CHECK> T;
@@ -1468,7 +1470,42 @@ __device__ __forceinline__ void eval_few(
     T.gy = L.gy;
   }
   const CollLanes CL = coll_lanes(g, coll, crange);
-  for (int qn = 0; qn < 9; ++qn) {
+  if (!DIP && scap > 0) {
+    // The colleague sources, concatenated, pass through this wave's LDS area
+    // scap at a time: every lane loads its share of a chunk at once (one
+    // memory round trip per chunk instead of one per colleague), then the
+    // source groups read them from LDS.
+    int cb[9], cp[10];   // colleague starts and prefix offsets (wave-uniform)
+    cp[0] = 0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      cb[q] = __builtin_amdgcn_readlane(CL.b, q);
+      const int ce = __builtin_amdgcn_readlane(CL.e, q);
+      cp[q + 1] = cp[q] + (ce > cb[q] ? ce - cb[q] : 0);
+    }
+    const int lane = threadIdx.x & 63;
+    for (int base = 0; base < cp[9]; base += scap) {
+      const int n = min(scap, cp[9] - base);
+      for (int v = lane; v < n; v += 64) {
+        const int vv = base + v;
+        int j = vv + cb[0];
+#pragma unroll
+        for (int q = 1; q < 9; ++q) j = vv >= cp[q] ? vv - cp[q] + cb[q] : j;
+        stage[v] = src[j];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int v = F.sg; v < n; v += F.ng) {
+        const double4 sj = stage[v];
+        T.pair(sj.x, sj.y, sj.z, tab);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+  for (int qn = 0; qn < 9 && (DIP || scap == 0); ++qn) {
     const int b = __builtin_amdgcn_readlane(CL.b, qn);
     const int e = __builtin_amdgcn_readlane(CL.e, qn);
     if (e <= b) continue;
@@ -1523,8 +1560,11 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
     const int* __restrict__ top, const double4* __restrict__ src,
     const double2* __restrict__ dip,
     const double* __restrict__ le, const double* __restrict__ ops, int64_t opsz, int p,
-    const double2* __restrict__ logtab, double kp, int64_t nt, double* __restrict__ out) {
+    const double2* __restrict__ logtab, double kp, int64_t nt, double* __restrict__ out,
+    int scap) {
   __shared__ double2 tab[kLogTableSize];
+  extern __shared__ double4 stage_lds[];   // scap per wave when launched with it
+  double4* stage = stage_lds + (int)(threadIdx.x >> 6) * scap;
   const int ni = *nitems;
   load_logtab(tab, logtab);
   __syncthreads();
@@ -1535,7 +1575,7 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
     if (t1 - t0 <= kFewTargets) {
       eval_few<PD, KER, NOUT, CHECK, DIP, CHEB>(g, t0, t1 - t0, tperm, tsx, tsy, nlevel, xmid,
                                                 ymid, coll, crange, bot, top, src, dip, le, ops, opsz, p,
-                                                tab, kp, nt, out);
+                                                tab, kp, nt, out, stage, scap);
       continue;
     }
     const int t = t0 + (int)(threadIdx.x & 63);
@@ -2645,13 +2685,19 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
               : ev == EVAL_DIPOLE ? eval_kernel_dipole(PD, check)
                                   : eval_kernel(PD, P.kernel_id, ev == EVAL_POT ? 1 : 3, check);
   P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
-  hipLaunchKernelGGL(kern, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, P.items.as<int4>(), P.ntot.as<int>(),
+  // few targets per evaluation box on average: the few-target items stage
+  // their colleague sources through LDS (the extra LDS would only cost the
+  // full-wave items occupancy)
+  const int scap = (ev != EVAL_DIPOLE && ne > 0 && nt < (int64_t)kFewTargets * ne)
+                       ? kStageSources : 0;
+  const size_t dyn = (size_t)scap * kEvalWaves * sizeof(double4);
+  hipLaunchKernelGGL(kern, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), dyn, s, P.items.as<int4>(), P.ntot.as<int>(),
                      t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(), P.tsx.as<double>(),
                      P.tsy.as<double>(), t.nlevel.as<int>(), t.xmid.as<double>(),
                      t.ymid.as<double>(), t.coll.as<int>(), t.crange.as<int2>(), t.bot.as<int>(), t.top.as<int>(),
                      P.src.as<double4>(), ev == EVAL_DIPOLE ? P.dip.as<double2>() : nullptr,
                      P.le.as<double>(), P.ops.as<double>(), opsz, P.cheb ? P.cheb : P.p,
-                     P.logtab.as<double2>(), P.kparam, nt, out);
+                     P.logtab.as<double2>(), P.kparam, nt, out, scap);
   FMM_LAUNCH_CHECK();
   P.mark(FMM2D_PHASE_EVAL_KERNEL, true, s);
 }
