@@ -2647,7 +2647,11 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
     P.mark(FMM2D_PHASE_EVAL_KERNEL, true, s);
     return;
   }
-  if (P.precision == PREC_MIXED && ev == EVAL_POT && P.kernel_id == KER_LAPLACE) {
+  // few targets per evaluation box on average: the FP64 near field with LDS
+  // staging (below) is faster than the FP32 one, whose per-lane source loads
+  // are latency-bound on short items, and it is the more accurate of the two
+  const bool few_targets = ne > 0 && nt < (int64_t)kFewTargets * ne;
+  if (P.precision == PREC_MIXED && ev == EVAL_POT && P.kernel_id == KER_LAPLACE && !few_targets) {
     LevelConsts lc{};
     for (int d = 0; d < t.L; ++d) {
       lc.invw[d] = 1.0 / t.lev[d].width;
@@ -2688,8 +2692,7 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   // few targets per evaluation box on average: the few-target items stage
   // their colleague sources through LDS (the extra LDS would only cost the
   // full-wave items occupancy)
-  const int scap = (ev != EVAL_DIPOLE && ne > 0 && nt < (int64_t)kFewTargets * ne)
-                       ? kStageSources : 0;
+  const int scap = (ev != EVAL_DIPOLE && few_targets) ? kStageSources : 0;
   const size_t dyn = (size_t)scap * kEvalWaves * sizeof(double4);
   hipLaunchKernelGGL(kern, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), dyn, s, P.items.as<int4>(), P.ntot.as<int>(),
                      t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(), P.tsx.as<double>(),

@@ -63,7 +63,9 @@ extern "C" {
 #define FMM2D_PREC_FP64 0  /* every step in FP64 */
 #define FMM2D_PREC_MIXED 1 /* FP32 MFMA M2L (constant mode in FP64) and, for the Laplace
                               potential, FP32 near field for leaf colleagues at
-                              separations >= w/16; FP64 elsewhere */
+                              separations >= w/16; FP64 elsewhere.  An evaluation
+                              with fewer than 32 targets per leaf on average takes
+                              the FP64 near field (faster there, and more accurate) */
 #define FMM2D_PREC_MIXED_M2L 2 /* FP32 MFMA M2L only; near field in FP64 */
 
 /* tree arrays for fmm2d_tree_export */
