@@ -223,7 +223,7 @@ __global__ __launch_bounds__(SEG_BS) void k_seg_offsets(
   const int nc = (int)ceil_div(sg.y, CHUNK);
   const int q = (nc + 3) >> 2;
   const int cb = c0 + min(nc, part * q), ce = c0 + min(nc, (part + 1) * q);
-  constexpr int B = 8;
+  constexpr int B = 16;
   int sum = 0;
   int c = cb;
   for (; c + B <= ce; c += B) {
@@ -467,6 +467,14 @@ bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
   // fin[finBigOff - nbig, finBigOff - nbig_early)
   int nfin_early = nfinal, nbig_early = nbig;
   bool ran = false;
+  // The round's input (kin, vin) and scatter target (kout, vout): a round whose
+  // segments all continue leaves its output in place for the next round to
+  // read (ping-pong); otherwise its chunks are copied back into keys / vals,
+  // where every finished segment must end.
+  uint64_t* kin = keys;
+  uint32_t* vin = vals;
+  uint64_t* kout = w.keysB.as<uint64_t>();
+  uint32_t* vout = w.valsB.as<uint32_t>();
   for (int r = 0; r < rounds && nseg > 0; ++r) {
     nfin_early = nfinal;
     nbig_early = nbig;
@@ -481,19 +489,15 @@ bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
     hipLaunchKernelGGL(k_make_chunks, dim3(1), dim3(1024), 0, s, segCur, nseg,
                        w.chunks.as<int4>(), w.segChunk0.as<int>(), cnt);
     FMM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_chunk_hist, dim3(cbound), dim3(SORT_BS), 0, s, keys, w.chunks.as<int4>(),
+    hipLaunchKernelGGL(k_chunk_hist, dim3(cbound), dim3(SORT_BS), 0, s, kin, w.chunks.as<int4>(),
                        cnt, shift, w.chunkHist.as<int>());
     FMM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_seg_offsets, dim3(nseg), dim3(SEG_BS), 0, s, segCur,
                        w.segChunk0.as<int>(), w.chunkHist.as<int>(), T, r == rounds - 1 ? 1 : 0, segNext,
                        w.fin.as<int2>(), finBigOff, cnt);
     FMM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_chunk_scatter, dim3(cbound), dim3(SORT_BS), 0, s, keys, vals,
-                       w.chunks.as<int4>(), cnt, shift, w.chunkHist.as<int>(),
-                       w.keysB.as<uint64_t>(), w.valsB.as<uint32_t>());
-    FMM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_chunk_copy, dim3(cbound), dim3(SORT_BS), 0, s, w.keysB.as<uint64_t>(),
-                       w.valsB.as<uint32_t>(), w.chunks.as<int4>(), cnt, keys, vals);
+    hipLaunchKernelGGL(k_chunk_scatter, dim3(cbound), dim3(SORT_BS), 0, s, kin, vin,
+                       w.chunks.as<int4>(), cnt, shift, w.chunkHist.as<int>(), kout, vout);
     FMM_LAUNCH_CHECK();
     FMM_HIP(hipMemcpyAsync(hc, cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, s));
     FMM_HIP(hipStreamSynchronize(s));
@@ -502,6 +506,20 @@ bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
     nfinal = hc[C_FINAL];
     nbig = hc[C_FINAL_BIG];
     cur ^= 1;
+    const bool finished_some = nfinal + nbig > nfin_early + nbig_early;
+    const bool more = nseg > 0 && r + 1 < rounds;
+    if (kout != keys && (finished_some || !more)) {
+      hipLaunchKernelGGL(k_chunk_copy, dim3(cbound), dim3(SORT_BS), 0, s, kout, vout,
+                         w.chunks.as<int4>(), cnt, keys, vals);
+      FMM_LAUNCH_CHECK();
+      kin = keys;
+      vin = vals;
+      kout = w.keysB.as<uint64_t>();
+      vout = w.valsB.as<uint32_t>();
+    } else {
+      std::swap(kin, kout);
+      std::swap(vin, vout);
+    }
   }
   FMM_REQUIRE(nfinal + nbig <= segcap, FMM_E_INTERNAL, "sort: segment list overflow");
   // prefix_only: the segments of the last round are left in radix order
