@@ -1900,15 +1900,8 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
 // the root centre c0, since r_i r_j / |r|^2 f_j = (X_i(t) - X_i(s)) f_j d_j log|r|:
 //   u_i = (1/4 pi) [ -L[f_i] + X_i(t) B - C_i ],  p = B / (2 pi),
 //   B = d_x L[f_x] + d_y L[f_y],  C_i = d_x L[X_i f_x] + d_y L[X_i f_y].
-// The far field therefore needs six Laplace densities per source
-//   (f_x, f_y, X f_x, X f_y, Y f_x, Y f_y),
-// run through the scalar passes as one density dimension; the near field is
-// the Stokeslet itself.  The Laplace passes carry -log(d^2)/(4 pi), so
-// L[g] = -2 pi Phi[g] and grad L[g] = sum_k a_k (t - e_k) / |t - e_k|^2 for a
-// local density a on the points e_k.
-constexpr int kStokesDens = 6;
 // Double layer (stresslet, stokes_fmm.py:64-101,187-236): for a source with
-// strength sigma and normal n, and r = t - s,
+// strength sigma and normal n,
 //   u = (1/pi) (r.n)(r.sigma) r / |r|^4,
 //   p = (1/pi) [-(n.sigma) / |r|^2 + 2 (r.n)(r.sigma) / |r|^4].
 // With g = n.sigma, a = n_x s_x - n_y s_y, b = n_x s_y + n_y s_x and the
@@ -1917,11 +1910,17 @@ constexpr int kStokesDens = 6;
 // - (X_i(t) - X_i(s)) d_j d_k log|r|):
 //   u_i = (1/2 pi) [ d_i L[g] - X_i(t) Q[a, b] + Q[X_i a, X_i b] ],
 //   p   = -(1/pi) Q[a, b].
-// The far field therefore needs four more densities (g as a single layer,
-// and the three quadrupole fields, whose check values are the fields
-// themselves); the near field evaluates the stresslet directly, with
+// Both layers together therefore need only three fields, each harmonic away
+// from the sources (a sum of log, d log and d d log terms):
+//   D_i = -L[f_i] - C_i + 2 (d_i L[g] + Q[X_i a, X_i b])   (i = x, y),
+//   D_2 = B - 2 Q[a, b],
+//   u_i = (1/4 pi) [ D_i + X_i(t) D_2 ],   p = D_2 / (2 pi).
+// They run through the scalar passes as a density dimension of three, with
+// the fields themselves as check values (S2M) and read back by L2T as the
+// potentials of their local equivalent densities; the near field evaluates
+// the Stokeslet and the stresslet directly, with
 // (r.n)(r.sigma) = (1/2) [g |r|^2 + a (r_x^2 - r_y^2) + 2 b r_x r_y].
-constexpr int kStokesDipDens = 10;
+constexpr int kStokesFields = 3;
 
 // Sources in tree order as (x, y, f_x, f_y); tau holds f_x[n] then f_y[n].
 __global__ void k_pack_stokes(const double* __restrict__ tau, const uint32_t* __restrict__ perm,
@@ -1951,10 +1950,10 @@ __global__ void k_pack_stokes_dipole(const double* __restrict__ tau,
   sdip[i] = make_double4(nx * gx + ny * gy, nx * gx - ny * gy, nx * gy + ny * gx, 0.0);
 }
 
-// S2M of the six densities: one log per (source, check point) serves all six.
-// DIP: also the double layer's four (g as a single layer; Q[a, b], Q[X a, X b]
-// and Q[Y a, Y b] as fields, with d_xx log|r| = (r_y^2 - r_x^2)/|r|^4 and
-// d_xy log|r| = -2 r_x r_y/|r|^4 at the check point).
+// S2M of the three fields D_x, D_y, D_2 at the check points (r = check - s):
+//   D_i += -f_i log|r| - X_i(s) (f.r)/|r|^2,  D_2 += (f.r)/|r|^2,
+// DIP: D_i += 2 (g r_i/|r|^2 + X_i(s) q),  D_2 -= 2 q,
+//   q = a d_xx log|r| + b d_xy log|r| = (a (r_y^2 - r_x^2) - 2 b r_x r_y)/|r|^4.
 template <int PD, bool DIP>
 __global__ __launch_bounds__(64 * kS2MWaves) void k_s2m_stokes(int nup,
                                                                const int* __restrict__ upnodes,
@@ -1986,43 +1985,42 @@ __global__ __launch_bounds__(64 * kS2MWaves) void k_s2m_stokes(int nup,
   const double ay = (k < p ? ly[k] : 1.0) + ymid[g];
   const int b = __builtin_amdgcn_readfirstlane(bot[g]);
   const int t = __builtin_amdgcn_readfirstlane(top[g]);
-  double acc[kStokesDens] = {};
-  double ag = 0.0, q1 = 0.0, qx = 0.0, qy = 0.0;
+  LogAcc lfx, lfy;                 // sum f_i log|r|^2
+  double ex = 0.0, ey = 0.0, e2 = 0.0;
   for (int j = b; j < t; ++j) {
     const double4 sj = src[j];
     const double dx = ax - sj.x;
     const double dy = ay - sj.y;
     const double d2 = __builtin_fma(dy, dy, dx * dx);
-    const double lg = log_tab(d2, tab);
+    double le_, lt_;
+    log_split(d2, tab, le_, lt_);
+    lfx.ae = __builtin_fma(sj.z, le_, lfx.ae);
+    lfx.at = __builtin_fma(sj.z, lt_, lfx.at);
+    lfy.ae = __builtin_fma(sj.w, le_, lfy.ae);
+    lfy.at = __builtin_fma(sj.w, lt_, lfy.at);
+    const double id2 = rcp_nr(d2);
     const double X = sj.x - c0x, Y = sj.y - c0y;
-    const double w[kStokesDens] = {sj.z, sj.w, X * sj.z, X * sj.w, Y * sj.z, Y * sj.w};
-#pragma unroll
-    for (int q = 0; q < kStokesDens; ++q) acc[q] = __builtin_fma(w[q], lg, acc[q]);
+    const double fr = __builtin_fma(sj.z, dx, sj.w * dy) * id2;
+    ex = __builtin_fma(-X, fr, ex);
+    ey = __builtin_fma(-Y, fr, ey);
+    e2 += fr;
     if constexpr (DIP) {
       const double4 e = sdip[j];
-      const double id2 = rcp_nr(d2);
-      const double id4 = id2 * id2;
-      const double qv = __builtin_fma(e.y, (dy - dx) * (dy + dx), -2.0 * e.z * dx * dy) * id4;
-      ag = __builtin_fma(e.x, lg, ag);
-      q1 += qv;
-      qx = __builtin_fma(X, qv, qx);
-      qy = __builtin_fma(Y, qv, qy);
+      const double qv = __builtin_fma(e.y, (dy - dx) * (dy + dx), -2.0 * e.z * dx * dy) * id2 * id2;
+      ex = __builtin_fma(2.0, __builtin_fma(e.x * dx, id2, X * qv), ex);
+      ey = __builtin_fma(2.0, __builtin_fma(e.x * dy, id2, Y * qv), ey);
+      e2 = __builtin_fma(-2.0, qv, e2);
     }
   }
   if (k < PD) {
-#pragma unroll
-    for (int q = 0; q < kStokesDens; ++q)
-      check[(q * rstride + r) * PD + k] = k < p ? -kInv4Pi * acc[q] : 0.0;
-    if constexpr (DIP) {
-      check[(6 * rstride + r) * PD + k] = k < p ? -kInv4Pi * ag : 0.0;
-      check[(7 * rstride + r) * PD + k] = k < p ? q1 : 0.0;
-      check[(8 * rstride + r) * PD + k] = k < p ? qx : 0.0;
-      check[(9 * rstride + r) * PD + k] = k < p ? qy : 0.0;
-    }
+    // -f_i log|r| = -(1/2) f_i log|r|^2
+    check[r * PD + k] = k < p ? __builtin_fma(-0.5, lfx.value(), ex) : 0.0;
+    check[(rstride + r) * PD + k] = k < p ? __builtin_fma(-0.5, lfy.value(), ey) : 0.0;
+    check[(2 * rstride + r) * PD + k] = k < p ? e2 : 0.0;
   }
 }
 
-// One item of k_eval_stokes: L2T of the six local densities + Stokeslet P2P,
+// One item of k_eval_stokes: L2T of the three local fields + Stokeslet P2P,
 // outputs (u, v, p).  FEW selects the (target slot, source group) lane layout
 // of eval_few; otherwise lane = target and the source loop index is uniform.
 template <int PD, bool CHECK, bool FEW, bool DIP>
@@ -2056,50 +2054,28 @@ __device__ __forceinline__ void stokes_item(
     const double* ly = lx + PD;
     const double* a = le + (int64_t)g * PD;
     const double xr = x - xmid[g], yr = y - ymid[g];
-    LogAcc l0, l1, lq1, lqx, lqy;
-    double B = 0.0, Cx = 0.0, Cy = 0.0, Gx = 0.0, Gy = 0.0;
+    // the three fields are the potentials -log(d^2)/(4 pi) of their local
+    // equivalent densities: one log per equivalent point serves all three
+    LogAcc l0, l1, l2;
     for (int k = F.sg; k < p; k += F.ng) {
       const double dx = xr - lx[k], dy = yr - ly[k];
       const double d2 = __builtin_fma(dy, dy, dx * dx);
-      const double id2 = rcp_nr(d2);
-      const double a0 = a[k], a1 = a[nstride * PD + k], a2 = a[2 * nstride * PD + k],
-                   a3 = a[3 * nstride * PD + k], a4 = a[4 * nstride * PD + k],
-                   a5 = a[5 * nstride * PD + k];
+      const double a0 = a[k], a1 = a[nstride * PD + k], a2 = a[2 * nstride * PD + k];
       double e, lt;
       log_split(d2, tab, e, lt);
       l0.ae = __builtin_fma(a0, e, l0.ae);
       l0.at = __builtin_fma(a0, lt, l0.at);
       l1.ae = __builtin_fma(a1, e, l1.ae);
       l1.at = __builtin_fma(a1, lt, l1.at);
-      B = __builtin_fma(__builtin_fma(a0, dx, a1 * dy), id2, B);
-      Cx = __builtin_fma(__builtin_fma(a2, dx, a3 * dy), id2, Cx);
-      Cy = __builtin_fma(__builtin_fma(a4, dx, a5 * dy), id2, Cy);
-      if constexpr (DIP) {
-        const double a6 = a[6 * nstride * PD + k], a7 = a[7 * nstride * PD + k],
-                     a8 = a[8 * nstride * PD + k], a9 = a[9 * nstride * PD + k];
-        Gx = __builtin_fma(a6 * dx, id2, Gx);
-        Gy = __builtin_fma(a6 * dy, id2, Gy);
-        lq1.ae = __builtin_fma(a7, e, lq1.ae);
-        lq1.at = __builtin_fma(a7, lt, lq1.at);
-        lqx.ae = __builtin_fma(a8, e, lqx.ae);
-        lqx.at = __builtin_fma(a8, lt, lqx.at);
-        lqy.ae = __builtin_fma(a9, e, lqy.ae);
-        lqy.at = __builtin_fma(a9, lt, lqy.at);
-      }
+      l2.ae = __builtin_fma(a2, e, l2.ae);
+      l2.at = __builtin_fma(a2, lt, l2.at);
     }
-    // -L[f] = -sum a (1/2) log d^2
+    const double Dx = -kInv4Pi * l0.value(), Dy = -kInv4Pi * l1.value(),
+                 D2 = -kInv4Pi * l2.value();
     const double X = x - c0x, Y = y - c0y;
-    u = kInv4Pi * (-0.5 * l0.value() + X * B - Cx);
-    v = kInv4Pi * (-0.5 * l1.value() + Y * B - Cy);
-    pr = kInv2Pi * B;
-    if constexpr (DIP) {
-      // fields of the quadrupole densities (single-layer representation)
-      const double Q1 = -kInv4Pi * lq1.value(), QX = -kInv4Pi * lqx.value(),
-                   QY = -kInv4Pi * lqy.value();
-      u += kInv2Pi * (Gx - X * Q1 + QX);
-      v += kInv2Pi * (Gy - Y * Q1 + QY);
-      pr -= 2.0 * kInv2Pi * Q1;
-    }
+    u = kInv4Pi * __builtin_fma(X, D2, Dx);
+    v = kInv4Pi * __builtin_fma(Y, D2, Dy);
+    pr = kInv2Pi * D2;
   }
   // near field: the Stokeslet itself (stokes_fmm.py:102-184)
   LogAcc lx_, ly_;
@@ -2187,7 +2163,7 @@ __device__ __forceinline__ void stokes_item(
   }
 }
 
-// L2T of the six local densities + Stokeslet P2P, outputs (u, v, p).
+// L2T of the three local fields + Stokeslet (and stresslet) P2P, outputs (u, v, p).
 // The Stokeslet-only instance is held to 96 VGPRs (5 waves per SIMD instead of
 // 4, no spills; 2% faster on 2M clustered points); with stresslets the same
 // bound would spill, so that instance is left unconstrained.
@@ -2434,7 +2410,7 @@ static void upward(Plan& P, const double* tau, const double* dvec, hipStream_t s
   const int PD = P.PD;
   const int64_t opsz = ops_per_level(PD);
   const bool stokes = P.kernel_id == KER_STOKES;
-  const int nd = stokes ? (dvec ? kStokesDipDens : kStokesDens) : 1;
+  const int nd = stokes ? kStokesFields : 1;
   P.nd = nd;
   P.tau_s.ensure(t.n * sizeof(double));
   P.src.ensure(t.n * sizeof(double4));

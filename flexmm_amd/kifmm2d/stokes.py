@@ -5,10 +5,12 @@ Ncutoff, bbox, precomputations, verbose)``, ``precompute()``,
 returning the velocity and pressure ``(u, v, p)``.
 
 The reference runs a vector KIFMM with (2p+1)-long bordered densities
-(stokes_fmm.py:473-596).  Here the far field is written with six Laplace
-densities per source (forces, and forces times coordinates), run through the
-scalar MFMA passes on one quadtree, and the near field evaluates the
-Stokeslet directly (the identity is spelled out in flexmm_amd/csrc/fmm.hip).
+(stokes_fmm.py:473-596).  Here the far field is written as three harmonic
+fields (sums of log, d log and d d log terms of the forces, and of the
+double-layer densities when present), run through the scalar MFMA passes on
+one quadtree as Laplace equivalent densities, and the near field evaluates
+the Stokeslet and stresslet directly (the identity is spelled out in
+flexmm_amd/csrc/fmm.hip).
 Both approximate the same direct sum; this form reuses the Laplace
 operators and reaches the Laplace FMM's accuracy.
 """

@@ -38,8 +38,9 @@ extern "C" {
                                              (examples/pure_kifmm_modified_helmholtz_gradient.py:36-68) */
 #define FMM2D_KERNEL_STOKESLET 2 /* 2D Stokeslet: velocity (-log r I + r r^T / r^2) f / (4 pi) and
                                     pressure (r . f) / (2 pi r^2) (kifmm2d/vector/stokes_fmm.py:33-61);
-                                    computed with six Laplace densities per source, so the plan
-                                    takes the Laplace operators */
+                                    its far field is three harmonic fields carried by Laplace
+                                    equivalent densities, so the plan takes the Laplace
+                                    operators */
 
 /* evaluators (register_evaluator, helpers.py:176-238) */
 #define FMM2D_EVAL_POTENTIAL 0 /* 1 output: u */

@@ -3,6 +3,7 @@ finite differences and explicit loops."""
 import math
 
 import numpy as np
+import pytest
 from scipy import special
 
 from oracle import direct as odirect
@@ -99,8 +100,10 @@ def test_stokeslet_solves_stokes_equations():
 
 
 def test_stokeslet_laplace_decomposition():
-    """The six-Laplace-density form of the Stokeslet that the device far field
-    uses reproduces the direct Stokeslet sum."""
+    """The Stokeslet as Laplace potentials and gradients of six densities
+    (f_x, f_y and their products with the source coordinates about a fixed
+    centre) reproduces the direct Stokeslet sum; the three harmonic fields of
+    the next tests regroup these terms."""
     from oracle import direct as odirect
     rng = np.random.default_rng(12)
     sx, sy = rng.random(400), rng.random(400)
@@ -275,3 +278,63 @@ def test_direct_src_matches_python_loops():
             d2 = dx * dx + dy * dy
             acc += -np.log(d2) / (4 * np.pi) * tau[0, j] + (ex[0, j] * dx + ex[1, j] * dy) / d2 / (2 * np.pi) * tau[1, j]
         assert abs(got[i] - acc) < 1e-13 * max(1.0, abs(acc))
+
+
+def _three_fields(sx, sy, fx, fy, gx, gy, nx, ny, tx, ty, c0):
+    """The three fields that the device far field carries, per source as the
+    S2M writes them at a check point t (r = t - s, X = s - c0):
+    D_i = sum -f_i log|r| - X_i (f.r)/|r|^2 + 2 (g r_i/|r|^2 + X_i q),
+    D_2 = sum (f.r)/|r|^2 - 2 q, with g = n.sigma, a = n_x sigma_x - n_y sigma_y,
+    b = n_x sigma_y + n_y sigma_x and q = (a (r_y^2 - r_x^2) - 2 b r_x r_y)/|r|^4."""
+    dx = tx[:, None] - sx[None, :]
+    dy = ty[:, None] - sy[None, :]
+    d2 = dx * dx + dy * dy
+    X, Y = sx - c0[0], sy - c0[1]
+    fr = (fx * dx + fy * dy) / d2
+    g = nx * gx + ny * gy
+    a = nx * gx - ny * gy
+    b = nx * gy + ny * gx
+    q = (a * (dy * dy - dx * dx) - 2 * b * dx * dy) / d2 ** 2
+    lg = 0.5 * np.log(d2)
+    Dx = (-fx * lg - X * fr + 2 * (g * dx / d2 + X * q)).sum(1)
+    Dy = (-fy * lg - Y * fr + 2 * (g * dy / d2 + Y * q)).sum(1)
+    D2 = (fr - 2 * q).sum(1)
+    return Dx, Dy, D2
+
+
+@pytest.mark.parametrize("stresslets", [False, True])
+def test_stokes_three_harmonic_fields(stresslets):
+    """u_i = (D_i + X_i(t) D_2)/(4 pi) and p = D_2/(2 pi) reproduce the direct
+    Stokeslet (+ stresslet) sums, and each field is harmonic away from the
+    sources (5-point Laplacian), which is what lets single-layer equivalent
+    densities represent it.  Planted mistake: the X_i(t) D_2 term with the
+    coordinate about another centre fails."""
+    from oracle import direct as odirect
+    rng = np.random.default_rng(31 + stresslets)
+    m = 300
+    sx, sy = rng.random(m), rng.random(m)
+    fx, fy = rng.standard_normal(m), rng.standard_normal(m)
+    gx, gy = (rng.standard_normal(m), rng.standard_normal(m)) if stresslets else (np.zeros(m),) * 2
+    th = rng.random(m) * 2 * np.pi
+    nx, ny = np.cos(th), np.sin(th)
+    tx, ty = rng.random(40) * 3 - 1, rng.random(40) + 1.2
+    c0 = (0.45, 0.55)
+    Dx, Dy, D2 = _three_fields(sx, sy, fx, fy, gx, gy, nx, ny, tx, ty, c0)
+    u = (Dx + (tx - c0[0]) * D2) / (4 * np.pi)
+    v = (Dy + (ty - c0[1]) * D2) / (4 * np.pi)
+    p = D2 / (2 * np.pi)
+    ref = odirect.direct_stokes_dipole(sx, sy, fx, fy, gx, gy, nx, ny, tx, ty)
+    for got, r in zip((u, v, p), ref):
+        assert np.abs(got - r).max() < 1e-13 * np.abs(r).max()
+    bad = (Dx + (tx - 0.5) * D2) / (4 * np.pi)
+    assert np.abs(bad - ref[0]).max() > 1e-3 * np.abs(ref[0]).max()
+    # the 5-point Laplacian of each field is only its O(h^2) truncation, a
+    # thousandth of the Laplacian 2 d_x D_2 of the non-harmonic X(t) D_2
+    h = 5e-4
+    c = _three_fields(sx, sy, fx, fy, gx, gy, nx, ny, tx, ty, c0)
+    nb = [_three_fields(sx, sy, fx, fy, gx, gy, nx, ny, tx + ex, ty + ey, c0)
+          for ex, ey in [(h, 0), (-h, 0), (0, h), (0, -h)]]
+    lap_xd2 = 2 * np.abs(nb[0][2] - nb[1][2]).max() / (2 * h)
+    for k in range(3):
+        lap = (sum(f[k] for f in nb) - 4 * c[k]) / h ** 2
+        assert np.abs(lap).max() < 1e-3 * lap_xd2
