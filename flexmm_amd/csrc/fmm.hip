@@ -2056,29 +2056,26 @@ __device__ __forceinline__ void stokes_item(
     const double xr = x - xmid[g], yr = y - ymid[g];
     // the three fields are the potentials -log(d^2)/(4 pi) of their local
     // equivalent densities: one log per equivalent point serves all three
-    LogAcc l0, l1, l2;
+    double l0 = 0.0, l1 = 0.0, l2 = 0.0;
     for (int k = F.sg; k < p; k += F.ng) {
       const double dx = xr - lx[k], dy = yr - ly[k];
       const double d2 = __builtin_fma(dy, dy, dx * dx);
       const double a0 = a[k], a1 = a[nstride * PD + k], a2 = a[2 * nstride * PD + k];
       double e, lt;
       log_split(d2, tab, e, lt);
-      l0.ae = __builtin_fma(a0, e, l0.ae);
-      l0.at = __builtin_fma(a0, lt, l0.at);
-      l1.ae = __builtin_fma(a1, e, l1.ae);
-      l1.at = __builtin_fma(a1, lt, l1.at);
-      l2.ae = __builtin_fma(a2, e, l2.ae);
-      l2.at = __builtin_fma(a2, lt, l2.at);
+      const double lg = __builtin_fma(e, kLn2, lt);
+      l0 = __builtin_fma(a0, lg, l0);
+      l1 = __builtin_fma(a1, lg, l1);
+      l2 = __builtin_fma(a2, lg, l2);
     }
-    const double Dx = -kInv4Pi * l0.value(), Dy = -kInv4Pi * l1.value(),
-                 D2 = -kInv4Pi * l2.value();
+    const double Dx = -kInv4Pi * l0, Dy = -kInv4Pi * l1, D2 = -kInv4Pi * l2;
     const double X = x - c0x, Y = y - c0y;
     u = kInv4Pi * __builtin_fma(X, D2, Dx);
     v = kInv4Pi * __builtin_fma(Y, D2, Dy);
     pr = kInv2Pi * D2;
   }
   // near field: the Stokeslet itself (stokes_fmm.py:102-184)
-  LogAcc lx_, ly_;
+  double lfx = 0.0, lfy = 0.0;   // sum f log(d^2)
   double su = 0.0, sv = 0.0, sp = 0.0;
   double du = 0.0, dv = 0.0, dp = 0.0;   // double layer
   double dmin = 1.0;
@@ -2087,37 +2084,31 @@ __device__ __forceinline__ void stokes_item(
     const int b = __builtin_amdgcn_readlane(CL.b, qn);
     const int e = __builtin_amdgcn_readlane(CL.e, qn);
     if (e <= b) continue;
+    // A coincident pair (d2 = 0, so dx = dy = 0) continues with d2 = 1: every
+    // term with a factor dx or dy then vanishes exactly, the stresslet
+    // pressure -g/d2 + g/d2 cancels, and only the log term is zeroed (under
+    // CHECK; otherwise the target is flagged through dmin).  Forces and
+    // stresslet densities stay in the wave-uniform registers they load into.
     auto pairS = [&](const double4& sj, int jj) {
       const double dx = x - sj.x, dy = y - sj.y;
       double d2 = __builtin_fma(dy, dy, dx * dx);
       const bool z = d2 == 0.0;
-      double fx = sj.z, fy = sj.w;
-      if (CHECK) {
-        fx = z ? 0.0 : fx;
-        fy = z ? 0.0 : fy;
-      } else {
-        dmin = __builtin_fmin(dmin, d2);
-      }
+      if (!CHECK) dmin = __builtin_fmin(dmin, d2);
       d2 = z ? 1.0 : d2;
       const double id2 = rcp_nr(d2);
       double le_, lt_;
       log_split(d2, tab, le_, lt_);
-      lx_.ae = __builtin_fma(fx, le_, lx_.ae);
-      lx_.at = __builtin_fma(fx, lt_, lx_.at);
-      ly_.ae = __builtin_fma(fy, le_, ly_.ae);
-      ly_.at = __builtin_fma(fy, lt_, ly_.at);
-      const double rf = __builtin_fma(dx, fx, dy * fy) * id2;
+      double lg = __builtin_fma(le_, kLn2, lt_);
+      if (CHECK) lg = z ? 0.0 : lg;
+      lfx = __builtin_fma(sj.z, lg, lfx);
+      lfy = __builtin_fma(sj.w, lg, lfy);
+      const double rf = __builtin_fma(dx, sj.z, dy * sj.w) * id2;
       su = __builtin_fma(dx, rf, su);
       sv = __builtin_fma(dy, rf, sv);
       sp += rf;
       if constexpr (DIP) {   // the stresslet itself
         const double4 e4 = sdip[jj];
-        double gg = e4.x, aa = e4.y, bb = e4.z;
-        if (CHECK) {
-          gg = z ? 0.0 : gg;
-          aa = z ? 0.0 : aa;
-          bb = z ? 0.0 : bb;
-        }
+        const double gg = e4.x, aa = e4.y, bb = e4.z;
         const double nrns =
             0.5 * __builtin_fma(gg, d2, __builtin_fma(aa, (dx - dy) * (dx + dy), 2.0 * bb * dx * dy));
         const double w = nrns * id2 * id2;
@@ -2138,8 +2129,8 @@ __device__ __forceinline__ void stokes_item(
     }
     for (; j < e; j += F.ng) pairS(src[j], j);
   }
-  u += kInv4Pi * (-0.5 * lx_.value() + su);
-  v += kInv4Pi * (-0.5 * ly_.value() + sv);
+  u += kInv4Pi * (-0.5 * lfx + su);
+  v += kInv4Pi * (-0.5 * lfy + sv);
   pr += kInv2Pi * sp;
   if constexpr (DIP) {
     u += 2.0 * kInv2Pi * du;
@@ -2966,9 +2957,11 @@ int fmm2d_evaluate(void* plan, const double* tx, const double* ty, int64_t nt, i
                    int check, double* out, void* stream) {
   ABI_BEGIN
   Plan* P = as_plan(plan);
-  NEED(tx);
-  NEED(ty);
-  NEED(out);
+  if (nt != 0) {   // an empty target set may come with null pointers
+    NEED(tx);
+    NEED(ty);
+    NEED(out);
+  }
   P->mark(FMM2D_PHASE_EVALUATE, false, (hipStream_t)stream);
   fmm::evaluate(*P, tx, ty, nt, evaluator, check, out, (hipStream_t)stream);
   P->mark(FMM2D_PHASE_EVALUATE, true, (hipStream_t)stream);

@@ -182,7 +182,8 @@ int fmm2d_upward_dipole(void* plan, const double* tau, const double* dvec, int64
 int fmm2d_downward(void* plan, void* stream);
 
 /* Evaluate at nt targets into out[outputs][nt] (input target order).
- * check != 0 skips sources that coincide exactly with the target.
+ * check != 0 skips sources that coincide exactly with the target.  nt = 0
+ * does nothing (tx, ty and out may then be null).
  * Replaces FMM.evaluate_to_points (fmm.py:182-199). */
 int fmm2d_evaluate(void* plan, const double* tx, const double* ty, int64_t nt, int evaluator,
                    int check, double* out, void* stream);

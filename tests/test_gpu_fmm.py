@@ -78,6 +78,24 @@ def test_gradient_evaluator():
         assert H.rel_max_err(g[k], ref[k]) < 1e-11
 
 
+def test_empty_target_set_and_single_source():
+    """No targets give empty outputs (torch and numpy inputs); one source is a
+    root leaf evaluated directly."""
+    import torch
+    x, y, tau = H.uniform(2000, 21)
+    f = _fmm(x, y, 30, 32)
+    f.build_expansions(tau)
+    e = np.zeros(0)
+    assert f.target_evaluation(e, e)[0].shape == (0,)
+    et = torch.zeros(0, dtype=torch.float64, device="cuda")
+    assert f.target_evaluation(et, et)[0].numel() == 0
+    g = _fmm(x[:1], y[:1], 30, 32, bbox=[0, 1, 0, 1])
+    g.build_expansions(tau[:1])
+    u = g.target_evaluation(x[1:9], y[1:9])[0]
+    ref = odirect.direct(okern.laplace, x[:1], y[:1], tau[:1], x[1:9], y[1:9])[0]
+    assert H.rel_max_err(u, ref) < 1e-14
+
+
 @pytest.mark.parametrize("n,ncut", [(50, 64), (100, 30), (300, 40)])
 def test_shallow_trees_are_direct(n, ncut):
     x, y, tau = H.uniform(n, n)
