@@ -198,12 +198,74 @@ struct PotAcc<KER_LAPLACE> {
   }
   __device__ __forceinline__ double result() const { return -kInv4Pi * a.value(); }
 };
+
+// K0(x) and k K1(x)/r at x = k r from d2 = r^2.  On the ascending-series
+// branch (x <= 2, i.e. q = k^2 d2 / 4 <= 1) neither a square root nor the
+// library log is needed: q is (k^2/4) d2, ln(x/2) = ln(d2)/2 + ln(k/2) with
+// ln(d2) from the table log, and
+//   k K1(x)/r = 1/d2 + (k^2/4) (2 ln(x/2) I1' - S1'),
+// I1' = sum q^j/(j!(j+1)!), S1' the psi sum of bessel_k01 (x I1' / 2 = I1).
+// The x > 2 branch is bessel_k0 / bessel_k01 at x = k sqrt(d2).
+struct HelmConsts {
+  double k, k2q, lnk2;   // k, k^2/4, ln(k/2)
+};
+__device__ __forceinline__ HelmConsts helm_consts(double k) {
+  return HelmConsts{k, 0.25 * k * k, log(0.5 * k)};
+}
+__device__ __forceinline__ double helm_lnx2(double d2, const double2* __restrict__ tab,
+                                            const HelmConsts& h) {
+  double e, t;
+  log_split(d2, tab, e, t);
+  return __builtin_fma(0.5, __builtin_fma(e, kLn2, t), h.lnk2);
+}
+__device__ __forceinline__ double helm_k0(double d2, const double2* __restrict__ tab,
+                                          const HelmConsts& h) {
+  const double q = h.k2q * d2;
+  if (q <= 1.0) {
+    const BesselSeries& c = bessel_series();
+    constexpr int n = kBesselSeriesTerms - 1;
+    double i0 = c.i0[n], s = c.s0[n];
+#pragma unroll
+    for (int j = n - 1; j >= 0; --j) {
+      i0 = __builtin_fma(i0, q, c.i0[j]);
+      s = __builtin_fma(s, q, c.s0[j]);
+    }
+    return __builtin_fma(-(helm_lnx2(d2, tab, h) + kEulerGamma), i0, s);
+  }
+  return bessel_k0(h.k * __builtin_sqrt(d2));
+}
+// K0 and k K1(x)/r; id2 = 1/d2
+__device__ __forceinline__ void helm_k01(double d2, double id2, const double2* __restrict__ tab,
+                                         const HelmConsts& h, double& k0, double& gk1) {
+  const double q = h.k2q * d2;
+  if (q <= 1.0) {
+    const BesselSeries& c = kBesselSeries;
+    constexpr int n = kBesselSeriesTerms - 1;
+    double i0 = c.i0[n], s0 = c.s0[n], i1 = c.i1[n], s1 = c.s1[n];
+#pragma unroll
+    for (int j = n - 1; j >= 0; --j) {
+      i0 = __builtin_fma(i0, q, c.i0[j]);
+      s0 = __builtin_fma(s0, q, c.s0[j]);
+      i1 = __builtin_fma(i1, q, c.i1[j]);
+      s1 = __builtin_fma(s1, q, c.s1[j]);
+    }
+    const double lg = helm_lnx2(d2, tab, h);
+    k0 = __builtin_fma(-(lg + kEulerGamma), i0, s0);
+    gk1 = __builtin_fma(h.k2q, __builtin_fma(2.0 * lg, i1, -s1), id2);
+    return;
+  }
+  const double r = __builtin_sqrt(d2);
+  double k1;
+  bessel_k01(h.k * r, k0, k1);
+  gk1 = h.k * k1 / r;
+}
+
 template <>
 struct PotAcc<KER_MODHELM> {
   double s = 0.0;
-  __device__ __forceinline__ void add(double w, double d2, const double2* __restrict__,
+  __device__ __forceinline__ void add(double w, double d2, const double2* __restrict__ tab,
                                       double k) {
-    s = __builtin_fma(w, bessel_k0(k * __builtin_sqrt(d2)), s);
+    s = __builtin_fma(w, helm_k0(d2, tab, helm_consts(k)), s);
   }
   __device__ __forceinline__ double result() const { return kInv2Pi * s; }
 };
@@ -1330,14 +1392,13 @@ struct Target {
       }
     } else {
       d2 = z ? 1.0 : d2;    // weight is 0 (or the result +inf) there
-      const double r = __builtin_sqrt(d2);
       if constexpr (NOUT == 1) {
         u.add(w, d2, tab, kp);
       } else {
-        double k0, k1;
-        bessel_k01(kp * r, k0, k1);
+        double k0, gk1;
+        helm_k01(d2, rcp_nr(d2), tab, helm_consts(kp), k0, gk1);
         u.s = __builtin_fma(w, k0, u.s);
-        const double qi = z ? 0.0 : -kp * k1 / r * w;
+        const double qi = z ? 0.0 : -gk1 * w;
         gx = __builtin_fma(qi, dx, gx);
         gy = __builtin_fma(qi, dy, gy);
       }
