@@ -184,21 +184,6 @@ struct LogAcc {
   __device__ __forceinline__ double value() const { return __builtin_fma(ae, kLn2, at); }
 };
 
-// Potential accumulators of the supported kernels: sum_j w_j K(d_j^2).
-//   Laplace           K = -log(d^2)/(4 pi)   (examples/pure_kifmm_laplace.py:24-29)
-//   modified Helmholtz K = K0(k d)/(2 pi)     (pure_kifmm_modified_helmholtz_gradient.py:36-42)
-template <int KER>
-struct PotAcc;
-template <>
-struct PotAcc<KER_LAPLACE> {
-  LogAcc a;
-  __device__ __forceinline__ void add(double w, double d2, const double2* __restrict__ tab,
-                                      double) {
-    a.add(w, d2, tab);
-  }
-  __device__ __forceinline__ double result() const { return -kInv4Pi * a.value(); }
-};
-
 // K0(x) and k K1(x)/r at x = k r from d2 = r^2.  On the ascending-series
 // branch (x <= 2, i.e. q = k^2 d2 / 4 <= 1) neither a square root nor the
 // library log is needed: q is (k^2/4) d2, ln(x/2) = ln(d2)/2 + ln(k/2) with
@@ -260,12 +245,28 @@ __device__ __forceinline__ void helm_k01(double d2, double id2, const double2* _
   gk1 = h.k * k1 / r;
 }
 
+
+// Potential accumulators of the supported kernels: sum_j w_j K(d_j^2).
+//   Laplace           K = -log(d^2)/(4 pi)   (examples/pure_kifmm_laplace.py:24-29)
+//   modified Helmholtz K = K0(k d)/(2 pi)     (pure_kifmm_modified_helmholtz_gradient.py:36-42)
+template <int KER>
+struct PotAcc;
+template <>
+struct PotAcc<KER_LAPLACE> {
+  LogAcc a;
+  __device__ __forceinline__ void add(double w, double d2, const double2* __restrict__ tab,
+                                      const HelmConsts&) {
+    a.add(w, d2, tab);
+  }
+  __device__ __forceinline__ double result() const { return -kInv4Pi * a.value(); }
+};
+
 template <>
 struct PotAcc<KER_MODHELM> {
   double s = 0.0;
   __device__ __forceinline__ void add(double w, double d2, const double2* __restrict__ tab,
-                                      double k) {
-    s = __builtin_fma(w, helm_k0(d2, tab, helm_consts(k)), s);
+                                      const HelmConsts& h) {
+    s = __builtin_fma(w, helm_k0(d2, tab, h), s);
   }
   __device__ __forceinline__ double result() const { return kInv2Pi * s; }
 };
@@ -301,11 +302,14 @@ __global__ __launch_bounds__(64 * kS2MWaves) void k_s2m(int nup, const int* __re
   const double* lx = ops + d * opsz + (int64_t)OP_NBLK * PD * PD;
   const double* ly = lx + PD;
   // check point k in absolute coordinates (box centre + large-circle offset)
-  const double ax = (k < p ? lx[k] : 1.0) + xmid[g];
-  const double ay = (k < p ? ly[k] : 1.0) + ymid[g];
+  // lanes k >= p repeat check point 0 (their values are not stored), so that
+  // they take the same branch of the kernel evaluation as the active lanes
+  const double ax = lx[k < p ? k : 0] + xmid[g];
+  const double ay = ly[k < p ? k : 0] + ymid[g];
   const int b = __builtin_amdgcn_readfirstlane(bot[g]);
   const int t = __builtin_amdgcn_readfirstlane(top[g]);
   PotAcc<KER> acc;
+  const HelmConsts hc = helm_consts(kp);
   double dsum = 0.0;   // double layer
   int j = b;
   for (; j + 4 <= t; j += 4) {
@@ -317,7 +321,7 @@ __global__ __launch_bounds__(64 * kS2MWaves) void k_s2m(int nup, const int* __re
       const double dx = ax - s4[u].x;
       const double dy = ay - s4[u].y;
       const double d2 = __builtin_fma(dy, dy, dx * dx);
-      acc.add(s4[u].z, d2, tab, kp);
+      acc.add(s4[u].z, d2, tab, hc);
       if constexpr (DIP) {
         const double2 q = dip[j + u];
         dsum = __builtin_fma(__builtin_fma(q.x, dx, q.y * dy), rcp_nr(d2), dsum);
@@ -329,7 +333,7 @@ __global__ __launch_bounds__(64 * kS2MWaves) void k_s2m(int nup, const int* __re
     const double dx = ax - sj.x;
     const double dy = ay - sj.y;
     const double d2 = __builtin_fma(dy, dy, dx * dx);
-    acc.add(sj.z, d2, tab, kp);
+    acc.add(sj.z, d2, tab, hc);
     if constexpr (DIP) {
       const double2 q = dip[j];
       dsum = __builtin_fma(__builtin_fma(q.x, dx, q.y * dy), rcp_nr(d2), dsum);
@@ -1349,7 +1353,7 @@ __device__ __forceinline__ ItemRange xcd_items(int ni) {
 template <int KER, int NOUT, bool CHECK>
 struct Target {
   double x, y;
-  double kp;        // kernel parameter (Helmholtz k)
+  HelmConsts kp;    // kernel parameter (Helmholtz k) and its derived constants
   PotAcc<KER> u;
   double gx = 0.0, gy = 0.0;
   // without self-exclusion: the smallest high word of d^2 seen; 0 means a
@@ -1396,7 +1400,7 @@ struct Target {
         u.add(w, d2, tab, kp);
       } else {
         double k0, gk1;
-        helm_k01(d2, rcp_nr(d2), tab, helm_consts(kp), k0, gk1);
+        helm_k01(d2, rcp_nr(d2), tab, kp, k0, gk1);
         u.s = __builtin_fma(w, k0, u.s);
         const double qi = z ? 0.0 : -gk1 * w;
         gx = __builtin_fma(qi, dx, gx);
@@ -1509,9 +1513,11 @@ __device__ __forceinline__ void eval_few(
   const FewLanes F = few_lanes(na);
   const int t = t0 + F.slot;
   Target<KER, NOUT, CHECK> T;
-  T.kp = kp;
-  T.x = F.act ? tsx[t] : 0.0;
-  T.y = F.act ? tsy[t] : 0.0;
+  T.kp = helm_consts(kp);
+  // idle slots repeat the item's first target (results not stored), so that
+  // they take the same branch of the kernel evaluation as the active ones
+  T.x = tsx[F.act ? t : t0];
+  T.y = tsy[F.act ? t : t0];
   const int d = nlevel[g];
   double far = 0.0;   // Chebyshev L2T
   if (CHEB && d >= 2) {
@@ -1519,7 +1525,7 @@ __device__ __forceinline__ void eval_few(
                    le + (int64_t)g * PD, p, F.sg, F.ng);
   } else if (d >= 2) {
     Target<KER, NOUT, false> L;
-    L.kp = kp;
+    L.kp = T.kp;
     L.x = T.x - xmid[g];
     L.y = T.y - ymid[g];
     const double* lx = ops + d * opsz + (int64_t)OP_NBLK * PD * PD;
@@ -1642,9 +1648,9 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
     const int t = t0 + (int)(threadIdx.x & 63);
     const bool act = t < t1;
     Target<KER, NOUT, CHECK> T;
-    T.kp = kp;
-    T.x = act ? tsx[t] : 0.0;
-    T.y = act ? tsy[t] : 0.0;
+    T.kp = helm_consts(kp);
+    T.x = tsx[act ? t : t0];   // idle lanes: as in eval_few
+    T.y = tsy[act ? t : t0];
     // L2T: helpers.py:219-235 (the equivalent density sits on the large circle);
     // Chebyshev expansions: bbfmm.py:185-199
     const int d = nlevel[g];
@@ -1654,7 +1660,7 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
                      le + (int64_t)g * PD, p, 0, 1);
     } else if (d >= 2) {
       Target<KER, NOUT, false> L;
-      L.kp = kp;
+      L.kp = T.kp;
       L.x = T.x - xmid[g];
       L.y = T.y - ymid[g];
       const double* lx = ops + d * opsz + (int64_t)OP_NBLK * PD * PD;
@@ -1760,7 +1766,7 @@ __device__ __forceinline__ void eval_few_mixed(
   const FewLanes F = few_lanes(na);
   const int t = t0 + F.slot;
   Target<KER_LAPLACE, 1, CHECK> T;
-  T.kp = 0.0;
+  T.kp = HelmConsts{};
   T.x = F.act ? tsx[t] : 0.0;
   T.y = F.act ? tsy[t] : 0.0;
   const int d = nlevel[g];
@@ -1770,7 +1776,7 @@ __device__ __forceinline__ void eval_few_mixed(
                    le + (int64_t)g * PD, p, F.sg, F.ng);
   } else if (d >= 2) {
     Target<KER_LAPLACE, 1, false> L;
-    L.kp = 0.0;
+    L.kp = HelmConsts{};
     L.x = T.x - xmid[g];
     L.y = T.y - ymid[g];
     const double* lx = ops + d * opsz + ops_off_circles(PD);
@@ -1865,7 +1871,7 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
     const int t = t0 + (int)(threadIdx.x & 63);
     const bool act = t < t1;
     Target<KER_LAPLACE, 1, CHECK> T;
-    T.kp = 0.0;
+    T.kp = HelmConsts{};
     T.x = act ? tsx[t] : 0.0;
     T.y = act ? tsy[t] : 0.0;
     const int d = nlevel[g];
@@ -1875,7 +1881,7 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
                      le + (int64_t)g * PD, p, 0, 1);
     } else if (d >= 2) {   // L2T in FP64 (the local densities carry large cancelling modes)
       Target<KER_LAPLACE, 1, false> L;
-      L.kp = 0.0;
+      L.kp = HelmConsts{};
       L.x = T.x - xmid[g];
       L.y = T.y - ymid[g];
       const double* lx = ops + d * opsz + ops_off_circles(PD);
