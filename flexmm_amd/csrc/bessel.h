@@ -97,8 +97,8 @@ __host__ __device__ inline void bessel_k01(double x, double& k0, double& k1) {
   if (x <= 2.0) {
     const double q = 0.25 * x * x;
     const double lg = log(0.5 * x);
-    // compile-time coefficients here: with four chains the gradient
-    // kernels measured faster with them in registers than in constant memory
+    // compile-time coefficients here (the host path; the device kernels use
+    // their own K0/K1 from d^2)
     const BesselSeries& c = kBesselSeries;
     constexpr int n = kBesselSeriesTerms - 1;
     double i0 = c.i0[n], s0 = c.s0[n], i1 = c.i1[n], s1 = c.s1[n];
