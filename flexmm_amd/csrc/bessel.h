@@ -75,6 +75,18 @@ __host__ __device__ inline const BesselSeries& bessel_series() {
 #endif
 }
 
+// x > 2 only
+__host__ __device__ inline double bessel_k0_large(double x) {
+  const double u = 4.0 / x - 1.0;
+  return exp(-x) / sqrt(x) * cheb_sum(kK0Cheb, u);
+}
+__host__ __device__ inline void bessel_k01_large(double x, double& k0, double& k1) {
+  const double u = 4.0 / x - 1.0;
+  const double e = exp(-x) / sqrt(x);
+  k0 = e * cheb_sum(kK0Cheb, u);
+  k1 = e * cheb_sum(kK1Cheb, u);
+}
+
 __host__ __device__ inline double bessel_k0(double x) {
   if (x <= 2.0) {
     const double q = 0.25 * x * x;
@@ -88,8 +100,7 @@ __host__ __device__ inline double bessel_k0(double x) {
     }
     return s - (log(0.5 * x) + kEulerGamma) * i0;
   }
-  const double u = 4.0 / x - 1.0;
-  return exp(-x) / sqrt(x) * cheb_sum(kK0Cheb, u);
+  return bessel_k0_large(x);
 }
 
 // K0 and K1 together (the gradient evaluator needs both).
@@ -113,10 +124,7 @@ __host__ __device__ inline void bessel_k01(double x, double& k0, double& k1) {
     k1 = 1.0 / x + lg * (0.5 * x * i1) - 0.25 * x * s1;
     return;
   }
-  const double u = 4.0 / x - 1.0;
-  const double e = exp(-x) / sqrt(x);
-  k0 = e * cheb_sum(kK0Cheb, u);
-  k1 = e * cheb_sum(kK1Cheb, u);
+  bessel_k01_large(x, k0, k1);
 }
 
 }  // namespace fmm

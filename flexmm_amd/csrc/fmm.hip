@@ -190,7 +190,7 @@ struct LogAcc {
 // ln(d2) from the table log, and
 //   k K1(x)/r = 1/d2 + (k^2/4) (2 ln(x/2) I1' - S1'),
 // I1' = sum q^j/(j!(j+1)!), S1' the psi sum of bessel_k01 (x I1' / 2 = I1).
-// The x > 2 branch is bessel_k0 / bessel_k01 at x = k sqrt(d2).
+// The x > 2 branch is bessel_k0_large / bessel_k01_large at x = k sqrt(d2).
 struct HelmConsts {
   double k, k2q, lnk2;   // k, k^2/4, ln(k/2)
 };
@@ -217,7 +217,7 @@ __device__ __forceinline__ double helm_k0(double d2, const double2* __restrict__
     }
     return __builtin_fma(-(helm_lnx2(d2, tab, h) + kEulerGamma), i0, s);
   }
-  return bessel_k0(h.k * __builtin_sqrt(d2));
+  return bessel_k0_large(h.k * __builtin_sqrt(d2));
 }
 // K0 and k K1(x)/r; id2 = 1/d2
 __device__ __forceinline__ void helm_k01(double d2, double id2, const double2* __restrict__ tab,
@@ -247,7 +247,7 @@ __device__ __forceinline__ void helm_k01(double d2, double id2, const double2* _
   }
   const double r = __builtin_sqrt(d2);
   double k1;
-  bessel_k01(h.k * r, k0, k1);
+  bessel_k01_large(h.k * r, k0, k1);
   gk1 = h.k * k1 / r;
 }
 
@@ -1623,8 +1623,11 @@ __device__ __forceinline__ void eval_few(
   }
 }
 
+// The Helmholtz gradient instance is held to 3 waves per SIMD (168 VGPRs;
+// it needs 169 unconstrained, which leaves 2).
 template <int PD, int KER, int NOUT, bool CHECK, bool DIP, bool CHEB>
-__global__ __launch_bounds__(64 * kEvalWaves) void k_eval(
+__global__ __launch_bounds__(64 * kEvalWaves)
+__attribute__((amdgpu_waves_per_eu(KER == KER_MODHELM && NOUT == 3 ? 3 : 1))) void k_eval(
     const int4* __restrict__ items, const int* __restrict__ nitems,
     const int* __restrict__ evalnode, const int* __restrict__ toff,
     const int* __restrict__ tperm, const double* __restrict__ tsx, const double* __restrict__ tsy,
