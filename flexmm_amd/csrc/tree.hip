@@ -144,26 +144,27 @@ __global__ __launch_bounds__(BS) void k_split_pos(const uint64_t* __restrict__ k
 }
 
 // Number of split cells per depth: entry j of the compacted split positions
-// starts the split cells of depths s_j+1 .. c_j.  One block; wave ballots
-// count each depth (no contended atomics).
+// starts the split cells of depths s_j+1 .. c_j.  One block per 1024 entries
+// (the grid covers the n >= np bound); wave ballots count each depth into LDS,
+// and each block adds its counts to dcount, which starts at zero.
 __global__ __launch_bounds__(1024) void k_depth_count(const int2* __restrict__ psc,
                                                       const int* __restrict__ np_dev,
                                                       int* __restrict__ dcount) {
   __shared__ int lc[kMaxLevels];
+  const int np = *np_dev;
+  if ((int64_t)blockIdx.x * 1024 >= np) return;   // block-uniform, before any barrier
   if (threadIdx.x < kMaxLevels) lc[threadIdx.x] = 0;
   __syncthreads();
-  const int np = *np_dev;
   const int lane = threadIdx.x & 63;
-  for (int base = threadIdx.x - lane; base < np; base += 1024) {
-    const int j = base + lane;
-    const int2 v = j < np ? psc[j] : make_int2(0, 0);
-    for (int d = 0; d < kMaxLevels; ++d) {
-      const unsigned long long m = __ballot(v.x < d && d <= v.y);
-      if (lane == 0 && m) atomicAdd(&lc[d], (int)__popcll(m));
-    }
+  const int j = blockIdx.x * 1024 + threadIdx.x;
+  const int2 v = j < np ? psc[j] : make_int2(0, 0);
+  for (int d = 0; d < kMaxLevels; ++d) {
+    const unsigned long long m = __ballot(v.x < d && d <= v.y);
+    if (lane == 0 && m) atomicAdd(&lc[d], (int)__popcll(m));
   }
   __syncthreads();
-  if (threadIdx.x < kMaxLevels) dcount[threadIdx.x] = lc[threadIdx.x];
+  if (threadIdx.x < kMaxLevels && lc[threadIdx.x] != 0)
+    atomicAdd(&dcount[threadIdx.x], lc[threadIdx.x]);
 }
 
 __global__ __launch_bounds__(BS) void k_compact_pos(const int* __restrict__ flag,
@@ -846,7 +847,9 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   hipLaunchKernelGGL(k_compact_pos, dim3(grid_for(n)), dim3(BS), 0, s, flag, rank, sc, nn, ppos,
                      psc);
   FMM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_depth_count, dim3(1), dim3(1024), 0, s, psc, cnt + CNT_P, cnt + CNT_D);
+  hipLaunchKernelGGL(k_depth_count, dim3((unsigned)std::max<int64_t>(1, ceil_div(n, 1024))),
+                     dim3(1024), 0, s, psc,
+                     cnt + CNT_P, cnt + CNT_D);
   FMM_LAUNCH_CHECK();
   FMM_HIP(hipMemcpyAsync(hc, cnt, CNT_N * sizeof(int), hipMemcpyDeviceToHost, s));
   FMM_HIP(hipStreamSynchronize(s));
