@@ -79,3 +79,29 @@ def test_helmholtz_mixed_m2l():
     err = H.rel_max_err(mix.target_evaluation(tx, ty)[0], f64.target_evaluation(tx, ty)[0])
     print("helmholtz mixed vs fp64 %.3e" % err)
     assert err < 1e-7
+
+
+@pytest.mark.parametrize("n,ncut", [(300, 400), (3000, 40)])
+def test_helmholtz_near_field_both_branches(n, ncut):
+    """Near-field pairs on both sides of x = k r = 2, where the device switches
+    from the ascending series (taken from d^2) to the large-argument
+    interpolant: a root leaf evaluated directly, and a shallow tree of big
+    leaves; potential and gradient against FP64 direct sums."""
+    from flexmm_amd import kernels
+    from flexmm_amd.kifmm2d import FMM
+    x, y, tau = H.uniform(n, 80 + ncut)
+    tx, ty, _ = H.uniform(500, 81)
+    print("helmholtz near field", n, ncut, flush=True)
+    ker, pot, grad = kernels.modified_helmholtz(K)
+    f = FMM(x, y, ker, ncut, 60, bbox=BOX)
+    f.register_evaluator(grad, grad, "gradient", 3)
+    f.build_expansions(tau)
+    g = f.evaluate_to_points(tx, ty, "gradient", False)
+    if ncut >= n:   # every pair is a near-field pair: both branches occur
+        r = np.hypot(tx[:, None] - x[None, :], ty[:, None] - y[None, :])
+        assert (K * r > 2).mean() > 0.3 and (K * r < 2).mean() > 0.01
+    mh, mhg = okern.make_modified_helmholtz(K)
+    refg = odirect.direct(mhg, x, y, tau, tx, ty)
+    errs = [H.rel_max_err(g[k], refg[k]) for k in range(3)]
+    print("levels", f.tree.levels, "vs direct", errs)
+    assert max(errs) < 1e-11
