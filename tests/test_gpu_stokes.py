@@ -75,3 +75,27 @@ def test_stokes_adjoint_forces():
     e2 = H.rel_max_err(FY.grad.cpu().numpy()[sel], ref[1])
     print("stokes adjoint errors", e1, e2)
     assert max(e1, e2) < 1e-10
+
+
+def test_stokes_duplicate_sources_and_coincident_targets():
+    """Sources repeated at exactly the same point: self-evaluation skips every
+    coincident pair (each duplicate as well as the point itself), as the
+    reference's kernel_apply_single_check does; without self-exclusion a
+    target on a source gives NaN (the reference's 0/0)."""
+    from flexmm_amd.kifmm2d import StokesFMM
+    rng = np.random.default_rng(97)
+    base = rng.random((2500, 2))
+    pts = np.concatenate([base, base[:700], base[:100]])   # triples and pairs
+    x, y = pts[:, 0].copy(), pts[:, 1].copy()
+    fx, fy = rng.standard_normal(x.size), rng.standard_normal(x.size)
+    print("stokes duplicates", x.size, flush=True)
+    f = StokesFMM(x, y, 64, 40, bbox=[0, 1, 0, 1])
+    f.build_expansions(fx, fy)
+    u, v, p = f.evaluate_to_points(x, y, True)
+    sel = np.arange(0, x.size, 7)
+    ref = odirect.direct_stokes(x, y, fx, fy, x[sel], y[sel], check=True)
+    errs = [H.rel_max_err(a[sel], r) for a, r in zip((u, v, p), ref)]
+    print("vs direct", errs)
+    assert max(errs) < 1e-11
+    u, v, p = f.evaluate_to_points(x[:5], y[:5], False)
+    assert np.isnan(u).all() and np.isnan(v).all() and np.isnan(p).all()
