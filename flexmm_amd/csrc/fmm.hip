@@ -2167,7 +2167,8 @@ __device__ __forceinline__ void stokes_item(
     // stresslet densities stay in the wave-uniform registers they load into.
     auto pairS = [&](const double4& sj, int jj) {
       const double dx = x - sj.x, dy = y - sj.y;
-      double d2 = __builtin_fma(dy, dy, dx * dx);
+      const double dxx = dx * dx;
+      double d2 = __builtin_fma(dy, dy, dxx);
       const bool z = d2 == 0.0;
       if (!CHECK) dmin = __builtin_fmin(dmin, d2);
       d2 = z ? 1.0 : d2;
@@ -2183,14 +2184,17 @@ __device__ __forceinline__ void stokes_item(
       sv = __builtin_fma(dy, rf, sv);
       sp += rf;
       if constexpr (DIP) {   // the stresslet itself
+        // w2 = 2 (r.n)(r.sigma)/d2^2, with 2 (r.n)(r.sigma) = g d2 +
+        // a (r_x^2 - r_y^2) + 2 b r_x r_y; du, dv carry twice the sums
         const double4 e4 = sdip[jj];
         const double gg = e4.x, aa = e4.y, bb = e4.z;
-        const double nrns =
-            0.5 * __builtin_fma(gg, d2, __builtin_fma(aa, (dx - dy) * (dx + dy), 2.0 * bb * dx * dy));
-        const double w = nrns * id2 * id2;
-        du = __builtin_fma(w, dx, du);
-        dv = __builtin_fma(w, dy, dv);
-        dp += __builtin_fma(-gg, id2, 2.0 * w);
+        const double xy = dx * dy;
+        const double t2 = __builtin_fma(-dy, dy, dxx);
+        const double w2 =
+            __builtin_fma(gg, d2, __builtin_fma(aa, t2, bb * (xy + xy))) * (id2 * id2);
+        du = __builtin_fma(w2, dx, du);
+        dv = __builtin_fma(w2, dy, dv);
+        dp += __builtin_fma(-gg, id2, w2);
       }
     };
     int j = b + F.sg;
@@ -2209,8 +2213,8 @@ __device__ __forceinline__ void stokes_item(
   v += kInv4Pi * (-0.5 * lfy + sv);
   pr += kInv2Pi * sp;
   if constexpr (DIP) {
-    u += 2.0 * kInv2Pi * du;
-    v += 2.0 * kInv2Pi * dv;
+    u += kInv2Pi * du;
+    v += kInv2Pi * dv;
     pr += 2.0 * kInv2Pi * dp;
   }
   if constexpr (FEW) {
