@@ -33,17 +33,35 @@ FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector = matrix)
 HBM_PEAK_GBS = 8000.0
 
 
-def measured_traffic():
-    """HBM bytes per launch of the FP64 evaluation kernel, from the rocprofv3
-    FETCH_SIZE/WRITE_SIZE runs summarised in profiles/ (scaled by the same
-    counters' readings for a copy kernel of known size); None if absent."""
-    path = os.path.join(HERE, "profiles", "laplace_1M_fp64_traffic.txt")
+def _profile_lines(name):
     try:
-        for line in open(path):
-            if "k_eval traffic_bytes=" in line:
-                return float(line.split("traffic_bytes=")[1])
+        return open(os.path.join(HERE, "profiles", name)).read().splitlines()
     except OSError:
-        pass
+        return []
+
+
+def measured_traffic(what):
+    """HBM bytes per launch (what = "k_eval") or per tree build (what =
+    "tree") from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
+    profiles/ (scaled by the same counters' readings for a copy kernel of known
+    size); None if absent."""
+    key = "%s traffic_bytes=" % what
+    for line in _profile_lines("laplace_1M_fp64_traffic.txt"):
+        if key in line:
+            return float(line.split(key)[1].split()[0])
+    return None
+
+
+def profiled_avg_us(kernel_prefix):
+    """Average duration of a kernel in the committed rocprofv3 --stats summary
+    of this workload (profiles/laplace_1M_fp64_kernels.txt); None if absent."""
+    for line in _profile_lines("laplace_1M_fp64_kernels.txt"):
+        if line.startswith(kernel_prefix):
+            parts = line[len(kernel_prefix):].split()
+            try:
+                return float(parts[-4])
+            except (IndexError, ValueError):
+                return None
     return None
 
 
@@ -67,7 +85,6 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-targets", type=int, default=1_000_000)
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the extra workloads (clustered, adjoint, Stokeslet, Stokes "
                          "double layer, black-box)")
@@ -100,56 +117,94 @@ def direct_sample(torch, xd, yd, taud, tx, ty, chunk=32):
     return torch.cat(out)
 
 
-def cpu_baseline(sample_targets, min_seconds=10.0, max_reps=6):
-    """Time the NumPy oracle (CPU restatement of the reference algorithm) on
-    one core: full tree + expansions and evaluation on the first
-    sample_targets grid targets (evaluation time scaled to all of them),
-    repeated until at least min_seconds of CPU work have run."""
+# README.md:27-33 (uniform sources, grid targets; thousands of points per
+# second per core on a 4-core MacBook Pro): form multipoles, target evaluation
+PUBLISHED_KPTS_PER_CORE = {"form": 947.0, "target_eval": 414.0}
+
+
+def host_cpu():
+    """Model name, logical CPUs this process may run on, and the physical cores
+    among them (from /proc/cpuinfo)."""
+    model, phys, cur = None, set(), {}
+    allowed = set(os.sched_getaffinity(0))
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "processor":
+                cur = {"proc": int(v)}
+            elif k == "model name" and model is None:
+                model = v
+            elif k in ("physical id", "core id"):
+                cur[k] = v
+                if "physical id" in cur and "core id" in cur and cur["proc"] in allowed:
+                    phys.add((cur["physical id"], cur["core id"]))
+    except OSError:
+        pass
+    return model, len(allowed), len(phys) or len(allowed)
+
+
+def cpu_baseline(min_seconds=10.0, max_reps=20):
+    """Time the oracle (NumPy restatement of the reference kifmm2d) on the
+    host's cores: the whole step on the full workload -- tree (serial, as the
+    reference's top-level partition), build_expansions (S2M over leaves on
+    forked workers, dense passes in threaded BLAS) and target_evaluation on
+    all 1M targets (forked workers over target chunks) -- repeated until at
+    least min_seconds have run.  Workers = the CPUs this process may use,
+    capped by OMP_NUM_THREADS."""
+    model, logical, physical = host_cpu()
+    workers = max(1, min(logical, int(os.environ.get("OMP_NUM_THREADS", logical))))
     code = r"""
 import json, sys, time
 import numpy as np
 sys.path.insert(0, %r)
-from oracle import kifmm2d, kernels
+from oracle import kernels, kifmm2d, parallel
 rng = np.random.default_rng(%d)
 n = %d
 x = rng.random(n); y = rng.random(n); tau = rng.standard_normal(n)
 g = np.linspace(0.0, 1.0, %d)
 tx, ty = np.meshgrid(g, g, indexing="ij"); tx = tx.ravel(); ty = ty.ravel()
-m = %d
 for lev in range(12):
     kifmm2d.get_precomputation(kernels.laplace, %d, 2.0 ** -lev)
-acc = dict(tree=0.0, expansions=0.0, eval_sample=0.0)
+acc = dict(tree=0.0, expansions=0.0, evaluation=0.0)
 reps = 0
 while reps < %d and (reps == 0 or sum(acc.values()) < %r):
     t0 = time.perf_counter()
-    f = kifmm2d.FMM(x, y, kernels.laplace, %d, %d, bbox=[0, 1, 0, 1])
+    f = parallel.ParallelFMM(x, y, kernels.laplace, %d, %d, bbox=[0, 1, 0, 1], workers=%d)
     t1 = time.perf_counter()
     f.build_expansions(tau)
     t2 = time.perf_counter()
-    u = f.target_evaluation(tx[:m], ty[:m])
+    u = f.target_evaluation(tx, ty)
     t3 = time.perf_counter()
-    acc["tree"] += t1 - t0; acc["expansions"] += t2 - t1; acc["eval_sample"] += t3 - t2
+    acc["tree"] += t1 - t0; acc["expansions"] += t2 - t1; acc["evaluation"] += t3 - t2
     reps += 1
     del f, u
-print(json.dumps(dict(acc, reps=reps, sample=m, nt=tx.size)))
-""" % (HERE, SEED, N_SRC, GRID, sample_targets, P, max_reps, float(min_seconds), NCUT, P)
-    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+print(json.dumps(dict(acc, reps=reps, nt=tx.size)))
+""" % (HERE, SEED, N_SRC, GRID, P, max_reps, float(min_seconds), NCUT, P, workers)
+    env = dict(os.environ, OMP_NUM_THREADS=str(workers), OPENBLAS_NUM_THREADS=str(workers),
+               MKL_NUM_THREADS=str(workers))
     res = subprocess.run([sys.executable, "-c", code], env=env, stdout=subprocess.PIPE,
-                         stderr=subprocess.PIPE, text=True, timeout=600)
+                         stderr=subprocess.PIPE, text=True, timeout=900)
     if res.returncode != 0:
         return dict(error=res.stderr[-500:])
     r = json.loads(res.stdout.strip().splitlines()[-1])
     reps = r["reps"]
-    total = (r["tree"] + r["expansions"] + r["eval_sample"] * r["nt"] / r["sample"]) / reps
+    total = (r["tree"] + r["expansions"] + r["evaluation"]) / reps
+    rate_pub = 1.0 / (1.0 / PUBLISHED_KPTS_PER_CORE["form"]
+                      + 1.0 / PUBLISHED_KPTS_PER_CORE["target_eval"]) * workers / 1e3
     return dict(
-        value=round(r["nt"] / total / 1e6, 4), unit="Mpts/s", cores=1, kind="port",
-        sample=("NumPy oracle (CPU restatement of the reference kifmm2d) on 1 core: full tree + "
-                "build_expansions for the 1M sources + target_evaluation on %d of the 1M grid "
-                "targets (scaled to 1M), the whole step repeated %d times"
-                % (r["sample"], reps)),
-        seconds=dict(tree=round(r["tree"], 3), expansions=round(r["expansions"], 3),
-                     eval_sample=round(r["eval_sample"], 3), reps=reps))
-
+        value=round(r["nt"] / total / 1e6, 4), unit="Mpts/s", cores=workers, kind="port",
+        cpu_model=model, logical_cpus=logical, physical_cores=physical,
+        sample=("NumPy oracle (CPU restatement of the reference kifmm2d) on %d worker "
+                "processes: the whole step on the full workload (tree serial, S2M over leaves "
+                "and evaluation of all 1M grid targets on forked workers, dense passes in "
+                "threaded BLAS), repeated %d times" % (workers, reps)),
+        seconds=dict(tree=round(r["tree"] / reps, 4), expansions=round(r["expansions"] / reps, 4),
+                     evaluation=round(r["evaluation"] / reps, 4), reps=reps),
+        published_reference_scaled=dict(
+            value=round(rate_pub, 4), unit="Mpts/s",
+            note="README.md:27-33 per-core rates (uniform sources, grid targets: form 947 and "
+                 "target evaluation 414 kpts/s/core, tree excluded) x %d cores" % workers))
 
 
 def _timed(torch, fn, steps, warmup):
@@ -398,18 +453,43 @@ def stokes_dipole_workload(torch, steps, warmup):
         f.evaluate_to_points(X, Y, True)
         f.close()
 
+    out = {}
+
     def double():
         f = StokesFMM(X, Y, 64, 100, bbox=box)
         f.build_expansions_dipole(FX, FY, GX, GY, NX, NY)
-        f.evaluate_to_points_dipole(X, Y, True)
+        out["uvp"] = f.evaluate_to_points_dipole(X, Y, True)
         f.close()
 
     ms_s = _timed(torch, single, steps, warmup)
     ms_d = _timed(torch, double, steps, warmup)
+    # FP64 direct sums (Stokeslet + stresslet, self-excluded) at 2,000 sources
+    sel = torch.from_numpy(np.sort(rng.choice(n, 2000, replace=False))).to(dev)
+    ref = [[], [], []]
+    for s0 in range(0, sel.numel(), 16):
+        ss = sel[s0:s0 + 16]
+        dx = X[ss, None] - X[None, :]
+        dy = Y[ss, None] - Y[None, :]
+        d2 = dx * dx + dy * dy
+        keep = d2 > 0
+        id2 = torch.where(keep, 1.0 / torch.where(keep, d2, 1.0), 0.0)
+        lg = torch.where(keep, -0.5 * torch.log(torch.where(keep, d2, 1.0)), 0.0)
+        w = (dx * NX + dy * NY) * (dx * GX + dy * GY) * id2 * id2
+        ref[0].append((((lg + dx * dx * id2) * FX + dx * dy * id2 * FY) / (4 * np.pi)
+                       + w * dx / np.pi).sum(1))
+        ref[1].append(((dx * dy * id2 * FX + (lg + dy * dy * id2) * FY) / (4 * np.pi)
+                       + w * dy / np.pi).sum(1))
+        ref[2].append(((dx * FX + dy * FY) * id2 / (2 * np.pi)
+                       + (-(NX * GX + NY * GY) * id2 + 2.0 * w) / np.pi).sum(1))
+    ref = [torch.cat(r) for r in ref]
+    errs = [float((a[sel] - r).abs().max() / r.abs().max()) for a, r in zip(out["uvp"], ref)]
     return {"workload": "Stokeslets + stresslets, 2M sources in 100 Gaussian clusters "
                         "(sigma 0.01), self-evaluation of u, v, p, Nequiv 64, Ncutoff 100, FP64",
             "single_layer_ms": round(ms_s, 4), "single_plus_double_layer_ms": round(ms_d, 4),
-            "single_plus_double_layer_Mpts_s": round(n / ms_d / 1e3, 3)}
+            "single_plus_double_layer_Mpts_s": round(n / ms_d / 1e3, 3),
+            "velocity_error_vs_direct": max(errs[:2]), "pressure_error_vs_direct": errs[2],
+            "error_sample": "max|err|/max|ref| at 2000 sampled sources vs FP64 direct sums "
+                            "with self-exclusion"}
 
 
 def main():
@@ -448,45 +528,48 @@ def main():
     ref = direct_sample(torch, xd, yd, taud, txd[sel], tyd[sel])
 
     def measure(precision):
-        def step(profile=False):
+        def step():
             f = FMM(xd, yd, kernels.laplace, NCUT, P, bbox=bbox, precision=precision)
-            if profile:
-                f.set_profiling(True)
-                f.build_tree()
             f.build_expansions(taud)
             u = f.target_evaluation(txd, tyd)
             return f, u
 
-        for _ in range(args.warmup):
+        # Every step reuses one pooled device plan.  Phase timing is switched
+        # on in the first warm-up step: each later run of a phase records its
+        # own pair of HIP events on the launch stream (torch's current
+        # stream), and phase_times() returns the mean over the runs since the
+        # previous call -- the timed steps below.
+        for i in range(max(1, args.warmup)):
             f, u = step()
+            if i == 0:
+                f.set_profiling(True)
+            last = f
             del f
+        last.phase_times()
+        del last
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             f, u = step()
+            last = f
             del f
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         ms = (t1 - t0) * 1000.0 / args.steps
-        # Phase and kernel durations (HIP events on the launch stream),
-        # averaged over separate profiled steps.
-        acc = {}
-        nprof = max(3, min(args.steps, 10))
-        for _ in range(nprof):
-            f, u = step(profile=True)
-            for k, v in f.phase_times().items():
-                acc[k] = acc.get(k, 0.0) + v / nprof
-            work = f.work_counts()
-            del f
-        f, u = step()
+        acc = last.phase_times()
+        last.set_profiling(False)
+        work = last.work_counts()
+        nodes = last.tree.n_nodes
+        del last
         # max-norm error relative to max|u| (examples/pure_kifmm_laplace.py:149-151)
         err = float((u[0][sel] - ref).abs().max() / ref.abs().max())
         return dict(ms=ms, value=tx.size / (ms / 1000.0) / 1e6, phases=acc, work=work, err=err,
-                    u=u[0])
+                    u=u[0], nodes=nodes)
 
     if args.only:
         r = measure(args.only)
         r.pop("u")
+        r.pop("nodes")
         print(json.dumps(dict(precision=args.only, ms_per_step=round(r["ms"], 4),
                               value=round(r["value"], 3), error_vs_direct=r["err"],
                               phase_ms={k: round(v, 4) for k, v in r["phases"].items()})))
@@ -511,7 +594,13 @@ def main():
     f_m2m = 8.0 * P * P * (work["m2m_parents"]) + 2.0 * P * P * work["l2l_children"]
     f_total = f_p2p + f_l2t + f_s2m + f_m2l + f_pinv + f_m2m
     ek = acc.get("eval_kernel", float("nan"))
-    achieved = (f_p2p + f_l2t) / (ek / 1000.0) / 1e12
+    f_eval = f_p2p + f_l2t
+    achieved = f_eval / (ek / 1000.0) / 1e12
+    prof_us = profiled_avg_us("k_eval<64, 0, 1, false, false, false>")
+    # tree build: algorithmic bytes = 40 B per source (read x, y; write sorted
+    # x, y; keys and permutation) + one write of the node table (252 B per node)
+    tree_bytes = 40.0 * N_SRC + 252.0 * r64["nodes"]
+    tree_ms = acc.get("tree", float("nan"))
 
     out = {
         "metric": "Mpts/s, 1M-src/1M-tgt Laplace KIFMM: FP64 vs mixed-prec at stated error",
@@ -542,6 +631,12 @@ def main():
             "error_vs_fp64": err_mx64,
             "dtype": "f32 M2L (MFMA) + f32 leaf near field, f64 elsewhere",
             "phase_ms": {k: round(v, 4) for k, v in rmx["phases"].items()},
+            "kernel_speedup_vs_fp64": {
+                k: round(acc[k] / rmx["phases"][k], 3) for k in ("eval_kernel", "m2l")
+                if k in acc and k in rmx["phases"] and rmx["phases"][k] > 0},
+            "shared_fp64_ms": round(sum(rmx["phases"].get(k, 0.0) for k in ("tree", "upward")), 4),
+            "shared_note": "the tree build and the upward pass (S2M, M2M, pseudo-inverse) run in "
+                           "FP64 in both forms",
         },
         "phase_ms": {k: round(v, 4) for k, v in acc.items()},
         "work": work,
@@ -549,16 +644,42 @@ def main():
         "end_to_end_tflops": round(f_total / (ms / 1000.0) / 1e12, 3),
         "roofline": {
             "kernel": "k_eval (fused L2T + P2P, FP64)",
-            "bound": "mfma",
-            "note": "FP64 VALU-bound (log per pair); MI355X FP64 vector peak = FP64 matrix peak",
+            "bound": "valu",
+            "note": "FP64 VALU issue bound (a table log per pair, 17.9 VALU instructions per "
+                    "pair; profiles/k_eval_1M_fp64_valu.txt); the peak is MI355X's FP64 "
+                    "vector rate, equal to its FP64 matrix rate",
             "achieved": round(achieved, 3),
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-            "traffic": measured_traffic(),
+            "algorithmic_flop": f_eval,
+            "kernel_ms": round(ek, 5),
+            "kernel_ms_note": "mean of HIP-event pairs around each launch of the kernel over the "
+                              "timed steps, on its launch stream",
+            "profile_avg_us": prof_us,
+            "frac_from_profile": (round(f_eval / (prof_us * 1e-6) / 1e12 / FP64_PEAK_TFLOPS, 4)
+                                  if prof_us else None),
+            "profile_note": "average duration in profiles/laplace_1M_fp64_kernels.txt "
+                            "(rocprofv3 --kernel-trace --stats of bench.py --only fp64)",
+            "traffic": measured_traffic("k_eval"),
             "traffic_note": "HBM bytes per launch of k_eval from rocprofv3 FETCH_SIZE/WRITE_SIZE "
                             "(profiles/laplace_1M_fp64_traffic.txt); algorithmic bytes ~6.9e7 "
                             "(targets, sources, local expansions, output)",
+        },
+        "roofline_tree": {
+            "kernel": "tree build (every kernel of fmm2d_tree_build)",
+            "bound": "hbm",
+            "achieved": round(tree_bytes / (tree_ms * 1e-3) / 1e9, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(tree_bytes / (tree_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes": tree_bytes,
+            "phase_ms": round(tree_ms, 5),
+            "traffic": measured_traffic("tree"),
+            "note": "algorithmic bytes = 40 B per source + 252 B per node; traffic = FETCH_SIZE "
+                    "+ WRITE_SIZE of all tree kernels per build (profiles/"
+                    "laplace_1M_fp64_traffic.txt); the build is a chain of short kernels with "
+                    "host read-backs, bound by launch and synchronisation latency",
         },
     }
     if not args.no_extra:
@@ -567,7 +688,7 @@ def main():
             out[key] = globals()[fn](torch, steps, warmup)
     if not args.no_cpu_baseline:
         progress("CPU baseline")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_sample_targets)
+        out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
 
 

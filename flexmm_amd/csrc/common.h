@@ -51,17 +51,22 @@ struct Failure {
 
 #define FMM_LAUNCH_CHECK() FMM_HIP(hipGetLastError())
 
+// Stream of the C-ABI call in progress on this thread (set by the entry
+// points for the duration of the call; null: the default stream).
+inline thread_local hipStream_t g_op_stream = nullptr;
+
 // Device buffer that only grows; reused across calls so that steady-state
 // calls allocate nothing.
 struct DevBuf {
   void* ptr = nullptr;
   size_t bytes = 0;
-  // Growing frees the old block; the device is synchronised first so that no
-  // queued kernel still uses it.
+  // Growing frees the old block after the stream of the current call has
+  // drained, so that no kernel queued on it still uses the block (calls on a
+  // plan are ordered on their streams; see the C-ABI header).
   void ensure(size_t nbytes) {
     if (nbytes <= bytes) return;
     if (ptr) {
-      FMM_HIP(hipDeviceSynchronize());
+      FMM_HIP(hipStreamSynchronize(g_op_stream));
       FMM_HIP(hipFree(ptr));
     }
     ptr = nullptr;
@@ -185,7 +190,7 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
 enum KernelId : int {
   KER_LAPLACE = 0,   // -log(d^2)/(4 pi)
   KER_MODHELM = 1,   // K0(k r)/(2 pi)
-  KER_STOKES = 2,    // 2D Stokeslet (velocity and pressure) via 6 Laplace densities
+  KER_STOKES = 2,    // 2D Stokeslet (velocity and pressure); far field as 3 harmonic fields
 };
 enum EvalId : int {
   EVAL_POT = 0,      // 1 output: u

@@ -18,6 +18,7 @@
 #include <cstdarg>
 #include <cstring>
 #include <exception>
+#include <mutex>
 
 #include "../../include/flexmm_amd.h"
 
@@ -57,11 +58,20 @@ enum {
   OP_NBLK = 50,
 };
 // ... followed, for the mixed-precision M2L, by the 40 row-mean vectors c_s
-// (FP64 [40][PD]) and the FP32 residuals R_s = M2L_s - c_s 1^T ([40][PD][PD]).
+// (FP64 [40][PD]) and the FP32 residuals R_s = M2L_s - c_s 1^T ([40][PD][PD]);
+// then the SVD-compressed M2L (helpers.py:17-45): 64 slot ranks (as doubles;
+// r > 0: compressed of rank r, 0: null slot, -1: dense) and 40 [PD][PD]
+// blocks, rows 0 .. PD/2-1 = M1 = diag(s) V^T (r x p) and rows PD/2 .. PD-1 =
+// M2 = U (p x r) two rows per block row: block[PD/2 + i/2][(i%2) PD/2 + j] =
+// M2[i][j].
 __host__ __device__ inline int64_t ops_off_circles(int PD) { return (int64_t)OP_NBLK * PD * PD; }
 __host__ __device__ inline int64_t ops_off_means(int PD) { return ops_off_circles(PD) + 2 * PD; }
 __host__ __device__ inline int64_t ops_off_r32(int PD) { return ops_off_means(PD) + 40 * PD; }
-inline int64_t ops_per_level(int PD) { return ops_off_r32(PD) + 20 * (int64_t)PD * PD; }
+__host__ __device__ inline int64_t ops_off_svd_rank(int PD) {
+  return ops_off_r32(PD) + 20 * (int64_t)PD * PD;
+}
+__host__ __device__ inline int64_t ops_off_svd(int PD) { return ops_off_svd_rank(PD) + 64; }
+inline int64_t ops_per_level(int PD) { return ops_off_svd(PD) + 40 * (int64_t)PD * PD; }
 
 // Chebyshev (black-box) expansions, bbfmm.py:95-231: order pc <= kMaxCheb,
 // p = pc^2 coefficients per box, index i pc + j for T_i(x) T_j(y), with
@@ -87,6 +97,7 @@ struct Plan {
   double kparam = 0.0;
   int p = 0, PD = 0, precision = PREC_FP64;
   int cheb = 0;        // Chebyshev order of a black-box plan (p = cheb^2); 0: KIFMM
+  int m2l_mode = FMM2D_M2L_SVD;   // FP64 M2L: compressed slots as in the reference, or dense
   Tree tree;
   DevBuf ops;
   int ops_levels = 0;
@@ -108,21 +119,57 @@ struct Plan {
   PrimWork pw;
   HostBuf hbuf;
   int device = 0;
-  // optional phase timing with HIP events on the caller's stream
+  // end of the last call's work on its stream (waited for by plan destroy)
+  hipEvent_t done_ev = nullptr;
+  bool done_recorded = false;
+  // the last evaluation's per-box target counts (work_counts)
+  const int* last_tcnt = nullptr;
+  // optional phase timing with HIP events on the caller's stream: one
+  // (start, end) pair per run of a phase, kept until phase_times reads them
   bool profile = false;
-  hipEvent_t ev[2 * FMM2D_NUM_PHASES] = {};
-  bool ev_used[FMM2D_NUM_PHASES] = {};
+  std::vector<hipEvent_t> ev[FMM2D_NUM_PHASES];   // start, end, start, end ...
+  int runs[FMM2D_NUM_PHASES] = {};
+  bool open_run[FMM2D_NUM_PHASES] = {};
   ~Plan() {
-    for (auto& e : ev)
-      if (e) (void)hipEventDestroy(e);
+    for (auto& v : ev)
+      for (auto e : v)
+        if (e) (void)hipEventDestroy(e);
+    if (done_ev) (void)hipEventDestroy(done_ev);
   }
   void mark(int phase, bool end, hipStream_t s) {
     if (!profile) return;
-    hipEvent_t& e = ev[2 * phase + (end ? 1 : 0)];
-    if (!e) FMM_HIP(hipEventCreate(&e));
-    FMM_HIP(hipEventRecord(e, s));
-    if (end) ev_used[phase] = true;
+    auto& v = ev[phase];
+    const size_t k = 2 * (size_t)runs[phase] + (end ? 1 : 0);
+    if (end && !open_run[phase]) return;
+    while (v.size() <= k) {
+      hipEvent_t e = nullptr;
+      FMM_HIP(hipEventCreate(&e));
+      v.push_back(e);
+    }
+    FMM_HIP(hipEventRecord(v[k], s));
+    open_run[phase] = !end;
+    if (end) ++runs[phase];
   }
+};
+
+// Binds a C-ABI call on a plan to its stream: buffers that grow during the
+// call wait for that stream (not the whole device) before freeing their old
+// block, and the end of the call is recorded so that destroying the plan
+// waits for exactly the work queued on it.
+struct PlanCall {
+  Plan& P;
+  hipStream_t s, prev;
+  bool record;
+  PlanCall(Plan& plan, void* stream, bool record_end = true)
+      : P(plan), s((hipStream_t)stream), prev(g_op_stream), record(record_end) {
+    g_op_stream = s;
+  }
+  ~PlanCall() {
+    g_op_stream = prev;
+    if (record && P.done_ev && hipEventRecord(P.done_ev, s) == hipSuccess) P.done_recorded = true;
+  }
+  PlanCall(const PlanCall&) = delete;
+  PlanCall& operator=(const PlanCall&) = delete;
 };
 
 namespace {
@@ -765,6 +812,13 @@ __global__ __launch_bounds__(kM2LOrderThreads) void k_m2l_order(M2LOrderTable ot
     perm[ot.poff[li] + c * ng + q0 + i] = q0 + (int)(key[i] & 2047u);
 }
 
+// FP64 M2L.  use_svd: slots whose rank table entry is r > 0 are applied in
+// the reference's compressed form, M2 (M1 q) (helpers.py:37-45), in two MFMA
+// stages: T = M1 q (r rows, ceil(r/16) row blocks, K = PD), then
+// acc += M2 T with K = 4 ceil(r/4); the first stage's result registers (lane
+// (col, kq) holds rows 16 rb + 4 v + kq) are exactly the B operand of the
+// second stage's k-steps j = 4 rb + v in natural k order, so no data moves
+// between lanes.  Null slots (r = 0) add nothing, as in the reference.
 template <int PD>
 __global__ __launch_bounds__(kM2LThreads) void k_m2l_all(LevelTable lt,
                                                         const int* __restrict__ flags,
@@ -772,18 +826,24 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all(LevelTable lt,
                                                         const double* __restrict__ mult,
                                                         const double* __restrict__ ops,
                                                         int64_t opsz, double* __restrict__ ple,
-                                                        int64_t rstride, int64_t nstride) {
+                                                        int64_t rstride, int64_t nstride,
+                                                        int use_svd) {
   // density blockIdx.y: multipole rows rstride rows further, local rows nstride nodes further
   mult += (int64_t)blockIdx.y * rstride * PD;
   ple += (int64_t)blockIdx.y * nstride * PD;
   constexpr int KS = Mfma<PD>::KS, RB = Mfma<PD>::RB, LDA = Mfma<PD>::LDA;
+  constexpr int H = PD / 2, R1 = H / 16, K2 = H / 4;
   __shared__ __attribute__((aligned(16))) double buf[2][PD * LDA];
   __shared__ int srcs[27][kM2LCols];
   __shared__ int pslot[28];
+  __shared__ int srank[kNumM2L];
   __shared__ unsigned long long smask;
   const M2LSetup S = m2l_setup(lt, flags, m2l, srcs, pslot, smask);
   const int l = threadIdx.x & 63;
   const double* opsL = ops + (int64_t)lt.d[S.li] * opsz;
+  if (threadIdx.x < kNumM2L)
+    srank[threadIdx.x] = use_svd ? (int)opsL[ops_off_svd_rank(PD) + threadIdx.x] : -1;
+  __syncthreads();
   d4 acc[RB];
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb) acc[rb] = d4{0.0, 0.0, 0.0, 0.0};
@@ -792,7 +852,9 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all(LevelTable lt,
     double bn[KS], bc[KS];
     auto fetch = [&](int k) {
       const int i = pslot[k];
-      st.load(opsL + (int64_t)(OP_M2L + c_valid.slot[S.c][i]) * PD * PD);
+      const int slot = c_valid.slot[S.c][i];
+      st.load(srank[slot] > 0 ? opsL + ops_off_svd(PD) + (int64_t)slot * PD * PD
+                              : opsL + (int64_t)(OP_M2L + slot) * PD * PD);
       const int src = srcs[i][S.col];
       const double* brow = src >= 0 ? mult + (int64_t)src * PD + 2 * S.kq : nullptr;
 #pragma unroll
@@ -808,17 +870,49 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all(LevelTable lt,
     for (int i = 0; i < S.nph; ++i) {
       double* A = (i & 1) ? buf[1] : buf[0];
       st.store(A);
+      const int r = srank[c_valid.slot[S.c][pslot[i]]];   // uniform in the block
       if (i + 1 < S.nph) fetch(i + 1);
       __syncthreads();
       const double* a0 = A + (l & 15) * LDA + 2 * S.kq;
+      if (r < 0) {
 #pragma unroll
-      for (int m = 0; m < KS / 2; ++m)
+        for (int m = 0; m < KS / 2; ++m)
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb) {
-          const dv2 a = *reinterpret_cast<const dv2*>(a0 + rb * 16 * LDA + 8 * m);
-          acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bc[2 * m], acc[rb], 0, 0, 0);
-          acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bc[2 * m + 1], acc[rb], 0, 0, 0);
-        }
+          for (int rb = 0; rb < RB; ++rb) {
+            const dv2 a = *reinterpret_cast<const dv2*>(a0 + rb * 16 * LDA + 8 * m);
+            acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bc[2 * m], acc[rb], 0, 0, 0);
+            acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bc[2 * m + 1], acc[rb], 0, 0, 0);
+          }
+      } else if (r > 0) {
+        // stage 1: T = M1 q, row blocks rb1 < ceil(r / 16)
+        const int r1b = (r + 15) >> 4;
+        d4 t1[R1];
+#pragma unroll
+        for (int rb1 = 0; rb1 < R1; ++rb1) t1[rb1] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int m = 0; m < KS / 2; ++m)
+#pragma unroll
+          for (int rb1 = 0; rb1 < R1; ++rb1)
+            if (rb1 < r1b) {
+              const dv2 a = *reinterpret_cast<const dv2*>(a0 + rb1 * 16 * LDA + 8 * m);
+              t1[rb1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bc[2 * m], t1[rb1], 0, 0, 0);
+              t1[rb1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bc[2 * m + 1], t1[rb1], 0, 0, 0);
+            }
+        // stage 2: acc += M2 T, k-steps j < ceil(r / 4), k = 4 j + kq
+        const int nk2 = (r + 3) >> 2;
+        const double* a2 = A + H * LDA + S.kq;
+#pragma unroll
+        for (int j = 0; j < K2; ++j)
+          if (j < nk2) {
+            const double b = t1[j >> 2][j & 3];
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) {
+              const int row = 16 * rb + (l & 15);
+              const double a = a2[(row >> 1) * LDA + (row & 1) * H + 4 * j];
+              acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[rb], 0, 0, 0);
+            }
+          }
+      }
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) bc[ks] = bn[ks];
     }
@@ -842,6 +936,13 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all(LevelTable lt,
 // Mixed-precision M2L: PLE = sum_s [ R_s q_src (FP32 MFMA, FP32 accumulation
 //                                                within the slot)
 //                               + c_s * qs_src  (FP64) ], slots summed in FP64.
+// FRAME (Stokeslet fields): the two fields D_i whose definition holds the
+// position X relative to the root centre are translated in the target box's
+// frame instead, D_i^(c) = D_i + c_i D_2 with c the target centre (formed in
+// FP64 from the source's D_i and D_2 rows before rounding), so that the FP32
+// operands carry no c_i D_2 part; the downward pass returns to the root
+// frame in FP64 (k_local_level), where that part cancels exactly against
+// X_i(t) D_2 at the targets.
 // v_mfma_f32_16x16x4f32: lane l holds A[l%16][l/16], B[l/16][l%16] and
 // D[4(l/16) + v][l%16].
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -873,12 +974,16 @@ struct AStageF {
   }
 };
 
-template <int PD>
+template <int PD, bool FRAME>
 __global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
     LevelTable lt, const int* __restrict__ flags, const int* __restrict__ m2l,
     const double* __restrict__ mult, const double* __restrict__ qs,
     const double* __restrict__ ops, int64_t opsz, double* __restrict__ ple, int64_t rstride,
-    int64_t nstride) {
+    int64_t nstride, const double* __restrict__ xmid, const double* __restrict__ ymid,
+    double c0x, double c0y) {
+  // FRAME: fields 0 and 1 take the source's field-2 row (D_2) too
+  const bool frame = FRAME && blockIdx.y < 2;
+  const int64_t to2 = (int64_t)(2 - (int)blockIdx.y) * rstride;   // rows to field 2
   mult += (int64_t)blockIdx.y * rstride * PD;
   qs += (int64_t)blockIdx.y * rstride;
   ple += (int64_t)blockIdx.y * nstride * PD;
@@ -893,6 +998,8 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
   const M2LSetup S = m2l_setup(lt, flags, m2l, srcs, pslot, smask);
   const int l = threadIdx.x & 63;
   const double* opsL = ops + (int64_t)lt.d[S.li] * opsz;
+  double ct = 0.0;   // target centre coordinate i in the root frame
+  if (frame && S.active) ct = blockIdx.y == 0 ? xmid[S.g] - c0x : ymid[S.g] - c0y;
   const double* means = opsL + ops_off_means(PD);
   const float* r32 = reinterpret_cast<const float*>(opsL + ops_off_r32(PD));
   double acc64[RB][4];
@@ -910,13 +1017,25 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
       if (threadIdx.x < PD) mn = means[c_valid.slot[S.c][i] * PD + threadIdx.x];
       const int src = srcs[i][S.col];
       const double* brow = src >= 0 ? mult + (int64_t)src * PD + 2 * S.kq : nullptr;
+      if (frame) {
 #pragma unroll
-      for (int m = 0; m < KS / 2; ++m) {
-        const dv2 v = brow ? *reinterpret_cast<const dv2*>(brow + 8 * m) : dv2{0.0, 0.0};
-        bn[2 * m] = (float)v.x;
-        bn[2 * m + 1] = (float)v.y;
+        for (int m = 0; m < KS / 2; ++m) {
+          const dv2 v = brow ? *reinterpret_cast<const dv2*>(brow + 8 * m) : dv2{0.0, 0.0};
+          const dv2 v2 = brow ? *reinterpret_cast<const dv2*>(brow + to2 * PD + 8 * m)
+                              : dv2{0.0, 0.0};
+          bn[2 * m] = (float)__builtin_fma(ct, v2.x, v.x);
+          bn[2 * m + 1] = (float)__builtin_fma(ct, v2.y, v.y);
+        }
+        qn = src >= 0 ? __builtin_fma(ct, qs[to2 + src], qs[src]) : 0.0;
+      } else {
+#pragma unroll
+        for (int m = 0; m < KS / 2; ++m) {
+          const dv2 v = brow ? *reinterpret_cast<const dv2*>(brow + 8 * m) : dv2{0.0, 0.0};
+          bn[2 * m] = (float)v.x;
+          bn[2 * m + 1] = (float)v.y;
+        }
+        qn = src >= 0 ? qs[src] : 0.0;
       }
-      qn = src >= 0 ? qs[src] : 0.0;
     };
     fetch(0);
 #pragma unroll
@@ -975,6 +1094,18 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
 // Downward, one level: PLE += L2L(parent LE) (parent width's L2LC,
 // fmm.py:142,162), LE = pinv2 * (pinv1 * PLE)  (fmm.py:156).  Tiled by child
 // position c like the M2L.
+// Return of the Stokeslet fields D_0, D_1 from the target-box frame of the
+// FP32 M2L to the root frame: PLE_i -= c_i PLE_2, c the box centre relative to
+// the root centre (xmid null: no change).
+struct FrameFix {
+  const double* xmid;
+  const double* ymid;
+  double c0x, c0y;
+  __device__ __forceinline__ double centre(int g) const {
+    return blockIdx.y == 0 ? xmid[g] - c0x : ymid[g] - c0y;
+  }
+};
+
 template <int PD>
 __global__ __launch_bounds__(256) void k_local_level(int g0, int ngroups, int do_l2l,
                                                     const int* __restrict__ flags,
@@ -982,7 +1113,9 @@ __global__ __launch_bounds__(256) void k_local_level(int g0, int ngroups, int do
                                                     const double* __restrict__ ple,
                                                     const double* __restrict__ opsL,
                                                     const double* __restrict__ opsParent,
-                                                    double* __restrict__ le, int64_t nstride) {
+                                                    double* __restrict__ le, int64_t nstride,
+                                                    FrameFix fx) {
+  const double* ple2 = ple + 2 * nstride * PD;   // field 2 (FrameFix)
   ple += (int64_t)blockIdx.y * nstride * PD;
   le += (int64_t)blockIdx.y * nstride * PD;
   __shared__ __attribute__((aligned(16))) double buf[2][PD * Mfma<PD>::LDA];
@@ -993,6 +1126,15 @@ __global__ __launch_bounds__(256) void k_local_level(int g0, int ngroups, int do
   const bool active = q < ngroups && !(flags[q < ngroups ? g : g0] & NF_FAKE);
   d4 acc[PD / 16];
   load_acc<PD>(acc, active ? ple + (int64_t)g * PD : nullptr);
+  if (fx.xmid && blockIdx.y < 2 && active) {
+    d4 acc2[PD / 16];
+    load_acc<PD>(acc2, ple2 + (int64_t)g * PD);
+    const double ct = fx.centre(g);
+#pragma unroll
+    for (int rb = 0; rb < PD / 16; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[rb][v] = __builtin_fma(-ct, acc2[rb][v], acc[rb][v]);
+  }
   const int pg = (active && do_l2l) ? parent[g] : -1;
   const int nl = do_l2l ? 1 : 0;
   int i = 0;
@@ -1131,9 +1273,10 @@ __global__ __launch_bounds__(64 * (PD / 16)) void k_local_level_small(
     int g0, int ngroups, int do_l2l, const int* __restrict__ flags,
     const int* __restrict__ parent, const double* __restrict__ ple,
     const double* __restrict__ opsL, const double* __restrict__ opsParent,
-    double* __restrict__ le, int64_t nstride) {
+    double* __restrict__ le, int64_t nstride, FrameFix fx) {
   __shared__ __attribute__((aligned(16))) double abuf[2][PD * Mfma<PD>::LDA];
   __shared__ double bbuf[2][PD][17];
+  const double* ple2 = ple + 2 * nstride * PD;   // field 2 (FrameFix)
   ple += (int64_t)blockIdx.y * nstride * PD;
   le += (int64_t)blockIdx.y * nstride * PD;
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1149,6 +1292,12 @@ __global__ __launch_bounds__(64 * (PD / 16)) void k_local_level_small(
   const double* src = ple + (int64_t)gs * PD;
 #pragma unroll
   for (int v = 0; v < 4; ++v) acc[v] = src[16 * w + 4 * v + kq];
+  if (fx.xmid && blockIdx.y < 2) {
+    const double* src2 = ple2 + (int64_t)gs * PD;
+    const double ct = fx.centre(gs);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v] = __builtin_fma(-ct, src2[16 * w + 4 * v + kq], acc[v]);
+  }
 #pragma unroll
   for (int v = 0; v < 4; ++v) acc[v] = active ? acc[v] : 0.0;
   const int pg = (active && do_l2l) ? par : -1;
@@ -2460,15 +2609,19 @@ static void m2l_launch(Plan& P, const M2LTables& T, int l0, int l1, hipStream_t 
     nb *= 2;
   }
   if (P.precision != PREC_FP64) {
-    auto kern = PD == 32 ? k_m2l_all_f32<32> : k_m2l_all_f32<64>;
+    const bool frame = P.kernel_id == KER_STOKES;
+    auto kern = PD == 32 ? (frame ? k_m2l_all_f32<32, true> : k_m2l_all_f32<32, false>)
+                         : (frame ? k_m2l_all_f32<64, true> : k_m2l_all_f32<64, false>);
     hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(kM2LThreads), 0, s, lt, t.flags.as<int>(),
                        t.m2l.as<int>(), P.mult.as<double>(), P.qs.as<double>(),
-                       P.ops.as<double>(), opsz, P.ple.as<double>(), rstride, nstride);
+                       P.ops.as<double>(), opsz, P.ple.as<double>(), rstride, nstride,
+                       t.xmid.as<double>(), t.ymid.as<double>(), t.xmin + 0.5 * t.w0,
+                       t.ymin + 0.5 * t.w0);
   } else {
     auto kern = PD == 32 ? k_m2l_all<32> : k_m2l_all<64>;
     hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(kM2LThreads), 0, s, lt, t.flags.as<int>(),
                        t.m2l.as<int>(), P.mult.as<double>(), P.ops.as<double>(), opsz,
-                       P.ple.as<double>(), rstride, nstride);
+                       P.ple.as<double>(), rstride, nstride, P.m2l_mode == FMM2D_M2L_SVD ? 1 : 0);
   }
   FMM_LAUNCH_CHECK();
 }
@@ -2587,7 +2740,13 @@ static void downward(Plan& P, hipStream_t s) {
   }
   const M2LTables T = m2l_tables(P);
   m2l_prepare(P, T, s);
+  P.mark(FMM2D_PHASE_M2L, false, s);
   m2l_launch(P, T, 0, T.lt.nlev, s);
+  P.mark(FMM2D_PHASE_M2L, true, s);
+  FrameFix fx{nullptr, nullptr, 0.0, 0.0};
+  if (P.kernel_id == KER_STOKES && P.precision != PREC_FP64)
+    fx = FrameFix{t.xmid.as<double>(), t.ymid.as<double>(), t.xmin + 0.5 * t.w0,
+                  t.ymin + 0.5 * t.w0};
   for (int d = 2; d < t.L; ++d) {
     const int ngroups = (int)(t.lev[d].n_nodes / 4);
     if (ngroups == 0) continue;
@@ -2597,7 +2756,7 @@ static void downward(Plan& P, hipStream_t s) {
       hipLaunchKernelGGL(ks, dim3(4 * ceil_div(ngroups, 16), nd), dim3(64 * (PD / 16)), 0, s,
                          (int)t.lev[d].node_off, ngroups, d >= 3 ? 1 : 0, t.flags.as<int>(),
                          t.parent.as<int>(), P.ple.as<double>(), opsL, opsL - opsz,
-                         P.le.as<double>(), nstride);
+                         P.le.as<double>(), nstride, fx);
       FMM_LAUNCH_CHECK();
       continue;
     }
@@ -2605,14 +2764,83 @@ static void downward(Plan& P, hipStream_t s) {
     hipLaunchKernelGGL(kern, dim3(4 * ceil_div(ngroups, 64), nd), dim3(256), 0, s,
                        (int)t.lev[d].node_off, ngroups, d >= 3 ? 1 : 0, t.flags.as<int>(),
                        t.parent.as<int>(), P.ple.as<double>(), opsL, opsL - opsz,
-                       P.le.as<double>(), nstride);
+                       P.le.as<double>(), nstride, fx);
     FMM_LAUNCH_CHECK();
   }
   P.have_exp = true;
 }
 
+// Target-side scratch of one evaluation: target boxes, per-box counts and
+// offsets, targets in leaf order and the work items.
+struct EvalScratch {
+  int *tnode, *tcnt, *toff, *tcur, *tperm, *icnt, *ioff, *ntot, *partials;
+  double *tsx, *tsy;
+  int4* items;
+};
+
+constexpr int64_t eval_item_cap(int64_t ne, int64_t nt) { return ne + nt / kTgtChunk + 1; }
+
+// The plan's own scratch, grown to nt targets and ne evaluation boxes.
+static EvalScratch plan_scratch(Plan& P, int64_t nt, int64_t ne) {
+  P.tnode.ensure(nt * sizeof(int));
+  P.tcnt.ensure((ne + 1) * sizeof(int));
+  P.toff.ensure((ne + 1) * sizeof(int));
+  P.tcur.ensure((nt + 1) * sizeof(int));   // per-target rank in its box
+  P.tperm.ensure(nt * sizeof(int));
+  P.tsx.ensure(nt * sizeof(double));
+  P.tsy.ensure(nt * sizeof(double));
+  P.icnt.ensure((ne + 1) * sizeof(int));
+  P.ioff.ensure((ne + 1) * sizeof(int));
+  P.items.ensure(eval_item_cap(ne, nt) * sizeof(int4));
+  P.ntot.ensure(4 * sizeof(int));
+  EvalScratch w;
+  w.tnode = P.tnode.as<int>();
+  w.tcnt = P.tcnt.as<int>();
+  w.toff = P.toff.as<int>();
+  w.tcur = P.tcur.as<int>();
+  w.tperm = P.tperm.as<int>();
+  w.icnt = P.icnt.as<int>();
+  w.ioff = P.ioff.as<int>();
+  w.ntot = P.ntot.as<int>();
+  w.partials = nullptr;   // scan partials from the plan's PrimWork
+  w.tsx = P.tsx.as<double>();
+  w.tsy = P.tsy.as<double>();
+  w.items = P.items.as<int4>();
+  return w;
+}
+
+// Leaf-relative FP32 sources and per-leaf charge sums of the mixed-precision
+// near field; formed once per set of expansions, by the downward pass, so
+// that evaluations never allocate plan state.
+static bool uses_mixed_near_field(const Plan& P) {
+  return P.precision == PREC_MIXED && P.kernel_id == KER_LAPLACE;
+}
+
+static LevelConsts level_consts(const Tree& t) {
+  LevelConsts lc{};
+  for (int d = 0; d < t.L; ++d) {
+    lc.invw[d] = 1.0 / t.lev[d].width;
+    lc.logw2[d] = std::log(t.lev[d].width * t.lev[d].width);
+  }
+  return lc;
+}
+
+static void prep_mixed(Plan& P, hipStream_t s) {
+  Tree& t = P.tree;
+  const LevelConsts lc = level_consts(t);
+  P.srcf.ensure((t.n + 1) * sizeof(f4));
+  P.tsum.ensure((t.n_nodes + 1) * sizeof(double));
+  if (t.n_eval > 0)
+    hipLaunchKernelGGL(k_prep_mixed, dim3(t.n_eval), dim3(64), 0, s, t.evalnode.as<int>(),
+                       t.nlevel.as<int>(), t.bot.as<int>(), t.top.as<int>(),
+                       t.xmid.as<double>(), t.ymid.as<double>(), P.src.as<double4>(), lc,
+                       P.srcf.as<f4>(), P.tsum.as<double>());
+  FMM_LAUNCH_CHECK();
+  P.have_srcf = true;
+}
+
 static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, int ev, int check,
-                     double* out, hipStream_t s) {
+                     double* out, hipStream_t s, const EvalScratch* given) {
   Tree& t = P.tree;
   FMM_REQUIRE(P.have_exp, FMM_E_STATE, "no expansions: call fmm2d_upward and fmm2d_downward first");
   const bool stokes = P.kernel_id == KER_STOKES;
@@ -2630,19 +2858,10 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   FMM_REQUIRE(nt >= 0 && nt < ((int64_t)1 << 31), FMM_E_LIMIT, "too many targets");
   if (nt == 0) return;
   const int64_t ne = t.n_eval;
-  P.tnode.ensure(nt * sizeof(int));
-  P.tcnt.ensure((ne + 1) * sizeof(int));
-  P.toff.ensure((ne + 1) * sizeof(int));
-  P.tcur.ensure((nt + 1) * sizeof(int));   // per-target rank in its box
-  P.tperm.ensure(nt * sizeof(int));
-  P.tsx.ensure(nt * sizeof(double));
-  P.tsy.ensure(nt * sizeof(double));
-  P.icnt.ensure((ne + 1) * sizeof(int));
-  P.ioff.ensure((ne + 1) * sizeof(int));
-  const int64_t icap = ne + nt / kTgtChunk + 1;
-  P.items.ensure(icap * sizeof(int4));
-  P.ntot.ensure(4 * sizeof(int));
-  FMM_HIP(hipMemsetAsync(P.tcnt.ptr, 0, (ne + 1) * sizeof(int), s));
+  const EvalScratch w = given ? *given : plan_scratch(P, nt, ne);
+  P.last_tcnt = w.tcnt;
+  const int64_t icap = eval_item_cap(ne, nt);
+  fill_i32(w.tcnt, ne + 1, 0, s);
   LocateTable lt{};
   lt.L = t.L;
   lt.xmin = t.xmin;
@@ -2656,22 +2875,22 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   lt.code = t.code.as<uint64_t>();
   hipLaunchKernelGGL(k_locate_count, dim3(grid256(nt)), dim3(256), 0, s, tx, ty, nt, lt,
                      t.flags.as<int>(), t.child.as<int>(), t.xmid.as<double>(),
-                     t.ymid.as<double>(), t.evalid.as<int>(), P.tnode.as<int>(),
-                     P.tcur.as<int>(), P.tcnt.as<int>());
+                     t.ymid.as<double>(), t.evalid.as<int>(), w.tnode,
+                     w.tcur, w.tcnt);
   FMM_LAUNCH_CHECK();
   // toff has ne + 1 entries: scanning ne + 1 counts (the last is zero) gives toff[ne] = nt.
-  exclusive_scan_i32(P.tcnt.as<int>(), P.toff.as<int>(), ne + 1, nullptr, P.pw, s);
+  exclusive_scan_i32(w.tcnt, w.toff, ne + 1, nullptr, P.pw, s, w.partials);
   hipLaunchKernelGGL(k_target_scatter, dim3(grid256(nt)), dim3(256), 0, s, tx, ty, nt,
-                     P.tnode.as<int>(), P.tcur.as<int>(), P.toff.as<int>(), P.tperm.as<int>(),
-                     P.tsx.as<double>(), P.tsy.as<double>());
+                     w.tnode, w.tcur, w.toff, w.tperm,
+                     w.tsx, w.tsy);
   FMM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_item_counts, dim3(grid256(ne)), dim3(256), 0, s, P.tcnt.as<int>(), (int)ne,
-                     P.icnt.as<int>());
+  hipLaunchKernelGGL(k_item_counts, dim3(grid256(ne)), dim3(256), 0, s, w.tcnt, (int)ne,
+                     w.icnt);
   FMM_LAUNCH_CHECK();
-  exclusive_scan_i32(P.icnt.as<int>(), P.ioff.as<int>(), ne, P.ntot.as<int>(), P.pw, s);
-  hipLaunchKernelGGL(k_make_items, dim3(grid256(ne)), dim3(256), 0, s, P.icnt.as<int>(),
-                     P.ioff.as<int>(), (int)ne, t.evalnode.as<int>(), P.toff.as<int>(),
-                     P.items.as<int4>());
+  exclusive_scan_i32(w.icnt, w.ioff, ne, w.ntot, P.pw, s, w.partials);
+  hipLaunchKernelGGL(k_make_items, dim3(grid256(ne)), dim3(256), 0, s, w.icnt,
+                     w.ioff, (int)ne, t.evalnode.as<int>(), w.toff,
+                     w.items);
   FMM_LAUNCH_CHECK();
   const int PD = P.PD;
   const int64_t opsz = ops_per_level(PD);
@@ -2682,9 +2901,9 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
                  : (PD == 32 ? (check ? k_eval_stokes<32, true, false> : k_eval_stokes<32, false, false>)
                              : (check ? k_eval_stokes<64, true, false> : k_eval_stokes<64, false, false>));
     P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
-    hipLaunchKernelGGL(sk, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, P.items.as<int4>(), P.ntot.as<int>(),
-                       t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(),
-                       P.tsx.as<double>(), P.tsy.as<double>(), t.nlevel.as<int>(),
+    hipLaunchKernelGGL(sk, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, w.items, w.ntot,
+                       t.evalnode.as<int>(), w.toff, w.tperm,
+                       w.tsx, w.tsy, t.nlevel.as<int>(),
                        t.xmid.as<double>(), t.ymid.as<double>(), t.coll.as<int>(), t.crange.as<int2>(),
                        t.bot.as<int>(), t.top.as<int>(), P.src.as<double4>(),
                        dl ? P.sdip.as<double4>() : nullptr, P.le.as<double>(),
@@ -2699,30 +2918,21 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   // are latency-bound on short items, and it is the more accurate of the two
   const bool few_targets = ne > 0 && nt < (int64_t)kFewTargets * ne;
   if (P.precision == PREC_MIXED && ev == EVAL_POT && P.kernel_id == KER_LAPLACE && !few_targets) {
-    LevelConsts lc{};
-    for (int d = 0; d < t.L; ++d) {
-      lc.invw[d] = 1.0 / t.lev[d].width;
-      lc.logw2[d] = std::log(t.lev[d].width * t.lev[d].width);
-    }
+    const LevelConsts lc = level_consts(t);
     if (!P.have_srcf) {
-      P.srcf.ensure((t.n + 1) * sizeof(f4));
-      P.tsum.ensure((t.n_nodes + 1) * sizeof(double));
-      if (t.n_eval > 0)
-        hipLaunchKernelGGL(k_prep_mixed, dim3(t.n_eval), dim3(64), 0, s, t.evalnode.as<int>(),
-                           t.nlevel.as<int>(), t.bot.as<int>(), t.top.as<int>(),
-                           t.xmid.as<double>(), t.ymid.as<double>(), P.src.as<double4>(), lc,
-                           P.srcf.as<f4>(), P.tsum.as<double>());
-      FMM_LAUNCH_CHECK();
-      P.have_srcf = true;
+      FMM_REQUIRE(!given, FMM_E_STATE,
+                  "the mixed-precision near field is prepared by fmm2d_downward: call it "
+                  "before an evaluation that allocates nothing");
+      prep_mixed(P, s);
     }
     auto mk = P.cheb ? (check ? (PD == 32 ? k_eval_mixed<32, true, true> : k_eval_mixed<64, true, true>)
                               : (PD == 32 ? k_eval_mixed<32, false, true> : k_eval_mixed<64, false, true>))
                      : (check ? (PD == 32 ? k_eval_mixed<32, true, false> : k_eval_mixed<64, true, false>)
                               : (PD == 32 ? k_eval_mixed<32, false, false> : k_eval_mixed<64, false, false>));
     P.mark(FMM2D_PHASE_EVAL_KERNEL, false, s);
-    hipLaunchKernelGGL(mk, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, P.items.as<int4>(), P.ntot.as<int>(),
-                       t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(),
-                       P.tsx.as<double>(), P.tsy.as<double>(), t.nlevel.as<int>(),
+    hipLaunchKernelGGL(mk, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), 0, s, w.items, w.ntot,
+                       t.evalnode.as<int>(), w.toff, w.tperm,
+                       w.tsx, w.tsy, t.nlevel.as<int>(),
                        t.flags.as<int>(), t.xmid.as<double>(), t.ymid.as<double>(),
                        t.coll.as<int>(), t.crange.as<int2>(), t.bot.as<int>(), t.top.as<int>(), P.src.as<double4>(),
                        P.srcf.as<f4>(), P.tsum.as<double>(), lc, P.le.as<double>(),
@@ -2741,9 +2951,9 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   // full-wave items occupancy)
   const int scap = (ev != EVAL_DIPOLE && few_targets) ? kStageSources : 0;
   const size_t dyn = (size_t)scap * kEvalWaves * sizeof(double4);
-  hipLaunchKernelGGL(kern, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), dyn, s, P.items.as<int4>(), P.ntot.as<int>(),
-                     t.evalnode.as<int>(), P.toff.as<int>(), P.tperm.as<int>(), P.tsx.as<double>(),
-                     P.tsy.as<double>(), t.nlevel.as<int>(), t.xmid.as<double>(),
+  hipLaunchKernelGGL(kern, dim3(eval_grid(icap)), dim3(64 * kEvalWaves), dyn, s, w.items, w.ntot,
+                     t.evalnode.as<int>(), w.toff, w.tperm, w.tsx,
+                     w.tsy, t.nlevel.as<int>(), t.xmid.as<double>(),
                      t.ymid.as<double>(), t.coll.as<int>(), t.crange.as<int2>(), t.bot.as<int>(), t.top.as<int>(),
                      P.src.as<double4>(), ev == EVAL_DIPOLE ? P.dip.as<double2>() : nullptr,
                      P.le.as<double>(), P.ops.as<double>(), opsz, P.cheb ? P.cheb : P.p,
@@ -2813,6 +3023,7 @@ void create_plan(int kernel_id, const double* kparams, int nkparams, int p, int 
     P->PD = p <= 32 ? 32 : 64;
     P->precision = precision;
     FMM_HIP(hipGetDevice(&P->device));
+    FMM_HIP(hipEventCreateWithFlags(&P->done_ev, hipEventDisableTiming));
     fmm::SlotTable st;
     for (int s = 0; s < fmm::kNumM2L; ++s) fmm::m2l_slot_offset(s, &st.ox[s], &st.oy[s]);
     FMM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(fmm::c_slots), &st, sizeof st));
@@ -2864,8 +3075,9 @@ int fmm2d_plan_create_cheb(int kernel_id, const double* kparams, int nkparams, i
 int fmm2d_plan_destroy(void* plan) {
   ABI_BEGIN
   if (plan) {
-    (void)hipDeviceSynchronize();
-    delete static_cast<Plan*>(plan);
+    Plan* P = static_cast<Plan*>(plan);
+    if (P->done_recorded) (void)hipEventSynchronize(P->done_ev);
+    delete P;
   }
   ABI_END
 }
@@ -2874,6 +3086,7 @@ int fmm2d_tree_build(void* plan, const double* x, const double* y, int64_t n, co
                      int64_t ncut, void* stream) {
   ABI_BEGIN
   Plan* P = as_plan(plan);
+  fmm::PlanCall call(*P, stream);
   NEED(x);
   NEED(y);
   NEED(bbox);
@@ -2917,6 +3130,7 @@ int fmm2d_tree_info(void* plan, int64_t* out, int64_t nout) {
 int fmm2d_tree_export(void* plan, int which, void* dst, int64_t bytes, void* stream) {
   ABI_BEGIN
   Plan* P = as_plan(plan);
+  fmm::PlanCall call(*P, stream);
   NEED(dst);
   const fmm::Tree& t = P->tree;
   FMM_REQUIRE(t.built, fmm::FMM_E_STATE, "no tree");
@@ -2958,6 +3172,7 @@ int64_t fmm2d_operator_size(int p) {
 int fmm2d_set_operators(void* plan, int level, const double* dev_ops, int64_t n, void* stream) {
   ABI_BEGIN
   Plan* P = as_plan(plan);
+  fmm::PlanCall call(*P, stream);
   NEED(dev_ops);
   const int64_t opsz = fmm::ops_per_level(P->PD);
   FMM_REQUIRE(n == opsz, fmm::FMM_E_ARG, "operator block has %lld values, expected %lld",
@@ -2994,6 +3209,7 @@ int fmm2d_clear_operators(void* plan) {
 int fmm2d_upward(void* plan, const double* tau, int64_t n, void* stream) {
   ABI_BEGIN
   Plan* P = as_plan(plan);
+  fmm::PlanCall call(*P, stream);
   NEED(tau);
   FMM_REQUIRE(P->tree.built, fmm::FMM_E_STATE, "no tree");
   FMM_REQUIRE(n == P->tree.n, fmm::FMM_E_ARG, "tau has %lld entries, tree has %lld sources",
@@ -3008,6 +3224,7 @@ int fmm2d_upward_dipole(void* plan, const double* tau, const double* dvec, int64
                         void* stream) {
   ABI_BEGIN
   Plan* P = as_plan(plan);
+  fmm::PlanCall call(*P, stream);
   NEED(tau);
   NEED(dvec);
   FMM_REQUIRE(P->kernel_id == fmm::KER_LAPLACE || P->kernel_id == fmm::KER_STOKES,
@@ -3026,9 +3243,11 @@ int fmm2d_upward_dipole(void* plan, const double* tau, const double* dvec, int64
 int fmm2d_downward(void* plan, void* stream) {
   ABI_BEGIN
   Plan* P = as_plan(plan);
+  fmm::PlanCall call(*P, stream);
   FMM_REQUIRE(P->tree.built && P->up_done, fmm::FMM_E_STATE, "call fmm2d_upward first");
   P->mark(FMM2D_PHASE_DOWNWARD, false, (hipStream_t)stream);
   fmm::downward(*P, (hipStream_t)stream);
+  if (fmm::uses_mixed_near_field(*P)) fmm::prep_mixed(*P, (hipStream_t)stream);
   P->mark(FMM2D_PHASE_DOWNWARD, true, (hipStream_t)stream);
   ABI_END
 }
@@ -3037,13 +3256,14 @@ int fmm2d_evaluate(void* plan, const double* tx, const double* ty, int64_t nt, i
                    int check, double* out, void* stream) {
   ABI_BEGIN
   Plan* P = as_plan(plan);
+  fmm::PlanCall call(*P, stream);
   if (nt != 0) {   // an empty target set may come with null pointers
     NEED(tx);
     NEED(ty);
     NEED(out);
   }
   P->mark(FMM2D_PHASE_EVALUATE, false, (hipStream_t)stream);
-  fmm::evaluate(*P, tx, ty, nt, evaluator, check, out, (hipStream_t)stream);
+  fmm::evaluate(*P, tx, ty, nt, evaluator, check, out, (hipStream_t)stream, nullptr);
   P->mark(FMM2D_PHASE_EVALUATE, true, (hipStream_t)stream);
   ABI_END
 }
@@ -3058,9 +3278,9 @@ int fmm2d_work_counts(void* plan, int64_t* out, int n) {
   fmm::DevBuf acc;
   acc.ensure(16 * sizeof(unsigned long long));
   FMM_HIP(hipMemset(acc.ptr, 0, 16 * sizeof(unsigned long long)));
-  if (P->tcnt.ptr && P->have_exp)
+  if (P->last_tcnt && P->have_exp)
     hipLaunchKernelGGL(fmm::k_eval_stats, dim3(fmm::grid256(t.n_eval)), dim3(256), 0, 0,
-                       t.evalnode.as<int>(), (int)t.n_eval, P->tcnt.as<int>(), t.coll.as<int>(),
+                       t.evalnode.as<int>(), (int)t.n_eval, P->last_tcnt, t.coll.as<int>(),
                        t.bot.as<int>(), t.top.as<int>(), t.nlevel.as<int>(),
                        acc.as<unsigned long long>());
   hipLaunchKernelGGL(fmm::k_tree_stats, dim3(fmm::grid256(t.n_nodes)), dim3(256), 0, 0,
@@ -3075,11 +3295,23 @@ int fmm2d_work_counts(void* plan, int64_t* out, int n) {
   ABI_END
 }
 
+int fmm2d_set_m2l_mode(void* plan, int mode) {
+  ABI_BEGIN
+  Plan* P = as_plan(plan);
+  FMM_REQUIRE(mode == FMM2D_M2L_SVD || mode == FMM2D_M2L_DENSE, fmm::FMM_E_ARG,
+              "unknown M2L mode %d", mode);
+  P->m2l_mode = mode;
+  ABI_END
+}
+
 int fmm2d_set_profiling(void* plan, int enable) {
   ABI_BEGIN
   Plan* P = as_plan(plan);
   P->profile = enable != 0;
-  for (auto& u : P->ev_used) u = false;
+  for (int i = 0; i < FMM2D_NUM_PHASES; ++i) {
+    P->runs[i] = 0;
+    P->open_run[i] = false;
+  }
   ABI_END
 }
 
@@ -3091,11 +3323,17 @@ int fmm2d_phase_times(void* plan, double* ms, int n) {
               FMM2D_NUM_PHASES);
   for (int i = 0; i < FMM2D_NUM_PHASES; ++i) {
     ms[i] = -1.0;
-    if (!P->ev_used[i]) continue;
-    FMM_HIP(hipEventSynchronize(P->ev[2 * i + 1]));
-    float t = 0.f;
-    FMM_HIP(hipEventElapsedTime(&t, P->ev[2 * i], P->ev[2 * i + 1]));
-    ms[i] = t;
+    const int r = P->runs[i];
+    if (r == 0) continue;
+    double sum = 0.0;
+    for (int k = 0; k < r; ++k) {
+      FMM_HIP(hipEventSynchronize(P->ev[i][2 * k + 1]));
+      float t = 0.f;
+      FMM_HIP(hipEventElapsedTime(&t, P->ev[i][2 * k], P->ev[i][2 * k + 1]));
+      sum += t;
+    }
+    ms[i] = sum / r;
+    P->runs[i] = 0;
   }
   ABI_END
 }
@@ -3103,6 +3341,7 @@ int fmm2d_phase_times(void* plan, double* ms, int n) {
 int fmm2d_export_expansions(void* plan, int which, double* dst, int64_t nvals, void* stream) {
   ABI_BEGIN
   Plan* P = as_plan(plan);
+  fmm::PlanCall call(*P, stream);
   NEED(dst);
   const fmm::Tree& t = P->tree;
   FMM_REQUIRE(t.built, fmm::FMM_E_STATE, "no tree");

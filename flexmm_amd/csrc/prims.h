@@ -60,9 +60,15 @@ struct PrimWork {
 };
 
 // out[i] = sum_{j<i} in[i]; *total_dev (optional) = sum of all in.
-// in and out may alias.
+// in and out may alias.  partials (optional): caller-owned space for
+// scan_partials_i32(n) ints; without it the block partials live in w.
 void exclusive_scan_i32(const int* in, int* out, int64_t n, int* total_dev, PrimWork& w,
-                        hipStream_t s);
+                        hipStream_t s, int* partials = nullptr);
+int64_t scan_partials_i32(int64_t n);
+
+// p[0..n) = v with a fill kernel (no memset node, so the call can sit in a
+// captured HIP graph).
+void fill_i32(int* p, int64_t n, int v, hipStream_t s);
 
 // Sort (keys, vals) of length n by (key, val) ascending, in place.  Segments
 // whose length drops to <= threshold are finished by an LDS sort; the radix

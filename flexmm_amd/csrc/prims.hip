@@ -113,10 +113,23 @@ __global__ __launch_bounds__(1024) void k_scan_small(const int* in, int* out, in
 
 }  // namespace
 
+__global__ __launch_bounds__(256) void k_fill_i32(int* __restrict__ p, int64_t n, int v) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void fill_i32(int* p, int64_t n, int v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_i32, dim3(ceil_div(n, 256)), dim3(256), 0, s, p, n, v);
+  FMM_LAUNCH_CHECK();
+}
+
+int64_t scan_partials_i32(int64_t n) { return n <= kScanSmallMax ? 0 : ceil_div(n, SCAN_TILE); }
+
 void exclusive_scan_i32(const int* in, int* out, int64_t n, int* total_dev, PrimWork& w,
-                        hipStream_t s) {
+                        hipStream_t s, int* partials) {
   if (n <= 0) {
-    if (total_dev) FMM_HIP(hipMemsetAsync(total_dev, 0, sizeof(int), s));
+    if (total_dev) fill_i32(total_dev, 1, 0, s);
     return;
   }
   if (n <= kScanSmallMax) {
@@ -125,8 +138,11 @@ void exclusive_scan_i32(const int* in, int* out, int64_t n, int* total_dev, Prim
     return;
   }
   int64_t nb = ceil_div(n, SCAN_TILE);
-  w.blocksums.ensure(nb * sizeof(int));
-  int* sums = w.blocksums.as<int>();
+  int* sums = partials;
+  if (!sums) {
+    w.blocksums.ensure(nb * sizeof(int));
+    sums = w.blocksums.as<int>();
+  }
   hipLaunchKernelGGL(k_scan_block_sums, dim3(nb), dim3(SCAN_BS), 0, s, in, n, sums);
   FMM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nb, total_dev);

@@ -94,11 +94,13 @@ __global__ __launch_bounds__(BS) void k_point_keys(const double* __restrict__ x,
                                                    double xmin0, double ymin0, double w0,
                                                    uint64_t* __restrict__ keys,
                                                    uint32_t* __restrict__ vals,
-                                                   double2* __restrict__ xy) {
+                                                   double2* __restrict__ xy,
+                                                   int* __restrict__ nonfinite) {
   int i = blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
   const double px = x[i], py = y[i];
   xy[i] = make_double2(px, py);
+  if (!(__builtin_isfinite(px) && __builtin_isfinite(py))) atomicOr(nonfinite, 1);
   double xmin = xmin0, ymin = ymin0, w = w0;
   uint64_t key = 0;
   for (int l = 1; l <= kMaxDepth; ++l) {
@@ -792,8 +794,16 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   uint64_t* keys = t.keys.as<uint64_t>();
   uint32_t* perm = t.perm.as<uint32_t>();
   t.xy.ensure(n * sizeof(double2));
+  // -- counters (zeroed before the first kernel, which flags non-finite points)
+  enum { CNT_D = 0, CNT_P = 32, CNT_R = 33, CNT_PC = 34, CNT_ROWS = 66, CNT_EVAL = 67,
+         CNT_UP = 68, CNT_UNIQ = 69, CNT_LR = 72, CNT_BAL = 104, CNT_BAD = 136, CNT_N = 137 };
+  t.w_cnt.ensure(CNT_N * sizeof(int));
+  t.h_cnt.ensure(CNT_N * sizeof(int));
+  int* cnt = t.w_cnt.as<int>();
+  int* hc = t.h_cnt.as<int>();
+  FMM_HIP(hipMemsetAsync(cnt, 0, CNT_N * sizeof(int), s));
   hipLaunchKernelGGL(k_point_keys, dim3(grid_for(n)), dim3(BS), 0, s, x, y, nn, xmin, ymin, w0,
-                     keys, perm, t.xy.as<double2>());
+                     keys, perm, t.xy.as<double2>(), cnt + CNT_BAD);
   FMM_LAUNCH_CHECK();
   // Segments of at most Ncutoff points that end in the last radix round sit
   // at the deepest radix depth D, which no cell of the tree goes below (2:1
@@ -806,14 +816,6 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
                      nn, t.sx.as<double>(), t.sy.as<double>());
   FMM_LAUNCH_CHECK();
 
-  // -- counters ------------------------------------------------------------------
-  enum { CNT_D = 0, CNT_P = 32, CNT_R = 33, CNT_PC = 34, CNT_ROWS = 66, CNT_EVAL = 67,
-         CNT_UP = 68, CNT_UNIQ = 69, CNT_LR = 72, CNT_BAL = 104, CNT_N = 136 };
-  t.w_cnt.ensure(CNT_N * sizeof(int));
-  t.h_cnt.ensure(CNT_N * sizeof(int));
-  int* cnt = t.w_cnt.as<int>();
-  int* hc = t.h_cnt.as<int>();
-  FMM_HIP(hipMemsetAsync(cnt, 0, CNT_N * sizeof(int), s));
   // Small host->device uploads go through distinct regions of a pinned buffer
   // so that the host never rewrites memory a queued copy still reads.
   constexpr int kUpRegion = kMaxLevels + 2;
@@ -853,6 +855,8 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   FMM_LAUNCH_CHECK();
   FMM_HIP(hipMemcpyAsync(hc, cnt, CNT_N * sizeof(int), hipMemcpyDeviceToHost, s));
   FMM_HIP(hipStreamSynchronize(s));
+  FMM_REQUIRE(hc[CNT_BAD] == 0, FMM_E_ARG,
+              "tree: the source coordinates hold a NaN or infinite value");
   const int np = hc[CNT_P];
   int maxd = -1;
   int64_t dcount[kMaxLevels];

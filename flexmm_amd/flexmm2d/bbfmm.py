@@ -32,7 +32,7 @@ from .. import _lib
 from .. import kernels as _kernels
 from ..kifmm2d.fmm import (FMM as _KIFMM, Helper, _PRECISIONS, _acquire_plan, _print,
                            _release_plan, _stream, _to_device)
-from ..kifmm2d.precompute import LevelOperators, kernel_matrix, m2l_slots
+from ..kifmm2d.precompute import LevelOperators, compress_m2l, kernel_matrix, m2l_slots
 
 MAX_ORDER = 8
 
@@ -82,6 +82,9 @@ class ChebOperators(LevelOperators):
         snx, sny = nx * 0.5 * width, ny * 0.5 * width
         self.m2l = [VI @ kernel_matrix(kernel, snx + ox * width, sny + oy * width, snx, sny) @ VI.T
                     for ox, oy in m2l_slots()]
+        # compress(): the M2L_Evaluator rule of flexmm2d/helpers.py:5-33
+        # (rank at 1e-14, compressed if 2 rank < p^2), bbfmm.py:99
+        self.m2l_svd = [compress_m2l(m, n) for m in self.m2l]
         self.pinv1 = np.eye(n)
         self.pinv2 = np.eye(n)
         self.pinv_rank = n
@@ -123,7 +126,7 @@ class BB_FMM(_KIFMM):
     """flexmm2d ``BB_FMM`` (bbfmm.py:200-231; passes flexmm2d/fmm.py:33-177) on the GPU."""
 
     def __init__(self, x, y, kernel_eval, Ncutoff=50, p=8, dtype=float, bbox=None, helper=None,
-                 verbose=False, precision="fp64"):
+                 verbose=False, precision="fp64", m2l="svd"):
         kernel = _kernels.resolve_kernel(kernel_eval)
         if kernel.kernel_id not in (_lib.KERNEL_LAPLACE, _lib.KERNEL_MODIFIED_HELMHOLTZ):
             raise NotImplementedError("black-box FMM kernels: Laplace, modified Helmholtz")
@@ -170,8 +173,9 @@ class BB_FMM(_KIFMM):
             self.bbox = self.helper.get_bbox(None, None, None, None, [float(b) for b in bbox])
         self._dev_key = str(self.device)
         self._prec = _PRECISIONS[precision]
+        self.m2l = m2l
         self._plan = _acquire_plan(self.lib, kernel, self.Nequiv, self._prec, self._dev_key,
-                                   cheb=p)
+                                   cheb=p, m2l=m2l)
         self.tau = None
         self.neighbor_evaluators = {}
         self.build_tree()

@@ -133,16 +133,30 @@ class _Plan:
 _plan_pool = {}
 
 
-def _acquire_plan(lib, kernel, p, precision, device, cheb=0):
+_M2L_MODES = {"svd": _lib.M2L_SVD, "dense": _lib.M2L_DENSE}
+
+
+def _m2l_mode(m2l):
+    if m2l not in _M2L_MODES:
+        raise ValueError("m2l must be one of %s" % sorted(_M2L_MODES))
+    return _M2L_MODES[m2l]
+
+
+def _acquire_plan(lib, kernel, p, precision, device, cheb=0, m2l="svd"):
     key = (kernel.key, p, precision, device, cheb)
     pool = _plan_pool.setdefault(key, [])
+    plan = None
     while pool:
-        plan = pool.pop()
+        cand = pool.pop()
         # a plan finalised by the cyclic garbage collector together with the
         # FMM that returned it to the pool is no longer usable
-        if plan.h:
-            return plan
-    return _Plan(lib, kernel, p, precision, cheb)
+        if cand.h:
+            plan = cand
+            break
+    if plan is None:
+        plan = _Plan(lib, kernel, p, precision, cheb)
+    _lib.check(lib.fmm2d_set_m2l_mode(plan.h, _m2l_mode(m2l)))
+    return plan
 
 
 def _release_plan(plan, kernel, p, precision, device, cheb=0):
@@ -198,7 +212,7 @@ class FMM:
     """flexmm kifmm2d ``FMM`` (fmm.py:37-199) on the GPU."""
 
     def __init__(self, x, y, kernel_eval, Ncutoff, Nequiv, dtype=float, bbox=None, verbose=False,
-                 precision="fp64"):
+                 precision="fp64", m2l="svd"):
         kernel = _kernels.resolve_kernel(kernel_eval)
         if np.dtype(dtype) != np.float64:
             raise NotImplementedError("dtype must be float64 (the reference computes in float64)")
@@ -236,7 +250,10 @@ class FMM:
             self.bbox = self.helper.get_bbox(None, None, None, None, [float(b) for b in bbox])
         self._dev_key = str(self.device)
         self._prec = _PRECISIONS[precision]
-        self._plan = _acquire_plan(self.lib, kernel, self.Nequiv, self._prec, self._dev_key)
+        _m2l_mode(m2l)
+        self.m2l = m2l
+        self._plan = _acquire_plan(self.lib, kernel, self.Nequiv, self._prec, self._dev_key,
+                                   m2l=m2l)
         self.tau = None
         self.build_tree()
         self.register_evaluator(_kernels.default_evaluator(kernel), None, "potential_evaluation", 1)
@@ -325,7 +342,8 @@ class FMM:
         _lib.check(self.lib.fmm2d_set_profiling(self.handle, 1 if enable else 0))
 
     def phase_times(self):
-        """Milliseconds of the last recorded run of each phase (waits for it)."""
+        """Mean milliseconds of each phase over its runs recorded since
+        profiling was enabled or since the previous call (waits for them)."""
         out = (_lib.ctypes.c_double * len(_lib.PHASES))()
         _lib.check(self.lib.fmm2d_phase_times(self.handle, out, len(out)))
         return {k: float(v) for k, v in zip(_lib.PHASES, out) if v >= 0}

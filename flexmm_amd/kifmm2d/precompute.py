@@ -12,10 +12,12 @@ Mirrors ``Precomputation`` of the reference (flexmm/kifmm2d/scalar/helpers.py:
 * M2MC = K(children's small circles -> parent's large circle) (p x 4p,
   children ordered (-,-), (-,+), (+,-), (+,+); helpers.py:270-284), L2LC = M2MC^T;
 * the 40 M2L matrices K(small circle shifted by (ox, oy) w -> small circle)
-  (helpers.py:288-296), stored dense.  The reference applies some of them in
-  SVD-compressed form (rank at absolute tolerance 1e-14, helpers.py:17-36); the
-  dense matrix differs from that product by at most the dropped singular
-  values, below 1e-14.
+  (helpers.py:288-296), stored dense, and their SVD-compressed form by the
+  reference's rule (M2L_Evaluator, helpers.py:17-36): r = the number of
+  singular values above 1e-14 (absolute); if 2 r < p the slot is applied as
+  M2 (M1 q) with M1 = diag(s_:r) V_:r^T (r x p) and M2 = U_:r (p x r); if
+  r = 0 it contributes nothing; otherwise it stays dense.  The device applies
+  the compressed product (rank > 0) or the dense matrix (rank -1) per slot.
 
 For the mixed-precision form each M2L matrix is split as M = c 1^T + R with
 c_i the mean of row i: R goes to FP32 matrix cores, while c times the source
@@ -28,6 +30,7 @@ of the evaluation path.
 import numpy as np
 
 PINV_TOL = 1e-15   # helpers.py:241
+M2L_SVD_TOL = 1e-14   # helpers.py:251
 OP_NBLK = 50
 
 
@@ -81,8 +84,14 @@ class LevelOperators:
         self.M2MC = kernel_matrix(kernel, cex, cey, lx, ly)
         self.m2l = [kernel_matrix(kernel, sx + ox * width, sy + oy * width, sx, sy)
                     for ox, oy in m2l_slots()]
+        self.m2l_svd = [compress_m2l(m, p) for m in self.m2l]
         self.small = (sx, sy)
         self.large = (lx, ly)
+
+    @property
+    def m2l_ranks(self):
+        """Per slot: the compressed rank r (2 r < p), 0 for a null slot, -1 dense."""
+        return [c[0] for c in self.m2l_svd]
 
     def pack(self):
         """Flat FP64 array in the fmm2d_set_operators layout."""
@@ -106,5 +115,40 @@ class LevelOperators:
         circ = np.zeros((2, PD))
         circ[0, :p] = self.large[0]
         circ[1, :p] = self.large[1]
+        ranks = np.full(64, -1.0)
+        svd = np.zeros((40, PD, PD))
+        for s, (r, M1, M2) in enumerate(self.m2l_svd):
+            ranks[s] = r
+            if r > 0:
+                svd[s] = pack_svd_block(M1, M2, PD)
         return np.concatenate([blocks.ravel(), circ.ravel(), means.ravel(),
-                               resid.ravel().view(np.float64)])
+                               resid.ravel().view(np.float64), ranks, svd.ravel()])
+
+
+def compress_m2l(m, p, tol=M2L_SVD_TOL):
+    """helpers.py:17-36: (rank, M1, M2) with M2 @ M1 the compressed matrix when
+    2 rank < p; (0, None, None) for a null matrix; (-1, None, None) dense."""
+    U, S, VH = np.linalg.svd(m)
+    r = int((S > tol).sum())
+    if r == 0:
+        return 0, None, None
+    if 2 * r < p:
+        return r, S[:r, None] * VH[:r], U[:, :r].copy()
+    return -1, None, None
+
+
+def pack_svd_block(M1, M2, PD):
+    """One [PD][PD] block holding a compressed slot for the device: rows
+    0 .. PD/2-1 are M1 (r x p, zero padded); rows PD/2 .. PD-1 hold M2 (p x r)
+    two rows of M2 per block row: block[PD/2 + i // 2][(i % 2) PD/2 + j] =
+    M2[i, j], so that a lane of the second MFMA stage reads its 4 consecutive
+    columns of one M2 row from one place."""
+    H = PD // 2
+    r, p = M1.shape
+    assert r <= H and M2.shape == (p, r)
+    b = np.zeros((PD, PD))
+    b[:r, :p] = M1
+    m2 = np.zeros((PD, H))
+    m2[:p, :r] = M2
+    b[H:, :] = m2.reshape(H, 2 * H)
+    return b

@@ -30,7 +30,7 @@ class FMM:
     """stokes_fmm.FMM (stokes_fmm.py:598-842) on the GPU."""
 
     def __init__(self, x, y, Nequiv=48, Ncutoff=50, bbox=None, precomputations=None,
-                 verbose=False, precision="fp64"):
+                 verbose=False, precision="fp64", m2l="svd"):
         if precision not in _PRECISIONS:
             raise ValueError("precision must be one of %s" % sorted(_PRECISIONS))
         if not torch.cuda.is_available():
@@ -60,8 +60,9 @@ class FMM:
         self.adjust_bbox()
         self._dev_key = str(self.device)
         self._prec = _PRECISIONS[precision]
+        self.m2l = m2l
         self._plan = _acquire_plan(self.lib, _kernels.stokeslet, self.Nequiv, self._prec,
-                                   self._dev_key)
+                                   self._dev_key, m2l=m2l)
         self._prepared = False
         self.build_tree()
 

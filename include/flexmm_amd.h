@@ -9,6 +9,24 @@
  * or NULL for the default stream.  Work is enqueued on that stream; the tree
  * build synchronises it to size its per-level arrays.
  *
+ * Memory: a plan owns its device buffers (tree, expansions, target scratch).
+ * They are allocated with hipMalloc when a call first needs them and grow
+ * only when a later call needs more (steady-state calls of the same sizes
+ * allocate nothing); a buffer that grows first waits for the stream of the
+ * call in progress, never for the whole device.  fmm2d_plan_destroy waits
+ * for the work of the plan's last call and frees them.  Calls on one plan
+ * are ordered by their streams: a call on another stream than the previous
+ * one must be ordered after it by the caller (an event), as for any
+ * stream-bound handle.  fmm2d_evaluate_noalloc allocates nothing at all.
+ *
+ * Inputs: source coordinates that are NaN or infinite are refused by
+ * fmm2d_tree_build (FMM2D_E_ARG).  A NaN or infinite target coordinate gives
+ * non-finite outputs for that target only.  Sources and targets outside the
+ * root box are located by the same midpoint comparisons as inside (to an
+ * edge cell, as the reference's Tree and locate_points do, tree.py:127-189,
+ * 703-727), so targets outside it are extrapolated from that cell's local
+ * expansion exactly as in the reference.
+ *
  * Return value: 0 on success, < 0 for an argument/state/limit error
  * (FMM2D_E_*), > 0 for a HIP runtime error (hipError_t).  The message of the
  * last failure on the calling thread is returned by fmm2d_last_error().
@@ -69,6 +87,12 @@ extern "C" {
                               the FP64 near field (faster there, and more accurate) */
 #define FMM2D_PREC_MIXED_M2L 2 /* FP32 MFMA M2L only; near field in FP64 */
 
+/* FP64 M2L form (fmm2d_set_m2l_mode) */
+#define FMM2D_M2L_SVD 0   /* default: the reference's M2L_Evaluator (helpers.py:17-45): a slot
+                             of numerical rank r (singular values > 1e-14) with 2 r < p is
+                             applied as M2 (M1 q), a null slot adds nothing, others dense */
+#define FMM2D_M2L_DENSE 1 /* every slot applied as its dense matrix */
+
 /* tree arrays for fmm2d_tree_export */
 #define FMM2D_TREE_KEYS 0       /* u64[n]   Morton keys of the sorted sources            */
 #define FMM2D_TREE_PERM 1       /* u32[n]   tree position -> input index (Tree.ordv)      */
@@ -103,7 +127,8 @@ extern "C" {
 #define FMM2D_PHASE_EVALUATE 3    /* fmm2d_evaluate (locate, sort, L2T + P2P) */
 #define FMM2D_PHASE_EVAL_KERNEL 4 /* the fused L2T + P2P kernel alone         */
 #define FMM2D_PHASE_S2M 5         /* the S2M kernel alone                     */
-#define FMM2D_NUM_PHASES 6
+#define FMM2D_PHASE_M2L 6         /* the M2L kernel(s) of the downward pass   */
+#define FMM2D_NUM_PHASES 7
 
 int fmm2d_abi_version(void);
 const char* fmm2d_last_error(void);
@@ -142,6 +167,11 @@ int fmm2d_tree_info(void* plan, int64_t* out, int64_t nout);
 /* Copy one tree array (FMM2D_TREE_*) into a device buffer of `bytes` bytes. */
 int fmm2d_tree_export(void* plan, int which, void* dst, int64_t bytes, void* stream);
 
+/* Choose how the FP64 M2L applies the translation matrices (FMM2D_M2L_*);
+ * takes effect at the next fmm2d_downward.  The FP32 M2L of the mixed forms
+ * always applies the dense residuals. */
+int fmm2d_set_m2l_mode(void* plan, int mode);
+
 /* Doubles per level of the operator block for Nequiv = p, or -1. */
 int64_t fmm2d_operator_size(int p);
 
@@ -151,7 +181,12 @@ int64_t fmm2d_operator_size(int p);
  * their 4 transposes (L2L), the 40 M2L matrices in slot order -- then the
  * large-circle x and y (PD each), then for the mixed-precision M2L the row
  * means c_s of each M2L matrix ([40][PD] doubles) and the FP32 residuals
- * M_s - c_s 1^T ([40][PD][PD] floats, i.e. 20 PD^2 doubles of storage).
+ * M_s - c_s 1^T ([40][PD][PD] floats, i.e. 20 PD^2 doubles of storage), then
+ * the SVD-compressed M2L of M2L_Evaluator (helpers.py:17-45): 64 slot ranks
+ * as doubles (r > 0 compressed, 0 null, -1 dense; slots 40..63 unused) and 40
+ * [PD][PD] blocks, rows 0 .. PD/2-1 holding M1 = diag(s_:r) V_:r^T and rows
+ * PD/2 .. PD-1 holding M2 = U_:r two rows per block row
+ * (block[PD/2 + i/2][(i%2) PD/2 + j] = M2[i][j]).
  * PD = 32 for p <= 32, else 64.  Replaces Precomputation (helpers.py:250-317). */
 int fmm2d_set_operators(void* plan, int level, const double* dev_ops, int64_t n, void* stream);
 int fmm2d_clear_operators(void* plan);
@@ -192,8 +227,10 @@ int fmm2d_evaluate(void* plan, const double* tx, const double* ty, int64_t nt, i
 int fmm2d_export_expansions(void* plan, int which, double* dst, int64_t nvals, void* stream);
 
 /* Phase timing with HIP events recorded on the caller's stream (off by
- * default).  fmm2d_phase_times waits for the events and writes the last
- * duration of each FMM2D_PHASE_* in milliseconds (-1 if not recorded). */
+ * default): every run of a phase records its own pair of events.
+ * fmm2d_phase_times waits for them and writes, for each FMM2D_PHASE_*, the
+ * mean duration in milliseconds of the runs recorded since profiling was
+ * enabled or since the previous fmm2d_phase_times call (-1 if none). */
 int fmm2d_set_profiling(void* plan, int enable);
 
 /* Algorithmic work of the current tree and the last evaluation (synchronous):

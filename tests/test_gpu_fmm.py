@@ -14,11 +14,11 @@ pytestmark = pytest.mark.gpu
 PARITY_TOL = 1e-12
 
 
-def _fmm(x, y, ncut, p, bbox=None):
+def _fmm(x, y, ncut, p, bbox=None, m2l="svd"):
     from flexmm_amd import kernels
     from flexmm_amd.kifmm2d import FMM, the_help
     the_help.clear()
-    return FMM(x, y, kernels.laplace, ncut, p, bbox=bbox)
+    return FMM(x, y, kernels.laplace, ncut, p, bbox=bbox, m2l=m2l)
 
 
 def _oracle(x, y, ncut, p, bbox=None):
@@ -142,3 +142,29 @@ def test_reference_cpu_case_100k():
     print("levels", f.tree.levels, "vs oracle %.3e, vs direct %.3e" % (err, errd))
     assert err < PARITY_TOL
     assert errd < 3e-8   # p = 32 truncation: 7.2e-9 measured
+
+
+@pytest.mark.parametrize("m2l", ["svd", "dense"])
+@pytest.mark.parametrize("case", ["uniform", "clustered"])
+def test_m2l_compressed_and_dense_forms(m2l, case):
+    """Both FP64 M2L forms against the oracle (which applies the reference's
+    compressed form, helpers.py:17-45): slots of rank r with 2 r < p as
+    M2 (M1 q), the others dense; and the two forms against each other."""
+    if case == "uniform":
+        x, y, tau = H.uniform(60000, 301)
+    else:
+        x, y, tau = H.clustered(60000, 6, 0.01, 302)
+    tx, ty = H.grid(150)
+    print("m2l", m2l, case, flush=True)
+    f = _fmm(x, y, 60, 60, bbox=[0, 1, 0, 1], m2l=m2l)
+    f.build_expansions(tau)
+    u = f.target_evaluation(tx, ty)[0]
+    o = _oracle(x, y, 60, 60, bbox=[0, 1, 0, 1])
+    o.build_expansions(tau)
+    uo = o.target_evaluation(tx, ty)[0]
+    err = H.rel_max_err(u, uo)
+    other = _fmm(x, y, 60, 60, bbox=[0, 1, 0, 1], m2l="dense" if m2l == "svd" else "svd")
+    other.build_expansions(tau)
+    e2 = H.rel_max_err(u, other.target_evaluation(tx, ty)[0])
+    print(m2l, case, "levels", f.tree.levels, "vs oracle %.3e, vs the other form %.3e" % (err, e2))
+    assert err < PARITY_TOL and e2 < PARITY_TOL

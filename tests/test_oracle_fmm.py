@@ -216,3 +216,21 @@ def test_slp_dlp_upwardor_p64():
     us = f.evaluate_to_points(x[:300], y[:300], "slp+dlp->u", True)[0]
     refs = odirect.direct_src(okern.laplace_slp_dlp, x, y, tau, ex, x[:300], y[:300], check=True)[0]
     assert _err(us, refs) < 1e-12
+
+
+def test_parallel_oracle_equals_serial():
+    """The forked-worker oracle (the CPU baseline of the benchmark) gives the
+    serial oracle's potentials: S2M over leaf chunks and evaluation over target
+    chunks only regroup additions."""
+    from oracle import parallel as opar
+    x, y, tau = H.uniform(20000, 31)
+    tx, ty = H.grid(90)
+    a = okifmm.FMM(x, y, okern.laplace, 40, 24, bbox=[0, 1, 0, 1])
+    a.build_expansions(tau)
+    b = opar.ParallelFMM(x, y, okern.laplace, 40, 24, bbox=[0, 1, 0, 1], workers=3)
+    b.build_expansions(tau)
+    for m in (a.multipoles, b.multipoles):
+        assert len(m) == a.tree.levels
+    assert H.rel_max_err(b.multipoles[-1], a.multipoles[-1]) < 1e-13
+    assert H.rel_max_err(b.target_evaluation(tx, ty), a.target_evaluation(tx, ty)) < 1e-14
+    assert H.rel_max_err(b.source_evaluation(x, y), a.source_evaluation(x, y)) < 1e-14

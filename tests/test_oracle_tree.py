@@ -234,3 +234,25 @@ def test_planted_wrong_colleague_radius_is_caught(monkeypatch):
     monkeypatch.setattr(otree.Level, "tag_colleagues", wrong)
     with pytest.raises(RuntimeError):   # more than 9 colleagues
         otree.Tree(x, y, 40, [0, 1, 0, 1])
+
+
+def test_fast_tree_comparison_accepts_equal_and_catches_planted_differences():
+    """The vectorised full-size tree comparison used by the GPU tests at the
+    headline sizes: it accepts the oracle's own tree laid out as device arrays
+    (X-list and balance nodes included) and rejects a moved point, a dropped
+    colleague, a dropped M2L partner and a flipped flag."""
+    from tests import helpers as H
+    x, y, _ = H.clustered(6000, 4, 0.004, 17)
+    t = otree.Tree(x, y, 20, [0, 1, 0, 1])
+    assert any(lev.fake_leaf.any() for lev in t.Levels)
+    dev = H.OracleAsDevice(t)
+    assert H.compare_trees_fast(dev, t) == sum(lev.n_node for lev in t.Levels)
+    for name, plant in (
+            ("perm", lambda a: a.__setitem__(slice(None), np.roll(a, 1))),
+            ("colleagues", lambda a: a.__setitem__(int(np.flatnonzero(a >= 0)[-1]), -1)),
+            ("m2l", lambda a: a.__setitem__(int(np.flatnonzero(a >= 0)[7]), -1)),
+            ("flags", lambda a: a.__setitem__(5, a[5] ^ H.NF_UP))):
+        bad = H.OracleAsDevice(t)
+        plant(bad._a[name])
+        with pytest.raises(AssertionError):
+            H.compare_trees_fast(bad, t)

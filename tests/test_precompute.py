@@ -34,7 +34,10 @@ def test_pack_layout():
     blocks = flat[:nb].reshape(pc.OP_NBLK, PD, PD)
     circ = flat[nb:nb + 2 * PD].reshape(2, PD)
     means = flat[nb + 2 * PD:nb + 42 * PD].reshape(40, PD)
-    resid = flat[nb + 42 * PD:].view(np.float32).reshape(40, PD, PD)
+    r32 = nb + 42 * PD
+    resid = flat[r32:r32 + 20 * PD * PD].view(np.float32).reshape(40, PD, PD)
+    ranks = flat[r32 + 20 * PD * PD:r32 + 20 * PD * PD + 64]
+    assert list(ranks[:40]) == ops.m2l_ranks and np.all(ranks[40:] == -1)
     for s in (0, 13, 39):
         m = ops.m2l[s]
         np.testing.assert_allclose(means[s, :p], m.mean(axis=1), rtol=1e-15)
@@ -49,7 +52,49 @@ def test_pack_layout():
     assert np.all(blocks[:, p:, :] == 0) and np.all(blocks[:, :, p:] == 0)
     np.testing.assert_array_equal(circ[0, :p], ops.large[0])
     assert np.all(circ[:, p:] == 0)
-    assert flat.size == 50 * PD * PD + 2 * PD + 40 * PD + 20 * PD * PD
+    assert flat.size == 50 * PD * PD + 2 * PD + 40 * PD + 20 * PD * PD + 64 + 40 * PD * PD
+
+
+@pytest.mark.parametrize("p,w", [(60, 1.0), (60, 1.0 / 128), (64, 0.25), (24, 0.5)])
+def test_compressed_m2l_matches_reference_rule(p, w):
+    """The SVD-compressed M2L slots: ranks, kinds and products equal to the
+    oracle's M2LEvaluator (helpers.py:17-45), and the packed block, read with
+    the device kernel's two-stage index arithmetic (stage 1 rows of M1, stage 2
+    block[PD/2 + i/2][(i%2) PD/2 + 4 j + kq]), applies M2 (M1 q)."""
+    ops = pc.LevelOperators(kernels.laplace, p, w)
+    ora = okifmm.Precomputation(w, okern.laplace, p)
+    PD = pc.padded_dim(p)
+    H = PD // 2
+    flat = ops.pack()
+    off = 50 * PD * PD + 2 * PD + 40 * PD + 20 * PD * PD
+    ranks = flat[off:off + 64]
+    svd = flat[off + 64:].reshape(40, PD, PD)
+    q = np.random.default_rng(p).standard_normal(p)
+    ncomp = 0
+    for s, (ox, oy) in enumerate(pc.m2l_slots()):
+        ev = ora.M2LF[ox + 3, oy + 3]
+        r = int(ranks[s])
+        assert r == (ev.rank if ev.kind == "svd" else (0 if ev.kind == "null" else -1))
+        if r <= 0:
+            continue
+        ncomp += 1
+        blk = svd[s]
+        T = np.zeros(PD)
+        T[:H] = blk[:H, :p] @ q                        # stage 1 over the padded rows
+        assert np.all(T[r:] == 0)
+        out = np.zeros(PD)
+        for j in range((r + 3) // 4):                   # stage 2
+            for kq in range(4):
+                k = 4 * j + kq
+                for row in range(PD):
+                    out[row] += blk[H + row // 2, (row % 2) * H + k] * T[k]
+        ref = ev(q)
+        np.testing.assert_allclose(out[:p], ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+        assert np.all(out[p:] == 0)
+        np.testing.assert_allclose(out[:p], ev.m2l @ q, rtol=0,
+                                   atol=1e-12 * np.abs(ev.m2l @ q).max())
+    if p >= 60:
+        assert ncomp >= 28
 
 
 def test_padded_dim_limits():
