@@ -22,6 +22,7 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_MIXED_M2L = 2
 M2L_SVD, M2L_DENSE = 0, 1
+TREE_AUTO, TREE_GENERAL = 0, 1
 TREE_ARRAYS = dict(keys=0, perm=1, sx=2, sy=3, code=4, level=5, bot=6, top=7, parent=8, child=9,
                    flags=10, colleagues=11, xmid=12, ymid=13, row=14, m2l=15, evalid=16)
 EXP_ARRAYS = dict(check=0, multipole=1, local=2, tau_sorted=3)
@@ -49,6 +50,7 @@ SIGNATURES = {
     "fmm2d_evaluate": (_int, [_vp, _vp, _vp, _i64, _int, _int, _vp, _vp]),
     "fmm2d_export_expansions": (_int, [_vp, _int, _vp, _i64, _vp]),
     "fmm2d_set_m2l_mode": (_int, [_vp, _int]),
+    "fmm2d_set_tree_mode": (_int, [_vp, _int]),
     "fmm2d_set_profiling": (_int, [_vp, _int]),
     "fmm2d_phase_times": (_int, [_vp, ctypes.POINTER(ctypes.c_double), _int]),
     "fmm2d_work_counts": (_int, [_vp, ctypes.POINTER(_i64), _int]),

@@ -2580,6 +2580,14 @@ static void m2l_prepare(Plan& P, const M2LTables& T, hipStream_t s) {
   P.perm_tree = t.build_id;
 }
 // M2L of the table's levels [l0, l1).
+// The mixed forms run the M2L on FP32 matrix cores, except for Chebyshev
+// (black-box) plans: there the FP32 accumulation error of a slot's product
+// alone reaches ~1e-7 of the potential (no dominant mode to split off, unlike
+// the KIFMM's row means), while their translations are of low rank (15-23 of
+// 64 at order 8), so the SVD-compressed FP64 M2L is the cheaper way to the
+// error bound.
+static bool m2l_in_fp32(const Plan& P) { return P.precision != PREC_FP64 && !P.cheb; }
+
 static void m2l_launch(Plan& P, const M2LTables& T, int l0, int l1, hipStream_t s) {
   const Tree& t = P.tree;
   const int PD = P.PD;
@@ -2603,12 +2611,12 @@ static void m2l_launch(Plan& P, const M2LTables& T, int l0, int l1, hipStream_t 
   // fewer than 3 tiles per CU: split the tiles' phases over two blocks each
   // (FP64 only: the FP32 phases are short enough that the second block
   // setup costs more than the balance gains, as measured on 1M points)
-  lt.split = P.precision == PREC_FP64 && (int64_t)nb * nd < 3 * kNumCUs ? 2 : 1;
+  lt.split = !m2l_in_fp32(P) && (int64_t)nb * nd < 3 * kNumCUs ? 2 : 1;
   if (lt.split == 2) {
     FMM_HIP(hipMemsetAsync(P.ple.ptr, 0, nstride * PD * nd * sizeof(double), s));
     nb *= 2;
   }
-  if (P.precision != PREC_FP64) {
+  if (m2l_in_fp32(P)) {
     const bool frame = P.kernel_id == KER_STOKES;
     auto kern = PD == 32 ? (frame ? k_m2l_all_f32<32, true> : k_m2l_all_f32<32, false>)
                          : (frame ? k_m2l_all_f32<64, true> : k_m2l_all_f32<64, false>);
@@ -3292,6 +3300,15 @@ int fmm2d_work_counts(void* plan, int64_t* out, int n) {
   FMM_HIP(hipMemcpy(h, acc.ptr, sizeof h, hipMemcpyDeviceToHost));
   for (int i = 0; i < 9; ++i) out[i] = (int64_t)h[i];
   out[5] = t.n_rows;
+  ABI_END
+}
+
+int fmm2d_set_tree_mode(void* plan, int mode) {
+  ABI_BEGIN
+  Plan* P = as_plan(plan);
+  FMM_REQUIRE(mode == FMM2D_TREE_AUTO || mode == FMM2D_TREE_GENERAL, fmm::FMM_E_ARG,
+              "unknown tree mode %d", mode);
+  P->tree.allow_full = mode == FMM2D_TREE_AUTO;
   ABI_END
 }
 

@@ -28,6 +28,7 @@ struct TreeLevel {
 
 struct Tree {
   bool built = false;
+  bool allow_full = true; // complete trees take the closed-form build (tree.hip)
   int64_t build_id = 0;   // incremented by every build
   int64_t n = 0;           // number of sources
   int ncut = 0;

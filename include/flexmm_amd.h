@@ -80,18 +80,30 @@ extern "C" {
 
 /* precision */
 #define FMM2D_PREC_FP64 0  /* every step in FP64 */
-#define FMM2D_PREC_MIXED 1 /* FP32 MFMA M2L (constant mode in FP64) and, for the Laplace
+#define FMM2D_PREC_MIXED 1 /* FP32 MFMA M2L (constant mode in FP64; the Stokeslet fields
+                              translated in the target box's frame) and, for the Laplace
                               potential, FP32 near field for leaf colleagues at
                               separations >= w/16; FP64 elsewhere.  An evaluation
                               with fewer than 32 targets per leaf on average takes
-                              the FP64 near field (faster there, and more accurate) */
-#define FMM2D_PREC_MIXED_M2L 2 /* FP32 MFMA M2L only; near field in FP64 */
+                              the FP64 near field (faster there, and more accurate).
+                              Black-box plans keep the (SVD-compressed) FP64 M2L: an
+                              FP32 accumulation alone costs them ~1e-7 */
+#define FMM2D_PREC_MIXED_M2L 2 /* FP32 MFMA M2L only (black-box plans: FP64); near field
+                                  in FP64 */
 
 /* FP64 M2L form (fmm2d_set_m2l_mode) */
 #define FMM2D_M2L_SVD 0   /* default: the reference's M2L_Evaluator (helpers.py:17-45): a slot
                              of numerical rank r (singular values > 1e-14) with 2 r < p is
                              applied as M2 (M1 q), a null slot adds nothing, others dense */
 #define FMM2D_M2L_DENSE 1 /* every slot applied as its dense matrix */
+
+/* tree build (fmm2d_set_tree_mode) */
+#define FMM2D_TREE_AUTO 0     /* default: points that form a complete quadtree (every cell of
+                                 depth D-1 holds more than Ncutoff points, every cell of depth D
+                                 between 1 and Ncutoff, 2 <= D < 10) take the closed-form build
+                                 of that tree; all others the general build */
+#define FMM2D_TREE_GENERAL 1  /* always the general build (sort, refinement, 2:1 balance,
+                                 X-list); the two give identical trees */
 
 /* tree arrays for fmm2d_tree_export */
 #define FMM2D_TREE_KEYS 0       /* u64[n]   Morton keys of the sorted sources            */
@@ -157,6 +169,9 @@ int fmm2d_plan_destroy(void* plan);
  * Replaces Tree(x, y, Ncutoff, bbox) (fmm.py:68-72, tree.py:451-518). */
 int fmm2d_tree_build(void* plan, const double* x, const double* y, int64_t n, const double* bbox,
                      int64_t ncut, void* stream);
+
+/* Choose the tree build (FMM2D_TREE_*) of later fmm2d_tree_build calls. */
+int fmm2d_set_tree_mode(void* plan, int mode);
 
 /* Host summary: out[0..7] = levels L, nodes, multipole rows, leaves, S2M boxes,
  * sources, row stride PD, ncut; then per level d: out[8+3d] node offset,

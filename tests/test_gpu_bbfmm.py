@@ -89,10 +89,11 @@ def test_bbfmm_modified_helmholtz_matches_oracle():
     assert e < 1e-12
 
 
-# FP32 M2L on Chebyshev coefficients: 2.8e-7, the size of the order-8
-# truncation error itself (the row-mean split that helps the KIFMM's
-# equivalent densities removes no dominant mode here)
-@pytest.mark.parametrize("precision,tol", [("mixed_m2l", 6e-7), ("mixed", 6e-7)])
+# Black-box plans keep the SVD-compressed FP64 M2L under the mixed forms (an
+# FP32 M2L accumulation alone costs Chebyshev coefficients ~1e-7, with no
+# dominant mode to split off); "mixed" then differs from FP64 by its FP32
+# near field only, "mixed_m2l" not at all.
+@pytest.mark.parametrize("precision,tol", [("mixed_m2l", 1e-12), ("mixed", 1e-7)])
 def test_bbfmm_mixed(precision, tol):
     from flexmm_amd import kernels
     from flexmm_amd.flexmm2d import BB_FMM

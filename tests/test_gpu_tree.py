@@ -92,3 +92,71 @@ def test_tree_uniform_counts_full():
                 ok &= (ix + ox >= 0) & (ix + ox < n) & (iy + oy >= 0) & (iy + oy < n)
                 expect += int(ok.sum())
         assert cnt == expect, (d, cnt, expect)
+
+
+# complete trees (uniform points): depth D from the depth-(D+1) histogram
+# (D = Dh - 1) or from the depth-D one (D = Dh); and cases the closed-form
+# build must refuse (a cell of depth D-1 at or under Ncutoff, an empty leaf,
+# clustered points)
+FULL_CASES = {
+    "uniform_1M_ncut100": lambda: (*H.uniform(1_000_000, 2)[:2], 100, [0, 1, 0, 1]),
+    "uniform_100k_ncut64": lambda: (*H.uniform(100_000, 1)[:2], 64, None),
+    "uniform_300k_ncut40": lambda: (*H.uniform(300_000, 12)[:2], 40, [0, 1, 0, 1]),
+    "uniform_50k_ncut200": lambda: (*H.uniform(50_000, 13)[:2], 200, [0, 1, 0, 1]),
+    "grid_65536_ncut16": lambda: (*[a.copy() for a in H.grid(256)], 16, [0, 1, 0, 1]),
+    "uniform_with_hole": lambda: (*_with_hole(200_000, 14), 64, [0, 1, 0, 1]),
+    "clustered_100k": lambda: (*H.clustered(100_000, 10, 0.01, 15)[:2], 64, [0, 1, 0, 1]),
+}
+
+
+def _with_hole(n, seed):
+    x, y, _ = H.uniform(n, seed)
+    keep = ~((np.abs(x - 0.5) < 0.02) & (np.abs(y - 0.5) < 0.02))
+    return x[keep].copy(), y[keep].copy()
+
+
+@pytest.mark.parametrize("case", sorted(FULL_CASES))
+def test_complete_tree_build_equals_general_build(case):
+    """The closed-form build of complete trees (FMM2D_TREE_AUTO) gives the
+    general build's tree array for array, bit for bit (keys, permutation,
+    sorted points, every node array), and both equal the oracle's tree."""
+    import torch
+    from flexmm_amd import _lib
+    x, y, ncut, bbox = FULL_CASES[case]()
+    print("complete tree", case, x.size, flush=True)
+    f = _build(x, y, ncut, bbox=bbox)
+    names = ["keys", "perm", "sx", "sy", "code", "level", "bot", "top", "parent", "child",
+             "flags", "colleagues", "xmid", "ymid", "row", "m2l", "evalid"]
+    auto = {k: f.tree_array(k).cpu() for k in names}
+    info_auto = (f.tree.levels, f.tree.n_nodes, f.tree.n_rows, f.tree.n_leaves, f.tree.n_s2m)
+    _lib.check(f.lib.fmm2d_set_tree_mode(f.handle, _lib.TREE_GENERAL))
+    try:
+        f.build_tree()
+        gen = {k: f.tree_array(k).cpu() for k in names}
+        info_gen = (f.tree.levels, f.tree.n_nodes, f.tree.n_rows, f.tree.n_leaves, f.tree.n_s2m)
+    finally:
+        _lib.check(f.lib.fmm2d_set_tree_mode(f.handle, _lib.TREE_AUTO))
+    assert info_auto == info_gen, (info_auto, info_gen)
+    for k in names[4:]:
+        a, b = auto[k], gen[k]
+        if k == "m2l":   # levels 0 and 1 carry no lists
+            start = f.tree.node_off[2] * 40 if f.tree.levels > 2 else a.numel()
+            a, b = a[start:], b[start:]
+        assert torch.equal(a, b), "%s: array %s differs" % (case, k)
+    # points: the same set in every leaf (the general build's order inside a
+    # leaf follows its radix scatter's atomics), each with its own key and
+    # coordinates
+    bot, top = auto["bot"].numpy(), auto["top"].numpy()
+    leaf = (auto["flags"].numpy() & H.NF_LEAF) != 0
+    owner = H._point_owner(np.arange(x.size), bot[leaf], top[leaf], x.size)
+    assert (owner >= 0).all()
+    for d in (auto, gen):
+        perm = d["perm"].numpy().astype(np.int64)
+        o = np.lexsort((perm, owner))
+        d["order"] = o
+        assert np.array_equal(d["sx"].numpy(), x[perm]) and np.array_equal(d["sy"].numpy(), y[perm])
+    for k in ("perm", "keys"):
+        assert np.array_equal(auto[k].numpy()[auto["order"]], gen[k].numpy()[gen["order"]]), k
+    f.build_tree()
+    n = H.compare_trees_fast(f, _oracle_tree(x, y, ncut, f.bbox))
+    print(case, "levels", f.tree.levels, "nodes", n)
