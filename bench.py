@@ -534,33 +534,42 @@ def main():
             u = f.target_evaluation(txd, tyd)
             return f, u
 
-        # Every step reuses one pooled device plan.  Phase timing is switched
-        # on in the first warm-up step: each later run of a phase records its
-        # own pair of HIP events on the launch stream (torch's current
-        # stream), and phase_times() returns the mean over the runs since the
-        # previous call -- the timed steps below.
-        for i in range(max(1, args.warmup)):
+        # Every step reuses one pooled device plan (each FMM is released
+        # before the next one is made).  Phase timing is switched on in the
+        # first warm-up step: each later run of a phase records its own pair
+        # of HIP events on the launch stream (torch's current stream), and
+        # phase_times returns the mean over the runs since its previous call
+        # -- below, the timed steps.
+        import ctypes
+        from flexmm_amd import _lib
+
+        def phase_times(plan):
+            out = (ctypes.c_double * len(_lib.PHASES))()
+            _lib.check(plan.lib.fmm2d_phase_times(plan.h, out, len(out)))
+            return {k: float(v) for k, v in zip(_lib.PHASES, out) if v >= 0}
+
+        f, u = step()
+        f.set_profiling(True)
+        plan = f._plan
+        f.close()
+        for _ in range(args.warmup - 1):
             f, u = step()
-            if i == 0:
-                f.set_profiling(True)
-            last = f
-            del f
-        last.phase_times()
-        del last
+            f.close()
+        phase_times(plan)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             f, u = step()
-            last = f
-            del f
+            f.close()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         ms = (t1 - t0) * 1000.0 / args.steps
-        acc = last.phase_times()
-        last.set_profiling(False)
-        work = last.work_counts()
-        nodes = last.tree.n_nodes
-        del last
+        acc = phase_times(plan)
+        _lib.check(plan.lib.fmm2d_set_profiling(plan.h, 0))
+        f, u = step()
+        work = f.work_counts()
+        nodes = f.tree.n_nodes
+        f.close()
         # max-norm error relative to max|u| (examples/pure_kifmm_laplace.py:149-151)
         err = float((u[0][sel] - ref).abs().max() / ref.abs().max())
         return dict(ms=ms, value=tx.size / (ms / 1000.0) / 1e6, phases=acc, work=work, err=err,

@@ -675,11 +675,17 @@ __constant__ ValidSlots c_valid;
 // load feeds two steps: step 2m + h of lane group kq multiplies k = 8m + 2kq
 // + h (A is read with the same permutation, which leaves the product
 // unchanged).  Loads of phase i + 1 are issued before the barrier of phase i.
-constexpr int kM2LThreads = 256;
+// With a row split RS, a block holds 4 RS waves: wave w takes columns
+// 16 (w mod 4) .. + 15 and the row blocks of half (or quarter) w / 4, which
+// puts RS times more waves on the SIMDs for the same tiles (a 1M-point tree
+// gives about 1.4 waves per SIMD with RS = 1).
+constexpr int kM2LThreads = 256;   // threads per block and row split
 constexpr int kM2LCols = 64;
+constexpr int kM2LRowSplit = 2;
+constexpr int kFlushM = 8;   // FP32 M2L: k-step pairs per FP32 run (8: one run per slot at PD 64)
 
 struct M2LSetup {
-  int li, c, col, kq, g, nph;
+  int li, c, col, kq, g, nph, rh;
   bool active;
 };
 
@@ -701,8 +707,9 @@ __device__ __forceinline__ M2LSetup m2l_setup(const LevelTable& lt, const int* _
   const int tile = lb >> 2;
   const int ngroups = lt.ngroups[li];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-  S.col = w * 16 + (l & 15);
+  S.col = (w & 3) * 16 + (l & 15);
   S.kq = l >> 4;
+  S.rh = w >> 2;
   const int qq = tile * kM2LCols + S.col;
   const int q = (lt.perm && qq < ngroups) ? lt.perm[lt.poff[li] + S.c * ngroups + qq] : qq;
   S.g = lt.g0[li] + 4 * q + S.c;
@@ -819,8 +826,8 @@ __global__ __launch_bounds__(kM2LOrderThreads) void k_m2l_order(M2LOrderTable ot
 // (col, kq) holds rows 16 rb + 4 v + kq) are exactly the B operand of the
 // second stage's k-steps j = 4 rb + v in natural k order, so no data moves
 // between lanes.  Null slots (r = 0) add nothing, as in the reference.
-template <int PD>
-__global__ __launch_bounds__(kM2LThreads) void k_m2l_all(LevelTable lt,
+template <int PD, int RS>
+__global__ __launch_bounds__(kM2LThreads * RS) void k_m2l_all(LevelTable lt,
                                                         const int* __restrict__ flags,
                                                         const int* __restrict__ m2l,
                                                         const double* __restrict__ mult,
@@ -831,7 +838,7 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all(LevelTable lt,
   // density blockIdx.y: multipole rows rstride rows further, local rows nstride nodes further
   mult += (int64_t)blockIdx.y * rstride * PD;
   ple += (int64_t)blockIdx.y * nstride * PD;
-  constexpr int KS = Mfma<PD>::KS, RB = Mfma<PD>::RB, LDA = Mfma<PD>::LDA;
+  constexpr int KS = Mfma<PD>::KS, RB = Mfma<PD>::RB / RS, LDA = Mfma<PD>::LDA;
   constexpr int H = PD / 2, R1 = H / 16, K2 = H / 4;
   __shared__ __attribute__((aligned(16))) double buf[2][PD * LDA];
   __shared__ int srcs[27][kM2LCols];
@@ -840,6 +847,7 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all(LevelTable lt,
   __shared__ unsigned long long smask;
   const M2LSetup S = m2l_setup(lt, flags, m2l, srcs, pslot, smask);
   const int l = threadIdx.x & 63;
+  const int rb0 = S.rh * RB;   // this wave's first row block
   const double* opsL = ops + (int64_t)lt.d[S.li] * opsz;
   if (threadIdx.x < kNumM2L)
     srank[threadIdx.x] = use_svd ? (int)opsL[ops_off_svd_rank(PD) + threadIdx.x] : -1;
@@ -848,7 +856,7 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all(LevelTable lt,
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb) acc[rb] = d4{0.0, 0.0, 0.0, 0.0};
   if (S.nph > 0) {
-    AStage<PD, kM2LThreads> st;
+    AStage<PD, kM2LThreads * RS> st;
     double bn[KS], bc[KS];
     auto fetch = [&](int k) {
       const int i = pslot[k];
@@ -874,12 +882,13 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all(LevelTable lt,
       if (i + 1 < S.nph) fetch(i + 1);
       __syncthreads();
       const double* a0 = A + (l & 15) * LDA + 2 * S.kq;
+      const double* ar = a0 + rb0 * 16 * LDA;   // this wave's rows
       if (r < 0) {
 #pragma unroll
         for (int m = 0; m < KS / 2; ++m)
 #pragma unroll
           for (int rb = 0; rb < RB; ++rb) {
-            const dv2 a = *reinterpret_cast<const dv2*>(a0 + rb * 16 * LDA + 8 * m);
+            const dv2 a = *reinterpret_cast<const dv2*>(ar + rb * 16 * LDA + 8 * m);
             acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bc[2 * m], acc[rb], 0, 0, 0);
             acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bc[2 * m + 1], acc[rb], 0, 0, 0);
           }
@@ -907,7 +916,7 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all(LevelTable lt,
             const double b = t1[j >> 2][j & 3];
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb) {
-              const int row = 16 * rb + (l & 15);
+              const int row = 16 * (rb0 + rb) + (l & 15);
               const double a = a2[(row >> 1) * LDA + (row & 1) * H + 4 * j];
               acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[rb], 0, 0, 0);
             }
@@ -918,7 +927,7 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all(LevelTable lt,
     }
   }
   if (S.active) {
-    double* rowp = ple + (int64_t)S.g * PD;
+    double* rowp = ple + (int64_t)S.g * PD + 16 * rb0;
     if (lt.split == 2) {
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb)
@@ -974,8 +983,8 @@ struct AStageF {
   }
 };
 
-template <int PD, bool FRAME>
-__global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
+template <int PD, bool FRAME, int RS>
+__global__ __launch_bounds__(kM2LThreads * RS) void k_m2l_all_f32(
     LevelTable lt, const int* __restrict__ flags, const int* __restrict__ m2l,
     const double* __restrict__ mult, const double* __restrict__ qs,
     const double* __restrict__ ops, int64_t opsz, double* __restrict__ ple, int64_t rstride,
@@ -987,7 +996,7 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
   mult += (int64_t)blockIdx.y * rstride * PD;
   qs += (int64_t)blockIdx.y * rstride;
   ple += (int64_t)blockIdx.y * nstride * PD;
-  constexpr int KS = PD / 4, RB = PD / 16, LDAF = PD + 4;
+  constexpr int KS = PD / 4, RB = PD / 16 / RS, LDAF = PD + 4;
   __shared__ __attribute__((aligned(16))) float buf[2][PD * LDAF];
   // row means of the phase's slot, staged with its residual block (keeping
   // all 27 resident would hold the block to 2 per CU by LDS)
@@ -997,6 +1006,7 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
   __shared__ unsigned long long smask;
   const M2LSetup S = m2l_setup(lt, flags, m2l, srcs, pslot, smask);
   const int l = threadIdx.x & 63;
+  const int rb0 = S.rh * RB;   // this wave's first row block
   const double* opsL = ops + (int64_t)lt.d[S.li] * opsz;
   double ct = 0.0;   // target centre coordinate i in the root frame
   if (frame && S.active) ct = blockIdx.y == 0 ? xmid[S.g] - c0x : ymid[S.g] - c0y;
@@ -1008,7 +1018,7 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
 #pragma unroll
     for (int v = 0; v < 4; ++v) acc64[rb][v] = 0.0;
   if (S.nph > 0) {
-    AStageF<PD, kM2LThreads> st;
+    AStageF<PD, kM2LThreads * RS> st;
     float bn[KS], bc[KS];
     double qn = 0.0, qc, mn = 0.0;
     auto fetch = [&](int k) {
@@ -1047,14 +1057,16 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
       if (threadIdx.x < PD) cmbuf[i & 1][threadIdx.x] = mn;
       if (i + 1 < S.nph) fetch(i + 1);
       __syncthreads();
-      // one FP32 accumulation per slot (p terms), slots summed in FP64: a
-      // single FP32 chain over all 27 slots loses about 4x more
-      const float* a0 = A + (l & 15) * LDAF + 2 * S.kq;
+      // FP32 accumulation over runs of 8 kFlushM products per slot, each run
+      // added to the FP64 sums (a single FP32 chain over all 27 slots loses
+      // about 4x more than one per slot; the FP32 rounding of the running
+      // sum, not of the operands, dominates the error)
+      const float* a0 = A + (rb0 * 16 + (l & 15)) * LDAF + 2 * S.kq;
       f4 acc[RB];
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb) acc[rb] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int m = 0; m < KS / 2; ++m)
+      for (int m = 0; m < KS / 2; ++m) {
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb) {
           typedef float fv2 __attribute__((ext_vector_type(2)));
@@ -1062,8 +1074,17 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
           acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, bc[2 * m], acc[rb], 0, 0, 0);
           acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, bc[2 * m + 1], acc[rb], 0, 0, 0);
         }
+        if ((m + 1) % kFlushM == 0 && m + 1 < KS / 2) {
+#pragma unroll
+          for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc64[rb][v] += (double)acc[rb][v];
+            acc[rb] = f4{0.f, 0.f, 0.f, 0.f};
+          }
+        }
+      }
       // constant mode c_s * (sum of the source's equivalent density), FP64
-      const double* cm = cmbuf[i & 1] + 4 * S.kq;
+      const double* cm = cmbuf[i & 1] + 16 * rb0 + 4 * S.kq;
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
@@ -1075,7 +1096,7 @@ __global__ __launch_bounds__(kM2LThreads) void k_m2l_all_f32(
     }
   }
   if (S.active) {
-    double* rowp = ple + (int64_t)S.g * PD;
+    double* rowp = ple + (int64_t)S.g * PD + 16 * rb0;
     if (lt.split == 2) {
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb)
@@ -2618,16 +2639,18 @@ static void m2l_launch(Plan& P, const M2LTables& T, int l0, int l1, hipStream_t 
   }
   if (m2l_in_fp32(P)) {
     const bool frame = P.kernel_id == KER_STOKES;
-    auto kern = PD == 32 ? (frame ? k_m2l_all_f32<32, true> : k_m2l_all_f32<32, false>)
-                         : (frame ? k_m2l_all_f32<64, true> : k_m2l_all_f32<64, false>);
-    hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(kM2LThreads), 0, s, lt, t.flags.as<int>(),
+    constexpr int RS = kM2LRowSplit;
+    auto kern = PD == 32 ? (frame ? k_m2l_all_f32<32, true, RS> : k_m2l_all_f32<32, false, RS>)
+                         : (frame ? k_m2l_all_f32<64, true, RS> : k_m2l_all_f32<64, false, RS>);
+    hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(kM2LThreads * RS), 0, s, lt, t.flags.as<int>(),
                        t.m2l.as<int>(), P.mult.as<double>(), P.qs.as<double>(),
                        P.ops.as<double>(), opsz, P.ple.as<double>(), rstride, nstride,
                        t.xmid.as<double>(), t.ymid.as<double>(), t.xmin + 0.5 * t.w0,
                        t.ymin + 0.5 * t.w0);
   } else {
-    auto kern = PD == 32 ? k_m2l_all<32> : k_m2l_all<64>;
-    hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(kM2LThreads), 0, s, lt, t.flags.as<int>(),
+    constexpr int RS = kM2LRowSplit;
+    auto kern = PD == 32 ? k_m2l_all<32, RS> : k_m2l_all<64, RS>;
+    hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(kM2LThreads * RS), 0, s, lt, t.flags.as<int>(),
                        t.m2l.as<int>(), P.mult.as<double>(), P.ops.as<double>(), opsz,
                        P.ple.as<double>(), rstride, nstride, P.m2l_mode == FMM2D_M2L_SVD ? 1 : 0);
   }

@@ -770,163 +770,49 @@ __global__ __launch_bounds__(BS) void k_locate(const double* __restrict__ tx,
 inline unsigned grid_for(int64_t n) { return (unsigned)(n <= 0 ? 1 : ceil_div(n, BS)); }
 
 // --- complete trees --------------------------------------------------------------
-// When every cell of some depth D-1 holds more than ncut points, every cell
-// of depth D at most ncut and at least one, the reference's refinement
-// (split iff > ncut, tree.py:204) gives the complete quadtree of depth D: its
-// leaves all sit on level D, so no 2:1 balance split and no X-list node
-// exists (both need leaves on different levels).  Uniform data give such
-// trees.  They are built from a histogram of the points over the cells of
-// depth Dh (= D or D + 1), a scan, a scatter into cell order and closed forms
-// of every node array, with one host read-back (of the five count extremes
-// that decide D) instead of the general path's sort rounds and size reads.
-// Points inside a depth-Dh cell keep input order, the order the general
-// path's stable radix rounds leave them in.
-constexpr int kFullMaxDepth = 10;      // histograms of at most 4^10 cells
-constexpr int kFullCellMax = 64;       // points of a depth-Dh cell ordered by one wave
-enum { FS_MAXH = 0, FS_MINH = 1, FS_MAXH1 = 2, FS_MINH1 = 3, FS_MINH2 = 4, FS_N = 5 };
+// When the split cells of every depth d < D are all 4^d cells of that depth
+// and no cell of depth D splits, the reference's refinement (split iff more
+// than ncut points, tree.py:204) has built the complete quadtree of depth D:
+// its leaves all sit on level D, so no 2:1 balance split and no X-list node
+// exists (both need leaves on different levels), and every sibling group of
+// the levels >= 2 holds a source (its parent has more than ncut).  Uniform
+// data give such trees.  After the sort and the split-cell counts that every
+// build reads back, such a tree takes closed forms of all node arrays in one
+// kernel instead of the split lists, their scans and the two further read-backs.
+constexpr int kFullMaxDepth = 15;   // depth-D cell index tables of at most 4^15 ints
 
-__global__ __launch_bounds__(BS) void k_full_init(int* __restrict__ hist, int64_t nh,
-                                                  int* __restrict__ stats) {
-  const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (i < nh) hist[i] = 0;
-  if (i < FS_N) stats[i] = (i == FS_MAXH || i == FS_MAXH1) ? 0 : 0x7fffffff;
-}
-
-// Keys (as k_point_keys) and the depth-dh histogram.
-__global__ __launch_bounds__(BS) void k_full_hist(const double* __restrict__ x,
-                                                  const double* __restrict__ y, int n,
-                                                  double xmin0, double ymin0, double w0, int dh,
-                                                  uint64_t* __restrict__ keys_in,
-                                                  int* __restrict__ hist,
-                                                  int* __restrict__ nonfinite) {
+// first sorted point of each nonempty depth-D cell (the others stay -1)
+__global__ __launch_bounds__(BS) void k_full_starts(const uint64_t* __restrict__ keys,
+                                                    const int2* __restrict__ sc, int n, int D,
+                                                    int* __restrict__ start) {
   const int i = blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
-  const double px = x[i], py = y[i];
-  if (!(__builtin_isfinite(px) && __builtin_isfinite(py))) atomicOr(nonfinite, 1);
-  double xmin = xmin0, ymin = ymin0, w = w0;
-  uint64_t key = 0;
-  for (int l = 1; l <= kMaxDepth; ++l) {
-    const double h = 0.5 * w;
-    const double xm = xmin + h;
-    const double ym = ymin + h;
-    const int bx = px > xm;
-    const int by = py > ym;
-    key |= (uint64_t)(2 * bx + by) << (64 - 2 * l);
-    xmin = bx ? xm : xm - h;
-    ymin = by ? ym : ym - h;
-    w = h;
-  }
-  keys_in[i] = key;
-  atomicAdd(&hist[key >> (64 - 2 * dh)], 1);
+  if (sc[i].x < D) start[keys[i] >> (64 - 2 * D)] = i;
 }
 
-// Extremes of the cell counts at depths dh, dh-1 (max and min) and dh-2 (min);
-// one thread per depth-(dh-2) cell (its 16 depth-dh cells).
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__global__ __launch_bounds__(BS) void k_full_check(const int* __restrict__ hist, int64_t nc2,
-                                                   int* __restrict__ stats) {
-  const int64_t c = (int64_t)blockIdx.x * BS + threadIdx.x;
-  int maxh = 0, minh = 0x7fffffff, maxh1 = 0, minh1 = 0x7fffffff, minh2 = 0x7fffffff;
-  if (c < nc2) {
-    const int4* h4 = reinterpret_cast<const int4*>(hist + 16 * c);
-    int s2 = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int4 v = h4[q];
-      const int s1 = v.x + v.y + v.z + v.w;
-      maxh = max(maxh, max(max(v.x, v.y), max(v.z, v.w)));
-      minh = min(minh, min(min(v.x, v.y), min(v.z, v.w)));
-      maxh1 = max(maxh1, s1);
-      minh1 = min(minh1, s1);
-      s2 += s1;
-    }
-    minh2 = s2;
-  }
-  maxh = wave_max(maxh);
-  minh = wave_min(minh);
-  maxh1 = wave_max(maxh1);
-  minh1 = wave_min(minh1);
-  minh2 = wave_min(minh2);
-  if ((threadIdx.x & 63) == 0) {
-    atomicMax(stats + FS_MAXH, maxh);
-    atomicMin(stats + FS_MINH, minh);
-    atomicMax(stats + FS_MAXH1, maxh1);
-    atomicMin(stats + FS_MINH1, minh1);
-    atomicMin(stats + FS_MINH2, minh2);
-  }
+// points of a depth-d cell c: [bot, top) from the depth-D starts (a cell
+// without points sits where its keys would be)
+__device__ __forceinline__ int full_pos(const int* __restrict__ start,
+                                        const uint64_t* __restrict__ keys, int n, int D, int d,
+                                        uint64_t c) {
+  const uint64_t leaf = c << (2 * (D - d));
+  if (leaf >= ((uint64_t)1 << (2 * D))) return n;
+  const int v = start[leaf];
+  return v >= 0 ? v : (int)lower_bound_u64(keys, n, leaf << (64 - 2 * D));
 }
 
-// Points into cell order (off = exclusive scan of the counts; the counts are
-// used up as cursors).
-__global__ __launch_bounds__(BS) void k_full_scatter(const uint64_t* __restrict__ keys_in,
-                                                     const double* __restrict__ x,
-                                                     const double* __restrict__ y, int n,
-                                                     int dh, const int* __restrict__ off,
-                                                     int* __restrict__ cursor,
-                                                     uint64_t* __restrict__ keys,
-                                                     uint32_t* __restrict__ perm,
-                                                     double* __restrict__ sx,
-                                                     double* __restrict__ sy) {
-  const int i = blockIdx.x * BS + threadIdx.x;
-  if (i >= n) return;
-  const uint64_t k = keys_in[i];
-  const uint64_t cell = k >> (64 - 2 * dh);
-  const int pos = off[cell] + atomicSub(&cursor[cell], 1) - 1;
-  keys[pos] = k;
-  perm[pos] = (uint32_t)i;
-  sx[pos] = x[i];
-  sy[pos] = y[i];
-}
-
-// Input order inside each depth-dh cell: one wave per cell ranks its (at most
-// 64) points by input index.
-__global__ __launch_bounds__(BS) void k_full_order(const int* __restrict__ off, int64_t ncell,
-                                                   uint64_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ perm,
-                                                   double* __restrict__ sx,
-                                                   double* __restrict__ sy) {
-  const int64_t cell = (int64_t)blockIdx.x * (BS / 64) + (threadIdx.x >> 6);
-  if (cell >= ncell) return;
-  const int lane = threadIdx.x & 63;
-  const int b = off[cell], cnt = off[cell + 1] - b;
-  if (cnt <= 1) return;
-  const bool act = lane < cnt;
-  const uint32_t pv = act ? perm[b + lane] : 0xffffffffu;
-  const uint64_t kv = act ? keys[b + lane] : 0;
-  const double xv = act ? sx[b + lane] : 0.0, yv = act ? sy[b + lane] : 0.0;
-  int rank = 0;
-  for (int j = 0; j < cnt; ++j) rank += __shfl(pv, j, 64) < pv;
-  if (act) {
-    keys[b + rank] = kv;
-    perm[b + rank] = pv;
-    sx[b + rank] = xv;
-    sy[b + rank] = yv;
-  }
-}
+__device__ __forceinline__ int full_levoff(int d) { return (int)((((int64_t)1 << (2 * d)) - 1) / 3); }
 
 // Every node array of the complete tree of depth D (levels 0 .. D, level d =
 // the 4^d cells in Morton order from levoff(d) = (4^d - 1) / 3).
-__device__ __forceinline__ int full_levoff(int d) { return (int)((((int64_t)1 << (2 * d)) - 1) / 3); }
-__device__ __forceinline__ int full_bot(const int* __restrict__ off, int dh, int d, uint64_t c) {
-  return off[c << (2 * (dh - d))];
-}
 __global__ __launch_bounds__(BS) void k_full_nodes(
-    int D, int nn, int dh, const int* __restrict__ off, double xmin0, double ymin0, double w0,
-    uint64_t* __restrict__ code, int* __restrict__ nlev, int* __restrict__ flags,
-    int* __restrict__ bot, int* __restrict__ top, int* __restrict__ parent,
-    int* __restrict__ child, int* __restrict__ coll, int2* __restrict__ crange,
-    double* __restrict__ xmid, double* __restrict__ ymid, int* __restrict__ row,
-    int* __restrict__ m2l, int* __restrict__ evalid, int* __restrict__ evalnode,
-    int* __restrict__ upnodes, int2* __restrict__ rowinfo) {
+    int D, int nn, const int* __restrict__ start, const uint64_t* __restrict__ keys, int npts,
+    double xmin0, double ymin0, double w0, uint64_t* __restrict__ code, int* __restrict__ nlev,
+    int* __restrict__ flags, int* __restrict__ bot, int* __restrict__ top,
+    int* __restrict__ parent, int* __restrict__ child, int* __restrict__ coll,
+    int2* __restrict__ crange, double* __restrict__ xmid, double* __restrict__ ymid,
+    int* __restrict__ row, int* __restrict__ m2l, int* __restrict__ evalid,
+    int* __restrict__ evalnode, int* __restrict__ upnodes, int2* __restrict__ rowinfo) {
   const int g = blockIdx.x * BS + threadIdx.x;
   if (g >= nn) return;
   int d = 0;
@@ -934,7 +820,7 @@ __global__ __launch_bounds__(BS) void k_full_nodes(
   const int lo = full_levoff(d);
   const uint64_t c = (uint64_t)(g - lo);
   const bool leaf = d == D;
-  const int b = full_bot(off, dh, d, c), t = full_bot(off, dh, d, c + 1);
+  const int b = full_pos(start, keys, npts, D, d, c), t = full_pos(start, keys, npts, D, d, c + 1);
   int f = leaf ? NF_LEAF : (NF_CHILDREN | NF_DOWN);   // tree.py:557-562
   if (t > b) f |= NF_SOURCE | (leaf ? NF_UP : 0);
   code[g] = c;
@@ -967,7 +853,8 @@ __global__ __launch_bounds__(BS) void k_full_nodes(
     if (nx >= 0 && ny >= 0 && nx < lim && ny < lim) {
       const uint64_t cc = morton((uint32_t)nx, (uint32_t)ny);
       r = lo + (int)cc;
-      rg = make_int2(full_bot(off, dh, d, cc), full_bot(off, dh, d, cc + 1));
+      rg = make_int2(full_pos(start, keys, npts, D, d, cc),
+                     full_pos(start, keys, npts, D, d, cc + 1));
     }
     coll[(int64_t)g * 9 + q] = r;
     crange[(int64_t)g * 9 + q] = rg;
@@ -980,7 +867,7 @@ __global__ __launch_bounds__(BS) void k_full_nodes(
   evalid[g] = leaf ? (int)c : -1;
   if (leaf) {
     evalnode[c] = g;
-    upnodes[c] = g;
+    upnodes[c] = g;   // every leaf: an empty one has S2M check values 0 (no sources)
   }
   if (d >= 2) {
     const int cx = ix & 1, cy = iy & 1;
@@ -999,61 +886,10 @@ __global__ __launch_bounds__(BS) void k_full_nodes(
 
 }  // namespace
 
-// Complete-tree build (see k_full_init above); false: the points do not form
-// a complete tree (or it is out of the fast path's limits), nothing is set.
-static bool tree_build_full(Tree& t, const double* x, const double* y, int64_t n, double xmin,
-                            double ymin, double w0, int ncut, hipStream_t s) {
-  int dstar = 0;
-  while (dstar < kFullMaxDepth && ((int64_t)ncut << (2 * dstar)) < n) ++dstar;
-  const int dh = dstar + 1;
-  if (dstar < 2 || dh > kFullMaxDepth) return false;
-  const int nn_pts = (int)n;
-  const int64_t ncell = (int64_t)1 << (2 * dh);
-  t.keys.ensure(n * sizeof(uint64_t));
-  t.perm.ensure(n * sizeof(uint32_t));
-  t.sx.ensure(n * sizeof(double));
-  t.sy.ensure(n * sizeof(double));
-  t.xy.ensure(n * sizeof(uint64_t));   // keys in input order
-  t.w_c.ensure((ncell + 1 + 64) * sizeof(int));
-  t.w_cnt.ensure((FS_N + 1) * sizeof(int));
-  t.h_cnt.ensure((FS_N + 1) * sizeof(int));
-  int* hist = t.w_c.as<int>();
-  int* stats = t.w_cnt.as<int>();
-  int* bad = stats + FS_N;
-  int* hc = t.h_cnt.as<int>();
-  hipLaunchKernelGGL(k_full_init, dim3(grid_for(ncell + 1 + FS_N + 1)), dim3(BS), 0, s, hist,
-                     ncell + 1, stats);
-  FMM_LAUNCH_CHECK();
-  fill_i32(bad, 1, 0, s);
-  uint64_t* keys_in = t.xy.as<uint64_t>();
-  hipLaunchKernelGGL(k_full_hist, dim3(grid_for(n)), dim3(BS), 0, s, x, y, nn_pts, xmin, ymin, w0,
-                     dh, keys_in, hist, bad);
-  FMM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_full_check, dim3(grid_for(ncell / 16)), dim3(BS), 0, s, hist, ncell / 16,
-                     stats);
-  FMM_LAUNCH_CHECK();
-  FMM_HIP(hipMemcpyAsync(hc, stats, (FS_N + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
-  FMM_HIP(hipStreamSynchronize(s));
-  FMM_REQUIRE(hc[FS_N] == 0, FMM_E_ARG, "tree: the source coordinates hold a NaN or infinite value");
-  int D = -1;
-  if (hc[FS_MAXH1] <= ncut && hc[FS_MINH2] > ncut && hc[FS_MINH1] >= 1)
-    D = dh - 1;
-  else if (hc[FS_MAXH] <= ncut && hc[FS_MINH1] > ncut && hc[FS_MINH] >= 1)
-    D = dh;
-  if (D < 2 || hc[FS_MAXH] > kFullCellMax) return false;
-  // cell offsets (ncell + 1 entries: the last count is zero, so off[ncell] = n)
-  t.w_a.ensure((ncell + 2) * sizeof(int));
-  int* off = t.w_a.as<int>();
-  exclusive_scan_i32(hist, off, ncell + 1, nullptr, t.pw, s);
-  hipLaunchKernelGGL(k_full_scatter, dim3(grid_for(n)), dim3(BS), 0, s, keys_in, x, y, nn_pts, dh,
-                     off, hist, t.keys.as<uint64_t>(), t.perm.as<uint32_t>(), t.sx.as<double>(),
-                     t.sy.as<double>());
-  FMM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_full_order, dim3((unsigned)ceil_div(ncell, BS / 64)), dim3(BS), 0, s, off,
-                     ncell, t.keys.as<uint64_t>(), t.perm.as<uint32_t>(), t.sx.as<double>(),
-                     t.sy.as<double>());
-  FMM_LAUNCH_CHECK();
-  // node arrays
+// Node arrays of a complete tree of depth D from the sorted points (keys and
+// the common depths sc of neighbouring keys); see k_full_nodes.
+template <class Upload>
+static void tree_build_full(Tree& t, int D, int npts, hipStream_t s, Upload&& upload) {
   const int L = D + 1;
   t.L = L;
   t.lev.assign(L, TreeLevel());
@@ -1062,7 +898,7 @@ static bool tree_build_full(Tree& t, const double* x, const double* y, int64_t n
   for (int d = 0; d < L; ++d) {
     t.lev[d].node_off = levoff[d];
     t.lev[d].n_nodes = (int64_t)1 << (2 * d);
-    t.lev[d].width = std::ldexp(w0, -d);
+    t.lev[d].width = std::ldexp(t.w0, -d);
   }
   const int NN = levoff[L];
   t.n_nodes = NN;
@@ -1074,10 +910,14 @@ static bool tree_build_full(Tree& t, const double* x, const double* y, int64_t n
   t.n_up = t.n_eval;
   t.nidx_off.assign(kMaxLevels, -1);
   t.levoff.ensure((L + 1) * sizeof(int));
-  t.h_up.ensure(8 * (kMaxLevels + 2) * sizeof(int64_t));
-  std::memcpy(t.h_up.ptr, levoff.data(), (L + 1) * sizeof(int));
-  FMM_HIP(hipMemcpyAsync(t.levoff.ptr, t.h_up.ptr, (L + 1) * sizeof(int), hipMemcpyHostToDevice,
-                         s));
+  upload(t.levoff.ptr, levoff.data(), (L + 1) * sizeof(int));
+  const int64_t ncell = (int64_t)1 << (2 * D);
+  t.w_flag.ensure((ncell + 1) * sizeof(int));
+  int* start = t.w_flag.as<int>();
+  fill_i32(start, ncell, -1, s);
+  hipLaunchKernelGGL(k_full_starts, dim3(grid_for(npts)), dim3(BS), 0, s, t.keys.as<uint64_t>(),
+                     t.w_a.as<int2>(), npts, D, start);
+  FMM_LAUNCH_CHECK();
   t.code.ensure(NN * sizeof(uint64_t));
   t.nlevel.ensure(NN * sizeof(int));
   t.bot.ensure(NN * sizeof(int));
@@ -1095,15 +935,16 @@ static bool tree_build_full(Tree& t, const double* x, const double* y, int64_t n
   t.evalnode.ensure((t.n_eval + 1) * sizeof(int));
   t.upnodes.ensure((t.n_up + 1) * sizeof(int));
   t.rowinfo.ensure((t.n_rows + 1) * sizeof(int2));
-  // levels 0 and 1 have no M2L lists; their rows stay unset as in the general path
-  hipLaunchKernelGGL(k_full_nodes, dim3(grid_for(NN)), dim3(BS), 0, s, D, NN, dh, off, xmin, ymin,
-                     w0, t.code.as<uint64_t>(), t.nlevel.as<int>(), t.flags.as<int>(),
-                     t.bot.as<int>(), t.top.as<int>(), t.parent.as<int>(), t.child.as<int>(),
-                     t.coll.as<int>(), t.crange.as<int2>(), t.xmid.as<double>(),
-                     t.ymid.as<double>(), t.row.as<int>(), t.m2l.as<int>(), t.evalid.as<int>(),
-                     t.evalnode.as<int>(), t.upnodes.as<int>(), t.rowinfo.as<int2>());
+  // levels 0 and 1 have no M2L lists (their entries are never read), as in
+  // the general build
+  hipLaunchKernelGGL(k_full_nodes, dim3(grid_for(NN)), dim3(BS), 0, s, D, NN, start,
+                     t.keys.as<uint64_t>(), npts, t.xmin, t.ymin, t.w0, t.code.as<uint64_t>(),
+                     t.nlevel.as<int>(), t.flags.as<int>(), t.bot.as<int>(), t.top.as<int>(),
+                     t.parent.as<int>(), t.child.as<int>(), t.coll.as<int>(), t.crange.as<int2>(),
+                     t.xmid.as<double>(), t.ymid.as<double>(), t.row.as<int>(), t.m2l.as<int>(),
+                     t.evalid.as<int>(), t.evalnode.as<int>(), t.upnodes.as<int>(),
+                     t.rowinfo.as<int2>());
   FMM_LAUNCH_CHECK();
-  return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -1120,11 +961,6 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   t.ymin = ymin;
   t.w0 = w0;
   const int nn = (int)n;
-  if (t.allow_full && tree_build_full(t, x, y, n, xmin, ymin, w0, ncut, s)) {
-    ++t.build_id;
-    t.built = true;
-    return;
-  }
 
   // -- sorted points ----------------------------------------------------------
   t.keys.ensure(n * sizeof(uint64_t));
@@ -1207,6 +1043,19 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   const int L = maxd + 2;   // deepest split depth + children level + 1
   t.L = L;
   FMM_REQUIRE(L <= kMaxLevels, FMM_E_INTERNAL, "tree: %d levels exceed the depth cap", L);
+
+  // a complete tree (every cell of the depths above the deepest split cells
+  // split): closed forms of the node arrays (k_full_nodes)
+  if (t.allow_full && L >= 3 && L - 1 <= kFullMaxDepth) {
+    bool complete = true;
+    for (int d = 0; d <= L - 2 && complete; ++d) complete = dcount[d] == ((int64_t)1 << (2 * d));
+    if (complete) {
+      tree_build_full(t, L - 1, nn, s, upload);
+      ++t.build_id;
+      t.built = true;
+      return;
+    }
+  }
 
   // split-cell lists per depth (S_ref)
   t.s_off.assign(L + 1, 0);
