@@ -681,8 +681,8 @@ __constant__ ValidSlots c_valid;
 // gives about 1.4 waves per SIMD with RS = 1).
 constexpr int kM2LThreads = 256;   // threads per block and row split
 constexpr int kM2LCols = 64;
-constexpr int kM2LRowSplit = 2;
-constexpr int kFlushM = 8;   // FP32 M2L: k-step pairs per FP32 run (8: one run per slot at PD 64)
+constexpr int kM2LRowSplit = 1;   // blocks of 4 waves, each wave all PD rows of 16 boxes
+constexpr int kFlushM = 2;   // FP32 M2L: k-step pairs per FP32 run (4 runs per slot at PD 64)
 
 struct M2LSetup {
   int li, c, col, kq, g, nph, rh;
@@ -2634,7 +2634,8 @@ static void m2l_launch(Plan& P, const M2LTables& T, int l0, int l1, hipStream_t 
   // setup costs more than the balance gains, as measured on 1M points)
   lt.split = !m2l_in_fp32(P) && (int64_t)nb * nd < 3 * kNumCUs ? 2 : 1;
   if (lt.split == 2) {
-    FMM_HIP(hipMemsetAsync(P.ple.ptr, 0, nstride * PD * nd * sizeof(double), s));
+    // zero with a fill kernel (no memset node: the call may sit in a captured graph)
+    fill_i32(P.ple.as<int>(), 2 * nstride * PD * nd, 0, s);
     nb *= 2;
   }
   if (m2l_in_fp32(P)) {
@@ -2764,7 +2765,7 @@ static void downward(Plan& P, hipStream_t s) {
   const int64_t rstride = t.n_rows + 1, nstride = t.n_nodes + 1;
   P.le.ensure(nstride * PD * nd * sizeof(double));
   P.ple.ensure(nstride * PD * nd * sizeof(double));
-  FMM_HIP(hipMemsetAsync(P.le.ptr, 0, nstride * PD * nd * sizeof(double), s));
+  fill_i32(P.le.as<int>(), 2 * nstride * PD * nd, 0, s);   // 0.0 (no memset node)
   if (t.L <= 2) {
     P.have_exp = true;
     return;

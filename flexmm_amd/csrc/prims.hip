@@ -495,6 +495,7 @@ bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
     nfin_early = nfinal;
     nbig_early = nbig;
     ran = true;
+    const int64_t active_in = active;
     const int shift = 56 - 8 * r;
     const int64_t cbound = nseg + active / CHUNK + 1;
     w.chunks.ensure(cbound * sizeof(int4));
@@ -522,7 +523,11 @@ bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
     nfinal = hc[C_FINAL];
     nbig = hc[C_FINAL_BIG];
     cur ^= 1;
-    const bool finished_some = nfinal + nbig > nfin_early + nbig_early;
+    // Elements that left the active set finished in this round, in segments of
+    // any length (one-element segments are on no list, but their place in
+    // kout is final too): they must reach keys / vals before a later round
+    // writes its output there instead.
+    const bool finished_some = active < active_in;
     const bool more = nseg > 0 && r + 1 < rounds;
     if (kout != keys && (finished_some || !more)) {
       hipLaunchKernelGGL(k_chunk_copy, dim3(cbound), dim3(SORT_BS), 0, s, kout, vout,

@@ -97,11 +97,17 @@ def test_bbfmm_modified_helmholtz_matches_oracle():
 def test_bbfmm_mixed(precision, tol):
     from flexmm_amd import kernels
     from flexmm_amd.flexmm2d import BB_FMM
+    from flexmm_amd.flexmm2d import bbfmm
     x, y, tau = H.uniform(40000, 350)
     tx, ty, _ = H.uniform(5000, 351)
+    # a cleared helper cache each time: a warm one snaps the second root box
+    # to its cached widths (helpers.py:72-80), a different tree
+    bbfmm.the_bb_help.clear()
     ref = BB_FMM(x, y, kernels.laplace, Ncutoff=50, p=8)
     ref.build_expansions(tau)
+    bbfmm.the_bb_help.clear()
     f = BB_FMM(x, y, kernels.laplace, Ncutoff=50, p=8, precision=precision)
+    assert f.bbox == ref.bbox
     f.build_expansions(tau)
     e1 = H.rel_max_err(f.source_evaluation(x, y), ref.source_evaluation(x, y))
     e2 = H.rel_max_err(f.target_evaluation(tx, ty), ref.target_evaluation(tx, ty))

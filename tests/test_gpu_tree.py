@@ -67,6 +67,23 @@ def test_tree_duplicates_below_cutoff():
     H.compare_canonical(H.gpu_canonical(f), ora.canonical())
 
 
+def test_tree_isolated_points_finish_alone_in_a_sort_round():
+    """Points alone in their depth-4 cell while every other depth-4 cell holds
+    more than Ncutoff: the first radix round of the key sort (depths 1-4)
+    finishes one-element segments only.  The sorted permutation still holds
+    every point once, and the tree equals the oracle's."""
+    rng = np.random.default_rng(31)
+    n = 60000
+    # the central block [0.25, 0.75]^2 is exactly 8 x 8 depth-4 cells
+    x = np.concatenate([0.25 + 0.5 * rng.random(n), [0.01, 0.97, 0.02]])
+    y = np.concatenate([0.25 + 0.5 * rng.random(n), [0.02, 0.015, 0.99]])
+    f = _build(x, y, 64, bbox=[0, 1, 0, 1])
+    perm = f.tree_array("perm").cpu().numpy().astype(np.int64)
+    assert np.array_equal(np.sort(perm), np.arange(x.size))
+    ora = _oracle_tree(x, y, 64, f.bbox)
+    assert H.compare_trees_fast(f, ora) == sum(l.n_node for l in ora.Levels)
+
+
 def test_tree_uniform_counts_full():
     """A full uniform tree: levels 0..L-1 hold 4^d nodes and the M2L lists hold
     the closed-form number of pairs."""

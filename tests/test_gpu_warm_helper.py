@@ -63,6 +63,48 @@ def test_warm_helper_snaps_bbox_and_reuses_operators():
     assert okifmm.get_bbox(x2, y2, None)[1] - okifmm.get_bbox(x2, y2, None)[0] < 0.6
 
 
+def test_second_fmm_on_the_same_points_takes_twice_the_box():
+    """The common reuse: a second FMM on the very same points.  The cached
+    smallest width (the first tree's deepest, rounded to 12 decimals,
+    float_dict.py:23-24) divides the data's extent 64 (1 + 3e-11) times, so
+    the snapped root box is twice the first one (helpers.py:72-80): the points
+    fill its central quarter and the extreme ones sit alone in outer cells.
+    The device tree and potentials equal the oracle's with the same widths
+    and cache history.  The two FMMs differ by more than parity (6.5e-12
+    measured): the second one's deepest level takes operators formed for a
+    width 3e-11 away from its own, which the reference does as well."""
+    from flexmm_amd import kernels
+    from flexmm_amd.kifmm2d import FMM, the_help
+    the_help.clear()
+    x, y, tau = H.uniform(120000, 43)
+    f1 = FMM(x, y, kernels.laplace, 64, 60)
+    f1.build_expansions(tau)
+    u1 = f1.source_evaluation(x, y)[0]
+    widths = f1.helper.collect_widths().tolist()
+    print("same points, second FMM", flush=True)
+    f2 = FMM(x, y, kernels.laplace, 64, 60)
+    assert f2.bbox == okifmm.get_bbox(x, y, None, widths=widths)
+    ratio = (f2.bbox[1] - f2.bbox[0]) / (f1.bbox[1] - f1.bbox[0])
+    assert abs(ratio - 2.0) < 1e-9
+    # the oracle's operator cache (keyed by round(width, 12) like the
+    # reference's FloatDict) gets the first FMM's widths first, as the
+    # device helper did: the second tree's deepest width, round(w0 / 64, 12),
+    # then reuses the operators formed for w0 / 64 itself
+    okifmm._PRECOMP_CACHE.clear()
+    o1 = okifmm.FMM(x, y, okern.laplace, 64, 60)
+    assert o1.bbox == f1.bbox
+    o = okifmm.FMM(x, y, okern.laplace, 64, 60, widths=widths)
+    assert o.bbox == f2.bbox
+    assert H.compare_trees_fast(f2, o.tree) == sum(l.n_node for l in o.tree.Levels)
+    f2.build_expansions(tau)
+    u2 = f2.source_evaluation(x, y)[0]
+    o.build_expansions(tau)
+    uo = o.source_evaluation(x, y)[0]
+    err, e12 = H.rel_max_err(u2, uo), H.rel_max_err(u2, u1)
+    print("snapped second FMM vs oracle %.3e, vs the first FMM %.3e" % (err, e12), flush=True)
+    assert err < 1e-12 and e12 < 1e-10
+
+
 def test_nonfinite_sources_are_refused():
     from flexmm_amd import _lib, kernels
     from flexmm_amd.kifmm2d import FMM, the_help
@@ -101,22 +143,32 @@ def test_nonfinite_targets_give_nan_outputs_only_there(precision):
 
 
 def test_targets_outside_the_root_box_follow_the_reference():
+    """Targets outside the root box reach an edge leaf (tree.py:703-727) and
+    take its local expansion and near field, extrapolated.  Within 3e-3 of
+    the box (a fifth of a leaf here) that matches the oracle to parity; up to
+    1e-2 outside the extrapolation multiplies the rounding of the local
+    equivalent densities (their pseudo-inverse leaves large near-null
+    components) by the distance: device and oracle then differ from the
+    direct sum by up to ~2e-10 and from each other by up to ~1e-12."""
+    import torch
     from flexmm_amd import kernels
     from flexmm_amd.kifmm2d import FMM, the_help
     the_help.clear()
     x, y, tau = H.uniform(30000, 206)
     rng = np.random.default_rng(207)
-    # just outside each side of the unit box (far outside, the extrapolation
-    # multiplies the rounding of the local equivalent densities, whose
-    # pseudo-inverse leaves large near-null components, by the distance)
-    tx = np.concatenate([-0.01 * rng.random(200), 1 + 0.01 * rng.random(200), rng.random(400)])
-    ty = np.concatenate([rng.random(400), -0.01 * rng.random(200), 1 + 0.01 * rng.random(200)])
     f = FMM(x, y, kernels.laplace, 64, 60, bbox=[0, 1, 0, 1])
     f.build_expansions(tau)
-    u = f.target_evaluation(tx, ty)[0]
     o = okifmm.FMM(x, y, okern.laplace, 64, 60, bbox=[0, 1, 0, 1])
     o.build_expansions(tau)
-    uo = o.target_evaluation(tx, ty)[0]
-    err = H.rel_max_err(u, uo)
-    print("outside targets vs oracle %.3e" % err, flush=True)
-    assert err < 1e-12
+    for dist, tol_oracle, tol_direct in ((3e-3, 1e-12, 1e-12), (1e-2, 1e-11, 1e-9)):
+        r, t = rng.random(200), rng.random(200)
+        # 200 targets beyond each side of the unit box, up to dist outside
+        tx = np.concatenate([-dist * r, 1 + dist * r, t, t])
+        ty = np.concatenate([t, t, -dist * r, 1 + dist * r])
+        u = f.target_evaluation(tx, ty)[0]
+        uo = o.target_evaluation(tx, ty)[0]
+        dev = lambda a: torch.from_numpy(a).cuda()
+        ref = H.torch_direct_laplace(torch, dev(x), dev(y), dev(tau), dev(tx), dev(ty)).cpu().numpy()
+        err, errd = H.rel_max_err(u, uo), H.rel_max_err(u, ref)
+        print("outside by <= %g: vs oracle %.3e, vs direct %.3e" % (dist, err, errd), flush=True)
+        assert err < tol_oracle and errd < tol_direct
