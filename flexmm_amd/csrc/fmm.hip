@@ -97,7 +97,7 @@ struct Plan {
   double kparam = 0.0;
   int p = 0, PD = 0, precision = PREC_FP64;
   int cheb = 0;        // Chebyshev order of a black-box plan (p = cheb^2); 0: KIFMM
-  int m2l_mode = FMM2D_M2L_SVD;   // FP64 M2L: compressed slots as in the reference, or dense
+  int m2l_mode = FMM2D_M2L_DENSE;   // FP64 M2L: every slot dense, or compressed as the reference
   Tree tree;
   DevBuf ops;
   int ops_levels = 0;
@@ -655,9 +655,10 @@ struct LevelTable {
   // child position c at level li is perm[poff[li] + c * ngroups[li] + qq]
   const int* perm;
   int poff[kMaxLevels];
-  // 2: each tile runs as two blocks taking the first and second half of its
-  // phases and adding into a zeroed PLE (two addends onto 0 give the same sum
-  // in either order); used when few tiles would leave CUs unevenly loaded
+  // > 1: each tile runs as `split` neighbouring blocks, each taking an equal
+  // share of its phases and adding into a zeroed PLE with FP64 atomics; used
+  // when few tiles would leave CUs idle.  Two addends onto 0 give the same sum
+  // in either order; more parts make the sum's rounding depend on the order.
   int split;
 };
 
@@ -682,6 +683,8 @@ __constant__ ValidSlots c_valid;
 constexpr int kM2LThreads = 256;   // threads per block and row split
 constexpr int kM2LCols = 64;
 constexpr int kM2LRowSplit = 1;   // blocks of 4 waves, each wave all PD rows of 16 boxes
+constexpr int kM2LSplitF64 = 2;   // blocks per tile when tiles are few (FP64, FP32 M2L)
+constexpr int kM2LSplitF32 = 1;
 constexpr int kFlushM = 2;   // FP32 M2L: k-step pairs per FP32 run (4 runs per slot at PD 64)
 
 struct M2LSetup {
@@ -697,8 +700,8 @@ __device__ __forceinline__ M2LSetup m2l_setup(const LevelTable& lt, const int* _
                                               unsigned long long& smask) {
   M2LSetup S;
   const int bid0 = xcd_block(blockIdx.x, gridDim.x);
-  const int half = lt.split == 2 ? (bid0 & 1) : 0;   // the two halves of a tile are neighbours
-  const int bid = lt.split == 2 ? (bid0 >> 1) : bid0;
+  const int part = bid0 % lt.split;   // the parts of a tile are neighbours
+  const int bid = bid0 / lt.split;
   int li = 0;
   while (li + 1 < lt.nlev && bid >= lt.boff[li + 1]) ++li;
   const int lb = bid - lt.boff[li];
@@ -733,9 +736,8 @@ __device__ __forceinline__ M2LSetup m2l_setup(const LevelTable& lt, const int* _
       pslot[k++] = __builtin_ctzll(m);
       m &= m - 1ull;
     }
-    if (lt.split == 2) {   // this block's half of the phases
-      const int h = (k + 1) / 2;
-      const int lo = half ? h : 0, hi = half ? k : h;
+    if (lt.split > 1) {   // this block's share of the phases
+      const int lo = k * part / lt.split, hi = k * (part + 1) / lt.split;
       for (int i = lo; i < hi; ++i) pslot[i - lo] = pslot[i];
       k = hi - lo;
     }
@@ -928,7 +930,7 @@ __global__ __launch_bounds__(kM2LThreads * RS) void k_m2l_all(LevelTable lt,
   }
   if (S.active) {
     double* rowp = ple + (int64_t)S.g * PD + 16 * rb0;
-    if (lt.split == 2) {
+    if (lt.split > 1) {
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
@@ -1097,7 +1099,7 @@ __global__ __launch_bounds__(kM2LThreads * RS) void k_m2l_all_f32(
   }
   if (S.active) {
     double* rowp = ple + (int64_t)S.g * PD + 16 * rb0;
-    if (lt.split == 2) {
+    if (lt.split > 1) {
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
@@ -1899,7 +1901,7 @@ __attribute__((amdgpu_waves_per_eu(KER == KER_MODHELM && NOUT == 3 ? 3 : 1))) vo
 // pair gives exactly zero and is excluded as in FP64.  The log is the hardware
 // log2 of d^2 / w^2 (O(1) values keep its absolute error small);
 // sum_j t_j log d_j^2 = ln2 * sum_j t_j log2(d_j^2/w^2) + log(w^2) * sum_j t_j.
-constexpr float kNearF32 = 1.0f / 256.0f;   // (w / 16)^2 in units of w^2
+constexpr float kNearF32 = 1.0f / 1024.0f;   // (w / 32)^2 in units of w^2
 
 struct LevelConsts {
   double invw[kMaxLevels];
@@ -1946,8 +1948,9 @@ __device__ __forceinline__ void eval_few_mixed(
   const int t = t0 + F.slot;
   Target<KER_LAPLACE, 1, CHECK> T;
   T.kp = HelmConsts{};
-  T.x = F.act ? tsx[t] : 0.0;
-  T.y = F.act ? tsy[t] : 0.0;
+  // idle lanes repeat the item's first target (same branches as a real lane)
+  T.x = tsx[F.act ? t : t0];
+  T.y = tsy[F.act ? t : t0];
   const int d = nlevel[g];
   double far = 0.0;   // Chebyshev L2T
   if (CHEB && d >= 2) {
@@ -1995,8 +1998,21 @@ __device__ __forceinline__ void eval_few_mixed(
         f4 s4[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) s4[u] = srcf[j + u * F.ng];
+        float d2[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) pair32(s4[u], j + u * F.ng);
+        for (int u = 0; u < 4; ++u) {
+          const float dx = xr - s4[u].x, dy = yr - s4[u].y;
+          d2[u] = __builtin_fmaf(dy, dy, dx * dx);
+        }
+        const float dm = fminf(fminf(d2[0], d2[1]), fminf(d2[2], d2[3]));
+        if (__builtin_expect(__ballot(dm < kNearF32) == 0, 1)) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            acc = __builtin_fmaf(s4[u].z, __builtin_amdgcn_logf(d2[u]), acc);
+        } else {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) pair32(s4[u], j + u * F.ng);
+        }
       }
       for (; j < e; j += F.ng) pair32(srcf[j], j);
       u32 += (double)acc;
@@ -2051,8 +2067,10 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
     const bool act = t < t1;
     Target<KER_LAPLACE, 1, CHECK> T;
     T.kp = HelmConsts{};
-    T.x = act ? tsx[t] : 0.0;
-    T.y = act ? tsy[t] : 0.0;
+    // idle lanes repeat the item's first target: they then take the same
+    // branches as a real lane (no near-pair path of their own)
+    T.x = tsx[act ? t : t0];
+    T.y = tsy[act ? t : t0];
     const int d = nlevel[g];
     double far = 0.0;   // Chebyshev L2T
     if (CHEB && d >= 2) {
@@ -2083,7 +2101,7 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
         const float yr = (float)((T.y - ymid[c]) * iw);
         float acc = 0.f;
         double excl = 0.0;   // charges of the pairs taken in FP64
-        // pairs closer than w/16 (and coincident ones) go to FP64: the rounding
+        // pairs closer than w/32 (and coincident ones) go to FP64: the rounding
         // of the FP32 relative coordinates costs ~1e-7 w / d in log d
         auto near64 = [&](int jj) {
           const double4 sd = src[jj];
@@ -2093,19 +2111,29 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
         int j = b;
         for (; j + 4 <= e; j += 4) {
           f4 s4[4];
-  #pragma unroll
+#pragma unroll
           for (int u = 0; u < 4; ++u) s4[u] = srcf[j + u];
-          bool nr[4];
-  #pragma unroll
+          float d2[4];
+#pragma unroll
           for (int u = 0; u < 4; ++u) {
             const float dx = xr - s4[u].x, dy = yr - s4[u].y;
-            const float d2 = __builtin_fmaf(dy, dy, dx * dx);
-            nr[u] = d2 < kNearF32;
-            const float lg2 = __builtin_amdgcn_logf(nr[u] ? 1.0f : d2);
-            acc = __builtin_fmaf(nr[u] ? 0.f : s4[u].z, lg2, acc);
+            d2[u] = __builtin_fmaf(dy, dy, dx * dx);
           }
-          if (nr[0] | nr[1] | nr[2] | nr[3]) {
-  #pragma unroll
+          const float dm = fminf(fminf(d2[0], d2[1]), fminf(d2[2], d2[3]));
+          if (__builtin_expect(__ballot(dm < kNearF32) == 0, 1)) {
+            // no near pair in the wave's four sources: no selects
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              acc = __builtin_fmaf(s4[u].z, __builtin_amdgcn_logf(d2[u]), acc);
+          } else {
+            bool nr[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              nr[u] = d2[u] < kNearF32;
+              const float lg2 = __builtin_amdgcn_logf(nr[u] ? 1.0f : d2[u]);
+              acc = __builtin_fmaf(nr[u] ? 0.f : s4[u].z, lg2, acc);
+            }
+#pragma unroll
             for (int u = 0; u < 4; ++u)
               if (nr[u]) near64(j + u);
           }
@@ -2629,14 +2657,13 @@ static void m2l_launch(Plan& P, const M2LTables& T, int l0, int l1, hipStream_t 
   lt.perm = T.adaptive ? P.m2lperm.as<int>() : nullptr;
   int nb = lt.boff[lt.nlev];
   if (nb == 0) return;
-  // fewer than 3 tiles per CU: split the tiles' phases over two blocks each
-  // (FP64 only: the FP32 phases are short enough that the second block
-  // setup costs more than the balance gains, as measured on 1M points)
-  lt.split = !m2l_in_fp32(P) && (int64_t)nb * nd < 3 * kNumCUs ? 2 : 1;
-  if (lt.split == 2) {
+  // fewer than 3 tiles per CU: split the tiles' phases over several blocks
+  const bool few = (int64_t)nb * nd < 3 * kNumCUs;
+  lt.split = few ? (m2l_in_fp32(P) ? kM2LSplitF32 : kM2LSplitF64) : 1;
+  if (lt.split > 1) {
     // zero with a fill kernel (no memset node: the call may sit in a captured graph)
     fill_i32(P.ple.as<int>(), 2 * nstride * PD * nd, 0, s);
-    nb *= 2;
+    nb *= lt.split;
   }
   if (m2l_in_fp32(P)) {
     const bool frame = P.kernel_id == KER_STOKES;

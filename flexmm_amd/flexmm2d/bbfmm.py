@@ -126,7 +126,7 @@ class BB_FMM(_KIFMM):
     """flexmm2d ``BB_FMM`` (bbfmm.py:200-231; passes flexmm2d/fmm.py:33-177) on the GPU."""
 
     def __init__(self, x, y, kernel_eval, Ncutoff=50, p=8, dtype=float, bbox=None, helper=None,
-                 verbose=False, precision="fp64", m2l="svd"):
+                 verbose=False, precision="fp64", m2l="dense"):
         kernel = _kernels.resolve_kernel(kernel_eval)
         if kernel.kernel_id not in (_lib.KERNEL_LAPLACE, _lib.KERNEL_MODIFIED_HELMHOLTZ):
             raise NotImplementedError("black-box FMM kernels: Laplace, modified Helmholtz")

@@ -142,7 +142,7 @@ def _m2l_mode(m2l):
     return _M2L_MODES[m2l]
 
 
-def _acquire_plan(lib, kernel, p, precision, device, cheb=0, m2l="svd"):
+def _acquire_plan(lib, kernel, p, precision, device, cheb=0, m2l="dense"):
     key = (kernel.key, p, precision, device, cheb)
     pool = _plan_pool.setdefault(key, [])
     plan = None
@@ -212,7 +212,7 @@ class FMM:
     """flexmm kifmm2d ``FMM`` (fmm.py:37-199) on the GPU."""
 
     def __init__(self, x, y, kernel_eval, Ncutoff, Nequiv, dtype=float, bbox=None, verbose=False,
-                 precision="fp64", m2l="svd"):
+                 precision="fp64", m2l="dense"):
         kernel = _kernels.resolve_kernel(kernel_eval)
         if np.dtype(dtype) != np.float64:
             raise NotImplementedError("dtype must be float64 (the reference computes in float64)")

@@ -30,7 +30,7 @@ class FMM:
     """stokes_fmm.FMM (stokes_fmm.py:598-842) on the GPU."""
 
     def __init__(self, x, y, Nequiv=48, Ncutoff=50, bbox=None, precomputations=None,
-                 verbose=False, precision="fp64", m2l="svd"):
+                 verbose=False, precision="fp64", m2l="dense"):
         if precision not in _PRECISIONS:
             raise ValueError("precision must be one of %s" % sorted(_PRECISIONS))
         if not torch.cuda.is_available():

@@ -83,7 +83,7 @@ extern "C" {
 #define FMM2D_PREC_MIXED 1 /* FP32 MFMA M2L (constant mode in FP64; the Stokeslet fields
                               translated in the target box's frame) and, for the Laplace
                               potential, FP32 near field for leaf colleagues at
-                              separations >= w/16; FP64 elsewhere.  An evaluation
+                              separations >= w/32; FP64 elsewhere.  An evaluation
                               with fewer than 32 targets per leaf on average takes
                               the FP64 near field (faster there, and more accurate).
                               Black-box plans keep the (SVD-compressed) FP64 M2L: an
@@ -92,10 +92,12 @@ extern "C" {
                                   in FP64 */
 
 /* FP64 M2L form (fmm2d_set_m2l_mode) */
-#define FMM2D_M2L_SVD 0   /* default: the reference's M2L_Evaluator (helpers.py:17-45): a slot
-                             of numerical rank r (singular values > 1e-14) with 2 r < p is
+#define FMM2D_M2L_SVD 0   /* the reference's M2L_Evaluator (helpers.py:17-45): a slot of
+                             numerical rank r (singular values > 1e-14) with 2 r < p is
                              applied as M2 (M1 q), a null slot adds nothing, others dense */
-#define FMM2D_M2L_DENSE 1 /* every slot applied as its dense matrix */
+#define FMM2D_M2L_DENSE 1 /* default: every slot applied as its dense matrix.  On MFMA the
+                             compressed form saves little (its ranks, 15-29 at p = 60,
+                             pad to 32 rows) and measured slower at every size tried */
 
 /* tree build (fmm2d_set_tree_mode) */
 #define FMM2D_TREE_AUTO 0     /* default: points that form a complete quadtree (every cell of
