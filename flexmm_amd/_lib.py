@@ -48,6 +48,7 @@ SIGNATURES = {
     "fmm2d_upward_dipole": (_int, [_vp, _vp, _vp, _i64, _vp]),
     "fmm2d_downward": (_int, [_vp, _vp]),
     "fmm2d_evaluate": (_int, [_vp, _vp, _vp, _i64, _int, _int, _vp, _vp]),
+    "fmm2d_evaluate_noalloc": (_int, [_vp, _vp, _vp, _i64, _int, _int, _vp, _vp]),
     "fmm2d_export_expansions": (_int, [_vp, _int, _vp, _i64, _vp]),
     "fmm2d_set_m2l_mode": (_int, [_vp, _int]),
     "fmm2d_set_tree_mode": (_int, [_vp, _int]),

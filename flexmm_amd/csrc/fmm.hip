@@ -3021,6 +3021,79 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   P.mark(FMM2D_PHASE_EVAL_KERNEL, true, s);
 }
 
+// ---------------------------------------------------------------------------
+// Evaluation that allocates nothing: the target-side scratch lives in one
+// file-scope device array, sized for kNoAllocMaxTargets targets and
+// kNoAllocMaxBoxes evaluation boxes (each GPU the library runs on holds its
+// own copy until the process ends).  Every call on a GPU shares that GPU's
+// copy, so calls that overlap on one GPU race.
+constexpr int64_t kNoAllocMaxTargets = (int64_t)1 << 20;
+constexpr int64_t kNoAllocMaxBoxes = (int64_t)1 << 18;
+constexpr int64_t kNoAllocMaxPartials = 1024;   // scan partials of kNoAllocMaxBoxes + 1 counts
+
+constexpr int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+
+// Byte offsets of the slices, in the order of EvalScratch's members.
+struct NoAllocLayout {
+  int64_t tnode, tcnt, toff, tcur, tperm, icnt, ioff, ntot, partials, tsx, tsy, items, total;
+};
+constexpr NoAllocLayout noalloc_layout() {
+  NoAllocLayout l{};
+  int64_t o = 0;
+  l.tnode = o; o += align256(kNoAllocMaxTargets * 4);
+  l.tcnt = o; o += align256((kNoAllocMaxBoxes + 1) * 4);
+  l.toff = o; o += align256((kNoAllocMaxBoxes + 1) * 4);
+  l.tcur = o; o += align256((kNoAllocMaxTargets + 1) * 4);
+  l.tperm = o; o += align256(kNoAllocMaxTargets * 4);
+  l.icnt = o; o += align256((kNoAllocMaxBoxes + 1) * 4);
+  l.ioff = o; o += align256((kNoAllocMaxBoxes + 1) * 4);
+  l.ntot = o; o += align256(4 * 4);
+  l.partials = o; o += align256(kNoAllocMaxPartials * 4);
+  l.tsx = o; o += align256(kNoAllocMaxTargets * 8);
+  l.tsy = o; o += align256(kNoAllocMaxTargets * 8);
+  l.items = o; o += align256(eval_item_cap(kNoAllocMaxBoxes, kNoAllocMaxTargets) * 16);
+  l.total = o;
+  return l;
+}
+constexpr NoAllocLayout kNoAlloc = noalloc_layout();
+
+static __device__ __align__(256) unsigned char scratch[kNoAlloc.total];
+
+// Device address of `scratch` on each GPU, looked up once per device with
+// that device current (the host-side name is only a stand-in).
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+static unsigned char* noalloc_scratch(int dev) {
+  FMM_REQUIRE(dev >= 0 && dev < kMaxDevices, FMM_E_LIMIT, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    FMM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    FMM_REQUIRE(ptr != nullptr, FMM_E_INTERNAL, "no device address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+
+static EvalScratch noalloc_slices(unsigned char* base) {
+  EvalScratch w;
+  w.tnode = reinterpret_cast<int*>(base + kNoAlloc.tnode);
+  w.tcnt = reinterpret_cast<int*>(base + kNoAlloc.tcnt);
+  w.toff = reinterpret_cast<int*>(base + kNoAlloc.toff);
+  w.tcur = reinterpret_cast<int*>(base + kNoAlloc.tcur);
+  w.tperm = reinterpret_cast<int*>(base + kNoAlloc.tperm);
+  w.icnt = reinterpret_cast<int*>(base + kNoAlloc.icnt);
+  w.ioff = reinterpret_cast<int*>(base + kNoAlloc.ioff);
+  w.ntot = reinterpret_cast<int*>(base + kNoAlloc.ntot);
+  w.partials = reinterpret_cast<int*>(base + kNoAlloc.partials);
+  w.tsx = reinterpret_cast<double*>(base + kNoAlloc.tsx);
+  w.tsy = reinterpret_cast<double*>(base + kNoAlloc.tsy);
+  w.items = reinterpret_cast<int4*>(base + kNoAlloc.items);
+  return w;
+}
+
 }  // namespace fmm
 
 // ===========================================================================
@@ -3324,6 +3397,41 @@ int fmm2d_evaluate(void* plan, const double* tx, const double* ty, int64_t nt, i
   P->mark(FMM2D_PHASE_EVALUATE, false, (hipStream_t)stream);
   fmm::evaluate(*P, tx, ty, nt, evaluator, check, out, (hipStream_t)stream, nullptr);
   P->mark(FMM2D_PHASE_EVALUATE, true, (hipStream_t)stream);
+  ABI_END
+}
+
+int fmm2d_evaluate_noalloc(void* plan, const double* tx, const double* ty, int64_t nt,
+                           int evaluator, int check, double* out, void* stream) {
+  ABI_BEGIN
+  Plan* P = as_plan(plan);
+  fmm::PlanCall call(*P, stream, false);   // no event node: the call may be captured
+  if (nt != 0) {
+    NEED(tx);
+    NEED(ty);
+    NEED(out);
+  }
+  FMM_REQUIRE(nt >= 0 && nt <= fmm::kNoAllocMaxTargets, fmm::FMM_E_LIMIT,
+              "fmm2d_evaluate_noalloc takes at most %lld targets (got %lld); use fmm2d_evaluate",
+              (long long)fmm::kNoAllocMaxTargets, (long long)nt);
+  FMM_REQUIRE(P->tree.n_eval <= fmm::kNoAllocMaxBoxes, fmm::FMM_E_LIMIT,
+              "fmm2d_evaluate_noalloc takes trees of at most %lld evaluation boxes (this tree "
+              "has %lld); use fmm2d_evaluate",
+              (long long)fmm::kNoAllocMaxBoxes, (long long)P->tree.n_eval);
+  int dev = -1;
+  FMM_HIP(hipGetDevice(&dev));
+  FMM_REQUIRE(dev == P->device, fmm::FMM_E_ARG,
+              "the plan belongs to device %d but device %d is current", P->device, dev);
+  FMM_REQUIRE(fmm::scan_partials_i32(fmm::kNoAllocMaxBoxes + 1) <= fmm::kNoAllocMaxPartials,
+              fmm::FMM_E_INTERNAL, "scan partials exceed the scratch array");
+  const fmm::EvalScratch w = fmm::noalloc_slices(fmm::noalloc_scratch(dev));
+  // not timed: phase timing creates its HIP events on first use
+  struct NoTiming {
+    Plan* P;
+    bool was;
+    ~NoTiming() { P->profile = was; }
+  } nt_guard{P, P->profile};
+  P->profile = false;
+  fmm::evaluate(*P, tx, ty, nt, evaluator, check, out, (hipStream_t)stream, &w);
   ABI_END
 }
 

@@ -240,6 +240,24 @@ int fmm2d_downward(void* plan, void* stream);
 int fmm2d_evaluate(void* plan, const double* tx, const double* ty, int64_t nt, int evaluator,
                    int check, double* out, void* stream);
 
+/* fmm2d_evaluate without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as fmm2d_evaluate; the target-side scratch
+ * (target boxes, counts, leaf-ordered targets, work items) lives in one
+ * device array of the library (about 38 MB, held on each GPU the library runs
+ * on until the process ends), so:
+ *   - nt <= 1048576 targets and trees of <= 262144 evaluation boxes (leaves
+ *     and X-list parents), else FMM2D_E_LIMIT;
+ *   - every call on one GPU shares that GPU's array: calls that overlap on
+ *     one GPU (other streams, other threads) race;
+ *   - the plan's device must be current; a mixed-precision plan needs
+ *     fmm2d_downward to have run (it prepares the FP32 near-field sources);
+ *   - phase timing (fmm2d_set_profiling) does not time these calls.
+ * Zeroing uses a fill kernel (no memset node). */
+int fmm2d_evaluate_noalloc(void* plan, const double* tx, const double* ty, int64_t nt,
+                           int evaluator, int check, double* out, void* stream);
+
 /* Copy an expansion array (FMM2D_EXP_*) into a device buffer of nvals doubles. */
 int fmm2d_export_expansions(void* plan, int which, double* dst, int64_t nvals, void* stream);
 
