@@ -1113,6 +1113,185 @@ __global__ __launch_bounds__(kM2LThreads * RS) void k_m2l_all_f32(
   }
 }
 
+// FP32 M2L without a shared operator stage: block = 16 target boxes of one
+// level and child position (a "narrow tile") and 4 waves; the slots that any
+// of the 16 boxes uses are dealt round-robin to the waves, and each wave loads
+// the residual block of its slot straight from L2 into registers.  The MFMA
+// k index of step ks in lane group kq is k = KS kq + ks (KS = PD / 4), so a
+// lane's A operands for one 16-row block are KS consecutive floats of one
+// row and its B operands KS consecutive doubles of the source multipole: every
+// load is a contiguous 16-byte one and no LDS or barrier is needed per slot.
+// The waves' FP64 sums are added in a fixed order through LDS at the end.
+// Against the 64-box tiles of k_m2l_all_f32 this gives four times the waves
+// (a 1M-point tree: ~5 per SIMD instead of ~1.4) and no per-slot barrier, at
+// the price of reading each operator block once per wave instead of once per
+// block (from L2: 40 blocks of 16 KB per level).
+constexpr int kM2LWCols = 16;
+constexpr int kM2LWWaves = 4;
+
+template <int PD, bool FRAME>
+__global__ __launch_bounds__(64 * kM2LWWaves) void k_m2l_waves_f32(
+    LevelTable lt, const int* __restrict__ flags, const int* __restrict__ m2l,
+    const double* __restrict__ mult, const double* __restrict__ qs,
+    const double* __restrict__ ops, int64_t opsz, double* __restrict__ ple, int64_t rstride,
+    int64_t nstride, const double* __restrict__ xmid, const double* __restrict__ ymid,
+    double c0x, double c0y) {
+  constexpr int RB = PD / 16, KS = PD / 4, NW = kM2LWWaves;
+  typedef float fv4 __attribute__((ext_vector_type(4)));
+  // FRAME: fields 0 and 1 take the source's field-2 row (D_2) too
+  const bool frame = FRAME && blockIdx.y < 2;
+  const int64_t to2 = (int64_t)(2 - (int)blockIdx.y) * rstride;   // rows to field 2
+  mult += (int64_t)blockIdx.y * rstride * PD;
+  qs += (int64_t)blockIdx.y * rstride;
+  ple += (int64_t)blockIdx.y * nstride * PD;
+  __shared__ int srcs[27][kM2LWCols];
+  __shared__ int pslot[28];
+  __shared__ unsigned int smask;
+  __shared__ double red[NW - 1][PD][kM2LWCols + 1];
+  const int bid = xcd_block(blockIdx.x, gridDim.x);
+  int li = 0;
+  while (li + 1 < lt.nlev && bid >= lt.boff[li + 1]) ++li;
+  const int lb = bid - lt.boff[li];
+  const int c = lb & 3;
+  const int tile = lb >> 2;
+  const int ngroups = lt.ngroups[li];
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int col = l & 15, kq = l >> 4;
+  const int qq = tile * kM2LWCols + col;
+  const int q = (lt.perm && qq < ngroups) ? lt.perm[lt.poff[li] + c * ngroups + qq] : qq;
+  const int g = lt.g0[li] + 4 * q + c;
+  const bool active = q < ngroups && !(flags[q < ngroups ? g : lt.g0[li]] & NF_FAKE);
+  // interaction lists: lane group (w, kq) reads slot indices 4 w + kq and
+  // 4 w + kq + 16 of its column
+  const int nv = c_valid.n[c];
+  const int* lst = m2l + (int64_t)g * kNumM2L;
+  if (threadIdx.x == 0) smask = 0u;
+  __syncthreads();
+  unsigned int used = 0u;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int i = 4 * w + kq + 16 * j;
+    const int src = (active && i < nv) ? lst[c_valid.slot[c][i]] : -1;
+    if (i < nv) srcs[i][col] = src;
+    const unsigned long long b = __ballot(src >= 0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if ((b >> (16 * k)) & 0xffffull) used |= 1u << (4 * w + k + 16 * j);
+  }
+  if (l == 0 && used) atomicOr(&smask, used);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int m = smask;
+    int k = 0;
+    while (m) {
+      pslot[k++] = __builtin_ctz(m);
+      m &= m - 1u;
+    }
+    pslot[27] = k;
+  }
+  __syncthreads();
+  const int nph = pslot[27];
+  const double* opsL = ops + (int64_t)lt.d[li] * opsz;
+  const double* means = opsL + ops_off_means(PD);
+  const float* r32 = reinterpret_cast<const float*>(opsL + ops_off_r32(PD));
+  double ct = 0.0;   // target centre coordinate i in the root frame
+  if (frame && active) ct = blockIdx.y == 0 ? xmid[g] - c0x : ymid[g] - c0y;
+  double acc64[RB][4];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc64[rb][v] = 0.0;
+  for (int k = w; k < nph; k += NW) {
+    const int i = pslot[k];
+    const int slot = c_valid.slot[c][i];
+    const int src = srcs[i][col];
+    // B: KS consecutive doubles of the source row, rounded to FP32 (the
+    // Stokes fields 0 and 1 in the target box's frame, formed in FP64)
+    float b[KS];
+    double qv = 0.0;
+    if (src >= 0) {
+      const double* brow = mult + (int64_t)src * PD + KS * kq;
+#pragma unroll
+      for (int m = 0; m < KS / 2; ++m) {
+        dv2 v = *reinterpret_cast<const dv2*>(brow + 2 * m);
+        if (frame) {
+          const dv2 v2 = *reinterpret_cast<const dv2*>(brow + to2 * PD + 2 * m);
+          v.x = __builtin_fma(ct, v2.x, v.x);
+          v.y = __builtin_fma(ct, v2.y, v.y);
+        }
+        b[2 * m] = (float)v.x;
+        b[2 * m + 1] = (float)v.y;
+      }
+      qv = frame ? __builtin_fma(ct, qs[to2 + src], qs[src]) : qs[src];
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) b[ks] = 0.f;
+    }
+    // A: row 16 rb + (l mod 16), columns KS kq .. KS kq + KS - 1
+    const float* arow = r32 + (int64_t)slot * PD * PD + (int64_t)col * PD + KS * kq;
+    fv4 a[RB][KS / 4];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int j = 0; j < KS / 4; ++j)
+        a[rb][j] = *reinterpret_cast<const fv4*>(arow + (int64_t)rb * 16 * PD + 4 * j);
+    f4 acc[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) acc[rb] = f4{0.f, 0.f, 0.f, 0.f};
+    // FP32 runs of kFlushM step pairs, each added to the FP64 sums
+#pragma unroll
+    for (int m = 0; m < KS / 2; ++m) {
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) {
+        acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb][m >> 1][2 * (m & 1)], b[2 * m],
+                                                      acc[rb], 0, 0, 0);
+        acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb][m >> 1][2 * (m & 1) + 1],
+                                                      b[2 * m + 1], acc[rb], 0, 0, 0);
+      }
+      if ((m + 1) % kFlushM == 0 && m + 1 < KS / 2) {
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc64[rb][v] += (double)acc[rb][v];
+          acc[rb] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+      }
+    }
+    // constant mode c_s * (sum of the source's equivalent density), FP64;
+    // lane rows 16 rb + 4 kq + v
+    const double* cm = means + (int64_t)slot * PD + 4 * kq;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+      const dv2 c01 = *reinterpret_cast<const dv2*>(cm + 16 * rb);
+      const dv2 c23 = *reinterpret_cast<const dv2*>(cm + 16 * rb + 2);
+      acc64[rb][0] += __builtin_fma(c01.x, qv, (double)acc[rb][0]);
+      acc64[rb][1] += __builtin_fma(c01.y, qv, (double)acc[rb][1]);
+      acc64[rb][2] += __builtin_fma(c23.x, qv, (double)acc[rb][2]);
+      acc64[rb][3] += __builtin_fma(c23.y, qv, (double)acc[rb][3]);
+    }
+  }
+  // waves 1 .. NW-1 hand their sums to wave 0, which adds them in wave order
+  if (w > 0) {
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) red[w - 1][16 * rb + 4 * kq + v][col] = acc64[rb][v];
+  }
+  __syncthreads();
+  if (w == 0 && active) {
+    double* rowp = ple + (int64_t)g * PD;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double sum = acc64[rb][v];
+#pragma unroll
+        for (int ww = 0; ww < NW - 1; ++ww) sum += red[ww][16 * rb + 4 * kq + v][col];
+        rowp[16 * rb + 4 * kq + v] = sum;
+      }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Downward, one level: PLE += L2L(parent LE) (parent width's L2LC,
 // fmm.py:142,162), LE = pinv2 * (pinv1 * PLE)  (fmm.py:156).  Tiled by child
@@ -1901,7 +2080,11 @@ __attribute__((amdgpu_waves_per_eu(KER == KER_MODHELM && NOUT == 3 ? 3 : 1))) vo
 // pair gives exactly zero and is excluded as in FP64.  The log is the hardware
 // log2 of d^2 / w^2 (O(1) values keep its absolute error small);
 // sum_j t_j log d_j^2 = ln2 * sum_j t_j log2(d_j^2/w^2) + log(w^2) * sum_j t_j.
-constexpr float kNearF32 = 1.0f / 1024.0f;   // (w / 32)^2 in units of w^2
+// Pairs closer than w/64 go to FP64: the coordinates' rounding (~6e-8 w)
+// costs ~1e-7 w/d in log d.  The rarer such pairs, the more groups of four
+// sources the loop runs without selects (w/16 -> w/64: 0.263 -> 0.223 ms on
+// 1M points, error vs FP64 unchanged at ~1e-8).
+constexpr float kNearF32 = 1.0f / 4096.0f;   // (w / 64)^2 in units of w^2
 
 struct LevelConsts {
   double invw[kMaxLevels];
@@ -2101,7 +2284,7 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
         const float yr = (float)((T.y - ymid[c]) * iw);
         float acc = 0.f;
         double excl = 0.0;   // charges of the pairs taken in FP64
-        // pairs closer than w/32 (and coincident ones) go to FP64: the rounding
+        // pairs closer than w/64 (and coincident ones) go to FP64: the rounding
         // of the FP32 relative coordinates costs ~1e-7 w / d in log d
         auto near64 = [&](int jj) {
           const double4 sd = src[jj];
@@ -2657,6 +2840,27 @@ static void m2l_launch(Plan& P, const M2LTables& T, int l0, int l1, hipStream_t 
   lt.perm = T.adaptive ? P.m2lperm.as<int>() : nullptr;
   int nb = lt.boff[lt.nlev];
   if (nb == 0) return;
+  if (m2l_in_fp32(P)) {
+    // narrow tiles of kM2LWCols boxes, one block each
+    LevelTable lw = lt;
+    int nbw = 0;
+    for (int li = 0; li < lw.nlev; ++li) {
+      lw.boff[li] = nbw;
+      nbw += 4 * (int)ceil_div(lw.ngroups[li], kM2LWCols);
+    }
+    lw.boff[lw.nlev] = nbw;
+    lw.split = 1;
+    const bool frame = P.kernel_id == KER_STOKES;
+    auto kern = PD == 32 ? (frame ? k_m2l_waves_f32<32, true> : k_m2l_waves_f32<32, false>)
+                         : (frame ? k_m2l_waves_f32<64, true> : k_m2l_waves_f32<64, false>);
+    hipLaunchKernelGGL(kern, dim3(nbw, nd), dim3(64 * kM2LWWaves), 0, s, lw, t.flags.as<int>(),
+                       t.m2l.as<int>(), P.mult.as<double>(), P.qs.as<double>(),
+                       P.ops.as<double>(), opsz, P.ple.as<double>(), rstride, nstride,
+                       t.xmid.as<double>(), t.ymid.as<double>(), t.xmin + 0.5 * t.w0,
+                       t.ymin + 0.5 * t.w0);
+    FMM_LAUNCH_CHECK();
+    return;
+  }
   // fewer than 3 tiles per CU: split the tiles' phases over several blocks
   const bool few = (int64_t)nb * nd < 3 * kNumCUs;
   lt.split = few ? (m2l_in_fp32(P) ? kM2LSplitF32 : kM2LSplitF64) : 1;

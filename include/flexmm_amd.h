@@ -83,7 +83,7 @@ extern "C" {
 #define FMM2D_PREC_MIXED 1 /* FP32 MFMA M2L (constant mode in FP64; the Stokeslet fields
                               translated in the target box's frame) and, for the Laplace
                               potential, FP32 near field for leaf colleagues at
-                              separations >= w/32; FP64 elsewhere.  An evaluation
+                              separations >= w/64; FP64 elsewhere.  An evaluation
                               with fewer than 32 targets per leaf on average takes
                               the FP64 near field (faster there, and more accurate).
                               Black-box plans keep the (SVD-compressed) FP64 M2L: an
