@@ -683,9 +683,13 @@ __constant__ ValidSlots c_valid;
 constexpr int kM2LThreads = 256;   // threads per block and row split
 constexpr int kM2LCols = 64;
 constexpr int kM2LRowSplit = 1;   // blocks of 4 waves, each wave all PD rows of 16 boxes
-constexpr int kM2LSplitF64 = 2;   // blocks per tile when tiles are few (FP64, FP32 M2L)
-constexpr int kM2LSplitF32 = 1;
+constexpr int kM2LSplitF64 = 2;   // FP64 M2L: blocks per tile when tiles are few
 constexpr int kFlushM = 2;   // FP32 M2L: k-step pairs per FP32 run (4 runs per slot at PD 64)
+// Dense FP64 M2L in narrow tiles (k_m2l_waves_f64) up to this padded
+// dimension: at PD 32 it beats the LDS-staged tiles (100K points, p 32:
+// 0.034 -> 0.028 ms); at PD 64 each wave re-reading 32 KB per slot from L2
+// costs more than the shared stage (1M points, p 60: 0.151 -> 0.189 ms).
+constexpr int kM2LWavesF64MaxPD = 32;
 
 struct M2LSetup {
   int li, c, col, kq, g, nph, rh;
@@ -958,179 +962,23 @@ __global__ __launch_bounds__(kM2LThreads * RS) void k_m2l_all(LevelTable lt,
 // D[4(l/16) + v][l%16].
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-template <int PD, int NT>
-struct AStageF {
-  static constexpr int NV = PD * PD / 4;            // float4 per block
-  static constexpr int N4 = (NV + NT - 1) / NT;
-  f4 v[N4];
-  __device__ __forceinline__ void load(const float* __restrict__ G) {
-    const f4* G4 = reinterpret_cast<const f4*>(G);
-#pragma unroll
-    for (int i = 0; i < N4; ++i) {
-      const int idx = threadIdx.x + i * NT;
-      if (NV % NT == 0 || idx < NV) v[i] = G4[idx];
-    }
-  }
-  __device__ __forceinline__ void store(float* __restrict__ A) const {
-    constexpr int LDAF = PD + 4;
-#pragma unroll
-    for (int i = 0; i < N4; ++i) {
-      const int idx = threadIdx.x + i * NT;
-      if (NV % NT == 0 || idx < NV) {
-        const int e = 4 * idx;
-        const int r = e / PD, c = e - r * PD;
-        *reinterpret_cast<f4*>(A + r * LDAF + c) = v[i];
-      }
-    }
-  }
-};
-
-template <int PD, bool FRAME, int RS>
-__global__ __launch_bounds__(kM2LThreads * RS) void k_m2l_all_f32(
-    LevelTable lt, const int* __restrict__ flags, const int* __restrict__ m2l,
-    const double* __restrict__ mult, const double* __restrict__ qs,
-    const double* __restrict__ ops, int64_t opsz, double* __restrict__ ple, int64_t rstride,
-    int64_t nstride, const double* __restrict__ xmid, const double* __restrict__ ymid,
-    double c0x, double c0y) {
-  // FRAME: fields 0 and 1 take the source's field-2 row (D_2) too
-  const bool frame = FRAME && blockIdx.y < 2;
-  const int64_t to2 = (int64_t)(2 - (int)blockIdx.y) * rstride;   // rows to field 2
-  mult += (int64_t)blockIdx.y * rstride * PD;
-  qs += (int64_t)blockIdx.y * rstride;
-  ple += (int64_t)blockIdx.y * nstride * PD;
-  constexpr int KS = PD / 4, RB = PD / 16 / RS, LDAF = PD + 4;
-  __shared__ __attribute__((aligned(16))) float buf[2][PD * LDAF];
-  // row means of the phase's slot, staged with its residual block (keeping
-  // all 27 resident would hold the block to 2 per CU by LDS)
-  __shared__ __attribute__((aligned(16))) double cmbuf[2][PD];
-  __shared__ int srcs[27][kM2LCols];
-  __shared__ int pslot[28];
-  __shared__ unsigned long long smask;
-  const M2LSetup S = m2l_setup(lt, flags, m2l, srcs, pslot, smask);
-  const int l = threadIdx.x & 63;
-  const int rb0 = S.rh * RB;   // this wave's first row block
-  const double* opsL = ops + (int64_t)lt.d[S.li] * opsz;
-  double ct = 0.0;   // target centre coordinate i in the root frame
-  if (frame && S.active) ct = blockIdx.y == 0 ? xmid[S.g] - c0x : ymid[S.g] - c0y;
-  const double* means = opsL + ops_off_means(PD);
-  const float* r32 = reinterpret_cast<const float*>(opsL + ops_off_r32(PD));
-  double acc64[RB][4];
-#pragma unroll
-  for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) acc64[rb][v] = 0.0;
-  if (S.nph > 0) {
-    AStageF<PD, kM2LThreads * RS> st;
-    float bn[KS], bc[KS];
-    double qn = 0.0, qc, mn = 0.0;
-    auto fetch = [&](int k) {
-      const int i = pslot[k];
-      st.load(r32 + (int64_t)c_valid.slot[S.c][i] * PD * PD);
-      if (threadIdx.x < PD) mn = means[c_valid.slot[S.c][i] * PD + threadIdx.x];
-      const int src = srcs[i][S.col];
-      const double* brow = src >= 0 ? mult + (int64_t)src * PD + 2 * S.kq : nullptr;
-      if (frame) {
-#pragma unroll
-        for (int m = 0; m < KS / 2; ++m) {
-          const dv2 v = brow ? *reinterpret_cast<const dv2*>(brow + 8 * m) : dv2{0.0, 0.0};
-          const dv2 v2 = brow ? *reinterpret_cast<const dv2*>(brow + to2 * PD + 8 * m)
-                              : dv2{0.0, 0.0};
-          bn[2 * m] = (float)__builtin_fma(ct, v2.x, v.x);
-          bn[2 * m + 1] = (float)__builtin_fma(ct, v2.y, v.y);
-        }
-        qn = src >= 0 ? __builtin_fma(ct, qs[to2 + src], qs[src]) : 0.0;
-      } else {
-#pragma unroll
-        for (int m = 0; m < KS / 2; ++m) {
-          const dv2 v = brow ? *reinterpret_cast<const dv2*>(brow + 8 * m) : dv2{0.0, 0.0};
-          bn[2 * m] = (float)v.x;
-          bn[2 * m + 1] = (float)v.y;
-        }
-        qn = src >= 0 ? qs[src] : 0.0;
-      }
-    };
-    fetch(0);
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) bc[ks] = bn[ks];
-    qc = qn;
-    for (int i = 0; i < S.nph; ++i) {
-      float* A = (i & 1) ? buf[1] : buf[0];
-      st.store(A);
-      if (threadIdx.x < PD) cmbuf[i & 1][threadIdx.x] = mn;
-      if (i + 1 < S.nph) fetch(i + 1);
-      __syncthreads();
-      // FP32 accumulation over runs of 8 kFlushM products per slot, each run
-      // added to the FP64 sums (a single FP32 chain over all 27 slots loses
-      // about 4x more than one per slot; the FP32 rounding of the running
-      // sum, not of the operands, dominates the error)
-      const float* a0 = A + (rb0 * 16 + (l & 15)) * LDAF + 2 * S.kq;
-      f4 acc[RB];
-#pragma unroll
-      for (int rb = 0; rb < RB; ++rb) acc[rb] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int m = 0; m < KS / 2; ++m) {
-#pragma unroll
-        for (int rb = 0; rb < RB; ++rb) {
-          typedef float fv2 __attribute__((ext_vector_type(2)));
-          const fv2 a = *reinterpret_cast<const fv2*>(a0 + rb * 16 * LDAF + 8 * m);
-          acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, bc[2 * m], acc[rb], 0, 0, 0);
-          acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, bc[2 * m + 1], acc[rb], 0, 0, 0);
-        }
-        if ((m + 1) % kFlushM == 0 && m + 1 < KS / 2) {
-#pragma unroll
-          for (int rb = 0; rb < RB; ++rb) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc64[rb][v] += (double)acc[rb][v];
-            acc[rb] = f4{0.f, 0.f, 0.f, 0.f};
-          }
-        }
-      }
-      // constant mode c_s * (sum of the source's equivalent density), FP64
-      const double* cm = cmbuf[i & 1] + 16 * rb0 + 4 * S.kq;
-#pragma unroll
-      for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-          acc64[rb][v] += __builtin_fma(cm[16 * rb + v], qc, (double)acc[rb][v]);
-#pragma unroll
-      for (int ks = 0; ks < KS; ++ks) bc[ks] = bn[ks];
-      qc = qn;
-    }
-  }
-  if (S.active) {
-    double* rowp = ple + (int64_t)S.g * PD + 16 * rb0;
-    if (lt.split > 1) {
-#pragma unroll
-      for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) unsafeAtomicAdd(rowp + 16 * rb + 4 * S.kq + v, acc64[rb][v]);
-    } else {
-#pragma unroll
-      for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) rowp[16 * rb + 4 * S.kq + v] = acc64[rb][v];
-    }
-  }
-}
-
-// FP32 M2L without a shared operator stage: block = 16 target boxes of one
-// level and child position (a "narrow tile") and 4 waves; the slots that any
-// of the 16 boxes uses are dealt round-robin to the waves, and each wave loads
-// the residual block of its slot straight from L2 into registers.  The MFMA
-// k index of step ks in lane group kq is k = KS kq + ks (KS = PD / 4), so a
-// lane's A operands for one 16-row block are KS consecutive floats of one
-// row and its B operands KS consecutive doubles of the source multipole: every
-// load is a contiguous 16-byte one and no LDS or barrier is needed per slot.
-// The waves' FP64 sums are added in a fixed order through LDS at the end.
-// Against the 64-box tiles of k_m2l_all_f32 this gives four times the waves
-// (a 1M-point tree: ~5 per SIMD instead of ~1.4) and no per-slot barrier, at
-// the price of reading each operator block once per wave instead of once per
-// block (from L2: 40 blocks of 16 KB per level).
+// FP32 M2L in narrow tiles: block = 16 target boxes of one level and child
+// position and 4 waves; the slots that any of the 16 boxes uses are dealt
+// round-robin to the waves, and each wave loads the residual block of its
+// slot straight from L2 into registers, with no LDS stage and no barrier per
+// slot.  The residuals are stored in the MFMA operand order (host packing,
+// residual_fragment_order): one 16-byte load per lane feeds four MFMA steps
+// and a wave's load reads 1 KB in one piece.  Step 2 m + h of lane group kq
+// multiplies k = 8 m + 2 kq + h, so a lane's source-multipole operands for
+// two steps are one 16-byte load and the 4 lane groups of a row read 64
+// consecutive bytes.  The waves' FP64 sums are added in a fixed order
+// through LDS at the end.  Against 64-box tiles sharing an LDS stage this
+// gives four times the waves (a 1M-point tree: ~5 per SIMD instead of ~1.4).
 constexpr int kM2LWCols = 16;
 constexpr int kM2LWWaves = 4;
 
 template <int PD, bool FRAME>
-__global__ __launch_bounds__(64 * kM2LWWaves) void k_m2l_waves_f32(
+__global__ __launch_bounds__(64 * kM2LWWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_m2l_waves_f32(
     LevelTable lt, const int* __restrict__ flags, const int* __restrict__ m2l,
     const double* __restrict__ mult, const double* __restrict__ qs,
     const double* __restrict__ ops, int64_t opsz, double* __restrict__ ple, int64_t rstride,
@@ -1205,17 +1053,17 @@ __global__ __launch_bounds__(64 * kM2LWWaves) void k_m2l_waves_f32(
     const int i = pslot[k];
     const int slot = c_valid.slot[c][i];
     const int src = srcs[i][col];
-    // B: KS consecutive doubles of the source row, rounded to FP32 (the
+    // B: the source row's entries 8 m + 2 kq + {0, 1}, rounded to FP32 (the
     // Stokes fields 0 and 1 in the target box's frame, formed in FP64)
     float b[KS];
     double qv = 0.0;
     if (src >= 0) {
-      const double* brow = mult + (int64_t)src * PD + KS * kq;
+      const double* brow = mult + (int64_t)src * PD + 2 * kq;
 #pragma unroll
       for (int m = 0; m < KS / 2; ++m) {
-        dv2 v = *reinterpret_cast<const dv2*>(brow + 2 * m);
+        dv2 v = *reinterpret_cast<const dv2*>(brow + 8 * m);
         if (frame) {
-          const dv2 v2 = *reinterpret_cast<const dv2*>(brow + to2 * PD + 2 * m);
+          const dv2 v2 = *reinterpret_cast<const dv2*>(brow + to2 * PD + 8 * m);
           v.x = __builtin_fma(ct, v2.x, v.x);
           v.y = __builtin_fma(ct, v2.y, v.y);
         }
@@ -1227,47 +1075,50 @@ __global__ __launch_bounds__(64 * kM2LWWaves) void k_m2l_waves_f32(
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) b[ks] = 0.f;
     }
-    // A: row 16 rb + (l mod 16), columns KS kq .. KS kq + KS - 1
-    const float* arow = r32 + (int64_t)slot * PD * PD + (int64_t)col * PD + KS * kq;
-    fv4 a[RB][KS / 4];
+    // A in operand order, one pair of 16-row blocks at a time (64 registers
+    // for all four would hold the kernel to two waves per SIMD):
+    // a[r][mp] = row 16 (2 rp + r) + (l mod 16), columns 8 m + 2 kq + {0, 1}
+    // for m = 2 mp, 2 mp + 1
+    const fv4* afrag = reinterpret_cast<const fv4*>(r32 + (int64_t)slot * PD * PD) + l;
+    const double* cm = means + (int64_t)slot * PD + 4 * kq;   // rows 16 rb + 4 kq + v
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
+    for (int rp = 0; rp < RB / 2; ++rp) {
+      fv4 a[2][KS / 4];
 #pragma unroll
-      for (int j = 0; j < KS / 4; ++j)
-        a[rb][j] = *reinterpret_cast<const fv4*>(arow + (int64_t)rb * 16 * PD + 4 * j);
-    f4 acc[RB];
+      for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb) acc[rb] = f4{0.f, 0.f, 0.f, 0.f};
-    // FP32 runs of kFlushM step pairs, each added to the FP64 sums
+        for (int j = 0; j < KS / 4; ++j) a[r][j] = afrag[((2 * rp + r) * (KS / 4) + j) * 64];
+      f4 acc[2] = {f4{0.f, 0.f, 0.f, 0.f}, f4{0.f, 0.f, 0.f, 0.f}};
+      // FP32 runs of kFlushM step pairs, each added to the FP64 sums
 #pragma unroll
-    for (int m = 0; m < KS / 2; ++m) {
+      for (int m = 0; m < KS / 2; ++m) {
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb) {
-        acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb][m >> 1][2 * (m & 1)], b[2 * m],
-                                                      acc[rb], 0, 0, 0);
-        acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb][m >> 1][2 * (m & 1) + 1],
-                                                      b[2 * m + 1], acc[rb], 0, 0, 0);
-      }
-      if ((m + 1) % kFlushM == 0 && m + 1 < KS / 2) {
+        for (int r = 0; r < 2; ++r) {
+          acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r][m >> 1][2 * (m & 1)], b[2 * m],
+                                                       acc[r], 0, 0, 0);
+          acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r][m >> 1][2 * (m & 1) + 1],
+                                                       b[2 * m + 1], acc[r], 0, 0, 0);
+        }
+        if ((m + 1) % kFlushM == 0 && m + 1 < KS / 2) {
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb) {
+          for (int r = 0; r < 2; ++r) {
 #pragma unroll
-          for (int v = 0; v < 4; ++v) acc64[rb][v] += (double)acc[rb][v];
-          acc[rb] = f4{0.f, 0.f, 0.f, 0.f};
+            for (int v = 0; v < 4; ++v) acc64[2 * rp + r][v] += (double)acc[r][v];
+            acc[r] = f4{0.f, 0.f, 0.f, 0.f};
+          }
         }
       }
-    }
-    // constant mode c_s * (sum of the source's equivalent density), FP64;
-    // lane rows 16 rb + 4 kq + v
-    const double* cm = means + (int64_t)slot * PD + 4 * kq;
+      // constant mode c_s * (sum of the source's equivalent density), FP64
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb) {
-      const dv2 c01 = *reinterpret_cast<const dv2*>(cm + 16 * rb);
-      const dv2 c23 = *reinterpret_cast<const dv2*>(cm + 16 * rb + 2);
-      acc64[rb][0] += __builtin_fma(c01.x, qv, (double)acc[rb][0]);
-      acc64[rb][1] += __builtin_fma(c01.y, qv, (double)acc[rb][1]);
-      acc64[rb][2] += __builtin_fma(c23.x, qv, (double)acc[rb][2]);
-      acc64[rb][3] += __builtin_fma(c23.y, qv, (double)acc[rb][3]);
+      for (int r = 0; r < 2; ++r) {
+        const int rb = 2 * rp + r;
+        const dv2 c01 = *reinterpret_cast<const dv2*>(cm + 16 * rb);
+        const dv2 c23 = *reinterpret_cast<const dv2*>(cm + 16 * rb + 2);
+        acc64[rb][0] += __builtin_fma(c01.x, qv, (double)acc[r][0]);
+        acc64[rb][1] += __builtin_fma(c01.y, qv, (double)acc[r][1]);
+        acc64[rb][2] += __builtin_fma(c23.x, qv, (double)acc[r][2]);
+        acc64[rb][3] += __builtin_fma(c23.y, qv, (double)acc[r][3]);
+      }
     }
   }
   // waves 1 .. NW-1 hand their sums to wave 0, which adds them in wave order
@@ -1288,6 +1139,120 @@ __global__ __launch_bounds__(64 * kM2LWWaves) void k_m2l_waves_f32(
 #pragma unroll
         for (int ww = 0; ww < NW - 1; ++ww) sum += red[ww][16 * rb + 4 * kq + v][col];
         rowp[16 * rb + 4 * kq + v] = sum;
+      }
+  }
+}
+
+// FP64 M2L (dense form) in narrow tiles, as k_m2l_waves_f32: 16 boxes per
+// block, their slots dealt to 4 waves, sums added in wave order at the end.
+// The M2L blocks stay row-major: step 2 m + h of lane group kq multiplies
+// k = 8 m + 2 kq + h, so a lane's A operands for two steps are one 16-byte
+// load and the 4 lane groups of a row read 64 consecutive bytes (as for the
+// multipoles).  Row blocks run one after another, each loading its operands
+// just before its 16 steps, which keeps the kernel at 4 waves per SIMD.
+// D layout of v_mfma_f64_16x16x4f64: lane l holds D[4 v + l/16][l%16].
+template <int PD>
+__global__ __launch_bounds__(64 * kM2LWWaves) __attribute__((amdgpu_waves_per_eu(4)))
+void k_m2l_waves_f64(LevelTable lt, const int* __restrict__ flags, const int* __restrict__ m2l,
+                     const double* __restrict__ mult, const double* __restrict__ ops,
+                     int64_t opsz, double* __restrict__ ple, int64_t rstride, int64_t nstride) {
+  constexpr int RB = PD / 16, KS = PD / 4, NW = kM2LWWaves;
+  mult += (int64_t)blockIdx.y * rstride * PD;
+  ple += (int64_t)blockIdx.y * nstride * PD;
+  __shared__ int srcs[27][kM2LWCols];
+  __shared__ int pslot[28];
+  __shared__ unsigned int smask;
+  __shared__ double red[NW - 1][PD][kM2LWCols + 1];
+  const int bid = xcd_block(blockIdx.x, gridDim.x);
+  int li = 0;
+  while (li + 1 < lt.nlev && bid >= lt.boff[li + 1]) ++li;
+  const int lb = bid - lt.boff[li];
+  const int c = lb & 3;
+  const int tile = lb >> 2;
+  const int ngroups = lt.ngroups[li];
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int col = l & 15, kq = l >> 4;
+  const int qq = tile * kM2LWCols + col;
+  const int q = (lt.perm && qq < ngroups) ? lt.perm[lt.poff[li] + c * ngroups + qq] : qq;
+  const int g = lt.g0[li] + 4 * q + c;
+  const bool active = q < ngroups && !(flags[q < ngroups ? g : lt.g0[li]] & NF_FAKE);
+  const int nv = c_valid.n[c];
+  const int* lst = m2l + (int64_t)g * kNumM2L;
+  if (threadIdx.x == 0) smask = 0u;
+  __syncthreads();
+  unsigned int used = 0u;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int i = 4 * w + kq + 16 * j;
+    const int src = (active && i < nv) ? lst[c_valid.slot[c][i]] : -1;
+    if (i < nv) srcs[i][col] = src;
+    const unsigned long long b = __ballot(src >= 0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if ((b >> (16 * k)) & 0xffffull) used |= 1u << (4 * w + k + 16 * j);
+  }
+  if (l == 0 && used) atomicOr(&smask, used);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int m = smask;
+    int k = 0;
+    while (m) {
+      pslot[k++] = __builtin_ctz(m);
+      m &= m - 1u;
+    }
+    pslot[27] = k;
+  }
+  __syncthreads();
+  const int nph = pslot[27];
+  const double* opsL = ops + (int64_t)lt.d[li] * opsz;
+  d4 acc[RB];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb) acc[rb] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int k = w; k < nph; k += NW) {
+    const int i = pslot[k];
+    const int slot = c_valid.slot[c][i];
+    const int src = srcs[i][col];
+    dv2 b[KS / 2];   // source row entries 8 m + 2 kq + {0, 1}
+    if (src >= 0) {
+      const double* brow = mult + (int64_t)src * PD + 2 * kq;
+#pragma unroll
+      for (int m = 0; m < KS / 2; ++m) b[m] = *reinterpret_cast<const dv2*>(brow + 8 * m);
+    } else {
+#pragma unroll
+      for (int m = 0; m < KS / 2; ++m) b[m] = dv2{0.0, 0.0};
+    }
+    const double* arow =
+        opsL + (int64_t)(OP_M2L + slot) * PD * PD + (int64_t)col * PD + 2 * kq;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+      dv2 a[KS / 2];
+#pragma unroll
+      for (int m = 0; m < KS / 2; ++m)
+        a[m] = *reinterpret_cast<const dv2*>(arow + (int64_t)rb * 16 * PD + 8 * m);
+#pragma unroll
+      for (int m = 0; m < KS / 2; ++m) {
+        acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m].x, b[m].x, acc[rb], 0, 0, 0);
+        acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m].y, b[m].y, acc[rb], 0, 0, 0);
+      }
+    }
+  }
+  if (w > 0) {
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) red[w - 1][16 * rb + 4 * v + kq][col] = acc[rb][v];
+  }
+  __syncthreads();
+  if (w == 0 && active) {
+    double* rowp = ple + (int64_t)g * PD;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double sum = acc[rb][v];
+#pragma unroll
+        for (int ww = 0; ww < NW - 1; ++ww) sum += red[ww][16 * rb + 4 * v + kq][col];
+        rowp[16 * rb + 4 * v + kq] = sum;
       }
   }
 }
@@ -2815,9 +2780,7 @@ static void m2l_prepare(Plan& P, const M2LTables& T, hipStream_t s) {
 // The mixed forms run the M2L on FP32 matrix cores, except for Chebyshev
 // (black-box) plans: there the FP32 accumulation error of a slot's product
 // alone reaches ~1e-7 of the potential (no dominant mode to split off, unlike
-// the KIFMM's row means), while their translations are of low rank (15-23 of
-// 64 at order 8), so the SVD-compressed FP64 M2L is the cheaper way to the
-// error bound.
+// the KIFMM's row means), so they keep the FP64 M2L.
 static bool m2l_in_fp32(const Plan& P) { return P.precision != PREC_FP64 && !P.cheb; }
 
 static void m2l_launch(Plan& P, const M2LTables& T, int l0, int l1, hipStream_t s) {
@@ -2861,31 +2824,34 @@ static void m2l_launch(Plan& P, const M2LTables& T, int l0, int l1, hipStream_t 
     FMM_LAUNCH_CHECK();
     return;
   }
+  if (P.m2l_mode == FMM2D_M2L_DENSE && PD <= kM2LWavesF64MaxPD) {
+    LevelTable lw = lt;
+    int nbw = 0;
+    for (int li = 0; li < lw.nlev; ++li) {
+      lw.boff[li] = nbw;
+      nbw += 4 * (int)ceil_div(lw.ngroups[li], kM2LWCols);
+    }
+    lw.boff[lw.nlev] = nbw;
+    lw.split = 1;
+    hipLaunchKernelGGL(k_m2l_waves_f64<32>, dim3(nbw, nd), dim3(64 * kM2LWWaves), 0, s, lw,
+                       t.flags.as<int>(),
+                       t.m2l.as<int>(), P.mult.as<double>(), P.ops.as<double>(), opsz,
+                       P.ple.as<double>(), rstride, nstride);
+    FMM_LAUNCH_CHECK();
+    return;
+  }
   // fewer than 3 tiles per CU: split the tiles' phases over several blocks
-  const bool few = (int64_t)nb * nd < 3 * kNumCUs;
-  lt.split = few ? (m2l_in_fp32(P) ? kM2LSplitF32 : kM2LSplitF64) : 1;
+  lt.split = (int64_t)nb * nd < 3 * kNumCUs ? kM2LSplitF64 : 1;
   if (lt.split > 1) {
     // zero with a fill kernel (no memset node: the call may sit in a captured graph)
     fill_i32(P.ple.as<int>(), 2 * nstride * PD * nd, 0, s);
     nb *= lt.split;
   }
-  if (m2l_in_fp32(P)) {
-    const bool frame = P.kernel_id == KER_STOKES;
-    constexpr int RS = kM2LRowSplit;
-    auto kern = PD == 32 ? (frame ? k_m2l_all_f32<32, true, RS> : k_m2l_all_f32<32, false, RS>)
-                         : (frame ? k_m2l_all_f32<64, true, RS> : k_m2l_all_f32<64, false, RS>);
-    hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(kM2LThreads * RS), 0, s, lt, t.flags.as<int>(),
-                       t.m2l.as<int>(), P.mult.as<double>(), P.qs.as<double>(),
-                       P.ops.as<double>(), opsz, P.ple.as<double>(), rstride, nstride,
-                       t.xmid.as<double>(), t.ymid.as<double>(), t.xmin + 0.5 * t.w0,
-                       t.ymin + 0.5 * t.w0);
-  } else {
-    constexpr int RS = kM2LRowSplit;
-    auto kern = PD == 32 ? k_m2l_all<32, RS> : k_m2l_all<64, RS>;
-    hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(kM2LThreads * RS), 0, s, lt, t.flags.as<int>(),
-                       t.m2l.as<int>(), P.mult.as<double>(), P.ops.as<double>(), opsz,
-                       P.ple.as<double>(), rstride, nstride, P.m2l_mode == FMM2D_M2L_SVD ? 1 : 0);
-  }
+  constexpr int RS = kM2LRowSplit;
+  auto kern = PD == 32 ? k_m2l_all<32, RS> : k_m2l_all<64, RS>;
+  hipLaunchKernelGGL(kern, dim3(nb, nd), dim3(kM2LThreads * RS), 0, s, lt, t.flags.as<int>(),
+                     t.m2l.as<int>(), P.mult.as<double>(), P.ops.as<double>(), opsz,
+                     P.ple.as<double>(), rstride, nstride, P.m2l_mode == FMM2D_M2L_SVD ? 1 : 0);
   FMM_LAUNCH_CHECK();
 }
 

@@ -103,6 +103,8 @@ class LevelOperators:
             c = m.mean(axis=1)
             means[s, :p] = c
             resid[s, :p, :p] = (m - c[:, None]).astype(np.float32)
+        rows, cols = residual_fragment_order(PD)
+        resid = resid[:, rows, cols]
         blocks = np.zeros((OP_NBLK, PD, PD))
         blocks[0, :p, :p] = self.pinv1
         blocks[1, :p, :p] = self.pinv2
@@ -123,6 +125,20 @@ class LevelOperators:
                 svd[s] = pack_svd_block(M1, M2, PD)
         return np.concatenate([blocks.ravel(), circ.ravel(), means.ravel(),
                                resid.ravel().view(np.float64), ranks, svd.ravel()])
+
+
+def residual_fragment_order(PD):
+    """(rows, cols) such that an FP32 residual block R is stored as
+    R[rows, cols]: the operand order of the FP32 MFMA M2L, in which one
+    16-byte load per lane feeds four MFMA steps and a wave's load reads 1 KB
+    in one piece.  Entry ((rb MP + mp) 64 + lane) 4 + e, MP = PD / 16, holds
+    R[16 rb + lane % 16][8 m + 2 (lane // 16) + e % 2] with m = 2 mp + e // 2."""
+    MP = PD // 16
+    rb, mp, lane, e = np.meshgrid(np.arange(PD // 16), np.arange(MP), np.arange(64),
+                                  np.arange(4), indexing="ij")
+    rows = 16 * rb + lane % 16
+    cols = 8 * (2 * mp + e // 2) + 2 * (lane // 16) + e % 2
+    return rows.ravel(), cols.ravel()
 
 
 def compress_m2l(m, p, tol=M2L_SVD_TOL):

@@ -35,7 +35,19 @@ def test_pack_layout():
     circ = flat[nb:nb + 2 * PD].reshape(2, PD)
     means = flat[nb + 2 * PD:nb + 42 * PD].reshape(40, PD)
     r32 = nb + 42 * PD
-    resid = flat[r32:r32 + 20 * PD * PD].view(np.float32).reshape(40, PD, PD)
+    frag = flat[r32:r32 + 20 * PD * PD].view(np.float32).reshape(40, PD * PD)
+    # MFMA operand order: entry ((rb MP + mp) 64 + lane) 4 + e holds row
+    # 16 rb + lane % 16, column 8 m + 2 (lane // 16) + e % 2, m = 2 mp + e // 2
+    MP = PD // 16
+    resid = np.full((40, PD, PD), np.nan, dtype=np.float32)
+    for rb in range(PD // 16):
+        for mp in range(MP):
+            for lane in range(64):
+                for e in range(4):
+                    m = 2 * mp + e // 2
+                    resid[:, 16 * rb + lane % 16, 8 * m + 2 * (lane // 16) + e % 2] = \
+                        frag[:, ((rb * MP + mp) * 64 + lane) * 4 + e]
+    assert not np.isnan(resid).any()   # every entry stored once
     ranks = flat[r32 + 20 * PD * PD:r32 + 20 * PD * PD + 64]
     assert list(ranks[:40]) == ops.m2l_ranks and np.all(ranks[40:] == -1)
     for s in (0, 13, 39):
