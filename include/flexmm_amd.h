@@ -86,8 +86,8 @@ extern "C" {
                               separations >= w/64; FP64 elsewhere.  An evaluation
                               with fewer than 32 targets per leaf on average takes
                               the FP64 near field (faster there, and more accurate).
-                              Black-box plans keep the (SVD-compressed) FP64 M2L: an
-                              FP32 accumulation alone costs them ~1e-7 */
+                              Black-box plans keep the FP64 M2L: an FP32
+                              accumulation alone costs them ~1e-7 */
 #define FMM2D_PREC_MIXED_M2L 2 /* FP32 MFMA M2L only (black-box plans: FP64); near field
                                   in FP64 */
 
@@ -198,13 +198,22 @@ int64_t fmm2d_operator_size(int p);
  * their 4 transposes (L2L), the 40 M2L matrices in slot order -- then the
  * large-circle x and y (PD each), then for the mixed-precision M2L the row
  * means c_s of each M2L matrix ([40][PD] doubles) and the FP32 residuals
- * M_s - c_s 1^T ([40][PD][PD] floats, i.e. 20 PD^2 doubles of storage), then
+ * R_s = M_s - c_s 1^T (40 blocks of PD^2 floats, i.e. 20 PD^2 doubles of
+ * storage, each in MFMA operand order: entry ((rb PD/16 + mp) 64 + lane) 4 + e
+ * holds R[16 rb + lane%16][8 m + 2 (lane/16) + e%2], m = 2 mp + e/2), then
  * the SVD-compressed M2L of M2L_Evaluator (helpers.py:17-45): 64 slot ranks
  * as doubles (r > 0 compressed, 0 null, -1 dense; slots 40..63 unused) and 40
  * [PD][PD] blocks, rows 0 .. PD/2-1 holding M1 = diag(s_:r) V_:r^T and rows
  * PD/2 .. PD-1 holding M2 = U_:r two rows per block row
- * (block[PD/2 + i/2][(i%2) PD/2 + j] = M2[i][j]).
- * PD = 32 for p <= 32, else 64.  Replaces Precomputation (helpers.py:250-317). */
+ * (block[PD/2 + i/2][(i%2) PD/2 + j] = M2[i][j]), then for the Laplace kernel
+ * the L2T as a power series, [2][64][PD]: rows n >= 1 hold Re and Im of the
+ * factor of the equivalent density phi_k in b_n = (1/2 pi n) sum_k phi_k
+ * exp(-i n theta_k) (theta_k the large-circle angles), row 0 of the real part
+ * is all ones and entry [1][0][0] is -log(R)/(2 pi), R the large radius; the
+ * potential of the local expansion at t is then
+ * -(log R/2 pi) sum phi + Re sum_{n>=1} b_n ((t - c)/R)^n (zero for other
+ * kernels).  PD = 32 for p <= 32, else 64.  Replaces Precomputation
+ * (helpers.py:250-317). */
 int fmm2d_set_operators(void* plan, int level, const double* dev_ops, int64_t n, void* stream);
 int fmm2d_clear_operators(void* plan);
 

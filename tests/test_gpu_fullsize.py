@@ -191,9 +191,12 @@ def _stokes_clusters():
     return pts[:, 0].copy(), pts[:, 1].copy(), rng.standard_normal(n), rng.standard_normal(n), rng
 
 
-# The mixed form's velocity: the far field D_i + X_i(t) D_2 cancels in part,
-# which amplifies the FP32 M2L error of the local expansions.
-STOKES_MIXED_TOL = 1e-6
+# The mixed form's velocity: the FP32 M2L takes the fields D_0, D_1 in the
+# target box's frame, so the cancellation of D_i + X_i(t) D_2 at the targets
+# no longer amplifies it; 4.8e-8 to 9.2e-8 measured over runs (the point
+# order inside a leaf, set by the sort's atomic scatter, changes between runs
+# and with it the rounding).
+STOKES_MIXED_TOL = 2e-7
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("mixed", STOKES_MIXED_TOL)])

@@ -20,10 +20,9 @@ def _clusters(n, nc, sigma, seed):
     return pts[:, 0].copy(), pts[:, 1].copy(), rng.standard_normal(m), rng.standard_normal(m)
 
 
-# mixed: the far field D_i + X_i(t) D_2 cancels in part, which amplifies the
-# FP32 M2L error of the local expansions in the velocity (4e-7 measured; the
-# pressure, D_2 alone, stays at the Laplace mixed level)
-@pytest.mark.parametrize("precision,tol,ptol", [("fp64", 1e-11, 1e-11), ("mixed", 3e-6, 1e-7)])
+# mixed: the FP32 M2L takes D_0, D_1 in the target box's frame (1.4e-8 to
+# 2.0e-8 measured in the velocity, 7e-9 in the pressure)
+@pytest.mark.parametrize("precision,tol,ptol", [("fp64", 1e-11, 1e-11), ("mixed", 1e-7, 1e-7)])
 def test_stokes_self_eval_clustered(precision, tol, ptol):
     from flexmm_amd.kifmm2d import StokesFMM
     x, y, fx, fy = _clusters(40000, 10, 0.01, 91)
