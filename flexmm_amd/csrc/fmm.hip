@@ -71,15 +71,7 @@ __host__ __device__ inline int64_t ops_off_svd_rank(int PD) {
   return ops_off_r32(PD) + 20 * (int64_t)PD * PD;
 }
 __host__ __device__ inline int64_t ops_off_svd(int PD) { return ops_off_svd_rank(PD) + 64; }
-// ... then the Laplace L2T as a power series, [2][kTaylorTerms][PD]: rows
-// n >= 1 of Re and Im of b_n = (1 / 2 pi n) sum_k phi_k exp(-i n theta_k),
-// row 0 of the real part all ones, and entry [1][0][0] the level's
-// -log(R) / (2 pi) (zero for other kernels).
-constexpr int kTaylorTerms = 64;
-__host__ __device__ inline int64_t ops_off_taylor(int PD) {
-  return ops_off_svd(PD) + 40 * (int64_t)PD * PD;
-}
-inline int64_t ops_per_level(int PD) { return ops_off_taylor(PD) + 2 * kTaylorTerms * PD; }
+inline int64_t ops_per_level(int PD) { return ops_off_svd(PD) + 40 * (int64_t)PD * PD; }
 
 // Chebyshev (black-box) expansions, bbfmm.py:95-231: order pc <= kMaxCheb,
 // p = pc^2 coefficients per box, index i pc + j for T_i(x) T_j(y), with
@@ -113,10 +105,6 @@ struct Plan {
   DevBuf logtab;
   DevBuf tau_s, src, check, mult, le, ple, qs;
   DevBuf srcf, tsum;   // mixed precision: leaf-relative FP32 sources, per-leaf charge sums
-  // Laplace L2T as power series per evaluation box (k_taylor): b_0 .. b_63
-  // real parts, then imaginary parts; valid after a downward pass that made it
-  DevBuf taylor;
-  bool have_taylor = false;
   DevBuf dip;          // dipole moments of the sources in tree order (double layer)
   DevBuf sdip;         // Stokes double layer: (n.sigma, a, b, 0) per source in tree order
   bool have_dip = false;
@@ -1518,79 +1506,6 @@ __global__ __launch_bounds__(64 * (PD / 16)) void k_local_level_small(
                        opsParent, le, nstride, fx);
 }
 
-// Laplace L2T as a power series (ops_off_taylor): for every evaluation box,
-// b = T phi from its local equivalent densities phi, T the [2 x 64] x PD rows
-// of the series (the same at every level: the large circle's angles do not
-// depend on the width), then b_0 times the box level's -log(R) / (2 pi).
-// The evaluation then costs a target 4 FMAs per term (k_eval, k_eval_mixed)
-// instead of a log per equivalent point.
-constexpr double kTaylorMaxZ2 = 0.36;   // |z| <= 0.6: 64 terms reach FP64 rounding;
-                                        // targets in their box have |z| <= 0.547
-
-template <int PD>
-__global__ __launch_bounds__(256) void k_taylor(int ne, const int* __restrict__ evalnode,
-                                                const int* __restrict__ nlevel,
-                                                const double* __restrict__ le,
-                                                const double* __restrict__ ops, int64_t opsz,
-                                                int dref, double* __restrict__ taylor) {
-  static_assert(PD == kTaylorTerms, "one 64-row MFMA block per part");
-  // block = 16 evaluation boxes, wave rb = rows 16 rb .. 16 rb + 15 of both
-  // parts; step 2 m + h of lane group kq multiplies k = 8 m + 2 kq + h, so
-  // every operand load is a 16-byte one straight from L2 (no LDS stage)
-  // A short kernel: its time is the latency of its dependent loads, so the
-  // operator rows (independent of the box) are requested first, and the
-  // densities without waiting for the box's level (levels 0 and 1 hold zeros)
-  const int l = threadIdx.x & 63, rb = threadIdx.x >> 6;
-  const int col = l & 15, kq = l >> 4;
-  const double* T = ops + (int64_t)dref * opsz + ops_off_taylor(PD);
-  dv2 a[2][PD / 8];
-#pragma unroll
-  for (int part = 0; part < 2; ++part)
-#pragma unroll
-    for (int m = 0; m < PD / 8; ++m)
-      a[part][m] = *reinterpret_cast<const dv2*>(T + (int64_t)part * kTaylorTerms * PD +
-                                                 (16 * rb + col) * PD + 2 * kq + 8 * m);
-  const int e = blockIdx.x * 16 + col;
-  const bool active = e < ne;
-  const int g = evalnode[active ? e : 0];
-  dv2 b[PD / 8];
-#pragma unroll
-  for (int m = 0; m < PD / 8; ++m)
-    b[m] = *reinterpret_cast<const dv2*>(le + (int64_t)g * PD + 8 * m + 2 * kq);
-  const int d = nlevel[g];
-  const bool use = active && d >= 2;
-#pragma unroll
-  for (int part = 0; part < 2; ++part) {
-    d4 acc = d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int m = 0; m < PD / 8; ++m) {
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[part][m].x, b[m].x, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[part][m].y, b[m].y, acc, 0, 0, 0);
-    }
-    if (use) {
-      // lane rows 16 rb + 4 v + kq; row 0 of the real part is sum phi
-      if (part == 0 && rb == 0 && kq == 0)
-        acc[0] *= ops[(int64_t)d * opsz + ops_off_taylor(PD) + kTaylorTerms * PD];
-      double* dst = taylor + (int64_t)e * 2 * kTaylorTerms + part * kTaylorTerms + 16 * rb + kq;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) dst[4 * v] = acc[v];
-    }
-  }
-}
-
-// sum of the series at z (bc: b_0 .. b_63 real parts, then imaginary parts)
-__device__ __forceinline__ double taylor_l2t(const double* __restrict__ bc, double zr, double zi) {
-  double ar = bc[kTaylorTerms - 1], ai = bc[2 * kTaylorTerms - 1];
-#pragma unroll 8
-  for (int n = kTaylorTerms - 2; n >= 1; --n) {
-    const double br = bc[n], bi = bc[kTaylorTerms + n];
-    const double tr = __builtin_fma(ar, zr, __builtin_fma(-ai, zi, br));
-    ai = __builtin_fma(ar, zi, __builtin_fma(ai, zr, bi));
-    ar = tr;
-  }
-  return __builtin_fma(ar, zr, __builtin_fma(-ai, zi, bc[0]));
-}
-
 // Levels with at most this many multipole rows (or 4 x this many boxes) use
 // the row-split kernels.
 constexpr int kSmallLevelRows = 4096;
@@ -2061,7 +1976,7 @@ __attribute__((amdgpu_waves_per_eu(KER == KER_MODHELM && NOUT == 3 ? 3 : 1))) vo
     const double2* __restrict__ dip,
     const double* __restrict__ le, const double* __restrict__ ops, int64_t opsz, int p,
     const double2* __restrict__ logtab, double kp, int64_t nt, double* __restrict__ out,
-    int scap, const int* __restrict__ evalid, const double* __restrict__ taylor) {
+    int scap) {
   __shared__ double2 tab[kLogTableSize];
   extern __shared__ double4 stage_lds[];   // scap per wave when launched with it
   double4* stage = stage_lds + (int)(threadIdx.x >> 6) * scap;
@@ -2096,33 +2011,15 @@ __attribute__((amdgpu_waves_per_eu(KER == KER_MODHELM && NOUT == 3 ? 3 : 1))) vo
     } else if (d >= 2) {
       const double* lx = ops + d * opsz + (int64_t)OP_NBLK * PD * PD;
       const double* ly = lx + PD;
-      bool direct = true;
-      if constexpr (KER == KER_LAPLACE && NOUT == 1 && !DIP && PD == kTaylorTerms) {
-        // the power series of the local expansion, when every target of the
-        // wave lies within |z| <= 0.6 (the coefficients are then wave-uniform
-        // scalar loads); a wave with a target far outside its box takes the
-        // equivalent points directly
-        if (taylor) {
-          const double* bc = taylor + (int64_t)evalid[g] * 2 * kTaylorTerms;
-          const double invR = 1.0 / __builtin_sqrt(__builtin_fma(lx[0], lx[0], ly[0] * ly[0]));
-          const double zr = (T.x - xmid[g]) * invR, zi = (T.y - ymid[g]) * invR;
-          if (__ballot(__builtin_fma(zr, zr, zi * zi) > kTaylorMaxZ2) == 0) {
-            far = taylor_l2t(bc, zr, zi);
-            direct = false;
-          }
-        }
-      }
-      if (direct) {
-        Target<KER, NOUT, false> L;
-        L.kp = T.kp;
-        L.x = T.x - xmid[g];
-        L.y = T.y - ymid[g];
-        const double* lg = le + (int64_t)g * PD;
-        for (int k = 0; k < p; ++k) L.pair(lx[k], ly[k], lg[k], tab);
-        T.u = L.u;
-        T.gx = L.gx;
-        T.gy = L.gy;
-      }
+      Target<KER, NOUT, false> L;
+      L.kp = T.kp;
+      L.x = T.x - xmid[g];
+      L.y = T.y - ymid[g];
+      const double* lg = le + (int64_t)g * PD;
+      for (int k = 0; k < p; ++k) L.pair(lx[k], ly[k], lg[k], tab);
+      T.u = L.u;
+      T.gx = L.gx;
+      T.gy = L.gy;
     }
     // P2P over the leaf's colleagues: helpers.py:180-217
     const CollLanes CL = coll_lanes(g, coll, crange);
@@ -2324,7 +2221,7 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
     const double4* __restrict__ src, const f4* __restrict__ srcf,
     const double* __restrict__ tsum, LevelConsts lc, const double* __restrict__ le,
     const double* __restrict__ ops, int64_t opsz, int p, const double2* __restrict__ logtab,
-    double* __restrict__ out, const int* __restrict__ evalid, const double* __restrict__ taylor) {
+    double* __restrict__ out) {
   __shared__ double2 tab[kLogTableSize];
   const int ni = *nitems;
   load_logtab(tab, logtab);
@@ -2357,25 +2254,13 @@ __global__ __launch_bounds__(64 * kEvalWaves) void k_eval_mixed(
     } else if (d >= 2) {   // L2T in FP64 (the local densities carry large cancelling modes)
       const double* lx = ops + d * opsz + ops_off_circles(PD);
       const double* ly = lx + PD;
-      bool direct = true;
-      if (PD == kTaylorTerms && taylor) {   // the power series, as in k_eval
-        const double* bc = taylor + (int64_t)evalid[g] * 2 * kTaylorTerms;
-        const double invR = 1.0 / __builtin_sqrt(__builtin_fma(lx[0], lx[0], ly[0] * ly[0]));
-        const double zr = (T.x - xmid[g]) * invR, zi = (T.y - ymid[g]) * invR;
-        if (__ballot(__builtin_fma(zr, zr, zi * zi) > kTaylorMaxZ2) == 0) {
-          far = taylor_l2t(bc, zr, zi);
-          direct = false;
-        }
-      }
-      if (direct) {
-        Target<KER_LAPLACE, 1, false> L;
-        L.kp = HelmConsts{};
-        L.x = T.x - xmid[g];
-        L.y = T.y - ymid[g];
-        const double* lg = le + (int64_t)g * PD;
-        for (int k = 0; k < p; ++k) L.pair(lx[k], ly[k], lg[k], tab);
-        T.u = L.u;
-      }
+      Target<KER_LAPLACE, 1, false> L;
+      L.kp = HelmConsts{};
+      L.x = T.x - xmid[g];
+      L.y = T.y - ymid[g];
+      const double* lg = le + (int64_t)g * PD;
+      for (int k = 0; k < p; ++k) L.pair(lx[k], ly[k], lg[k], tab);
+      T.u = L.u;
     }
     const double iw = lc.invw[d];
     const double lw2 = lc.logw2[d];
@@ -3107,7 +2992,6 @@ static void downward(Plan& P, hipStream_t s) {
   P.le.ensure(nstride * PD * nd * sizeof(double));
   P.ple.ensure(nstride * PD * nd * sizeof(double));
   fill_i32(P.le.as<int>(), 2 * nstride * PD * nd, 0, s);   // 0.0 (no memset node)
-  P.have_taylor = false;
   if (t.L <= 2) {
     P.have_exp = true;
     return;
@@ -3140,15 +3024,6 @@ static void downward(Plan& P, hipStream_t s) {
                        t.parent.as<int>(), P.ple.as<double>(), opsL, opsL - opsz,
                        P.le.as<double>(), nstride, fx);
     FMM_LAUNCH_CHECK();
-  }
-  // Laplace KIFMM: the local expansions of the evaluation boxes as power series
-  if (P.kernel_id == KER_LAPLACE && !P.cheb && PD == kTaylorTerms && nd == 1 && t.n_eval > 0) {
-    P.taylor.ensure(t.n_eval * 2 * kTaylorTerms * sizeof(double));
-    hipLaunchKernelGGL(k_taylor<kTaylorTerms>, dim3(ceil_div(t.n_eval, 16)), dim3(256), 0, s,
-                       (int)t.n_eval, t.evalnode.as<int>(), t.nlevel.as<int>(),
-                       P.le.as<double>(), P.ops.as<double>(), opsz, 2, P.taylor.as<double>());
-    FMM_LAUNCH_CHECK();
-    P.have_taylor = true;
   }
   P.have_exp = true;
 }
@@ -3320,7 +3195,7 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
                        t.coll.as<int>(), t.crange.as<int2>(), t.bot.as<int>(), t.top.as<int>(), P.src.as<double4>(),
                        P.srcf.as<f4>(), P.tsum.as<double>(), lc, P.le.as<double>(),
                        P.ops.as<double>(), opsz, P.cheb ? P.cheb : P.p, P.logtab.as<double2>(),
-                       out, t.evalid.as<int>(), P.have_taylor ? P.taylor.as<double>() : nullptr);
+                       out);
     FMM_LAUNCH_CHECK();
     P.mark(FMM2D_PHASE_EVAL_KERNEL, true, s);
     return;
@@ -3340,8 +3215,7 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
                      t.ymid.as<double>(), t.coll.as<int>(), t.crange.as<int2>(), t.bot.as<int>(), t.top.as<int>(),
                      P.src.as<double4>(), ev == EVAL_DIPOLE ? P.dip.as<double2>() : nullptr,
                      P.le.as<double>(), P.ops.as<double>(), opsz, P.cheb ? P.cheb : P.p,
-                     P.logtab.as<double2>(), P.kparam, nt, out, scap, t.evalid.as<int>(),
-                     P.have_taylor ? P.taylor.as<double>() : nullptr);
+                     P.logtab.as<double2>(), P.kparam, nt, out, scap);
   FMM_LAUNCH_CHECK();
   P.mark(FMM2D_PHASE_EVAL_KERNEL, true, s);
 }

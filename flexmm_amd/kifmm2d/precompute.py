@@ -68,7 +68,6 @@ class LevelOperators:
     """FP64 operators of one box width."""
 
     def __init__(self, kernel, p, width):
-        self.kernel = kernel
         self.p = p
         self.width = float(width)
         sx, sy, lx, ly = circles(width, p)
@@ -125,41 +124,7 @@ class LevelOperators:
             if r > 0:
                 svd[s] = pack_svd_block(M1, M2, PD)
         return np.concatenate([blocks.ravel(), circ.ravel(), means.ravel(),
-                               resid.ravel().view(np.float64), ranks, svd.ravel(),
-                               self.taylor_block(PD).ravel()])
-
-    def taylor_block(self, PD):
-        """The Laplace local expansion as a power series, [2][TAYLOR_TERMS][PD]:
-        inside the large circle (radius R, angles theta_k around the box
-        centre) the L2T potential of the equivalent densities phi is
-            -(1/2 pi) sum_k phi_k log|t - y_k|
-              = -(log R / 2 pi) sum_k phi_k + Re sum_{n>=1} b_n z^n,
-            b_n = (1 / 2 pi n) sum_k phi_k exp(-i n theta_k),  z = (t - c) / R
-        (log|t - y| = log R - Re sum_n (z e^{-i theta})^n / n).  Rows n >= 1
-        hold Re and Im of the factor of phi_k in b_n; row 0 of the real part is
-        all ones (the sum of phi) and entry [1][0][0] the level constant
-        -log(R) / (2 pi).  Zero for kernels other than Laplace."""
-        tay = np.zeros((2, TAYLOR_TERMS, PD))
-        if getattr(self, "kernel", None) is not _laplace_kernel():
-            return tay
-        p = self.p
-        lx, ly = self.large
-        theta = np.arctan2(ly, lx)
-        R = np.hypot(lx[0], ly[0])
-        n = np.arange(1, TAYLOR_TERMS)[:, None]
-        tay[0, 1:, :p] = np.cos(n * theta[None, :]) / (2 * np.pi * n)
-        tay[1, 1:, :p] = -np.sin(n * theta[None, :]) / (2 * np.pi * n)
-        tay[0, 0, :p] = 1.0
-        tay[1, 0, 0] = -np.log(R) / (2 * np.pi)
-        return tay
-
-
-TAYLOR_TERMS = 64   # power-series terms of the Laplace L2T (z^0 .. z^63)
-
-
-def _laplace_kernel():
-    from .. import kernels
-    return kernels.laplace
+                               resid.ravel().view(np.float64), ranks, svd.ravel()])
 
 
 def residual_fragment_order(PD):

@@ -205,15 +205,8 @@ int64_t fmm2d_operator_size(int p);
  * as doubles (r > 0 compressed, 0 null, -1 dense; slots 40..63 unused) and 40
  * [PD][PD] blocks, rows 0 .. PD/2-1 holding M1 = diag(s_:r) V_:r^T and rows
  * PD/2 .. PD-1 holding M2 = U_:r two rows per block row
- * (block[PD/2 + i/2][(i%2) PD/2 + j] = M2[i][j]), then for the Laplace kernel
- * the L2T as a power series, [2][64][PD]: rows n >= 1 hold Re and Im of the
- * factor of the equivalent density phi_k in b_n = (1/2 pi n) sum_k phi_k
- * exp(-i n theta_k) (theta_k the large-circle angles), row 0 of the real part
- * is all ones and entry [1][0][0] is -log(R)/(2 pi), R the large radius; the
- * potential of the local expansion at t is then
- * -(log R/2 pi) sum phi + Re sum_{n>=1} b_n ((t - c)/R)^n (zero for other
- * kernels).  PD = 32 for p <= 32, else 64.  Replaces Precomputation
- * (helpers.py:250-317). */
+ * (block[PD/2 + i/2][(i%2) PD/2 + j] = M2[i][j]).
+ * PD = 32 for p <= 32, else 64.  Replaces Precomputation (helpers.py:250-317). */
 int fmm2d_set_operators(void* plan, int level, const double* dev_ops, int64_t n, void* stream);
 int fmm2d_clear_operators(void* plan);
 

@@ -64,45 +64,7 @@ def test_pack_layout():
     assert np.all(blocks[:, p:, :] == 0) and np.all(blocks[:, :, p:] == 0)
     np.testing.assert_array_equal(circ[0, :p], ops.large[0])
     assert np.all(circ[:, p:] == 0)
-    assert flat.size == (50 * PD * PD + 2 * PD + 40 * PD + 20 * PD * PD + 64 + 40 * PD * PD
-                         + 2 * pc.TAYLOR_TERMS * PD)
-
-
-@pytest.mark.parametrize("p,w", [(60, 1.0), (60, 1.0 / 256), (40, 0.5)])
-def test_taylor_block_reproduces_the_l2t_potential(p, w):
-    """The power-series form of the Laplace L2T equals the direct sum
-    -(1/2 pi) sum_k phi_k log|t - y_k| over the large circle, for random
-    equivalent densities (with large cancelling modes) at targets in the box
-    and just outside it (|z| <= 0.6), to 64 terms."""
-    ops = pc.LevelOperators(kernels.laplace, p, w)
-    flat = ops.pack()
-    PD = pc.padded_dim(p)
-    N = pc.TAYLOR_TERMS
-    tay = flat[-2 * N * PD:].reshape(2, N, PD)
-    lx, ly = ops.large
-    R = np.hypot(lx[0], ly[0])
-    np.testing.assert_allclose(np.hypot(lx, ly), R, rtol=1e-15)
-    rng = np.random.default_rng(p)
-    phi = rng.standard_normal(p) * 1e3 + np.cos(np.arange(p) * np.pi) * 1e4
-    tx = rng.uniform(-0.5, 0.5, 200) * w
-    ty = rng.uniform(-0.5, 0.5, 200) * w
-    tx[:20] = 0.6 * R * np.cos(np.linspace(0, 2 * np.pi, 20))
-    ty[:20] = 0.6 * R * np.sin(np.linspace(0, 2 * np.pi, 20))
-    direct = -(phi[None, :] * np.log(np.hypot(tx[:, None] - lx[None, :],
-                                              ty[:, None] - ly[None, :]))).sum(axis=1) / (2 * np.pi)
-    b = tay[0, :, :p] @ phi + 1j * (tay[1, :, :p] @ phi)
-    z = (tx + 1j * ty) / R
-    acc = np.zeros_like(z)
-    for n in range(N - 1, 0, -1):
-        acc = acc * z + b[n]
-    series = tay[1, 0, 0] * b[0].real + (acc * z).real
-    assert tay[1, 0, 0] == -np.log(R) / (2 * np.pi)
-    err = np.abs(series - direct).max() / np.abs(phi).sum()
-    print("p", p, "w", w, "series vs direct %.2e (relative to sum |phi|)" % err)
-    assert err < 1e-15
-    # kernels other than Laplace carry no series
-    other = pc.LevelOperators(kernels.modified_helmholtz(20.0)[0], p, w).pack()
-    assert np.all(other[-2 * N * PD:] == 0)
+    assert flat.size == 50 * PD * PD + 2 * PD + 40 * PD + 20 * PD * PD + 64 + 40 * PD * PD
 
 
 @pytest.mark.parametrize("p,w", [(60, 1.0), (60, 1.0 / 128), (64, 0.25), (24, 0.5)])
