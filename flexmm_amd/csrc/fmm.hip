@@ -1328,21 +1328,32 @@ constexpr int kMaxSmallPhases = 6;   // 4 M2M + 2 pseudo-inverse factors
 // Phases 0 .. nb-1 take their B column from row brow + i of `bbase`, where
 // brow is the lane's box's source row (-1: none); these rows are loaded once
 // per block, cooperatively and with 16-byte loads, into the LDS tile `btile`.
-// Later phases are chained.  `opA(i)` is the operator of phase i.
+// Later phases are chained.  `opA(i)` is the operator of phase i.  A wave
+// uses only its own 16 operator rows, so it loads them straight into
+// registers, three phases ahead, with 16-byte loads: step 2 m + h of lane
+// group kq multiplies k = 8 m + 2 kq + h.  (All six phases ahead was slower:
+// the B tile fill then waits for every operator load issued before it.)  The B tiles are box-major in LDS
+// (row padded to PD + 2), so that B arrives in the same order with one
+// 16-byte LDS read per step pair.  (Staging the operator in LDS left every
+// MFMA waiting on its own LDS read, with one block per CU to hide nothing.)
+constexpr int kSmallPad = 2;
 template <int PD, int NB, class OpFn>
-__device__ __forceinline__ void run_small(int n, int nb, OpFn opA,
-                                          const double* __restrict__ bbase, int brow, d4& acc,
-                                          double* __restrict__ abuf0, double* __restrict__ abuf1,
-                                          double (&bbuf)[2][PD][17],
-                                          double (&btile)[NB][PD][17], int (&srow)[16]) {
-  constexpr int KS = PD / 4, LDA = Mfma<PD>::LDA, NT = 64 * (PD / 16), H = PD / 2;
+__device__ __forceinline__ void run_small(
+    int n, int nb, OpFn opA, const double* __restrict__ bbase, int brow, d4& acc,
+    double (&bbuf)[2][16][PD + kSmallPad], double (&btile)[NB][16][PD + kSmallPad],
+    int (&srow)[16]) {
+  constexpr int KM = PD / 8, NT = 64 * (PD / 16), H = PD / 2;
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int col = l & 15, kq = l >> 4;
-  // operator blocks three phases ahead in a register ring (n <= kMaxSmallPhases)
-  AStage<PD, NT> st[3];
+  auto load_rows = [&](dv2 (&a)[KM], const double* __restrict__ A) {
+    const double* r = A + (16 * w + col) * PD + 2 * kq;
+#pragma unroll
+    for (int m = 0; m < KM; ++m) a[m] = *reinterpret_cast<const dv2*>(r + 8 * m);
+  };
+  dv2 ar[3][KM];   // operator rows of phases i, i + 1, i + 2 (n <= kMaxSmallPhases)
 #pragma unroll
   for (int i = 0; i < 3; ++i)
-    if (i < n) st[i].load(opA(i));
+    if (i < n) load_rows(ar[i], opA(i));
   if (w == 0 && kq == 0) srow[col] = brow;
   __syncthreads();
   for (int e = threadIdx.x; e < NB * 16 * H; e += NT) {
@@ -1351,38 +1362,39 @@ __device__ __forceinline__ void run_small(int n, int nb, OpFn opA,
     const int rr = srow[box];
     dv2 v = dv2{0.0, 0.0};
     if (rr >= 0 && i < nb) v = *reinterpret_cast<const dv2*>(bbase + (int64_t)(rr + i) * PD + 2 * k2);
-    btile[i][2 * k2][box] = v.x;
-    btile[i][2 * k2 + 1][box] = v.y;
+    *reinterpret_cast<dv2*>(&btile[i][box][2 * k2]) = v;
   }
 #pragma unroll
   for (int i = 0; i < kMaxSmallPhases; ++i) {
     if (i >= n) break;
     const bool chain = i >= nb;
-    double* A = (i & 1) ? abuf1 : abuf0;
-    st[i % 3].store(A);
-    if (i + 3 < n) st[i % 3].load(opA(i + 3));
-    double b[KS];
     if (chain) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) bbuf[i & 1][16 * w + 4 * v + kq][col] = acc[v];
+      for (int v = 0; v < 4; ++v) bbuf[i & 1][col][16 * w + 4 * v + kq] = acc[v];
       acc = d4{0.0, 0.0, 0.0, 0.0};
     }
     __syncthreads();   // also orders the btile fill before its first use
+    dv2 b[KM];
     if (chain) {
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) b[ks] = bbuf[i & 1][4 * ks + kq][col];
+      for (int m = 0; m < KM; ++m)
+        b[m] = *reinterpret_cast<const dv2*>(&bbuf[i & 1][col][8 * m + 2 * kq]);
     } else {
 #pragma unroll
       for (int j = 0; j < NB; ++j)
         if (j == i) {
 #pragma unroll
-          for (int ks = 0; ks < KS; ++ks) b[ks] = btile[j][4 * ks + kq][col];
+          for (int m = 0; m < KM; ++m)
+            b[m] = *reinterpret_cast<const dv2*>(&btile[j][col][8 * m + 2 * kq]);
         }
     }
-    const double* a0 = A + (16 * w + col) * LDA + kq;
+    __builtin_amdgcn_sched_barrier(0);   // every B read issued before the MFMA chain
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks)
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[4 * ks], b[ks], acc, 0, 0, 0);
+    for (int m = 0; m < KM; ++m) {
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i % 3][m].x, b[m].x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i % 3][m].y, b[m].y, acc, 0, 0, 0);
+    }
+    if (i + 3 < n) load_rows(ar[i % 3], opA(i + 3));
   }
 }
 
@@ -1393,8 +1405,7 @@ __device__ __forceinline__ void mult_small_unit(
     int bx, int by, int64_t row0, int nrows, int with_m2m, const int2* __restrict__ rowinfo,
     const double* __restrict__ check, const double* __restrict__ opsL, double* __restrict__ mult,
     double* __restrict__ qs, int64_t dstride) {
-  __shared__ __attribute__((aligned(16))) double abuf[2][PD * Mfma<PD>::LDA];
-  __shared__ double bbuf[2][PD][17];
+  __shared__ __attribute__((aligned(16))) double bbuf[2][16][PD + kSmallPad];
   __shared__ double qpart[PD / 16][16];
   check += (int64_t)by * dstride * PD;
   mult += (int64_t)by * dstride * PD;
@@ -1415,9 +1426,9 @@ __device__ __forceinline__ void mult_small_unit(
   auto opA = [&](int k) -> const double* {
     return opsL + (int64_t)(k < nm ? OP_M2M + k : OP_PINV1 + (k - nm)) * PD * PD;
   };
-  __shared__ double btile[4][PD][17];
+  __shared__ __attribute__((aligned(16))) double btile[4][16][PD + kSmallPad];
   __shared__ int srow[16];
-  run_small<PD, 4>(nm + 2, nm, opA, mult, crow, acc, abuf[0], abuf[1], bbuf, btile, srow);
+  run_small<PD, 4>(nm + 2, nm, opA, mult, crow, acc, bbuf, btile, srow);
   double part = acc[0] + acc[1] + acc[2] + acc[3];
   part += __shfl_xor(part, 16, 64);
   part += __shfl_xor(part, 32, 64);
@@ -1454,8 +1465,7 @@ __device__ __forceinline__ void local_small_unit(
     const int* __restrict__ parent, const double* __restrict__ ple,
     const double* __restrict__ opsL, const double* __restrict__ opsParent,
     double* __restrict__ le, int64_t nstride, const FrameFix& fx) {
-  __shared__ __attribute__((aligned(16))) double abuf[2][PD * Mfma<PD>::LDA];
-  __shared__ double bbuf[2][PD][17];
+  __shared__ __attribute__((aligned(16))) double bbuf[2][16][PD + kSmallPad];
   const double* ple2 = ple + 2 * nstride * PD;   // field 2 (FrameFix)
   ple += (int64_t)by * nstride * PD;
   le += (int64_t)by * nstride * PD;
@@ -1486,9 +1496,9 @@ __device__ __forceinline__ void local_small_unit(
     return k < nl ? opsParent + (int64_t)(OP_L2L + c) * PD * PD
                   : opsL + (int64_t)(OP_PINV1 + (k - nl)) * PD * PD;
   };
-  __shared__ double btile[1][PD][17];
+  __shared__ __attribute__((aligned(16))) double btile[1][16][PD + kSmallPad];
   __shared__ int srow[16];
-  run_small<PD, 1>(nl + 2, nl, opA, le, pg, acc, abuf[0], abuf[1], bbuf, btile, srow);
+  run_small<PD, 1>(nl + 2, nl, opA, le, pg, acc, bbuf, btile, srow);
   if (active) {
     double* dst = le + (int64_t)g * PD;
 #pragma unroll

@@ -83,7 +83,51 @@ void fill_i32(int* p, int64_t n, int v, hipStream_t s);
 // D holds at most Ncutoff points, so no cell of the tree below depth D exists
 // and none needs its points grouped; segments that ended earlier are sorted.
 constexpr int kLocalSortMax = 2048;
+// first_round_ready: the caller's kernel wrote the first round's records
+// (sort_first_round below), so the sort starts at that round's segment
+// offsets; it requires n > threshold.
 bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, PrimWork& w,
-                    hipStream_t s, int key_bits = 64, bool prefix_only = false);
+                    hipStream_t s, int key_bits = 64, bool prefix_only = false,
+                    bool first_round_ready = false);
+
+// Layout of the sort's device counters and chunks.
+namespace sort_detail {
+enum { C_NEXT = 0, C_FINAL = 1, C_CHUNKS = 2, C_ACTIVE = 3, C_FINAL_BIG = 4, C_COUNT = 5 };
+}
+constexpr int kSortChunk = 4096;   // keys per chunk of a radix round
+
+// The first radix round's records, written by a key-producing kernel with one
+// block of at least 256 threads per chunk of kSortChunk keys (chunk b = keys
+// [b kSortChunk, min(n, (b + 1) kSortChunk))): each block passes the counts of
+// its keys' top 8 bits (bits 56..63) to sort_first_round_block.  This saves
+// the sort three launches and a pass over the keys.
+struct SortFirstRound {
+  int4* chunks = nullptr;
+  int* chunk_hist = nullptr;
+  int* seg_chunk0 = nullptr;
+  int2* seg0 = nullptr;
+  int* counters = nullptr;
+  int n = 0;
+};
+// Grows the sort's work buffers for n keys (as sort_pairs_u64 would) and
+// returns where the records go.
+SortFirstRound sort_first_round(PrimWork& w, int64_t n);
+
+// Every thread of block b (at least 256 threads) calls this once hist (256
+// counts in LDS) is complete.
+__device__ inline void sort_first_round_block(const SortFirstRound& f, int b, const int* hist) {
+  const int t = threadIdx.x;
+  if (t < 256) f.chunk_hist[(int64_t)b * 256 + t] = hist[t];
+  if (t == 0) {
+    const int start = b * kSortChunk;
+    f.chunks[b] = make_int4(0, start, min(kSortChunk, f.n - start), 0);
+    if (b == 0) {
+      for (int i = 0; i < sort_detail::C_COUNT; ++i) f.counters[i] = 0;
+      f.counters[sort_detail::C_CHUNKS] = (int)ceil_div(f.n, kSortChunk);
+      f.seg0[0] = make_int2(0, f.n);
+      f.seg_chunk0[0] = 0;
+    }
+  }
+}
 
 }  // namespace fmm

@@ -155,10 +155,10 @@ void exclusive_scan_i32(const int* in, int* out, int64_t n, int* total_dev, Prim
 // Segmented MSD radix sort.
 namespace {
 
-constexpr int CHUNK = 4096;
+using namespace sort_detail;
+constexpr int CHUNK = kSortChunk;
 constexpr int SORT_BS = 256;
-
-enum { C_NEXT = 0, C_FINAL = 1, C_CHUNKS = 2, C_ACTIVE = 3, C_FINAL_BIG = 4, C_COUNT = 5 };
+constexpr int kChunkPer = CHUNK / SORT_BS;   // keys per thread of a chunk block
 constexpr int kWaveSortMax = 64;
 
 __global__ __launch_bounds__(1024) void k_make_chunks(const int2* __restrict__ seg, int nseg,
@@ -210,11 +210,17 @@ __global__ __launch_bounds__(SORT_BS) void k_chunk_hist(const uint64_t* __restri
   if (b >= counters[C_CHUNKS]) return;
   int4 ch = chunks[b];
   hist[threadIdx.x] = 0;
-  __syncthreads();
-  for (int i = threadIdx.x; i < ch.z; i += SORT_BS) {
-    int d = (int)((keys[ch.y + i] >> shift) & 255u);
-    atomicAdd(&hist[d], 1);
+  // all loads first: the LDS atomics would otherwise wait for each in turn
+  uint64_t k[kChunkPer];
+#pragma unroll
+  for (int u = 0; u < kChunkPer; ++u) {
+    const int i = threadIdx.x + u * SORT_BS;
+    k[u] = i < ch.z ? keys[ch.y + i] : 0;
   }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kChunkPer; ++u)
+    if (threadIdx.x + u * SORT_BS < ch.z) atomicAdd(&hist[(int)((k[u] >> shift) & 255u)], 1);
   __syncthreads();
   chunkHist[(int64_t)b * 256 + threadIdx.x] = hist[threadIdx.x];
 }
@@ -239,17 +245,29 @@ __global__ __launch_bounds__(SEG_BS) void k_seg_offsets(
   const int nc = (int)ceil_div(sg.y, CHUNK);
   const int q = (nc + 3) >> 2;
   const int cb = c0 + min(nc, part * q), ce = c0 + min(nc, (part + 1) * q);
-  constexpr int B = 16;
+  // A part of at most R chunks (segments of up to 4 R chunks) is read in one
+  // batch and kept in registers for the second pass; larger ones are read in
+  // batches of B, twice.  The kernel's time is the latency of these reads.
+  constexpr int R = 64, B = 16;
+  const bool kept = q > B && ce - cb <= R;
+  int hk[R];
   int sum = 0;
   int c = cb;
-  for (; c + B <= ce; c += B) {
-    int h[B];
+  if (kept) {
 #pragma unroll
-    for (int u = 0; u < B; ++u) h[u] = chunkHist[(int64_t)(c + u) * 256 + d];
+    for (int u = 0; u < R; ++u) hk[u] = cb + u < ce ? chunkHist[(int64_t)(cb + u) * 256 + d] : 0;
 #pragma unroll
-    for (int u = 0; u < B; ++u) sum += h[u];
+    for (int u = 0; u < R; ++u) sum += hk[u];
+  } else {
+    for (; c + B <= ce; c += B) {
+      int h[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u) h[u] = chunkHist[(int64_t)(c + u) * 256 + d];
+#pragma unroll
+      for (int u = 0; u < B; ++u) sum += h[u];
+    }
+    for (; c < ce; ++c) sum += chunkHist[(int64_t)c * 256 + d];
   }
-  for (; c < ce; ++c) sum += chunkHist[(int64_t)c * 256 + d];
   part_sum[part][d] = sum;
   __syncthreads();
   const int run = part_sum[0][d] + part_sum[1][d] + part_sum[2][d] + part_sum[3][d];
@@ -261,7 +279,14 @@ __global__ __launch_bounds__(SEG_BS) void k_seg_offsets(
   int pre = 0;
   for (int pp = 0; pp < part; ++pp) pre += part_sum[pp][d];
   int pos = sg.x + dstart_s[d] + pre;
-  c = cb;
+  if (kept) {
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      if (cb + u < ce) chunkHist[(int64_t)(cb + u) * 256 + d] = pos;
+      pos += hk[u];
+    }
+  }
+  c = kept ? ce : cb;
   for (; c + B <= ce; c += B) {
     int h[B];
 #pragma unroll
@@ -303,22 +328,63 @@ __global__ __launch_bounds__(SEG_BS) void k_seg_offsets(
   if (threadIdx.x == 0 && act_tot) atomicAdd(&counters[C_ACTIVE], act_tot);
 }
 
+// Scatter of one chunk by its digit, staged through LDS: the chunk is first
+// grouped by digit in LDS (rank within the digit from an LDS atomic, digit
+// starts from a scan of the counts), then written out in that order, so
+// consecutive lanes store consecutive places of one digit's run instead of 64
+// unrelated places per store instruction.
 __global__ __launch_bounds__(SORT_BS) void k_chunk_scatter(
     const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
     const int4* __restrict__ chunks, const int* __restrict__ counters, int shift,
     const int* __restrict__ chunkHist, uint64_t* __restrict__ keysB, uint32_t* __restrict__ valsB) {
-  __shared__ int off[256];
+  __shared__ int gdst[256];    // destination of the chunk's first element of digit d
+  __shared__ int lstart[256];  // count, then start, of digit d inside the chunk
+  __shared__ int scan_lds[SORT_BS / 64];
+  __shared__ uint64_t sk[CHUNK];
+  __shared__ uint32_t sv[CHUNK];
   const int b = blockIdx.x;
   if (b >= counters[C_CHUNKS]) return;
-  int4 ch = chunks[b];
-  off[threadIdx.x] = chunkHist[(int64_t)b * 256 + threadIdx.x];
+  const int4 ch = chunks[b];
+  gdst[threadIdx.x] = chunkHist[(int64_t)b * 256 + threadIdx.x];
+  lstart[threadIdx.x] = 0;
+  uint64_t k[kChunkPer];
+  uint32_t v[kChunkPer];
+#pragma unroll
+  for (int u = 0; u < kChunkPer; ++u) {
+    const int i = threadIdx.x + u * SORT_BS;
+    if (i < ch.z) {
+      k[u] = keys[ch.y + i];
+      v[u] = vals[ch.y + i];
+    }
+  }
   __syncthreads();
-  for (int i = threadIdx.x; i < ch.z; i += SORT_BS) {
-    uint64_t k = keys[ch.y + i];
-    int d = (int)((k >> shift) & 255u);
-    int pos = atomicAdd(&off[d], 1);
-    keysB[pos] = k;
-    valsB[pos] = vals[ch.y + i];
+  int r[kChunkPer];
+#pragma unroll
+  for (int u = 0; u < kChunkPer; ++u)
+    if (threadIdx.x + u * SORT_BS < ch.z) r[u] = atomicAdd(&lstart[(int)((k[u] >> shift) & 255u)], 1);
+  __syncthreads();
+  const int start = block_exclusive_scan<SORT_BS>(lstart[threadIdx.x], scan_lds, nullptr);
+  lstart[threadIdx.x] = start;
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kChunkPer; ++u) {
+    if (threadIdx.x + u * SORT_BS < ch.z) {
+      const int j = lstart[(int)((k[u] >> shift) & 255u)] + r[u];
+      sk[j] = k[u];
+      sv[j] = v[u];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kChunkPer; ++u) {
+    const int j = threadIdx.x + u * SORT_BS;
+    if (j < ch.z) {
+      const uint64_t kj = sk[j];
+      const int d = (int)((kj >> shift) & 255u);
+      const int pos = gdst[d] + (j - lstart[d]);
+      keysB[pos] = kj;
+      valsB[pos] = sv[j];
+    }
   }
 }
 
@@ -443,12 +509,41 @@ __global__ __launch_bounds__(SORT_BS) void k_local_sort(uint64_t* __restrict__ k
 
 }  // namespace
 
+// Work buffers of the first round for n keys (sort_pairs_u64 sizes them the
+// same way, so it finds them in place).
+static void first_round_buffers(PrimWork& w, int64_t n) {
+  const int64_t segcap = n / 2 + 2;
+  const int64_t cbound = 1 + n / CHUNK + 1;
+  w.seg[0].ensure(segcap * sizeof(int2));
+  w.counters.ensure(C_COUNT * sizeof(int));
+  w.chunks.ensure(cbound * sizeof(int4));
+  w.chunkHist.ensure(cbound * 256 * sizeof(int));
+  w.segChunk0.ensure(sizeof(int));
+}
+
+SortFirstRound sort_first_round(PrimWork& w, int64_t n) {
+  FMM_REQUIRE(n >= 2 && n < (int64_t)1 << 31, FMM_E_LIMIT, "sort: n = %lld outside [2, 2^31-1]",
+              (long long)n);
+  first_round_buffers(w, n);
+  SortFirstRound f;
+  f.chunks = w.chunks.as<int4>();
+  f.chunk_hist = w.chunkHist.as<int>();
+  f.seg_chunk0 = w.segChunk0.as<int>();
+  f.seg0 = w.seg[0].as<int2>();
+  f.counters = w.counters.as<int>();
+  f.n = (int)n;
+  return f;
+}
+
 bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, PrimWork& w,
-                    hipStream_t s, int key_bits, bool prefix_only) {
+                    hipStream_t s, int key_bits, bool prefix_only, bool first_round_ready) {
   if (n <= 1) return prefix_only;
   FMM_REQUIRE(n < (int64_t)1 << 31, FMM_E_LIMIT, "sort: n = %lld exceeds 2^31-1", (long long)n);
   int T = threshold < 1 ? 1 : threshold;
   if (T > kLocalSortMax) T = kLocalSortMax;
+  FMM_REQUIRE(!first_round_ready || n > T, FMM_E_INTERNAL,
+              "sort: a prepared first round needs more than %d keys", T);
+  first_round_buffers(w, n);
   const int64_t segcap = n / 2 + 2;
   w.keysB.ensure(n * sizeof(uint64_t));
   w.valsB.ensure(n * sizeof(uint32_t));
@@ -460,9 +555,11 @@ bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
   int* cnt = w.counters.as<int>();
   int* hc = w.hcounters.as<int>();
   const int finBigOff = (int)segcap;
-  hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(64), 0, s, w.seg[0].as<int2>(), w.fin.as<int2>(),
-                     finBigOff, cnt, (int)n, T);
-  FMM_LAUNCH_CHECK();
+  if (!first_round_ready) {
+    hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(64), 0, s, w.seg[0].as<int2>(),
+                       w.fin.as<int2>(), finBigOff, cnt, (int)n, T);
+    FMM_LAUNCH_CHECK();
+  }
 
   int nseg = 0;
   int64_t active = 0;
@@ -503,12 +600,14 @@ bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
     w.segChunk0.ensure((int64_t)nseg * sizeof(int));
     int2* segCur = w.seg[cur].as<int2>();
     int2* segNext = w.seg[cur ^ 1].as<int2>();
-    hipLaunchKernelGGL(k_make_chunks, dim3(1), dim3(1024), 0, s, segCur, nseg,
-                       w.chunks.as<int4>(), w.segChunk0.as<int>(), cnt);
-    FMM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_chunk_hist, dim3(cbound), dim3(SORT_BS), 0, s, kin, w.chunks.as<int4>(),
-                       cnt, shift, w.chunkHist.as<int>());
-    FMM_LAUNCH_CHECK();
+    if (!(first_round_ready && r == 0)) {
+      hipLaunchKernelGGL(k_make_chunks, dim3(1), dim3(1024), 0, s, segCur, nseg,
+                         w.chunks.as<int4>(), w.segChunk0.as<int>(), cnt);
+      FMM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_chunk_hist, dim3(cbound), dim3(SORT_BS), 0, s, kin,
+                         w.chunks.as<int4>(), cnt, shift, w.chunkHist.as<int>());
+      FMM_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(k_seg_offsets, dim3(nseg), dim3(SEG_BS), 0, s, segCur,
                        w.segChunk0.as<int>(), w.chunkHist.as<int>(), T, r == rounds - 1 ? 1 : 0, segNext,
                        w.fin.as<int2>(), finBigOff, cnt);

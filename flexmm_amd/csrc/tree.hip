@@ -14,6 +14,7 @@
 // Level d+1 holds the children of P_d in Morton order.  Colleagues, parents,
 // point ranges and M2L partners are binary searches in these sorted lists.
 #include "tree.h"
+#include "point_key.h"
 
 #include <cstring>
 
@@ -24,33 +25,6 @@ namespace {
 constexpr int BS = 256;
 
 __device__ __forceinline__ int clz64(uint64_t v) { return v ? __clzll((long long)v) : 64; }
-
-__host__ __device__ inline uint64_t spread_bits(uint32_t v) {
-  uint64_t x = v;
-  x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
-  x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
-  x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
-  x = (x | (x << 2)) & 0x3333333333333333ull;
-  x = (x | (x << 1)) & 0x5555555555555555ull;
-  return x;
-}
-__host__ __device__ inline uint32_t compact_bits(uint64_t x) {
-  x &= 0x5555555555555555ull;
-  x = (x | (x >> 1)) & 0x3333333333333333ull;
-  x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
-  x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
-  x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
-  x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
-  return (uint32_t)x;
-}
-// Morton code of a cell: the digit of level l (from the top) is 2*xbit + ybit.
-__host__ __device__ inline uint64_t morton(uint32_t ix, uint32_t iy) {
-  return (spread_bits(ix) << 1) | spread_bits(iy);
-}
-__host__ __device__ inline void demorton(uint64_t c, uint32_t* ix, uint32_t* iy) {
-  *ix = compact_bits(c >> 1);
-  *iy = compact_bits(c);
-}
 
 __device__ __forceinline__ int64_t bsearch_u64(const uint64_t* __restrict__ a, int64_t n,
                                                uint64_t v) {
@@ -89,33 +63,57 @@ __device__ __forceinline__ int find_level(const T* __restrict__ off, int L, int6
 // --- points ----------------------------------------------------------------
 // Morton keys of the points and an interleaved copy of their coordinates
 // (the gather into tree order then reads one 16-byte record per point).
-__global__ __launch_bounds__(BS) void k_point_keys(const double* __restrict__ x,
+// One block of 1024 threads per chunk of kSortChunk points (4 per thread,
+// strided by the block size; every load is issued before the first key is
+// formed).  The key
+// comes from the scaled-coordinate form where it provably equals the
+// reference's recurrence (point_key.h) and from the recurrence itself for the
+// few points within rounding reach of a midpoint.  With fr (more points than
+// the sort's threshold) the block also counts its keys' top 8 bits for the
+// sort's first round.
+constexpr int KEY_BS = 1024;
+constexpr int kKeyPer = kSortChunk / KEY_BS;
+static_assert(kKeyLevels == kMaxDepth, "keys cover every depth");
+__global__ __launch_bounds__(KEY_BS) void k_point_keys(const double* __restrict__ x,
                                                    const double* __restrict__ y, int n,
                                                    double xmin0, double ymin0, double w0,
-                                                   uint64_t* __restrict__ keys,
+                                                   KeyScale ks, uint64_t* __restrict__ keys,
                                                    uint32_t* __restrict__ vals,
                                                    double2* __restrict__ xy,
-                                                   int* __restrict__ nonfinite) {
-  int i = blockIdx.x * BS + threadIdx.x;
-  if (i >= n) return;
-  const double px = x[i], py = y[i];
-  xy[i] = make_double2(px, py);
-  if (!(__builtin_isfinite(px) && __builtin_isfinite(py))) atomicOr(nonfinite, 1);
-  double xmin = xmin0, ymin = ymin0, w = w0;
-  uint64_t key = 0;
-  for (int l = 1; l <= kMaxDepth; ++l) {
-    const double h = 0.5 * w;
-    const double xm = xmin + h;
-    const double ym = ymin + h;
-    const int bx = px > xm;
-    const int by = py > ym;
-    key |= (uint64_t)(2 * bx + by) << (64 - 2 * l);
-    xmin = bx ? xm : xm - h;
-    ymin = by ? ym : ym - h;
-    w = h;
+                                                   int* __restrict__ nonfinite, SortFirstRound fr,
+                                                   int with_hist) {
+  static_assert(kKeyPer * KEY_BS == kSortChunk, "one chunk per block");
+  __shared__ int hist[256];
+  if (with_hist && threadIdx.x < 256) hist[threadIdx.x] = 0;
+  const int i0 = blockIdx.x * kSortChunk + threadIdx.x;
+  double px[kKeyPer], py[kKeyPer];
+#pragma unroll
+  for (int u = 0; u < kKeyPer; ++u) {
+    const int i = i0 + u * KEY_BS;
+    px[u] = i < n ? x[i] : 0.0;
+    py[u] = i < n ? y[i] : 0.0;
   }
-  keys[i] = key;
-  vals[i] = (uint32_t)i;
+  if (with_hist) __syncthreads();
+  bool bad = false;
+#pragma unroll
+  for (int u = 0; u < kKeyPer; ++u) {
+    const int i = i0 + u * KEY_BS;
+    if (i < n) {
+      xy[i] = make_double2(px[u], py[u]);
+      bad |= !(__builtin_isfinite(px[u]) && __builtin_isfinite(py[u]));
+      bool ok;
+      uint64_t key = point_key_fast(px[u], py[u], xmin0, ymin0, ks, &ok);
+      if (!ok) key = point_key_recurrence(px[u], py[u], xmin0, ymin0, w0);
+      keys[i] = key;
+      vals[i] = (uint32_t)i;
+      if (with_hist) atomicAdd(&hist[(int)(key >> 56)], 1);
+    }
+  }
+  if (bad) atomicOr(nonfinite, 1);
+  if (with_hist) {
+    __syncthreads();
+    sort_first_round_block(fr, blockIdx.x, hist);
+  }
 }
 
 __global__ __launch_bounds__(BS) void k_gather_sorted(const double2* __restrict__ xy,
@@ -133,40 +131,50 @@ __global__ __launch_bounds__(BS) void k_gather_sorted(const double2* __restrict_
 // Position i starts the cells of depths s(i)+1.. (s = common depth with key
 // i-1); such a cell has more than ncut points iff key i+ncut is still inside
 // it, i.e. depth <= c(i) (common depth with key i+ncut).
-__global__ __launch_bounds__(BS) void k_split_pos(const uint64_t* __restrict__ keys, int n,
-                                                  int ncut, int* __restrict__ flag,
-                                                  int2* __restrict__ sc) {
-  int i = blockIdx.x * BS + threadIdx.x;
-  if (i >= n) return;
-  const uint64_t k = keys[i];
-  const int s = (i == 0) ? -1 : min(clz64(k ^ keys[i - 1]) >> 1, kMaxDepth);
-  const int c = ((int64_t)i + ncut < n) ? min(clz64(k ^ keys[i + ncut]) >> 1, kMaxDepth - 1) : -1;
-  flag[i] = s < c;
-  sc[i] = make_int2(s, c);
-}
-
-// Number of split cells per depth: entry j of the compacted split positions
-// starts the split cells of depths s_j+1 .. c_j.  One block per 1024 entries
-// (the grid covers the n >= np bound); wave ballots count each depth into LDS,
-// and each block adds its counts to dcount, which starts at zero.
-__global__ __launch_bounds__(1024) void k_depth_count(const int2* __restrict__ psc,
-                                                      const int* __restrict__ np_dev,
-                                                      int* __restrict__ dcount) {
-  __shared__ int lc[kMaxLevels];
-  const int np = *np_dev;
-  if ((int64_t)blockIdx.x * 1024 >= np) return;   // block-uniform, before any barrier
-  if (threadIdx.x < kMaxLevels) lc[threadIdx.x] = 0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int j = blockIdx.x * 1024 + threadIdx.x;
-  const int2 v = j < np ? psc[j] : make_int2(0, 0);
-  for (int d = 0; d < kMaxLevels; ++d) {
-    const unsigned long long m = __ballot(v.x < d && d <= v.y);
-    if (lane == 0 && m) atomicAdd(&lc[d], (int)__popcll(m));
+// The same kernel counts the split cells per depth (position i starts one at
+// each depth d with s < d <= c) and the split positions: each wave writes its
+// ballot counts to its own LDS row, the block adds the rows and adds them to
+// row b mod kSplitRows of `rows` ([kSplitRows][kMaxLevels + 1], zeroed; the
+// host adds the rows).  A thousand blocks adding to one set of counters
+// serialised on those addresses (10 us of a 15 us kernel at 1M points).  A
+// complete tree needs nothing more from the split positions.
+constexpr int kSplitRows = 64;
+__global__ __launch_bounds__(1024) void k_split_pos(const uint64_t* __restrict__ keys, int n,
+                                                    int ncut, int* __restrict__ flag,
+                                                    int2* __restrict__ sc, int* __restrict__ rows) {
+  __shared__ int lc[1024 / 64][kMaxLevels + 1];
+  const int i = blockIdx.x * 1024 + threadIdx.x;
+  int s = 0, c = -1;
+  if (i < n) {
+    // the three keys are loaded together (clamped indices): behind their
+    // conditions each load waited for the one before
+    const int ip = i > 0 ? i - 1 : 0;
+    const int ic = (int)min((int64_t)i + ncut, (int64_t)n - 1);
+    const uint64_t k = keys[i], kp = keys[ip], kc = keys[ic];
+    s = (i == 0) ? -1 : min(clz64(k ^ kp) >> 1, kMaxDepth);
+    c = ((int64_t)i + ncut < n) ? min(clz64(k ^ kc) >> 1, kMaxDepth - 1) : -1;
+    flag[i] = s < c;
+    sc[i] = make_int2(s, c);
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const unsigned long long fm = __ballot(s < c);
+  if (fm) {   // wave-uniform: most waves hold no split position
+    for (int d = 0; d < kMaxLevels; ++d) {
+      const int cnt = (int)__popcll(__ballot(s < d && d <= c));
+      if (lane == 0) lc[wv][d] = cnt;
+    }
+    if (lane == 0) lc[wv][kMaxLevels] = (int)__popcll(fm);
+  } else if (lane <= kMaxLevels) {
+    lc[wv][lane] = 0;
   }
   __syncthreads();
-  if (threadIdx.x < kMaxLevels && lc[threadIdx.x] != 0)
-    atomicAdd(&dcount[threadIdx.x], lc[threadIdx.x]);
+  if (threadIdx.x <= kMaxLevels) {
+    int t = 0;
+#pragma unroll
+    for (int w = 0; w < 1024 / 64; ++w) t += lc[w][threadIdx.x];
+    if (t != 0)
+      atomicAdd(&rows[(blockIdx.x % kSplitRows) * (kMaxLevels + 1) + threadIdx.x], t);
+  }
 }
 
 __global__ __launch_bounds__(BS) void k_compact_pos(const int* __restrict__ flag,
@@ -787,7 +795,9 @@ __global__ __launch_bounds__(BS) void k_full_starts(const uint64_t* __restrict__
                                                     int* __restrict__ start) {
   const int i = blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
-  if (sc[i].x < D) start[keys[i] >> (64 - 2 * D)] = i;
+  const int2 si = sc[i];
+  const uint64_t k = keys[i];   // loaded with sc, not behind its test
+  if (si.x < D) start[k >> (64 - 2 * D)] = i;
 }
 
 // points of a depth-d cell c: [bot, top) from the depth-D starts (a cell
@@ -804,7 +814,11 @@ __device__ __forceinline__ int full_pos(const int* __restrict__ start,
 __device__ __forceinline__ int full_levoff(int d) { return (int)((((int64_t)1 << (2 * d)) - 1) / 3); }
 
 // Every node array of the complete tree of depth D (levels 0 .. D, level d =
-// the 4^d cells in Morton order from levoff(d) = (4^d - 1) / 3).
+// the 4^d cells in Morton order from levoff(d) = (4^d - 1) / 3).  Sixteen
+// lanes per node: lane 0 writes the node's own fields, lanes 0..8 its nine
+// colleague slots and all sixteen its 40 M2L slots (j, j + 16, j + 32), so
+// the per-node arrays are written in contiguous runs.
+constexpr int kFullLanes = 16;
 __global__ __launch_bounds__(BS) void k_full_nodes(
     int D, int nn, const int* __restrict__ start, const uint64_t* __restrict__ keys, int npts,
     double xmin0, double ymin0, double w0, uint64_t* __restrict__ code, int* __restrict__ nlev,
@@ -813,41 +827,54 @@ __global__ __launch_bounds__(BS) void k_full_nodes(
     int2* __restrict__ crange, double* __restrict__ xmid, double* __restrict__ ymid,
     int* __restrict__ row, int* __restrict__ m2l, int* __restrict__ evalid,
     int* __restrict__ evalnode, int* __restrict__ upnodes, int2* __restrict__ rowinfo) {
-  const int g = blockIdx.x * BS + threadIdx.x;
+  const int64_t tid = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const int g = (int)(tid / kFullLanes), j = (int)(tid % kFullLanes);
   if (g >= nn) return;
   int d = 0;
   while (d < D && g >= full_levoff(d + 1)) ++d;
   const int lo = full_levoff(d);
   const uint64_t c = (uint64_t)(g - lo);
   const bool leaf = d == D;
-  const int b = full_pos(start, keys, npts, D, d, c), t = full_pos(start, keys, npts, D, d, c + 1);
-  int f = leaf ? NF_LEAF : (NF_CHILDREN | NF_DOWN);   // tree.py:557-562
-  if (t > b) f |= NF_SOURCE | (leaf ? NF_UP : 0);
-  code[g] = c;
-  nlev[g] = d;
-  flags[g] = f;
-  bot[g] = b;
-  top[g] = t;
-  parent[g] = d > 0 ? full_levoff(d - 1) + (int)(c >> 2) : -1;
-  const int ch = leaf ? -1 : full_levoff(d + 1) + 4 * (int)c;
-  child[g] = ch;
-  // centre by the reference recurrence (as k_nodes_b)
-  double xm0 = xmin0, ym0 = ymin0, w = w0;
-  for (int l = 1; l <= d; ++l) {
-    const double h = 0.5 * w;
-    const double xm = xm0 + h, ym = ym0 + h;
-    const int dig = (int)((c >> (2 * (d - l))) & 3);
-    xm0 = (dig & 2) ? xm : xm - h;
-    ym0 = (dig & 1) ? ym : ym - h;
-    w = h;
-  }
-  xmid[g] = xm0 + 0.5 * w;
-  ymid[g] = ym0 + 0.5 * w;
+  const int g0 = full_levoff(2);
   uint32_t ix, iy;
   demorton(c, &ix, &iy);
   const int64_t lim = (int64_t)1 << d;
-  for (int q = 0; q < 9; ++q) {
-    const int64_t nx = (int64_t)ix + q / 3 - 1, ny = (int64_t)iy + q % 3 - 1;
+  if (j == 0) {
+    const int b = full_pos(start, keys, npts, D, d, c), t = full_pos(start, keys, npts, D, d, c + 1);
+    int f = leaf ? NF_LEAF : (NF_CHILDREN | NF_DOWN);   // tree.py:557-562
+    if (t > b) f |= NF_SOURCE | (leaf ? NF_UP : 0);
+    code[g] = c;
+    nlev[g] = d;
+    flags[g] = f;
+    bot[g] = b;
+    top[g] = t;
+    parent[g] = d > 0 ? full_levoff(d - 1) + (int)(c >> 2) : -1;
+    const int ch = leaf ? -1 : full_levoff(d + 1) + 4 * (int)c;
+    child[g] = ch;
+    // centre by the reference recurrence (as k_nodes_b)
+    double xm0 = xmin0, ym0 = ymin0, w = w0;
+    for (int l = 1; l <= d; ++l) {
+      const double h = 0.5 * w;
+      const double xm = xm0 + h, ym = ym0 + h;
+      const int dig = (int)((c >> (2 * (d - l))) & 3);
+      xm0 = (dig & 2) ? xm : xm - h;
+      ym0 = (dig & 1) ? ym : ym - h;
+      w = h;
+    }
+    xmid[g] = xm0 + 0.5 * w;
+    ymid[g] = ym0 + 0.5 * w;
+    // rows: every sibling group of levels >= 2 holds sources, so row = g - g0
+    const int r = d >= 2 ? g - g0 : -1;
+    row[g] = r;
+    if (r >= 0) rowinfo[r] = make_int2(ch >= 0 ? ch - g0 : -1, f);
+    evalid[g] = leaf ? (int)c : -1;
+    if (leaf) {
+      evalnode[c] = g;
+      upnodes[c] = g;   // every leaf: an empty one has S2M check values 0 (no sources)
+    }
+  }
+  if (j < 9) {
+    const int64_t nx = (int64_t)ix + j / 3 - 1, ny = (int64_t)iy + j % 3 - 1;
     int r = -1;
     int2 rg = make_int2(0, 0);
     if (nx >= 0 && ny >= 0 && nx < lim && ny < lim) {
@@ -856,22 +883,12 @@ __global__ __launch_bounds__(BS) void k_full_nodes(
       rg = make_int2(full_pos(start, keys, npts, D, d, cc),
                      full_pos(start, keys, npts, D, d, cc + 1));
     }
-    coll[(int64_t)g * 9 + q] = r;
-    crange[(int64_t)g * 9 + q] = rg;
-  }
-  // rows: every sibling group of levels >= 2 holds sources, so row = g - g0
-  const int g0 = full_levoff(2);
-  const int r = d >= 2 ? g - g0 : -1;
-  row[g] = r;
-  if (r >= 0) rowinfo[r] = make_int2(ch >= 0 ? ch - g0 : -1, f);
-  evalid[g] = leaf ? (int)c : -1;
-  if (leaf) {
-    evalnode[c] = g;
-    upnodes[c] = g;   // every leaf: an empty one has S2M check values 0 (no sources)
+    coll[(int64_t)g * 9 + j] = r;
+    crange[(int64_t)g * 9 + j] = rg;
   }
   if (d >= 2) {
     const int cx = ix & 1, cy = iy & 1;
-    for (int s = 0; s < kNumM2L; ++s) {
+    for (int s = j; s < kNumM2L; s += kFullLanes) {
       const int ox = c_slot_off.ox[s], oy = c_slot_off.oy[s];
       int rr = -1;
       if (ox >= -2 - cx && ox <= 3 - cx && oy >= -2 - cy && oy <= 3 - cy) {
@@ -937,7 +954,8 @@ static void tree_build_full(Tree& t, int D, int npts, hipStream_t s, Upload&& up
   t.rowinfo.ensure((t.n_rows + 1) * sizeof(int2));
   // levels 0 and 1 have no M2L lists (their entries are never read), as in
   // the general build
-  hipLaunchKernelGGL(k_full_nodes, dim3(grid_for(NN)), dim3(BS), 0, s, D, NN, start,
+  hipLaunchKernelGGL(k_full_nodes, dim3(grid_for((int64_t)NN * kFullLanes)), dim3(BS), 0, s, D, NN,
+                     start,
                      t.keys.as<uint64_t>(), npts, t.xmin, t.ymin, t.w0, t.code.as<uint64_t>(),
                      t.nlevel.as<int>(), t.flags.as<int>(), t.bot.as<int>(), t.top.as<int>(),
                      t.parent.as<int>(), t.child.as<int>(), t.coll.as<int>(), t.crange.as<int2>(),
@@ -971,15 +989,20 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   uint32_t* perm = t.perm.as<uint32_t>();
   t.xy.ensure(n * sizeof(double2));
   // -- counters (zeroed before the first kernel, which flags non-finite points)
-  enum { CNT_D = 0, CNT_P = 32, CNT_R = 33, CNT_PC = 34, CNT_ROWS = 66, CNT_EVAL = 67,
-         CNT_UP = 68, CNT_UNIQ = 69, CNT_LR = 72, CNT_BAL = 104, CNT_BAD = 136, CNT_N = 137 };
-  t.w_cnt.ensure(CNT_N * sizeof(int));
-  t.h_cnt.ensure(CNT_N * sizeof(int));
+  enum { CNT_R = 33, CNT_PC = 34, CNT_ROWS = 66, CNT_EVAL = 67, CNT_UP = 68, CNT_UNIQ = 69,
+         CNT_LR = 72, CNT_BAL = 104, CNT_BAD = 136, CNT_N = 137,
+         CNT_SPLIT = CNT_N, CNT_ALL = CNT_SPLIT + kSplitRows * (kMaxLevels + 1) };
+  t.w_cnt.ensure(CNT_ALL * sizeof(int));
+  t.h_cnt.ensure(CNT_ALL * sizeof(int));
   int* cnt = t.w_cnt.as<int>();
   int* hc = t.h_cnt.as<int>();
-  FMM_HIP(hipMemsetAsync(cnt, 0, CNT_N * sizeof(int), s));
-  hipLaunchKernelGGL(k_point_keys, dim3(grid_for(n)), dim3(BS), 0, s, x, y, nn, xmin, ymin, w0,
-                     keys, perm, t.xy.as<double2>(), cnt + CNT_BAD);
+  FMM_HIP(hipMemsetAsync(cnt, 0, CNT_ALL * sizeof(int), s));
+  const int sort_T = ncut < kLocalSortMax ? ncut : kLocalSortMax;
+  const bool first_round = n > sort_T;
+  const SortFirstRound fr = first_round ? sort_first_round(t.pw, n) : SortFirstRound{};
+  hipLaunchKernelGGL(k_point_keys, dim3((unsigned)ceil_div(n, kSortChunk)), dim3(KEY_BS), 0, s, x, y,
+                     nn, xmin, ymin, w0, key_scale(xmin, ymin, w0), keys, perm,
+                     t.xy.as<double2>(), cnt + CNT_BAD, fr, first_round ? 1 : 0);
   FMM_LAUNCH_CHECK();
   // Segments of at most Ncutoff points that end in the last radix round sit
   // at the deepest radix depth D, which no cell of the tree goes below (2:1
@@ -987,7 +1010,7 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   // children at most at depth D): the radix order within them stands.
   // Segments that end earlier are sorted, since balance and X-list children
   // can split their cells further.
-  sort_pairs_u64(keys, perm, n, ncut < kLocalSortMax ? ncut : kLocalSortMax, t.pw, s, 64, true);
+  sort_pairs_u64(keys, perm, n, sort_T, t.pw, s, 64, true, first_round);
   hipLaunchKernelGGL(k_gather_sorted, dim3(grid_for(n)), dim3(BS), 0, s, t.xy.as<double2>(), perm,
                      nn, t.sx.as<double>(), t.sy.as<double>());
   FMM_LAUNCH_CHECK();
@@ -1014,32 +1037,20 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   int* flag = t.w_flag.as<int>();
   int* rank = t.w_rank.as<int>();
   int2* sc = t.w_a.as<int2>();
-  hipLaunchKernelGGL(k_split_pos, dim3(grid_for(n)), dim3(BS), 0, s, keys, nn, ncut, flag, sc);
+  hipLaunchKernelGGL(k_split_pos, dim3((unsigned)ceil_div(n, 1024)), dim3(1024), 0, s, keys, nn,
+                     ncut, flag, sc, cnt + CNT_SPLIT);
   FMM_LAUNCH_CHECK();
-  exclusive_scan_i32(flag, rank, n, cnt + CNT_P, t.pw, s);
-  // compacted split positions (sized for the worst case, so that no count is
-  // needed on the host first) and the split cells per depth
-  t.w_c.ensure((int64_t)(n + 2) * (sizeof(int) + sizeof(int2)) + 16);
-  int* ppos = t.w_c.as<int>();
-  int2* psc = reinterpret_cast<int2*>(ppos + n + (n & 1));
-  hipLaunchKernelGGL(k_compact_pos, dim3(grid_for(n)), dim3(BS), 0, s, flag, rank, sc, nn, ppos,
-                     psc);
-  FMM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_depth_count, dim3((unsigned)std::max<int64_t>(1, ceil_div(n, 1024))),
-                     dim3(1024), 0, s, psc,
-                     cnt + CNT_P, cnt + CNT_D);
-  FMM_LAUNCH_CHECK();
-  FMM_HIP(hipMemcpyAsync(hc, cnt, CNT_N * sizeof(int), hipMemcpyDeviceToHost, s));
+  FMM_HIP(hipMemcpyAsync(hc, cnt, CNT_ALL * sizeof(int), hipMemcpyDeviceToHost, s));
   FMM_HIP(hipStreamSynchronize(s));
   FMM_REQUIRE(hc[CNT_BAD] == 0, FMM_E_ARG,
               "tree: the source coordinates hold a NaN or infinite value");
-  const int np = hc[CNT_P];
+  int64_t dcount[kMaxLevels + 1] = {};   // per depth, then the split positions
+  for (int r = 0; r < kSplitRows; ++r)
+    for (int d = 0; d <= kMaxLevels; ++d) dcount[d] += hc[CNT_SPLIT + r * (kMaxLevels + 1) + d];
+  const int np = (int)dcount[kMaxLevels];
   int maxd = -1;
-  int64_t dcount[kMaxLevels];
-  for (int d = 0; d < kMaxLevels; ++d) {
-    dcount[d] = hc[CNT_D + d];
+  for (int d = 0; d < kMaxLevels; ++d)
     if (dcount[d] > 0) maxd = d;
-  }
   const int L = maxd + 2;   // deepest split depth + children level + 1
   t.L = L;
   FMM_REQUIRE(L <= kMaxLevels, FMM_E_INTERNAL, "tree: %d levels exceed the depth cap", L);
@@ -1057,7 +1068,14 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
     }
   }
 
-  // split-cell lists per depth (S_ref)
+  // compacted split positions and the split-cell lists per depth (S_ref)
+  exclusive_scan_i32(flag, rank, n, nullptr, t.pw, s);
+  t.w_c.ensure((int64_t)(n + 2) * (sizeof(int) + sizeof(int2)) + 16);
+  int* ppos = t.w_c.as<int>();
+  int2* psc = reinterpret_cast<int2*>(ppos + n + (n & 1));
+  hipLaunchKernelGGL(k_compact_pos, dim3(grid_for(n)), dim3(BS), 0, s, flag, rank, sc, nn, ppos,
+                     psc);
+  FMM_LAUNCH_CHECK();
   t.s_off.assign(L + 1, 0);
   for (int d = 0; d < L; ++d) t.s_off[d + 1] = t.s_off[d] + (d <= L - 2 ? dcount[d] : 0);
   const int64_t ns_total = t.s_off[L];
