@@ -976,15 +976,23 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 // gives four times the waves (a 1M-point tree: ~5 per SIMD instead of ~1.4).
 constexpr int kM2LWCols = 16;
 constexpr int kM2LWWaves = 4;
+// FP32 M2L: NCG column groups of 16 boxes and NW waves per block.  Measured
+// at 1M points against one group and 4 waves (0.095 ms): two groups, which
+// share each residual fragment load between twice the MFMAs, 0.103 ms at 3
+// waves per SIMD and 0.122 at 2; 8 waves per block (finer work per wave for
+// the last round of waves), 0.098 ms.
+constexpr int kM2LF32Groups = 1;
+constexpr int kM2LF32Waves = 4;
 
-template <int PD, bool FRAME>
-__global__ __launch_bounds__(64 * kM2LWWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_m2l_waves_f32(
+template <int PD, bool FRAME, int NCG, int NW>
+__global__ __launch_bounds__(64 * NW)
+__attribute__((amdgpu_waves_per_eu(NCG == 1 ? 4 : 3))) void k_m2l_waves_f32(
     LevelTable lt, const int* __restrict__ flags, const int* __restrict__ m2l,
     const double* __restrict__ mult, const double* __restrict__ qs,
     const double* __restrict__ ops, int64_t opsz, double* __restrict__ ple, int64_t rstride,
     int64_t nstride, const double* __restrict__ xmid, const double* __restrict__ ymid,
     double c0x, double c0y) {
-  constexpr int RB = PD / 16, KS = PD / 4, NW = kM2LWWaves;
+  constexpr int RB = PD / 16, KS = PD / 4, NC = 16 * NCG;
   typedef float fv4 __attribute__((ext_vector_type(4)));
   // FRAME: fields 0 and 1 take the source's field-2 row (D_2) too
   const bool frame = FRAME && blockIdx.y < 2;
@@ -992,10 +1000,10 @@ __global__ __launch_bounds__(64 * kM2LWWaves) __attribute__((amdgpu_waves_per_eu
   mult += (int64_t)blockIdx.y * rstride * PD;
   qs += (int64_t)blockIdx.y * rstride;
   ple += (int64_t)blockIdx.y * nstride * PD;
-  __shared__ int srcs[27][kM2LWCols];
+  __shared__ int srcs[27][NC];
   __shared__ int pslot[28];
   __shared__ unsigned int smask;
-  __shared__ double red[NW - 1][PD][kM2LWCols + 1];
+  __shared__ double red[NW - 1][PD][NC + 1];
   const int bid = xcd_block(blockIdx.x, gridDim.x);
   int li = 0;
   while (li + 1 < lt.nlev && bid >= lt.boff[li + 1]) ++li;
@@ -1005,26 +1013,36 @@ __global__ __launch_bounds__(64 * kM2LWWaves) __attribute__((amdgpu_waves_per_eu
   const int ngroups = lt.ngroups[li];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int col = l & 15, kq = l >> 4;
-  const int qq = tile * kM2LWCols + col;
-  const int q = (lt.perm && qq < ngroups) ? lt.perm[lt.poff[li] + c * ngroups + qq] : qq;
-  const int g = lt.g0[li] + 4 * q + c;
-  const bool active = q < ngroups && !(flags[q < ngroups ? g : lt.g0[li]] & NF_FAKE);
-  // interaction lists: lane group (w, kq) reads slot indices 4 w + kq and
-  // 4 w + kq + 16 of its column
+  int g[NCG];
+  bool active[NCG];
+#pragma unroll
+  for (int cg = 0; cg < NCG; ++cg) {
+    const int qq = tile * NC + 16 * cg + col;
+    const int q = (lt.perm && qq < ngroups) ? lt.perm[lt.poff[li] + c * ngroups + qq] : qq;
+    g[cg] = lt.g0[li] + 4 * q + c;
+    active[cg] = q < ngroups && !(flags[q < ngroups ? g[cg] : lt.g0[li]] & NF_FAKE);
+  }
+  // interaction lists: the 4 NW lane groups (w, kq) read slot indices
+  // 4 w + kq + 4 NW j < 32 of their column in each group
+  constexpr int LG = 4 * NW, NJ = 32 / LG;
+  static_assert(NJ * LG == 32, "lane groups cover the 32 slot indices");
   const int nv = c_valid.n[c];
-  const int* lst = m2l + (int64_t)g * kNumM2L;
   if (threadIdx.x == 0) smask = 0u;
   __syncthreads();
   unsigned int used = 0u;
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int i = 4 * w + kq + 16 * j;
-    const int src = (active && i < nv) ? lst[c_valid.slot[c][i]] : -1;
-    if (i < nv) srcs[i][col] = src;
-    const unsigned long long b = __ballot(src >= 0);
+  for (int cg = 0; cg < NCG; ++cg) {
+    const int* lst = m2l + (int64_t)g[cg] * kNumM2L;
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if ((b >> (16 * k)) & 0xffffull) used |= 1u << (4 * w + k + 16 * j);
+    for (int j = 0; j < NJ; ++j) {
+      const int i = 4 * w + kq + LG * j;
+      const int src = (active[cg] && i < nv) ? lst[c_valid.slot[c][i]] : -1;
+      if (i < nv) srcs[i][16 * cg + col] = src;
+      const unsigned long long b = __ballot(src >= 0);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if ((b >> (16 * k)) & 0xffffull) used |= 1u << (4 * w + k + LG * j);
+    }
   }
   if (l == 0 && used) atomicOr(&smask, used);
   __syncthreads();
@@ -1042,41 +1060,51 @@ __global__ __launch_bounds__(64 * kM2LWWaves) __attribute__((amdgpu_waves_per_eu
   const double* opsL = ops + (int64_t)lt.d[li] * opsz;
   const double* means = opsL + ops_off_means(PD);
   const float* r32 = reinterpret_cast<const float*>(opsL + ops_off_r32(PD));
-  double ct = 0.0;   // target centre coordinate i in the root frame
-  if (frame && active) ct = blockIdx.y == 0 ? xmid[g] - c0x : ymid[g] - c0y;
-  double acc64[RB][4];
+  double ct[NCG];   // target centre coordinate i in the root frame
 #pragma unroll
-  for (int rb = 0; rb < RB; ++rb)
+  for (int cg = 0; cg < NCG; ++cg) {
+    ct[cg] = 0.0;
+    if (frame && active[cg]) ct[cg] = blockIdx.y == 0 ? xmid[g[cg]] - c0x : ymid[g[cg]] - c0y;
+  }
+  double acc64[NCG][RB][4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc64[rb][v] = 0.0;
+  for (int cg = 0; cg < NCG; ++cg)
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc64[cg][rb][v] = 0.0;
   for (int k = w; k < nph; k += NW) {
     const int i = pslot[k];
     const int slot = c_valid.slot[c][i];
-    const int src = srcs[i][col];
     // B: the source row's entries 8 m + 2 kq + {0, 1}, rounded to FP32 (the
     // Stokes fields 0 and 1 in the target box's frame, formed in FP64)
-    float b[KS];
-    double qv = 0.0;
-    if (src >= 0) {
-      const double* brow = mult + (int64_t)src * PD + 2 * kq;
+    float b[NCG][KS];
+    double qv[NCG];
 #pragma unroll
-      for (int m = 0; m < KS / 2; ++m) {
-        dv2 v = *reinterpret_cast<const dv2*>(brow + 8 * m);
-        if (frame) {
-          const dv2 v2 = *reinterpret_cast<const dv2*>(brow + to2 * PD + 8 * m);
-          v.x = __builtin_fma(ct, v2.x, v.x);
-          v.y = __builtin_fma(ct, v2.y, v.y);
+    for (int cg = 0; cg < NCG; ++cg) {
+      const int src = srcs[i][16 * cg + col];
+      qv[cg] = 0.0;
+      if (src >= 0) {
+        const double* brow = mult + (int64_t)src * PD + 2 * kq;
+#pragma unroll
+        for (int m = 0; m < KS / 2; ++m) {
+          dv2 v = *reinterpret_cast<const dv2*>(brow + 8 * m);
+          if (frame) {
+            const dv2 v2 = *reinterpret_cast<const dv2*>(brow + to2 * PD + 8 * m);
+            v.x = __builtin_fma(ct[cg], v2.x, v.x);
+            v.y = __builtin_fma(ct[cg], v2.y, v.y);
+          }
+          b[cg][2 * m] = (float)v.x;
+          b[cg][2 * m + 1] = (float)v.y;
         }
-        b[2 * m] = (float)v.x;
-        b[2 * m + 1] = (float)v.y;
-      }
-      qv = frame ? __builtin_fma(ct, qs[to2 + src], qs[src]) : qs[src];
-    } else {
+        qv[cg] = frame ? __builtin_fma(ct[cg], qs[to2 + src], qs[src]) : qs[src];
+      } else {
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) b[ks] = 0.f;
+        for (int ks = 0; ks < KS; ++ks) b[cg][ks] = 0.f;
+      }
     }
     // A in operand order, one pair of 16-row blocks at a time (64 registers
-    // for all four would hold the kernel to two waves per SIMD):
+    // for all four would cost the kernel its resident waves):
     // a[r][mp] = row 16 (2 rp + r) + (l mod 16), columns 8 m + 2 kq + {0, 1}
     // for m = 2 mp, 2 mp + 1
     const fv4* afrag = reinterpret_cast<const fv4*>(r32 + (int64_t)slot * PD * PD) + l;
@@ -1088,24 +1116,30 @@ __global__ __launch_bounds__(64 * kM2LWWaves) __attribute__((amdgpu_waves_per_eu
       for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int j = 0; j < KS / 4; ++j) a[r][j] = afrag[((2 * rp + r) * (KS / 4) + j) * 64];
-      f4 acc[2] = {f4{0.f, 0.f, 0.f, 0.f}, f4{0.f, 0.f, 0.f, 0.f}};
+      f4 acc[NCG][2];
+#pragma unroll
+      for (int cg = 0; cg < NCG; ++cg) acc[cg][0] = acc[cg][1] = f4{0.f, 0.f, 0.f, 0.f};
       // FP32 runs of kFlushM step pairs, each added to the FP64 sums
 #pragma unroll
       for (int m = 0; m < KS / 2; ++m) {
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
-          acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r][m >> 1][2 * (m & 1)], b[2 * m],
-                                                       acc[r], 0, 0, 0);
-          acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r][m >> 1][2 * (m & 1) + 1],
-                                                       b[2 * m + 1], acc[r], 0, 0, 0);
-        }
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+          for (int cg = 0; cg < NCG; ++cg) {
+            acc[cg][r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r][m >> 1][2 * (m & 1)],
+                                                             b[cg][2 * m], acc[cg][r], 0, 0, 0);
+            acc[cg][r] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                a[r][m >> 1][2 * (m & 1) + 1], b[cg][2 * m + 1], acc[cg][r], 0, 0, 0);
+          }
         if ((m + 1) % kFlushM == 0 && m + 1 < KS / 2) {
 #pragma unroll
-          for (int r = 0; r < 2; ++r) {
+          for (int cg = 0; cg < NCG; ++cg)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) acc64[2 * rp + r][v] += (double)acc[r][v];
-            acc[r] = f4{0.f, 0.f, 0.f, 0.f};
-          }
+            for (int r = 0; r < 2; ++r) {
+#pragma unroll
+              for (int v = 0; v < 4; ++v) acc64[cg][2 * rp + r][v] += (double)acc[cg][r][v];
+              acc[cg][r] = f4{0.f, 0.f, 0.f, 0.f};
+            }
         }
       }
       // constant mode c_s * (sum of the source's equivalent density), FP64
@@ -1114,32 +1148,41 @@ __global__ __launch_bounds__(64 * kM2LWWaves) __attribute__((amdgpu_waves_per_eu
         const int rb = 2 * rp + r;
         const dv2 c01 = *reinterpret_cast<const dv2*>(cm + 16 * rb);
         const dv2 c23 = *reinterpret_cast<const dv2*>(cm + 16 * rb + 2);
-        acc64[rb][0] += __builtin_fma(c01.x, qv, (double)acc[r][0]);
-        acc64[rb][1] += __builtin_fma(c01.y, qv, (double)acc[r][1]);
-        acc64[rb][2] += __builtin_fma(c23.x, qv, (double)acc[r][2]);
-        acc64[rb][3] += __builtin_fma(c23.y, qv, (double)acc[r][3]);
+#pragma unroll
+        for (int cg = 0; cg < NCG; ++cg) {
+          acc64[cg][rb][0] += __builtin_fma(c01.x, qv[cg], (double)acc[cg][r][0]);
+          acc64[cg][rb][1] += __builtin_fma(c01.y, qv[cg], (double)acc[cg][r][1]);
+          acc64[cg][rb][2] += __builtin_fma(c23.x, qv[cg], (double)acc[cg][r][2]);
+          acc64[cg][rb][3] += __builtin_fma(c23.y, qv[cg], (double)acc[cg][r][3]);
+        }
       }
     }
   }
   // waves 1 .. NW-1 hand their sums to wave 0, which adds them in wave order
   if (w > 0) {
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
+    for (int cg = 0; cg < NCG; ++cg)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) red[w - 1][16 * rb + 4 * kq + v][col] = acc64[rb][v];
+      for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[w - 1][16 * rb + 4 * kq + v][16 * cg + col] = acc64[cg][rb][v];
   }
   __syncthreads();
-  if (w == 0 && active) {
-    double* rowp = ple + (int64_t)g * PD;
+  if (w == 0) {
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
+    for (int cg = 0; cg < NCG; ++cg) {
+      if (!active[cg]) continue;
+      double* rowp = ple + (int64_t)g[cg] * PD;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        double sum = acc64[rb][v];
+      for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-        for (int ww = 0; ww < NW - 1; ++ww) sum += red[ww][16 * rb + 4 * kq + v][col];
-        rowp[16 * rb + 4 * kq + v] = sum;
-      }
+        for (int v = 0; v < 4; ++v) {
+          double sum = acc64[cg][rb][v];
+#pragma unroll
+          for (int ww = 0; ww < NW - 1; ++ww) sum += red[ww][16 * rb + 4 * kq + v][16 * cg + col];
+          rowp[16 * rb + 4 * kq + v] = sum;
+        }
+    }
   }
 }
 
@@ -2843,19 +2886,22 @@ static void m2l_launch(Plan& P, const M2LTables& T, int l0, int l1, hipStream_t 
   int nb = lt.boff[lt.nlev];
   if (nb == 0) return;
   if (m2l_in_fp32(P)) {
-    // narrow tiles of kM2LWCols boxes, one block each
+    // narrow tiles of 16 kM2LF32Groups boxes, one block each
+    constexpr int NG = kM2LF32Groups;
     LevelTable lw = lt;
     int nbw = 0;
     for (int li = 0; li < lw.nlev; ++li) {
       lw.boff[li] = nbw;
-      nbw += 4 * (int)ceil_div(lw.ngroups[li], kM2LWCols);
+      nbw += 4 * (int)ceil_div(lw.ngroups[li], 16 * NG);
     }
     lw.boff[lw.nlev] = nbw;
     lw.split = 1;
     const bool frame = P.kernel_id == KER_STOKES;
-    auto kern = PD == 32 ? (frame ? k_m2l_waves_f32<32, true> : k_m2l_waves_f32<32, false>)
-                         : (frame ? k_m2l_waves_f32<64, true> : k_m2l_waves_f32<64, false>);
-    hipLaunchKernelGGL(kern, dim3(nbw, nd), dim3(64 * kM2LWWaves), 0, s, lw, t.flags.as<int>(),
+    constexpr int NW = kM2LF32Waves;
+    auto kern = PD == 32
+                    ? (frame ? k_m2l_waves_f32<32, true, NG, NW> : k_m2l_waves_f32<32, false, NG, NW>)
+                    : (frame ? k_m2l_waves_f32<64, true, NG, NW> : k_m2l_waves_f32<64, false, NG, NW>);
+    hipLaunchKernelGGL(kern, dim3(nbw, nd), dim3(64 * NW), 0, s, lw, t.flags.as<int>(),
                        t.m2l.as<int>(), P.mult.as<double>(), P.qs.as<double>(),
                        P.ops.as<double>(), opsz, P.ple.as<double>(), rstride, nstride,
                        t.xmid.as<double>(), t.ymid.as<double>(), t.xmin + 0.5 * t.w0,
