@@ -2863,7 +2863,12 @@ static void m2l_prepare(Plan& P, const M2LTables& T, hipStream_t s) {
 // (black-box) plans: there the FP32 accumulation error of a slot's product
 // alone reaches ~1e-7 of the potential (no dominant mode to split off, unlike
 // the KIFMM's row means), so they keep the FP64 M2L.
-static bool m2l_in_fp32(const Plan& P) { return P.precision != PREC_FP64 && !P.cheb; }
+// Black-box plans and Stokes plans with stresslets keep the FP64 M2L under the
+// mixed forms: neither expansion has a dominant constant mode for the row-mean
+// split to take out, and an FP32 product alone costs them 1e-7 to 4e-7.
+static bool m2l_in_fp32(const Plan& P) {
+  return P.precision != PREC_FP64 && !P.cheb && !(P.kernel_id == KER_STOKES && P.have_dip);
+}
 
 static void m2l_launch(Plan& P, const M2LTables& T, int l0, int l1, hipStream_t s) {
   const Tree& t = P.tree;
@@ -3058,7 +3063,7 @@ static void downward(Plan& P, hipStream_t s) {
   m2l_launch(P, T, 0, T.lt.nlev, s);
   P.mark(FMM2D_PHASE_M2L, true, s);
   FrameFix fx{nullptr, nullptr, 0.0, 0.0};
-  if (P.kernel_id == KER_STOKES && P.precision != PREC_FP64)
+  if (P.kernel_id == KER_STOKES && m2l_in_fp32(P))   // the FP32 M2L ran in box frames
     fx = FrameFix{t.xmid.as<double>(), t.ymid.as<double>(), t.xmin + 0.5 * t.w0,
                   t.ymin + 0.5 * t.w0};
   for (int d = 2; d < t.L; ++d) {

@@ -52,6 +52,7 @@ struct PrimWork {
   HostBuf hcounters;    // pinned mirror of the counters
   // MSD sort
   DevBuf keysB, valsB;
+  DevBuf keysC, valsC;  // second spare pair of a prepared first round
   DevBuf seg[2];        // (start, len) int2
   DevBuf fin;           // final segments (start, len)
   DevBuf chunks;        // int4 (seg, start, len, -)
@@ -83,16 +84,24 @@ void fill_i32(int* p, int64_t n, int v, hipStream_t s);
 // D holds at most Ncutoff points, so no cell of the tree below depth D exists
 // and none needs its points grouped; segments that ended earlier are sorted.
 constexpr int kLocalSortMax = 2048;
-// first_round_ready: the caller's kernel wrote the first round's records
-// (sort_first_round below), so the sort starts at that round's segment
-// offsets; it requires n > threshold.
+// first_round_ready: the caller's kernel wrote the first round's records and
+// its keys and values into the sort's spare buffers (sort_first_round
+// below), so the sort starts at that round's segment offsets; it requires
+// n > threshold.  Such a sort never reads keys / vals: every round scatters
+// the elements of finished segments straight to their places there and the
+// others to the other spare buffer, so no round copies anything back.  The
+// second round is launched on the first round's counts as the device holds
+// them, without waiting for the host to read them.
 bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, PrimWork& w,
                     hipStream_t s, int key_bits = 64, bool prefix_only = false,
                     bool first_round_ready = false);
 
 // Layout of the sort's device counters and chunks.
 namespace sort_detail {
-enum { C_NEXT = 0, C_FINAL = 1, C_CHUNKS = 2, C_ACTIVE = 3, C_FINAL_BIG = 4, C_COUNT = 5 };
+// C_SEGS: segments of the current round; C_*_PREV: C_ACTIVE, C_FINAL and
+// C_FINAL_BIG as they stood when the round began
+enum { C_NEXT = 0, C_FINAL = 1, C_CHUNKS = 2, C_ACTIVE = 3, C_FINAL_BIG = 4, C_SEGS = 5,
+       C_ACTIVE_PREV = 6, C_FINAL_PREV = 7, C_BIG_PREV = 8, C_COUNT = 9 };
 }
 constexpr int kSortChunk = 4096;   // keys per chunk of a radix round
 
@@ -102,6 +111,8 @@ constexpr int kSortChunk = 4096;   // keys per chunk of a radix round
 // its keys' top 8 bits (bits 56..63) to sort_first_round_block.  This saves
 // the sort three launches and a pass over the keys.
 struct SortFirstRound {
+  uint64_t* keys = nullptr;   // where the key kernel writes the keys and values
+  uint32_t* vals = nullptr;
   int4* chunks = nullptr;
   int* chunk_hist = nullptr;
   int* seg_chunk0 = nullptr;
@@ -124,6 +135,7 @@ __device__ inline void sort_first_round_block(const SortFirstRound& f, int b, co
     if (b == 0) {
       for (int i = 0; i < sort_detail::C_COUNT; ++i) f.counters[i] = 0;
       f.counters[sort_detail::C_CHUNKS] = (int)ceil_div(f.n, kSortChunk);
+      f.counters[sort_detail::C_SEGS] = 1;
       f.seg0[0] = make_int2(0, f.n);
       f.seg_chunk0[0] = 0;
     }

@@ -113,14 +113,21 @@ __global__ __launch_bounds__(1024) void k_scan_small(const int* in, int* out, in
 
 }  // namespace
 
+// 16-byte stores for the aligned body (one int per thread ran at a third of
+// the bandwidth), single ints for the up to 3 before it and after it.
 __global__ __launch_bounds__(256) void k_fill_i32(int* __restrict__ p, int64_t n, int v) {
+  const int64_t head = min((int64_t)(((16 - ((uintptr_t)p & 15)) & 15) >> 2), n);
+  const int64_t nb = (n - head) >> 2;
+  const int64_t t0 = head + 4 * nb;
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) p[i] = v;
+  if (i < nb) reinterpret_cast<int4*>(p + head)[i] = make_int4(v, v, v, v);
+  if (i < head) p[i] = v;
+  if (i < n - t0) p[t0 + i] = v;
 }
 
 void fill_i32(int* p, int64_t n, int v, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_fill_i32, dim3(ceil_div(n, 256)), dim3(256), 0, s, p, n, v);
+  hipLaunchKernelGGL(k_fill_i32, dim3(ceil_div(n / 4 + 4, 256)), dim3(256), 0, s, p, n, v);
   FMM_LAUNCH_CHECK();
 }
 
@@ -161,18 +168,29 @@ constexpr int SORT_BS = 256;
 constexpr int kChunkPer = CHUNK / SORT_BS;   // keys per thread of a chunk block
 constexpr int kWaveSortMax = 64;
 
-__global__ __launch_bounds__(1024) void k_make_chunks(const int2* __restrict__ seg, int nseg,
+// nseg_host < 0: the round's segments are the C_NEXT of the round before, as
+// the device holds it (a round launched before the host read the counts).
+__global__ __launch_bounds__(1024) void k_make_chunks(const int2* __restrict__ seg, int nseg_host,
                                                      int4* __restrict__ chunks,
                                                      int* __restrict__ segChunk0,
                                                      int* __restrict__ counters) {
   __shared__ int lds[1024 / 64];
   __shared__ int tot;
+  __shared__ int s_nseg;
   // counters of this round's segment classification (k_seg_offsets adds to
-  // them, after this kernel)
+  // them, after this kernel), and the ones it starts from
   if (threadIdx.x == 0) {
+    const int ns = nseg_host >= 0 ? nseg_host : counters[C_NEXT];
+    s_nseg = ns;
+    counters[C_SEGS] = ns;
+    counters[C_ACTIVE_PREV] = counters[C_ACTIVE];
+    counters[C_FINAL_PREV] = counters[C_FINAL];
+    counters[C_BIG_PREV] = counters[C_FINAL_BIG];
     counters[C_NEXT] = 0;
     counters[C_ACTIVE] = 0;
   }
+  __syncthreads();
+  const int nseg = s_nseg;
   int carry = 0;
   for (int base = 0; base < nseg; base += 1024) {
     int si = base + threadIdx.x;
@@ -239,6 +257,7 @@ __global__ __launch_bounds__(SEG_BS) void k_seg_offsets(
   __shared__ int part_sum[4][256];
   __shared__ int bases[3];
   const int si = blockIdx.x;
+  if (si >= counters[C_SEGS]) return;   // block-uniform: a grid sized by a bound
   const int d = threadIdx.x & 255, part = threadIdx.x >> 8;
   const int2 sg = seg[si];
   const int c0 = segChunk0[si];
@@ -279,10 +298,12 @@ __global__ __launch_bounds__(SEG_BS) void k_seg_offsets(
   int pre = 0;
   for (int pp = 0; pp < part; ++pp) pre += part_sum[pp][d];
   int pos = sg.x + dstart_s[d] + pre;
+  // bit 31 of a destination: the digit's segment continues to the next round
+  const int cont = (run > threshold && !last_round) ? (int)0x80000000u : 0;
   if (kept) {
 #pragma unroll
     for (int u = 0; u < R; ++u) {
-      if (cb + u < ce) chunkHist[(int64_t)(cb + u) * 256 + d] = pos;
+      if (cb + u < ce) chunkHist[(int64_t)(cb + u) * 256 + d] = pos | cont;
       pos += hk[u];
     }
   }
@@ -293,13 +314,13 @@ __global__ __launch_bounds__(SEG_BS) void k_seg_offsets(
     for (int u = 0; u < B; ++u) h[u] = chunkHist[(int64_t)(c + u) * 256 + d];
 #pragma unroll
     for (int u = 0; u < B; ++u) {
-      chunkHist[(int64_t)(c + u) * 256 + d] = pos;
+      chunkHist[(int64_t)(c + u) * 256 + d] = pos | cont;
       pos += h[u];
     }
   }
   for (; c < ce; ++c) {
     const int h = chunkHist[(int64_t)c * 256 + d];
-    chunkHist[(int64_t)c * 256 + d] = pos;
+    chunkHist[(int64_t)c * 256 + d] = pos | cont;
     pos += h;
   }
   // classify the digit segments and append them with one atomic per list
@@ -336,13 +357,17 @@ __global__ __launch_bounds__(SEG_BS) void k_seg_offsets(
 __global__ __launch_bounds__(SORT_BS) void k_chunk_scatter(
     const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
     const int4* __restrict__ chunks, const int* __restrict__ counters, int shift,
-    const int* __restrict__ chunkHist, uint64_t* __restrict__ keysB, uint32_t* __restrict__ valsB) {
+    const int* __restrict__ chunkHist, uint64_t* __restrict__ keysB, uint32_t* __restrict__ valsB,
+    uint64_t* __restrict__ kfin, uint32_t* __restrict__ vfin, int* __restrict__ host_counters) {
   __shared__ int gdst[256];    // destination of the chunk's first element of digit d
   __shared__ int lstart[256];  // count, then start, of digit d inside the chunk
   __shared__ int scan_lds[SORT_BS / 64];
   __shared__ uint64_t sk[CHUNK];
   __shared__ uint32_t sv[CHUNK];
   const int b = blockIdx.x;
+  // the round's counters are final (k_seg_offsets ran before): block 0 hands
+  // them to the pinned host array, which saves the host a copy before it reads
+  if (b == 0 && threadIdx.x < C_COUNT) host_counters[threadIdx.x] = counters[threadIdx.x];
   if (b >= counters[C_CHUNKS]) return;
   const int4 ch = chunks[b];
   gdst[threadIdx.x] = chunkHist[(int64_t)b * 256 + threadIdx.x];
@@ -381,9 +406,15 @@ __global__ __launch_bounds__(SORT_BS) void k_chunk_scatter(
     if (j < ch.z) {
       const uint64_t kj = sk[j];
       const int d = (int)((kj >> shift) & 255u);
-      const int pos = gdst[d] + (j - lstart[d]);
-      keysB[pos] = kj;
-      valsB[pos] = sv[j];
+      const int gd = gdst[d];
+      const int pos = (gd & 0x7fffffff) + (j - lstart[d]);
+      if (kfin && gd >= 0) {   // the digit's segment finished: its final place
+        kfin[pos] = kj;
+        vfin[pos] = sv[j];
+      } else {
+        keysB[pos] = kj;
+        valsB[pos] = sv[j];
+      }
     }
   }
 }
@@ -511,21 +542,30 @@ __global__ __launch_bounds__(SORT_BS) void k_local_sort(uint64_t* __restrict__ k
 
 // Work buffers of the first round for n keys (sort_pairs_u64 sizes them the
 // same way, so it finds them in place).
+constexpr int kSecondRoundSegs = 256;   // segments a one-segment round can leave
+
 static void first_round_buffers(PrimWork& w, int64_t n) {
   const int64_t segcap = n / 2 + 2;
-  const int64_t cbound = 1 + n / CHUNK + 1;
+  const int64_t cbound = kSecondRoundSegs + n / CHUNK + 1;   // either of the first two rounds
+  w.keysB.ensure(n * sizeof(uint64_t));
+  w.valsB.ensure(n * sizeof(uint32_t));
   w.seg[0].ensure(segcap * sizeof(int2));
+  w.seg[1].ensure(segcap * sizeof(int2));
   w.counters.ensure(C_COUNT * sizeof(int));
   w.chunks.ensure(cbound * sizeof(int4));
   w.chunkHist.ensure(cbound * 256 * sizeof(int));
-  w.segChunk0.ensure(sizeof(int));
+  w.segChunk0.ensure(kSecondRoundSegs * sizeof(int));
 }
 
 SortFirstRound sort_first_round(PrimWork& w, int64_t n) {
   FMM_REQUIRE(n >= 2 && n < (int64_t)1 << 31, FMM_E_LIMIT, "sort: n = %lld outside [2, 2^31-1]",
               (long long)n);
   first_round_buffers(w, n);
+  w.keysC.ensure(n * sizeof(uint64_t));
+  w.valsC.ensure(n * sizeof(uint32_t));
   SortFirstRound f;
+  f.keys = w.keysB.as<uint64_t>();
+  f.vals = w.valsB.as<uint32_t>();
   f.chunks = w.chunks.as<int4>();
   f.chunk_hist = w.chunkHist.as<int>();
   f.seg_chunk0 = w.segChunk0.as<int>();
@@ -554,6 +594,8 @@ bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
   w.hcounters.ensure(C_COUNT * sizeof(int));
   int* cnt = w.counters.as<int>();
   int* hc = w.hcounters.as<int>();
+  int* hc_dev = nullptr;   // the same pinned array as the device addresses it
+  FMM_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&hc_dev), hc, 0));
   const int finBigOff = (int)segcap;
   if (!first_round_ready) {
     hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(64), 0, s, w.seg[0].as<int2>(),
@@ -583,52 +625,75 @@ bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
   // The round's input (kin, vin) and scatter target (kout, vout): a round whose
   // segments all continue leaves its output in place for the next round to
   // read (ping-pong); otherwise its chunks are copied back into keys / vals,
-  // where every finished segment must end.
-  uint64_t* kin = keys;
-  uint32_t* vin = vals;
-  uint64_t* kout = w.keysB.as<uint64_t>();
-  uint32_t* vout = w.valsB.as<uint32_t>();
+  // where every finished segment must end.  A prepared sort alternates
+  // between the two spare pairs and sends finished elements to keys / vals
+  // (kfin) as it scatters.
+  const bool ready = first_round_ready;
+  uint64_t* kin = ready ? w.keysB.as<uint64_t>() : keys;
+  uint32_t* vin = ready ? w.valsB.as<uint32_t>() : vals;
+  uint64_t* kout = ready ? w.keysC.as<uint64_t>() : w.keysB.as<uint64_t>();
+  uint32_t* vout = ready ? w.valsC.as<uint32_t>() : w.valsB.as<uint32_t>();
+  uint64_t* kfin = ready ? keys : nullptr;
+  uint32_t* vfin = ready ? vals : nullptr;
+  // After a prepared first round (one segment, so at most kSecondRoundSegs
+  // segments and n keys left) the second round is launched at once, on grids
+  // sized by those bounds; its kernels take the counts from the device.
+  const bool early_second = first_round_ready && rounds > 1;
   for (int r = 0; r < rounds && nseg > 0; ++r) {
+    const bool on_device_counts = early_second && r == 1;
     nfin_early = nfinal;
     nbig_early = nbig;
     ran = true;
-    const int64_t active_in = active;
+    int64_t active_in = active;
     const int shift = 56 - 8 * r;
-    const int64_t cbound = nseg + active / CHUNK + 1;
+    const int64_t cbound =
+        on_device_counts ? kSecondRoundSegs + n / CHUNK + 1 : nseg + active / CHUNK + 1;
+    const int nseg_grid = on_device_counts ? kSecondRoundSegs : nseg;
     w.chunks.ensure(cbound * sizeof(int4));
     w.chunkHist.ensure(cbound * 256 * sizeof(int));
-    w.segChunk0.ensure((int64_t)nseg * sizeof(int));
+    w.segChunk0.ensure((int64_t)nseg_grid * sizeof(int));
     int2* segCur = w.seg[cur].as<int2>();
     int2* segNext = w.seg[cur ^ 1].as<int2>();
     if (!(first_round_ready && r == 0)) {
-      hipLaunchKernelGGL(k_make_chunks, dim3(1), dim3(1024), 0, s, segCur, nseg,
-                         w.chunks.as<int4>(), w.segChunk0.as<int>(), cnt);
+      hipLaunchKernelGGL(k_make_chunks, dim3(1), dim3(1024), 0, s, segCur,
+                         on_device_counts ? -1 : nseg, w.chunks.as<int4>(),
+                         w.segChunk0.as<int>(), cnt);
       FMM_LAUNCH_CHECK();
       hipLaunchKernelGGL(k_chunk_hist, dim3(cbound), dim3(SORT_BS), 0, s, kin,
                          w.chunks.as<int4>(), cnt, shift, w.chunkHist.as<int>());
       FMM_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_seg_offsets, dim3(nseg), dim3(SEG_BS), 0, s, segCur,
+    hipLaunchKernelGGL(k_seg_offsets, dim3(nseg_grid), dim3(SEG_BS), 0, s, segCur,
                        w.segChunk0.as<int>(), w.chunkHist.as<int>(), T, r == rounds - 1 ? 1 : 0, segNext,
                        w.fin.as<int2>(), finBigOff, cnt);
     FMM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_chunk_scatter, dim3(cbound), dim3(SORT_BS), 0, s, kin, vin,
-                       w.chunks.as<int4>(), cnt, shift, w.chunkHist.as<int>(), kout, vout);
+                       w.chunks.as<int4>(), cnt, shift, w.chunkHist.as<int>(), kout, vout,
+                       kfin, vfin, hc_dev);
     FMM_LAUNCH_CHECK();
-    FMM_HIP(hipMemcpyAsync(hc, cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, s));
+    cur ^= 1;
+    if (early_second && r == 0) {
+      std::swap(kin, kout);
+      std::swap(vin, vout);
+      continue;
+    }
     FMM_HIP(hipStreamSynchronize(s));
     nseg = hc[C_NEXT];
     active = hc[C_ACTIVE];
     nfinal = hc[C_FINAL];
     nbig = hc[C_FINAL_BIG];
-    cur ^= 1;
+    if (on_device_counts) {   // the first round's results, as this round began
+      active_in = hc[C_ACTIVE_PREV];
+      nfin_early = hc[C_FINAL_PREV];
+      nbig_early = hc[C_BIG_PREV];
+    }
     // Elements that left the active set finished in this round, in segments of
     // any length (one-element segments are on no list, but their place in
     // kout is final too): they must reach keys / vals before a later round
     // writes its output there instead.
     const bool finished_some = active < active_in;
     const bool more = nseg > 0 && r + 1 < rounds;
-    if (kout != keys && (finished_some || !more)) {
+    if (!ready && kout != keys && (finished_some || !more)) {
       hipLaunchKernelGGL(k_chunk_copy, dim3(cbound), dim3(SORT_BS), 0, s, kout, vout,
                          w.chunks.as<int4>(), cnt, keys, vals);
       FMM_LAUNCH_CHECK();

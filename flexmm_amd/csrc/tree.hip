@@ -70,7 +70,7 @@ __device__ __forceinline__ int find_level(const T* __restrict__ off, int L, int6
 // reference's recurrence (point_key.h) and from the recurrence itself for the
 // few points within rounding reach of a midpoint.  With fr (more points than
 // the sort's threshold) the block also counts its keys' top 8 bits for the
-// sort's first round.
+// sort's first round, and keys / vals are the sort's spare buffers.
 constexpr int KEY_BS = 1024;
 constexpr int kKeyPer = kSortChunk / KEY_BS;
 static_assert(kKeyLevels == kMaxDepth, "keys cover every depth");
@@ -135,9 +135,11 @@ __global__ __launch_bounds__(BS) void k_gather_sorted(const double2* __restrict_
 // each depth d with s < d <= c) and the split positions: each wave writes its
 // ballot counts to its own LDS row, the block adds the rows and adds them to
 // row b mod kSplitRows of `rows` ([kSplitRows][kMaxLevels + 1], zeroed; the
-// host adds the rows).  A thousand blocks adding to one set of counters
-// serialised on those addresses (10 us of a 15 us kernel at 1M points).  A
-// complete tree needs nothing more from the split positions.
+// host adds the rows): a thousand blocks adding to one set of counters
+// serialised on those addresses (10 us of a 15 us kernel at 1M points).
+// (Having the last block add the rows costs every block an agent-scope fence,
+// which writes back its L2: 8.4 -> 32 us.)  A complete tree needs nothing
+// more from the split positions.
 constexpr int kSplitRows = 64;
 __global__ __launch_bounds__(1024) void k_split_pos(const uint64_t* __restrict__ keys, int n,
                                                     int ncut, int* __restrict__ flag,
@@ -826,9 +828,11 @@ __global__ __launch_bounds__(BS) void k_full_nodes(
     int* __restrict__ parent, int* __restrict__ child, int* __restrict__ coll,
     int2* __restrict__ crange, double* __restrict__ xmid, double* __restrict__ ymid,
     int* __restrict__ row, int* __restrict__ m2l, int* __restrict__ evalid,
-    int* __restrict__ evalnode, int* __restrict__ upnodes, int2* __restrict__ rowinfo) {
+    int* __restrict__ evalnode, int* __restrict__ upnodes, int2* __restrict__ rowinfo,
+    int* __restrict__ levoff) {
   const int64_t tid = (int64_t)blockIdx.x * BS + threadIdx.x;
   const int g = (int)(tid / kFullLanes), j = (int)(tid % kFullLanes);
+  if (tid <= D + 1) levoff[tid] = full_levoff((int)tid);   // levels 0 .. D and the end
   if (g >= nn) return;
   int d = 0;
   while (d < D && g >= full_levoff(d + 1)) ++d;
@@ -905,8 +909,7 @@ __global__ __launch_bounds__(BS) void k_full_nodes(
 
 // Node arrays of a complete tree of depth D from the sorted points (keys and
 // the common depths sc of neighbouring keys); see k_full_nodes.
-template <class Upload>
-static void tree_build_full(Tree& t, int D, int npts, hipStream_t s, Upload&& upload) {
+static void tree_build_full(Tree& t, int D, int npts, hipStream_t s) {
   const int L = D + 1;
   t.L = L;
   t.lev.assign(L, TreeLevel());
@@ -926,8 +929,7 @@ static void tree_build_full(Tree& t, int D, int npts, hipStream_t s, Upload&& up
   t.n_eval = (int64_t)1 << (2 * D);
   t.n_up = t.n_eval;
   t.nidx_off.assign(kMaxLevels, -1);
-  t.levoff.ensure((L + 1) * sizeof(int));
-  upload(t.levoff.ptr, levoff.data(), (L + 1) * sizeof(int));
+  t.levoff.ensure((L + 1) * sizeof(int));   // written by k_full_nodes
   const int64_t ncell = (int64_t)1 << (2 * D);
   t.w_flag.ensure((ncell + 1) * sizeof(int));
   int* start = t.w_flag.as<int>();
@@ -961,7 +963,7 @@ static void tree_build_full(Tree& t, int D, int npts, hipStream_t s, Upload&& up
                      t.parent.as<int>(), t.child.as<int>(), t.coll.as<int>(), t.crange.as<int2>(),
                      t.xmid.as<double>(), t.ymid.as<double>(), t.row.as<int>(), t.m2l.as<int>(),
                      t.evalid.as<int>(), t.evalnode.as<int>(), t.upnodes.as<int>(),
-                     t.rowinfo.as<int2>());
+                     t.rowinfo.as<int2>(), t.levoff.as<int>());
   FMM_LAUNCH_CHECK();
 }
 
@@ -1001,7 +1003,8 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   const bool first_round = n > sort_T;
   const SortFirstRound fr = first_round ? sort_first_round(t.pw, n) : SortFirstRound{};
   hipLaunchKernelGGL(k_point_keys, dim3((unsigned)ceil_div(n, kSortChunk)), dim3(KEY_BS), 0, s, x, y,
-                     nn, xmin, ymin, w0, key_scale(xmin, ymin, w0), keys, perm,
+                     nn, xmin, ymin, w0, key_scale(xmin, ymin, w0),
+                     first_round ? fr.keys : keys, first_round ? fr.vals : perm,
                      t.xy.as<double2>(), cnt + CNT_BAD, fr, first_round ? 1 : 0);
   FMM_LAUNCH_CHECK();
   // Segments of at most Ncutoff points that end in the last radix round sit
@@ -1061,7 +1064,7 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
     bool complete = true;
     for (int d = 0; d <= L - 2 && complete; ++d) complete = dcount[d] == ((int64_t)1 << (2 * d));
     if (complete) {
-      tree_build_full(t, L - 1, nn, s, upload);
+      tree_build_full(t, L - 1, nn, s);
       ++t.build_id;
       t.built = true;
       return;

@@ -86,10 +86,11 @@ extern "C" {
                               separations >= w/64; FP64 elsewhere.  An evaluation
                               with fewer than 32 targets per leaf on average takes
                               the FP64 near field (faster there, and more accurate).
-                              Black-box plans keep the FP64 M2L: an FP32
-                              accumulation alone costs them ~1e-7 */
-#define FMM2D_PREC_MIXED_M2L 2 /* FP32 MFMA M2L only (black-box plans: FP64); near field
-                                  in FP64 */
+                              Black-box plans and Stokes plans with stresslets
+                              (fmm2d_upward_dipole) keep the FP64 M2L: an FP32
+                              accumulation alone costs them 1e-7 to 4e-7 */
+#define FMM2D_PREC_MIXED_M2L 2 /* FP32 MFMA M2L only (black-box and stresslet plans:
+                                  FP64); near field in FP64 */
 
 /* FP64 M2L form (fmm2d_set_m2l_mode) */
 #define FMM2D_M2L_SVD 0   /* the reference's M2L_Evaluator (helpers.py:17-45): a slot of
