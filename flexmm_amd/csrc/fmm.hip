@@ -1570,6 +1570,7 @@ constexpr int kSmallLevelRows = 4096;
 // else a binary search in the level's sorted codes.
 struct LocateTable {
   int L;
+  int full_depth;   // a complete tree's leaf depth (evaluation box = leaf code), else -1
   double xmin, ymin, w0;
   int levoff[kMaxLevels + 1];
   int64_t tab_off[kMaxLevels];
@@ -1597,7 +1598,9 @@ __device__ __forceinline__ int locate_node(const LocateTable& lt, int d, uint64_
 // the stored node centres), so the node lookups of four depths are issued
 // together instead of one dependent descent step per level.  Lanes of a wave
 // that land in the same box share one atomic (targets arrive spatially
-// coherent, so a wave touches few boxes).
+// coherent, so a wave touches few boxes).  In a complete tree the first leaf
+// on the path is at the leaf depth D and its evaluation box is its code, so
+// the recurrence alone gives the box.
 __global__ __launch_bounds__(256) void k_locate_count(const double* __restrict__ tx,
                                                      const double* __restrict__ ty, int64_t nt,
                                                      LocateTable lt,
@@ -1611,7 +1614,21 @@ __global__ __launch_bounds__(256) void k_locate_count(const double* __restrict__
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const bool act = i < nt;
   int e = -1;
-  if (act) {
+  if (act && lt.full_depth > 0) {
+    const double x = tx[i], y = ty[i];
+    double xmin = lt.xmin, ymin = lt.ymin, w = lt.w0;
+    uint64_t c = 0;
+    for (int d = 1; d <= lt.full_depth; ++d) {
+      const double h = 0.5 * w;
+      const double xm = xmin + h, ym = ymin + h;
+      const int bx = x > xm, by = y > ym;
+      c = (c << 2) | (uint64_t)(2 * bx + by);
+      xmin = bx ? xm : xm - h;
+      ymin = by ? ym : ym - h;
+      w = h;
+    }
+    e = (int)c;
+  } else if (act) {
     const double x = tx[i], y = ty[i];
     int g = 0;
     bool found = (flags[0] & NF_LEAF) != 0;
@@ -3183,6 +3200,7 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
   fill_i32(w.tcnt, ne + 1, 0, s);
   LocateTable lt{};
   lt.L = t.L;
+  lt.full_depth = t.complete ? t.L - 1 : -1;
   lt.xmin = t.xmin;
   lt.ymin = t.ymin;
   lt.w0 = t.w0;

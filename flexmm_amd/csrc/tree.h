@@ -29,6 +29,8 @@ struct TreeLevel {
 struct Tree {
   bool built = false;
   bool allow_full = true; // complete trees take the closed-form build (tree.hip)
+  bool complete = false;  // the last build was such a tree: every leaf at depth L - 1, and
+                          // leaf (depth L - 1, code c) is evaluation box c
   int64_t build_id = 0;   // incremented by every build
   int64_t n = 0;           // number of sources
   int ncut = 0;

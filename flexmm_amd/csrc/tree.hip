@@ -975,6 +975,7 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   FMM_REQUIRE(ncut >= 1, FMM_E_ARG, "tree: Ncutoff must be >= 1 (got %d)", ncut);
   FMM_REQUIRE(w0 > 0.0 && std::isfinite(w0), FMM_E_ARG, "tree: root width must be positive");
   t.built = false;
+  t.complete = false;
   t.n = n;
   t.ncut = ncut;
   t.xmin = xmin;
@@ -1065,6 +1066,7 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
     for (int d = 0; d <= L - 2 && complete; ++d) complete = dcount[d] == ((int64_t)1 << (2 * d));
     if (complete) {
       tree_build_full(t, L - 1, nn, s);
+      t.complete = true;
       ++t.build_id;
       t.built = true;
       return;
