@@ -1726,18 +1726,80 @@ __global__ __launch_bounds__(256) void k_item_counts(const int* __restrict__ tcn
   if (b < ne) icnt[b] = (tcnt[b] + kTgtChunk - 1) / kTgtChunk;
 }
 
+// Box offsets of a tree of at most kBoxOffsetsMax evaluation boxes, in one
+// block: toff = exclusive scan of tcnt[0 .. ne] (tcnt[ne] = 0, so toff[ne] =
+// nt) and ioff = exclusive scan of the boxes' item counts ceil(tcnt /
+// kTgtChunk), with their total in *ntot: one pass over the counts for both
+// (two scans and a count kernel otherwise).  Element j 1024 + t is thread t's
+// (coalesced); tiles are scanned by wave scans and combined through LDS.
+constexpr int kBoxTiles = 32;
+constexpr int kBoxOffsetsMax = kBoxTiles * 1024 - 1;
+__global__ __launch_bounds__(1024) void k_box_offsets(const int* __restrict__ tcnt, int ne,
+                                                      int* __restrict__ toff,
+                                                      int* __restrict__ ioff,
+                                                      int* __restrict__ ntot) {
+  __shared__ int wsum[2][kBoxTiles][16];
+  __shared__ int woff[2][kBoxTiles][16];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int n = ne + 1;
+  const int J = (n + 1023) >> 10;
+  int exv[kBoxTiles], exu[kBoxTiles];   // in-wave exclusive prefixes (loaded counts first)
+#pragma unroll
+  for (int j = 0; j < kBoxTiles; ++j) {
+    const int idx = j * 1024 + t;
+    exv[j] = (j < J && idx < ne) ? tcnt[idx] : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < kBoxTiles; ++j) {
+    if (j < J) {
+      const int v = exv[j], u = (v + kTgtChunk - 1) / kTgtChunk;
+      const int iv = wave_inclusive_scan(v), iu = wave_inclusive_scan(u);
+      if (lane == 63) {
+        wsum[0][j][wid] = iv;
+        wsum[1][j][wid] = iu;
+      }
+      exv[j] = iv - v;
+      exu[j] = iu - u;
+    }
+  }
+  __syncthreads();
+  if (wid < 2) {   // wave a: offsets of array a (tile prefix, then the 16 waves of each tile)
+    const int a = wid;
+    int tile_tot = 0;
+    if (lane < J)
+      for (int w = 0; w < 16; ++w) tile_tot += wsum[a][lane][w];
+    const int tile_off = wave_inclusive_scan(lane < J ? tile_tot : 0) - (lane < J ? tile_tot : 0);
+    if (lane < J) {
+      int run = tile_off;
+      for (int w = 0; w < 16; ++w) {
+        woff[a][lane][w] = run;
+        run += wsum[a][lane][w];
+      }
+      if (a == 1 && lane == J - 1) *ntot = run;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kBoxTiles; ++j) {
+    const int idx = j * 1024 + t;
+    if (j < J && idx < n) {
+      toff[idx] = woff[0][j][wid] + exv[j];
+      ioff[idx] = woff[1][j][wid] + exu[j];
+    }
+  }
+}
+
 // Item = (evaluation node, first target, end target) of one chunk of at most
 // kTgtChunk targets, in sorted-target positions.
-__global__ __launch_bounds__(256) void k_make_items(const int* __restrict__ icnt,
-                                                   const int* __restrict__ ioff, int ne,
+__global__ __launch_bounds__(256) void k_make_items(const int* __restrict__ ioff, int ne,
                                                    const int* __restrict__ evalnode,
                                                    const int* __restrict__ toff,
                                                    int4* __restrict__ items) {
   int b = blockIdx.x * 256 + threadIdx.x;
   if (b >= ne) return;
-  const int n = icnt[b], o = ioff[b];
+  const int o = ioff[b];
   const int g = evalnode[b], t0 = toff[b], t1 = toff[b + 1];
-  for (int k = 0; k < n; ++k)
+  for (int k = 0; t0 + k * kTgtChunk < t1; ++k)
     items[o + k] = make_int4(g, t0 + k * kTgtChunk, min(t1, t0 + (k + 1) * kTgtChunk), 0);
 }
 
@@ -3215,19 +3277,24 @@ static void evaluate(Plan& P, const double* tx, const double* ty, int64_t nt, in
                      t.ymid.as<double>(), t.evalid.as<int>(), w.tnode,
                      w.tcur, w.tcnt);
   FMM_LAUNCH_CHECK();
-  // toff has ne + 1 entries: scanning ne + 1 counts (the last is zero) gives toff[ne] = nt.
-  exclusive_scan_i32(w.tcnt, w.toff, ne + 1, nullptr, P.pw, s, w.partials);
+  if (ne <= kBoxOffsetsMax) {
+    hipLaunchKernelGGL(k_box_offsets, dim3(1), dim3(1024), 0, s, w.tcnt, (int)ne, w.toff, w.ioff,
+                       w.ntot);
+    FMM_LAUNCH_CHECK();
+  } else {
+    // toff has ne + 1 entries: scanning ne + 1 counts (the last is zero) gives toff[ne] = nt.
+    exclusive_scan_i32(w.tcnt, w.toff, ne + 1, nullptr, P.pw, s, w.partials);
+    hipLaunchKernelGGL(k_item_counts, dim3(grid256(ne)), dim3(256), 0, s, w.tcnt, (int)ne,
+                       w.icnt);
+    FMM_LAUNCH_CHECK();
+    exclusive_scan_i32(w.icnt, w.ioff, ne, w.ntot, P.pw, s, w.partials);
+  }
   hipLaunchKernelGGL(k_target_scatter, dim3(grid256(nt)), dim3(256), 0, s, tx, ty, nt,
                      w.tnode, w.tcur, w.toff, w.tperm,
                      w.tsx, w.tsy);
   FMM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_item_counts, dim3(grid256(ne)), dim3(256), 0, s, w.tcnt, (int)ne,
-                     w.icnt);
-  FMM_LAUNCH_CHECK();
-  exclusive_scan_i32(w.icnt, w.ioff, ne, w.ntot, P.pw, s, w.partials);
-  hipLaunchKernelGGL(k_make_items, dim3(grid256(ne)), dim3(256), 0, s, w.icnt,
-                     w.ioff, (int)ne, t.evalnode.as<int>(), w.toff,
-                     w.items);
+  hipLaunchKernelGGL(k_make_items, dim3(grid256(ne)), dim3(256), 0, s, w.ioff, (int)ne,
+                     t.evalnode.as<int>(), w.toff, w.items);
   FMM_LAUNCH_CHECK();
   const int PD = P.PD;
   const int64_t opsz = ops_per_level(PD);
