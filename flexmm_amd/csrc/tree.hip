@@ -116,17 +116,6 @@ __global__ __launch_bounds__(KEY_BS) void k_point_keys(const double* __restrict_
   }
 }
 
-__global__ __launch_bounds__(BS) void k_gather_sorted(const double2* __restrict__ xy,
-                                                      const uint32_t* __restrict__ perm, int n,
-                                                      double* __restrict__ sx,
-                                                      double* __restrict__ sy) {
-  int i = blockIdx.x * BS + threadIdx.x;
-  if (i >= n) return;
-  const double2 v = xy[perm[i]];
-  sx[i] = v.x;
-  sy[i] = v.y;
-}
-
 // --- refinement: split cells from the sorted keys ----------------------------
 // Position i starts the cells of depths s(i)+1.. (s = common depth with key
 // i-1); such a cell has more than ncut points iff key i+ncut is still inside
@@ -141,9 +130,16 @@ __global__ __launch_bounds__(BS) void k_gather_sorted(const double2* __restrict_
 // which writes back its L2: 8.4 -> 32 us.)  A complete tree needs nothing
 // more from the split positions.
 constexpr int kSplitRows = 64;
+// It also gathers the points into tree order (sx, sy from the interleaved
+// copy xy through perm: one 16-byte gather per point), its loads in flight
+// with the keys'.
 __global__ __launch_bounds__(1024) void k_split_pos(const uint64_t* __restrict__ keys, int n,
                                                     int ncut, int* __restrict__ flag,
-                                                    int2* __restrict__ sc, int* __restrict__ rows) {
+                                                    int2* __restrict__ sc, int* __restrict__ rows,
+                                                    const uint32_t* __restrict__ perm,
+                                                    const double2* __restrict__ xy,
+                                                    double* __restrict__ sx,
+                                                    double* __restrict__ sy) {
   __shared__ int lc[1024 / 64][kMaxLevels + 1];
   const int i = blockIdx.x * 1024 + threadIdx.x;
   int s = 0, c = -1;
@@ -152,7 +148,10 @@ __global__ __launch_bounds__(1024) void k_split_pos(const uint64_t* __restrict__
     // conditions each load waited for the one before
     const int ip = i > 0 ? i - 1 : 0;
     const int ic = (int)min((int64_t)i + ncut, (int64_t)n - 1);
+    const double2 pt = xy[perm[i]];
     const uint64_t k = keys[i], kp = keys[ip], kc = keys[ic];
+    sx[i] = pt.x;
+    sy[i] = pt.y;
     s = (i == 0) ? -1 : min(clz64(k ^ kp) >> 1, kMaxDepth);
     c = ((int64_t)i + ncut < n) ? min(clz64(k ^ kc) >> 1, kMaxDepth - 1) : -1;
     flag[i] = s < c;
@@ -1015,9 +1014,6 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   // Segments that end earlier are sorted, since balance and X-list children
   // can split their cells further.
   sort_pairs_u64(keys, perm, n, sort_T, t.pw, s, 64, true, first_round);
-  hipLaunchKernelGGL(k_gather_sorted, dim3(grid_for(n)), dim3(BS), 0, s, t.xy.as<double2>(), perm,
-                     nn, t.sx.as<double>(), t.sy.as<double>());
-  FMM_LAUNCH_CHECK();
 
   // Small host->device uploads go through distinct regions of a pinned buffer
   // so that the host never rewrites memory a queued copy still reads.
@@ -1042,7 +1038,8 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   int* rank = t.w_rank.as<int>();
   int2* sc = t.w_a.as<int2>();
   hipLaunchKernelGGL(k_split_pos, dim3((unsigned)ceil_div(n, 1024)), dim3(1024), 0, s, keys, nn,
-                     ncut, flag, sc, cnt + CNT_SPLIT);
+                     ncut, flag, sc, cnt + CNT_SPLIT, perm, t.xy.as<double2>(), t.sx.as<double>(),
+                     t.sy.as<double>());
   FMM_LAUNCH_CHECK();
   FMM_HIP(hipMemcpyAsync(hc, cnt, CNT_ALL * sizeof(int), hipMemcpyDeviceToHost, s));
   FMM_HIP(hipStreamSynchronize(s));
