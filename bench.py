@@ -535,11 +535,12 @@ def main():
             return f, u
 
         # Every step reuses one pooled device plan (each FMM is released
-        # before the next one is made).  Phase timing is switched on in the
-        # first warm-up step: each later run of a phase records its own pair
-        # of HIP events on the launch stream (torch's current stream), and
-        # phase_times returns the mean over the runs since its previous call
-        # -- below, the timed steps.
+        # before the next one is made).  The timed steps run without phase
+        # events (an event record between two kernels idles the GPU for a
+        # few microseconds); the same number of steps then runs again with
+        # them, each run of a phase recording its own pair of HIP events on
+        # the launch stream (torch's current stream), and phase_times
+        # returns the mean over the runs since its previous call.
         import ctypes
         from flexmm_amd import _lib
 
@@ -549,13 +550,11 @@ def main():
             return {k: float(v) for k, v in zip(_lib.PHASES, out) if v >= 0}
 
         f, u = step()
-        f.set_profiling(True)
         plan = f._plan
         f.close()
         for _ in range(args.warmup - 1):
             f, u = step()
             f.close()
-        phase_times(plan)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -564,6 +563,11 @@ def main():
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         ms = (t1 - t0) * 1000.0 / args.steps
+        _lib.check(plan.lib.fmm2d_set_profiling(plan.h, 1))
+        phase_times(plan)
+        for _ in range(args.steps):
+            f, u = step()
+            f.close()
         acc = phase_times(plan)
         _lib.check(plan.lib.fmm2d_set_profiling(plan.h, 0))
         f, u = step()
