@@ -73,6 +73,33 @@ __host__ __device__ inline int64_t ops_off_svd_rank(int PD) {
 __host__ __device__ inline int64_t ops_off_svd(int PD) { return ops_off_svd_rank(PD) + 64; }
 inline int64_t ops_per_level(int PD) { return ops_off_svd(PD) + 40 * (int64_t)PD * PD; }
 
+// Position of entry k of a multipole row in the FP32 M2L's copy of it
+// (k_m2l_waves_f32): lane group kq = (k mod 8) / 2 reads KS = PD / 4
+// consecutive floats, in the order of its MFMA steps 2 m + h, k = 8 m + 2 kq + h.
+template <int PD>
+__host__ __device__ __forceinline__ int m32_pos(int k) {
+  return ((k & 7) >> 1) * (PD / 4) + 2 * (k >> 3) + (k & 1);
+}
+
+// Stores this lane's entries 16 rb + 4 v + kq (v = 0 .. 3) of a multipole row
+// into its FP32 copy row32, with the lane's partner l ^ 16 (lane group
+// kq ^ 1) as one 16-byte store each: the even lane group writes the v = 0, 2
+// entries of both, the odd one the v = 1, 3 entries (positions m32_pos).
+// Every lane of the wave takes part in the exchange; store says whether this
+// lane writes.
+template <int PD>
+__device__ __forceinline__ void store_m32(float* __restrict__ row32, int rb, const double (&v)[4],
+                                          bool store) {
+  const int kq = (threadIdx.x & 63) >> 4;
+  const bool odd = kq & 1;
+  const float f0 = (float)v[0], f1 = (float)v[1], f2 = (float)v[2], f3 = (float)v[3];
+  const float rx = __shfl_xor(odd ? f0 : f1, 16, 64);
+  const float ry = __shfl_xor(odd ? f2 : f3, 16, 64);
+  const float4 out = odd ? make_float4(rx, f1, ry, f3) : make_float4(f0, rx, f2, ry);
+  if (store)
+    *reinterpret_cast<float4*>(row32 + (PD / 4) * ((kq >> 1) + (odd ? 2 : 0)) + 4 * rb) = out;
+}
+
 // Chebyshev (black-box) expansions, bbfmm.py:95-231: order pc <= kMaxCheb,
 // p = pc^2 coefficients per box, index i pc + j for T_i(x) T_j(y), with
 // (x, y) = (point - box centre) * (2 / w).  The level's 2/w sits in the first
@@ -104,6 +131,7 @@ struct Plan {
   std::vector<char> ops_set;
   DevBuf logtab;
   DevBuf tau_s, src, check, mult, le, ple, qs;
+  DevBuf mult32;       // FP32 M2L: the multipoles rounded, in operand order (m32_pos)
   DevBuf srcf, tsum;   // mixed precision: leaf-relative FP32 sources, per-leaf charge sums
   DevBuf dip;          // dipole moments of the sources in tree order (double layer)
   DevBuf sdip;         // Stokes double layer: (n.sigma, a, b, 0) per source in tree order
@@ -600,7 +628,8 @@ __global__ __launch_bounds__(256) void k_mult_level(int64_t row0, int nrows, int
                                                    const double* __restrict__ check,
                                                    const double* __restrict__ opsL,
                                                    double* __restrict__ mult,
-                                                   double* __restrict__ qs, int64_t dstride) {
+                                                   double* __restrict__ qs, int64_t dstride,
+                                                   float* __restrict__ m32) {
   __shared__ __attribute__((aligned(16))) double buf[2][PD * Mfma<PD>::LDA];
   // density blockIdx.y: its rows sit dstride rows further in check, mult and qs
   check += (int64_t)blockIdx.y * dstride * PD;
@@ -637,6 +666,14 @@ __global__ __launch_bounds__(256) void k_mult_level(int64_t row0, int nrows, int
     store_acc<PD>(acc, mult + r * PD);
     if ((l >> 4) == 0) qs[r] = part;
   }
+  if (m32) {
+    float* row32 = m32 + ((int64_t)blockIdx.y * dstride + r) * PD;
+#pragma unroll
+    for (int rb = 0; rb < PD / 16; ++rb) {
+      const double v[4] = {acc[rb][0], acc[rb][1], acc[rb][2], acc[rb][3]};
+      store_m32<PD>(row32, rb, v, active);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -667,6 +704,9 @@ struct ValidSlots {
   int slot[4][kNumM2L];
 };
 __constant__ ValidSlots c_valid;
+// zeros: the multipole row of an absent M2L source (FP32 M2L), so that its
+// operand loads need no condition
+__device__ double c_zero_row[64] = {};   // never written
 
 // M2L block: 256 threads, 64 target boxes of one level and one child position
 // c (4 waves x 16 columns, every wave all PD rows), phases = the slots that
@@ -684,7 +724,6 @@ constexpr int kM2LThreads = 256;   // threads per block and row split
 constexpr int kM2LCols = 64;
 constexpr int kM2LRowSplit = 1;   // blocks of 4 waves, each wave all PD rows of 16 boxes
 constexpr int kM2LSplitF64 = 2;   // FP64 M2L: blocks per tile when tiles are few
-constexpr int kFlushM = 2;   // FP32 M2L: k-step pairs per FP32 run (4 runs per slot at PD 64)
 // Dense FP64 M2L in narrow tiles (k_m2l_waves_f64) up to this padded
 // dimension: at PD 32 it beats the LDS-staged tiles (100K points, p 32:
 // 0.034 -> 0.028 ms); at PD 64 each wave re-reading 32 KB per slot from L2
@@ -963,36 +1002,36 @@ __global__ __launch_bounds__(kM2LThreads * RS) void k_m2l_all(LevelTable lt,
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 // FP32 M2L in narrow tiles: block = 16 target boxes of one level and child
-// position and 4 waves; the slots that any of the 16 boxes uses are dealt
+// position and NW waves; the slots that any of the 16 boxes uses are dealt
 // round-robin to the waves, and each wave loads the residual block of its
 // slot straight from L2 into registers, with no LDS stage and no barrier per
 // slot.  The residuals are stored in the MFMA operand order (host packing,
 // residual_fragment_order): one 16-byte load per lane feeds four MFMA steps
 // and a wave's load reads 1 KB in one piece.  Step 2 m + h of lane group kq
-// multiplies k = 8 m + 2 kq + h, so a lane's source-multipole operands for
-// two steps are one 16-byte load and the 4 lane groups of a row read 64
-// consecutive bytes.  The waves' FP64 sums are added in a fixed order
-// through LDS at the end.  Against 64-box tiles sharing an LDS stage this
-// gives four times the waves (a 1M-point tree: ~5 per SIMD instead of ~1.4).
+// multiplies k = 8 m + 2 kq + h.  The source multipoles come from their FP32
+// copy in the same order (m32_pos: a lane's 16 operands are 64 contiguous
+// bytes), except in the Stokeslet frame fields, which form them in FP64 from
+// two rows.  Each wave streams through its slots' residual pieces with a
+// row block of them in flight and loads the next slot's multipoles at the
+// start of the current one, so its loads overlap the MFMAs before them
+// (measured at 1M points: 0.094 -> 0.077 ms).  The constant mode c_s * (sum of the
+// source's equivalent density) is an FP64 MFMA product over the wave's slots
+// at the end (rows permuted so that its output layout is the FP32 one).  The
+// waves' FP64 sums are added in a fixed order through LDS.  Against 64-box
+// tiles sharing an LDS stage this gives four times the waves (a 1M-point
+// tree: ~5 per SIMD instead of ~1.4).
 constexpr int kM2LWCols = 16;
 constexpr int kM2LWWaves = 4;
-// FP32 M2L: NCG column groups of 16 boxes and NW waves per block.  Measured
-// at 1M points against one group and 4 waves (0.095 ms): two groups, which
-// share each residual fragment load between twice the MFMAs, 0.103 ms at 3
-// waves per SIMD and 0.122 at 2; 8 waves per block (finer work per wave for
-// the last round of waves), 0.098 ms.
-constexpr int kM2LF32Groups = 1;
 constexpr int kM2LF32Waves = 4;
 
-template <int PD, bool FRAME, int NCG, int NW>
-__global__ __launch_bounds__(64 * NW)
-__attribute__((amdgpu_waves_per_eu(NCG == 1 ? 4 : 3))) void k_m2l_waves_f32(
+template <int PD, bool FRAME, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void k_m2l_waves_f32(
     LevelTable lt, const int* __restrict__ flags, const int* __restrict__ m2l,
-    const double* __restrict__ mult, const double* __restrict__ qs,
-    const double* __restrict__ ops, int64_t opsz, double* __restrict__ ple, int64_t rstride,
-    int64_t nstride, const double* __restrict__ xmid, const double* __restrict__ ymid,
-    double c0x, double c0y) {
-  constexpr int RB = PD / 16, KS = PD / 4, NC = 16 * NCG;
+    const double* __restrict__ mult, const float* __restrict__ m32,
+    const double* __restrict__ qs, const double* __restrict__ ops, int64_t opsz,
+    double* __restrict__ ple, int64_t rstride, int64_t nstride, const double* __restrict__ xmid,
+    const double* __restrict__ ymid, double c0x, double c0y) {
+  constexpr int RB = PD / 16, KS = PD / 4, NA = KS / 4;
   typedef float fv4 __attribute__((ext_vector_type(4)));
   // FRAME: fields 0 and 1 take the source's field-2 row (D_2) too
   const bool frame = FRAME && blockIdx.y < 2;
@@ -1000,10 +1039,12 @@ __attribute__((amdgpu_waves_per_eu(NCG == 1 ? 4 : 3))) void k_m2l_waves_f32(
   mult += (int64_t)blockIdx.y * rstride * PD;
   qs += (int64_t)blockIdx.y * rstride;
   ple += (int64_t)blockIdx.y * nstride * PD;
-  __shared__ int srcs[27][NC];
+  __shared__ int srcs[27][16];
+  __shared__ double qsl[27][16];   // source charge sums (0: no source)
   __shared__ int pslot[28];
+  __shared__ int pslotv[27];   // the M2L slot (offset) of each used slot index
   __shared__ unsigned int smask;
-  __shared__ double red[NW - 1][PD][NC + 1];
+  __shared__ double red[NW - 1][PD][17];
   const int bid = xcd_block(blockIdx.x, gridDim.x);
   int li = 0;
   while (li + 1 < lt.nlev && bid >= lt.boff[li + 1]) ++li;
@@ -1013,31 +1054,31 @@ __attribute__((amdgpu_waves_per_eu(NCG == 1 ? 4 : 3))) void k_m2l_waves_f32(
   const int ngroups = lt.ngroups[li];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int col = l & 15, kq = l >> 4;
-  int g[NCG];
-  bool active[NCG];
-#pragma unroll
-  for (int cg = 0; cg < NCG; ++cg) {
-    const int qq = tile * NC + 16 * cg + col;
-    const int q = (lt.perm && qq < ngroups) ? lt.perm[lt.poff[li] + c * ngroups + qq] : qq;
-    g[cg] = lt.g0[li] + 4 * q + c;
-    active[cg] = q < ngroups && !(flags[q < ngroups ? g[cg] : lt.g0[li]] & NF_FAKE);
-  }
+  const int qq = tile * 16 + col;
+  const int q = (lt.perm && qq < ngroups) ? lt.perm[lt.poff[li] + c * ngroups + qq] : qq;
+  const int g = lt.g0[li] + 4 * q + c;
+  const bool active = q < ngroups && !(flags[q < ngroups ? g : lt.g0[li]] & NF_FAKE);
+  // target centre coordinate blockIdx.y in the root frame (frame fields)
+  const double ct =
+      (frame && active) ? (blockIdx.y == 0 ? xmid[g] - c0x : ymid[g] - c0y) : 0.0;
   // interaction lists: the 4 NW lane groups (w, kq) read slot indices
-  // 4 w + kq + 4 NW j < 32 of their column in each group
+  // 4 w + kq + 4 NW j < 32 of their column
   constexpr int LG = 4 * NW, NJ = 32 / LG;
   static_assert(NJ * LG == 32, "lane groups cover the 32 slot indices");
   const int nv = c_valid.n[c];
   if (threadIdx.x == 0) smask = 0u;
   __syncthreads();
   unsigned int used = 0u;
-#pragma unroll
-  for (int cg = 0; cg < NCG; ++cg) {
-    const int* lst = m2l + (int64_t)g[cg] * kNumM2L;
+  {
+    const int* lst = m2l + (int64_t)g * kNumM2L;
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
       const int i = 4 * w + kq + LG * j;
-      const int src = (active[cg] && i < nv) ? lst[c_valid.slot[c][i]] : -1;
-      if (i < nv) srcs[i][16 * cg + col] = src;
+      const int src = (active && i < nv) ? lst[c_valid.slot[c][i]] : -1;
+      if (i < nv) {
+        srcs[i][col] = src;
+        qsl[i][col] = src < 0 ? 0.0 : frame ? __builtin_fma(ct, qs[to2 + src], qs[src]) : qs[src];
+      }
       const unsigned long long b = __ballot(src >= 0);
 #pragma unroll
       for (int k = 0; k < 4; ++k)
@@ -1050,7 +1091,9 @@ __attribute__((amdgpu_waves_per_eu(NCG == 1 ? 4 : 3))) void k_m2l_waves_f32(
     unsigned int m = smask;
     int k = 0;
     while (m) {
-      pslot[k++] = __builtin_ctz(m);
+      const int i = __builtin_ctz(m);
+      pslot[k] = i;
+      pslotv[k++] = c_valid.slot[c][i];
       m &= m - 1u;
     }
     pslot[27] = k;
@@ -1059,130 +1102,155 @@ __attribute__((amdgpu_waves_per_eu(NCG == 1 ? 4 : 3))) void k_m2l_waves_f32(
   const int nph = pslot[27];
   const double* opsL = ops + (int64_t)lt.d[li] * opsz;
   const double* means = opsL + ops_off_means(PD);
-  const float* r32 = reinterpret_cast<const float*>(opsL + ops_off_r32(PD));
-  double ct[NCG];   // target centre coordinate i in the root frame
+  // residual piece j of row block rb of a slot: afr[(slot RB + rb) NA + j) 64]
+  const fv4* afr = reinterpret_cast<const fv4*>(opsL + ops_off_r32(PD)) + l;
+  d4 acc64[RB];
 #pragma unroll
-  for (int cg = 0; cg < NCG; ++cg) {
-    ct[cg] = 0.0;
-    if (frame && active[cg]) ct[cg] = blockIdx.y == 0 ? xmid[g[cg]] - c0x : ymid[g[cg]] - c0y;
-  }
-  double acc64[NCG][RB][4];
-#pragma unroll
-  for (int cg = 0; cg < NCG; ++cg)
-#pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) acc64[cg][rb][v] = 0.0;
-  for (int k = w; k < nph; k += NW) {
-    const int i = pslot[k];
-    const int slot = c_valid.slot[c][i];
-    // B: the source row's entries 8 m + 2 kq + {0, 1}, rounded to FP32 (the
-    // Stokes fields 0 and 1 in the target box's frame, formed in FP64)
-    float b[NCG][KS];
-    double qv[NCG];
-#pragma unroll
-    for (int cg = 0; cg < NCG; ++cg) {
-      const int src = srcs[i][16 * cg + col];
-      qv[cg] = 0.0;
-      if (src >= 0) {
-        const double* brow = mult + (int64_t)src * PD + 2 * kq;
+  for (int rb = 0; rb < RB; ++rb) acc64[rb] = d4{0.0, 0.0, 0.0, 0.0};
+  // this wave's slots: pslot[w + NW j], j < nmine
+  const int nmine = w < nph ? (nph - 1 - w) / NW + 1 : 0;
+  if (nmine > 0) {
+    if constexpr (FRAME) {
+      // Stokeslet fields: per slot, entries 8 m + 2 kq + {0, 1} of the
+      // source rows formed in FP64 (the frame fields from two rows), then
+      // each row block's residual pieces loaded before its steps
+      for (int jj = 0; jj < nmine; ++jj) {
+        const int k = w + NW * jj;
+        const int slot = __builtin_amdgcn_readfirstlane(pslotv[k]);
+        const int src = srcs[pslot[k]][col];
+        const double* brow = src >= 0 ? mult + (int64_t)src * PD + 2 * kq : c_zero_row;
+        fv4 bu[NA];
 #pragma unroll
         for (int m = 0; m < KS / 2; ++m) {
           dv2 v = *reinterpret_cast<const dv2*>(brow + 8 * m);
-          if (frame) {
+          if (frame && src >= 0) {
             const dv2 v2 = *reinterpret_cast<const dv2*>(brow + to2 * PD + 8 * m);
-            v.x = __builtin_fma(ct[cg], v2.x, v.x);
-            v.y = __builtin_fma(ct[cg], v2.y, v.y);
+            v.x = __builtin_fma(ct, v2.x, v.x);
+            v.y = __builtin_fma(ct, v2.y, v.y);
           }
-          b[cg][2 * m] = (float)v.x;
-          b[cg][2 * m + 1] = (float)v.y;
+          bu[m >> 1][2 * (m & 1)] = (float)v.x;
+          bu[m >> 1][2 * (m & 1) + 1] = (float)v.y;
         }
-        qv[cg] = frame ? __builtin_fma(ct[cg], qs[to2 + src], qs[src]) : qs[src];
-      } else {
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) b[cg][ks] = 0.f;
+        for (int rb = 0; rb < RB; ++rb) {
+          fv4 a[NA];
+#pragma unroll
+          for (int j = 0; j < NA; ++j) a[j] = afr[((int64_t)slot * RB * NA + rb * NA + j) * 64];
+#pragma unroll
+          for (int j = 0; j < NA; ++j) {
+            f4 s0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j][0], bu[j][0], f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+            f4 s1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j][1], bu[j][1], f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+            s0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j][2], bu[j][2], s0, 0, 0, 0);
+            s1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j][3], bu[j][3], s1, 0, 0, 0);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc64[rb][v] += (double)(s0[v] + s1[v]);
+          }
+        }
+      }
+    } else {
+      // operator pieces pc = rb NA + j of a slot (16 bytes per lane, four
+      // MFMA steps each), NA of them in flight: consuming a piece issues the
+      // load of the piece NA further on, across row block and slot
+      // boundaries.  The multipole operands alternate between two buffers: a
+      // slot's first NA pieces also load the next slot's operands into the
+      // other one.  Absent sources read a zero row.
+      constexpr int NP = RB * NA;
+      const float* m32y = m32 + (int64_t)blockIdx.y * rstride * PD;
+      auto bptr = [&](int src) -> const fv4* {
+        return reinterpret_cast<const fv4*>(
+            (src >= 0 ? m32y + (int64_t)src * PD : reinterpret_cast<const float*>(c_zero_row)) +
+            kq * KS);
+      };
+      auto apiece = [&](int sl, int pc) -> const fv4* {
+        return afr + ((int64_t)sl * NP + pc) * 64;
+      };
+      int slot = __builtin_amdgcn_readfirstlane(pslotv[w]);
+      fv4 a[NA], b0[NA], b1[NA];
+      // first loads in the order the loop consumes them, so that its waits
+      // for the first pieces hold for every slot
+#pragma unroll
+      for (int h = 0; h < NA; ++h) {
+        a[h] = *apiece(slot, h);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      {
+        const fv4* bp = bptr(srcs[pslot[w]][col]);
+#pragma unroll
+        for (int j = 0; j < NA; ++j) {
+          b0[j] = bp[j];
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      auto one_slot = [&](fv4 (&bu)[NA], fv4 (&bl)[NA], int jj) {
+        // the next slot of this wave (the current one again after the last)
+        const int kn = jj + 1 < nmine ? w + NW * (jj + 1) : w + NW * jj;
+        const int slot_n = __builtin_amdgcn_readfirstlane(pslotv[kn]);
+        const fv4* bpn = bptr(srcs[pslot[kn]][col]);
+#pragma unroll
+        for (int pc = 0; pc < NP; ++pc) {
+          const int rb = pc / NA, j = pc % NA;
+          // an FP32 run of four steps, added to the FP64 sums; two
+          // accumulators keep consecutive MFMAs independent
+          f4 s0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j][0], bu[j][0], f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+          f4 s1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j][1], bu[j][1], f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+          s0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j][2], bu[j][2], s0, 0, 0, 0);
+          s1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j][3], bu[j][3], s1, 0, 0, 0);
+          // the piece just used is reloaded into the same registers
+          a[j] = pc + NA < NP ? *apiece(slot, pc + NA) : *apiece(slot_n, pc + NA - NP);
+          if (pc < NA) bl[pc] = bpn[pc];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc64[rb][v] += (double)(s0[v] + s1[v]);
+          // in program order: the scheduler would otherwise move each load
+          // down to its use
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        slot = slot_n;
+      };
+      for (int jj = 0; jj < nmine; jj += 2) {
+        one_slot(b0, b1, jj);
+        if (jj + 1 < nmine) one_slot(b1, b0, jj + 1);
+        // keeps the next slot's loads here, ahead of the loop's back edge
+        // (otherwise they are merged with the first slot's into one load at
+        // the top of the loop, which then waits for them at once)
+        __asm__ volatile("" ::: "memory");
       }
     }
-    // A in operand order, one pair of 16-row blocks at a time (64 registers
-    // for all four would cost the kernel its resident waves):
-    // a[r][mp] = row 16 (2 rp + r) + (l mod 16), columns 8 m + 2 kq + {0, 1}
-    // for m = 2 mp, 2 mp + 1
-    const fv4* afrag = reinterpret_cast<const fv4*>(r32 + (int64_t)slot * PD * PD) + l;
-    const double* cm = means + (int64_t)slot * PD + 4 * kq;   // rows 16 rb + 4 kq + v
+    // constant mode: acc64[rb] += sum over this wave's slots of
+    // means[slot][row] * qsl[slot][col], four slots per FP64 MFMA step.  Lane
+    // (kq, col) supplies A[col][kq] = means[slot_kq][16 rb + pi(col)] with
+    // pi(4 a + b) = 4 b + a, so that D[4 v + kq][col] (the FP64 layout) is row
+    // 16 rb + 4 kq + v, the row acc64[rb][v] holds.
+    const int prow = 4 * (col & 3) + (col >> 2);
+    for (int k0 = w; k0 < nph; k0 += 4 * NW) {
+      const int kk = k0 + NW * kq;
+      const bool ok = kk < nph;
+      const int ks = ok ? kk : k0;
+      const double bq = ok ? qsl[pslot[ks]][col] : 0.0;
+      const double* mrow = means + (int64_t)pslotv[ks] * PD + prow;
 #pragma unroll
-    for (int rp = 0; rp < RB / 2; ++rp) {
-      fv4 a[2][KS / 4];
-#pragma unroll
-      for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int j = 0; j < KS / 4; ++j) a[r][j] = afrag[((2 * rp + r) * (KS / 4) + j) * 64];
-      f4 acc[NCG][2];
-#pragma unroll
-      for (int cg = 0; cg < NCG; ++cg) acc[cg][0] = acc[cg][1] = f4{0.f, 0.f, 0.f, 0.f};
-      // FP32 runs of kFlushM step pairs, each added to the FP64 sums
-#pragma unroll
-      for (int m = 0; m < KS / 2; ++m) {
-#pragma unroll
-        for (int r = 0; r < 2; ++r)
-#pragma unroll
-          for (int cg = 0; cg < NCG; ++cg) {
-            acc[cg][r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r][m >> 1][2 * (m & 1)],
-                                                             b[cg][2 * m], acc[cg][r], 0, 0, 0);
-            acc[cg][r] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                a[r][m >> 1][2 * (m & 1) + 1], b[cg][2 * m + 1], acc[cg][r], 0, 0, 0);
-          }
-        if ((m + 1) % kFlushM == 0 && m + 1 < KS / 2) {
-#pragma unroll
-          for (int cg = 0; cg < NCG; ++cg)
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-#pragma unroll
-              for (int v = 0; v < 4; ++v) acc64[cg][2 * rp + r][v] += (double)acc[cg][r][v];
-              acc[cg][r] = f4{0.f, 0.f, 0.f, 0.f};
-            }
-        }
-      }
-      // constant mode c_s * (sum of the source's equivalent density), FP64
-#pragma unroll
-      for (int r = 0; r < 2; ++r) {
-        const int rb = 2 * rp + r;
-        const dv2 c01 = *reinterpret_cast<const dv2*>(cm + 16 * rb);
-        const dv2 c23 = *reinterpret_cast<const dv2*>(cm + 16 * rb + 2);
-#pragma unroll
-        for (int cg = 0; cg < NCG; ++cg) {
-          acc64[cg][rb][0] += __builtin_fma(c01.x, qv[cg], (double)acc[cg][r][0]);
-          acc64[cg][rb][1] += __builtin_fma(c01.y, qv[cg], (double)acc[cg][r][1]);
-          acc64[cg][rb][2] += __builtin_fma(c23.x, qv[cg], (double)acc[cg][r][2]);
-          acc64[cg][rb][3] += __builtin_fma(c23.y, qv[cg], (double)acc[cg][r][3]);
-        }
-      }
+      for (int rb = 0; rb < RB; ++rb)
+        acc64[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(mrow[16 * rb], bq, acc64[rb], 0, 0, 0);
     }
   }
   // waves 1 .. NW-1 hand their sums to wave 0, which adds them in wave order
   if (w > 0) {
 #pragma unroll
-    for (int cg = 0; cg < NCG; ++cg)
+    for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) red[w - 1][16 * rb + 4 * kq + v][16 * cg + col] = acc64[cg][rb][v];
+      for (int v = 0; v < 4; ++v) red[w - 1][16 * rb + 4 * kq + v][col] = acc64[rb][v];
   }
   __syncthreads();
-  if (w == 0) {
+  if (w == 0 && active) {
+    double* rowp = ple + (int64_t)g * PD;
 #pragma unroll
-    for (int cg = 0; cg < NCG; ++cg) {
-      if (!active[cg]) continue;
-      double* rowp = ple + (int64_t)g[cg] * PD;
+    for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb)
+      for (int v = 0; v < 4; ++v) {
+        double sum = acc64[rb][v];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          double sum = acc64[cg][rb][v];
-#pragma unroll
-          for (int ww = 0; ww < NW - 1; ++ww) sum += red[ww][16 * rb + 4 * kq + v][16 * cg + col];
-          rowp[16 * rb + 4 * kq + v] = sum;
-        }
-    }
+        for (int ww = 0; ww < NW - 1; ++ww) sum += red[ww][16 * rb + 4 * kq + v][col];
+        rowp[16 * rb + 4 * kq + v] = sum;
+      }
   }
 }
 
@@ -1447,7 +1515,7 @@ template <int PD>
 __device__ __forceinline__ void mult_small_unit(
     int bx, int by, int64_t row0, int nrows, int with_m2m, const int2* __restrict__ rowinfo,
     const double* __restrict__ check, const double* __restrict__ opsL, double* __restrict__ mult,
-    double* __restrict__ qs, int64_t dstride) {
+    double* __restrict__ qs, int64_t dstride, float* __restrict__ m32) {
   __shared__ __attribute__((aligned(16))) double bbuf[2][16][PD + kSmallPad];
   __shared__ double qpart[PD / 16][16];
   check += (int64_t)by * dstride * PD;
@@ -1476,6 +1544,10 @@ __device__ __forceinline__ void mult_small_unit(
   part += __shfl_xor(part, 16, 64);
   part += __shfl_xor(part, 32, 64);
   if (kq == 0) qpart[w][col] = part;
+  if (m32) {
+    const double v[4] = {acc[0], acc[1], acc[2], acc[3]};
+    store_m32<PD>(m32 + ((int64_t)by * dstride + r) * PD, w, v, active);
+  }
   __syncthreads();
   if (active) {
     double* dst = mult + r * PD;
@@ -1494,9 +1566,9 @@ template <int PD>
 __global__ __launch_bounds__(64 * (PD / 16)) void k_mult_level_small(
     int64_t row0, int nrows, int with_m2m, const int2* __restrict__ rowinfo,
     const double* __restrict__ check, const double* __restrict__ opsL, double* __restrict__ mult,
-    double* __restrict__ qs, int64_t dstride) {
+    double* __restrict__ qs, int64_t dstride, float* __restrict__ m32) {
   mult_small_unit<PD>(blockIdx.x, blockIdx.y, row0, nrows, with_m2m, rowinfo, check, opsL, mult,
-                      qs, dstride);
+                      qs, dstride, m32);
 }
 
 // Downward level (as k_local_level): 16 boxes of child position c per unit;
@@ -2970,23 +3042,25 @@ static void m2l_launch(Plan& P, const M2LTables& T, int l0, int l1, hipStream_t 
   int nb = lt.boff[lt.nlev];
   if (nb == 0) return;
   if (m2l_in_fp32(P)) {
-    // narrow tiles of 16 kM2LF32Groups boxes, one block each
-    constexpr int NG = kM2LF32Groups;
+    // narrow tiles of 16 boxes, one block each
     LevelTable lw = lt;
     int nbw = 0;
     for (int li = 0; li < lw.nlev; ++li) {
       lw.boff[li] = nbw;
-      nbw += 4 * (int)ceil_div(lw.ngroups[li], 16 * NG);
+      nbw += 4 * (int)ceil_div(lw.ngroups[li], kM2LWCols);
     }
     lw.boff[lw.nlev] = nbw;
     lw.split = 1;
     const bool frame = P.kernel_id == KER_STOKES;
     constexpr int NW = kM2LF32Waves;
     auto kern = PD == 32
-                    ? (frame ? k_m2l_waves_f32<32, true, NG, NW> : k_m2l_waves_f32<32, false, NG, NW>)
-                    : (frame ? k_m2l_waves_f32<64, true, NG, NW> : k_m2l_waves_f32<64, false, NG, NW>);
+                    ? (frame ? k_m2l_waves_f32<32, true, NW> : k_m2l_waves_f32<32, false, NW>)
+                    : (frame ? k_m2l_waves_f32<64, true, NW> : k_m2l_waves_f32<64, false, NW>);
+    FMM_REQUIRE(frame || P.mult32.bytes >= (size_t)(rstride * PD * nd * sizeof(float)),
+                FMM_E_INTERNAL, "FP32 multipole copy missing");
     hipLaunchKernelGGL(kern, dim3(nbw, nd), dim3(64 * NW), 0, s, lw, t.flags.as<int>(),
-                       t.m2l.as<int>(), P.mult.as<double>(), P.qs.as<double>(),
+                       t.m2l.as<int>(), P.mult.as<double>(),
+                       frame ? nullptr : P.mult32.as<float>(), P.qs.as<double>(),
                        P.ops.as<double>(), opsz, P.ple.as<double>(), rstride, nstride,
                        t.xmid.as<double>(), t.ymid.as<double>(), t.xmin + 0.5 * t.w0,
                        t.ymin + 0.5 * t.w0);
@@ -3064,6 +3138,13 @@ static void upward(Plan& P, const double* tau, const double* dvec, hipStream_t s
   P.check.ensure(rstride * PD * nd * sizeof(double));
   P.mult.ensure(rstride * PD * nd * sizeof(double));
   P.qs.ensure(rstride * nd * sizeof(double));
+  // the FP32 M2L's operand copy of the multipoles (the Stokeslet's frame
+  // fields form theirs from the FP64 rows)
+  float* m32 = nullptr;
+  if (m2l_in_fp32(P) && !stokes) {
+    P.mult32.ensure(rstride * PD * nd * sizeof(float));
+    m32 = P.mult32.as<float>();
+  }
   if (t.L <= 2 || nr == 0) {
     P.have_exp = true;
     return;
@@ -3109,14 +3190,14 @@ static void upward(Plan& P, const double* tau, const double* dvec, hipStream_t s
       auto ks = PD == 32 ? k_mult_level_small<32> : k_mult_level_small<64>;
       hipLaunchKernelGGL(ks, dim3(ceil_div(nrows, 16), nd), dim3(64 * (PD / 16)), 0, s, r0,
                          nrows, d < t.L - 1 ? 1 : 0, t.rowinfo.as<int2>(), P.check.as<double>(), opsL,
-                         P.mult.as<double>(), P.qs.as<double>(), rstride);
+                         P.mult.as<double>(), P.qs.as<double>(), rstride, m32);
       FMM_LAUNCH_CHECK();
       continue;
     }
     auto kern = PD == 32 ? k_mult_level<32> : k_mult_level<64>;
     hipLaunchKernelGGL(kern, dim3(ceil_div(nrows, 64), nd), dim3(256), 0, s, r0, nrows,
                        d < t.L - 1 ? 1 : 0, t.rowinfo.as<int2>(), P.check.as<double>(), opsL,
-                       P.mult.as<double>(), P.qs.as<double>(), rstride);
+                       P.mult.as<double>(), P.qs.as<double>(), rstride, m32);
     FMM_LAUNCH_CHECK();
   }
   P.have_exp = false;
