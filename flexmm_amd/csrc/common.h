@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -124,6 +125,75 @@ struct HostBuf {
 };
 
 __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Pinned, fine-grained host words that a kernel writes while its stream
+// still runs: each word is (sequence << 32 | value), written by one 64-bit
+// system-scope store, so the host can take a few results as soon as they are
+// final instead of draining the stream first (the kernels queued after them
+// then follow without a gap).  Every use writes all the words it reads under
+// a new sequence number and the host takes each word once it carries that
+// number; no order between words is assumed.
+struct HostMailbox {
+  static constexpr int kWords = 64;
+  uint64_t* host = nullptr;   // host address of the words
+  uint64_t* dev = nullptr;    // the same words as kernels address them
+  uint32_t seq = 0;
+  void ensure() {
+    if (host) return;
+    void* p = nullptr;
+    FMM_HIP(hipHostMalloc(&p, kWords * sizeof(uint64_t), hipHostMallocCoherent));
+    host = static_cast<uint64_t*>(p);
+    for (int i = 0; i < kWords; ++i) host[i] = 0;   // sequence 0 is never used
+    FMM_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&dev), host, 0));
+  }
+  uint32_t next() {
+    seq = seq + 1 == 0 ? 1 : seq + 1;
+    return seq;
+  }
+  // Words [0, n) of sequence sq into out: spins for up to 2 ms, then drains
+  // the stream s (whose kernels write them), after which they must be there.
+  void take(int n, uint32_t sq, hipStream_t s, int64_t* out) const {
+    FMM_REQUIRE(n <= kWords, FMM_E_INTERNAL, "host mailbox: %d words requested", n);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int spin = 0;; ++spin) {
+      int got = 0;
+      for (; got < n; ++got) {
+        const uint64_t v = __atomic_load_n(host + got, __ATOMIC_ACQUIRE);
+        if ((uint32_t)(v >> 32) != sq) break;
+        out[got] = (int32_t)(uint32_t)v;
+      }
+      if (got == n) return;
+      if ((spin & 63) == 63 &&
+          std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2))
+        break;
+#if defined(__x86_64__)
+      __builtin_ia32_pause();
+#endif
+    }
+    FMM_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < n; ++i) {
+      const uint64_t v = __atomic_load_n(host + i, __ATOMIC_ACQUIRE);
+      FMM_REQUIRE((uint32_t)(v >> 32) == sq, FMM_E_INTERNAL,
+                  "host mailbox: word %d not written by its kernel", i);
+      out[i] = (int32_t)(uint32_t)v;
+    }
+  }
+  ~HostMailbox() {
+    if (host) (void)hipHostFree(host);
+  }
+  HostMailbox() = default;
+  HostMailbox(const HostMailbox&) = delete;
+  HostMailbox& operator=(const HostMailbox&) = delete;
+};
+
+#if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
+// word i of a HostMailbox (device address mb) for sequence seq
+__device__ __forceinline__ void mailbox_put(uint64_t* mb, int i, uint32_t seq, int value) {
+  __hip_atomic_store(mb + i, ((uint64_t)seq << 32) | (uint32_t)value, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_SYSTEM);
+}
+#endif
+
 
 // ---------------------------------------------------------------------------
 // Table-driven FP64 logarithm (P. T. P. Tang, "Table-driven implementation of

@@ -49,7 +49,7 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int* total)
 struct PrimWork {
   DevBuf blocksums;     // scan partials
   DevBuf counters;      // small device counters
-  HostBuf hcounters;    // pinned mirror of the counters
+  HostMailbox mbox;     // the counters the host reads between rounds
   // MSD sort
   DevBuf keysB, valsB;
   DevBuf keysC, valsC;  // second spare pair of a prepared first round

@@ -358,7 +358,8 @@ __global__ __launch_bounds__(SORT_BS) void k_chunk_scatter(
     const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
     const int4* __restrict__ chunks, const int* __restrict__ counters, int shift,
     const int* __restrict__ chunkHist, uint64_t* __restrict__ keysB, uint32_t* __restrict__ valsB,
-    uint64_t* __restrict__ kfin, uint32_t* __restrict__ vfin, int* __restrict__ host_counters) {
+    uint64_t* __restrict__ kfin, uint32_t* __restrict__ vfin, uint64_t* __restrict__ mbox,
+    uint32_t seq) {
   __shared__ int gdst[256];    // destination of the chunk's first element of digit d
   __shared__ int lstart[256];  // count, then start, of digit d inside the chunk
   __shared__ int scan_lds[SORT_BS / 64];
@@ -366,8 +367,9 @@ __global__ __launch_bounds__(SORT_BS) void k_chunk_scatter(
   __shared__ uint32_t sv[CHUNK];
   const int b = blockIdx.x;
   // the round's counters are final (k_seg_offsets ran before): block 0 hands
-  // them to the pinned host array, which saves the host a copy before it reads
-  if (b == 0 && threadIdx.x < C_COUNT) host_counters[threadIdx.x] = counters[threadIdx.x];
+  // them to the host at once, so the host can queue the next round's kernels
+  // while this one scatters
+  if (b == 0 && threadIdx.x < C_COUNT) mailbox_put(mbox, threadIdx.x, seq, counters[threadIdx.x]);
   if (b >= counters[C_CHUNKS]) return;
   const int4 ch = chunks[b];
   gdst[threadIdx.x] = chunkHist[(int64_t)b * 256 + threadIdx.x];
@@ -591,11 +593,9 @@ bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
   w.seg[1].ensure(segcap * sizeof(int2));
   w.fin.ensure(segcap * sizeof(int2));
   w.counters.ensure(C_COUNT * sizeof(int));
-  w.hcounters.ensure(C_COUNT * sizeof(int));
+  w.mbox.ensure();
   int* cnt = w.counters.as<int>();
-  int* hc = w.hcounters.as<int>();
-  int* hc_dev = nullptr;   // the same pinned array as the device addresses it
-  FMM_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&hc_dev), hc, 0));
+  int64_t hc[C_COUNT];
   const int finBigOff = (int)segcap;
   if (!first_round_ready) {
     hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(64), 0, s, w.seg[0].as<int2>(),
@@ -667,9 +667,10 @@ bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
                        w.segChunk0.as<int>(), w.chunkHist.as<int>(), T, r == rounds - 1 ? 1 : 0, segNext,
                        w.fin.as<int2>(), finBigOff, cnt);
     FMM_LAUNCH_CHECK();
+    const uint32_t seq = w.mbox.next();
     hipLaunchKernelGGL(k_chunk_scatter, dim3(cbound), dim3(SORT_BS), 0, s, kin, vin,
                        w.chunks.as<int4>(), cnt, shift, w.chunkHist.as<int>(), kout, vout,
-                       kfin, vfin, hc_dev);
+                       kfin, vfin, w.mbox.dev, seq);
     FMM_LAUNCH_CHECK();
     cur ^= 1;
     if (early_second && r == 0) {
@@ -677,8 +678,8 @@ bool sort_pairs_u64(uint64_t* keys, uint32_t* vals, int64_t n, int threshold, Pr
       std::swap(vin, vout);
       continue;
     }
-    FMM_HIP(hipStreamSynchronize(s));
-    nseg = hc[C_NEXT];
+    w.mbox.take(C_COUNT, seq, s, hc);
+    nseg = (int)hc[C_NEXT];
     active = hc[C_ACTIVE];
     nfinal = hc[C_FINAL];
     nbig = hc[C_FINAL_BIG];

@@ -178,6 +178,22 @@ __global__ __launch_bounds__(1024) void k_split_pos(const uint64_t* __restrict__
   }
 }
 
+// The host's share of the split counts: per depth d <= kMaxLevels the sum of
+// the kSplitRows rows, and the non-finite flag, into words 0 .. kMaxLevels + 1
+// of the host mailbox.
+__global__ __launch_bounds__(64) void k_split_counts(const int* __restrict__ rows,
+                                                     const int* __restrict__ bad,
+                                                     uint64_t* __restrict__ mbox, uint32_t seq) {
+  const int d = threadIdx.x;
+  if (d <= kMaxLevels) {
+    int sum = 0;
+    for (int r = 0; r < kSplitRows; ++r) sum += rows[r * (kMaxLevels + 1) + d];
+    mailbox_put(mbox, d, seq, sum);
+  } else if (d == kMaxLevels + 1) {
+    mailbox_put(mbox, d, seq, *bad);
+  }
+}
+
 __global__ __launch_bounds__(BS) void k_compact_pos(const int* __restrict__ flag,
                                                     const int* __restrict__ rank,
                                                     const int2* __restrict__ sc, int n,
@@ -1041,13 +1057,19 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
                      ncut, flag, sc, cnt + CNT_SPLIT, perm, t.xy.as<double2>(), t.sx.as<double>(),
                      t.sy.as<double>());
   FMM_LAUNCH_CHECK();
-  FMM_HIP(hipMemcpyAsync(hc, cnt, CNT_ALL * sizeof(int), hipMemcpyDeviceToHost, s));
-  FMM_HIP(hipStreamSynchronize(s));
-  FMM_REQUIRE(hc[CNT_BAD] == 0, FMM_E_ARG,
+  // per depth, then the split positions; then the non-finite flag
+  int64_t dcount[kMaxLevels + 2];
+  {
+    HostMailbox& mb = t.pw.mbox;
+    mb.ensure();
+    const uint32_t seq = mb.next();
+    hipLaunchKernelGGL(k_split_counts, dim3(1), dim3(64), 0, s, cnt + CNT_SPLIT, cnt + CNT_BAD,
+                       mb.dev, seq);
+    FMM_LAUNCH_CHECK();
+    mb.take(kMaxLevels + 2, seq, s, dcount);
+  }
+  FMM_REQUIRE(dcount[kMaxLevels + 1] == 0, FMM_E_ARG,
               "tree: the source coordinates hold a NaN or infinite value");
-  int64_t dcount[kMaxLevels + 1] = {};   // per depth, then the split positions
-  for (int r = 0; r < kSplitRows; ++r)
-    for (int d = 0; d <= kMaxLevels; ++d) dcount[d] += hc[CNT_SPLIT + r * (kMaxLevels + 1) + d];
   const int np = (int)dcount[kMaxLevels];
   int maxd = -1;
   for (int d = 0; d < kMaxLevels; ++d)

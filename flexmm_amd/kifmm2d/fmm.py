@@ -16,6 +16,7 @@ Differences from the reference:
 * two live FMMs never share mutable per-level state (the reference's
   Helper.prepare overwrites helper-wide surface lists, helpers.py:95-108).
 """
+import math
 import sys
 import time
 
@@ -69,9 +70,8 @@ class Helper:
         """helpers.py:68-88."""
         if bbox is None:
             bbox = [px_min, px_max, py_min, py_max]
-        widths = self.collect_widths()
-        if len(widths) > 0:
-            small_width = np.min(widths)
+        if self.precomputations:
+            small_width = min(self.precomputations)   # the keys are the widths
             bw = max(bbox[1] - bbox[0], bbox[3] - bbox[2])
             width = 2 ** np.ceil(np.log2(bw / small_width)) * small_width
         else:
@@ -168,11 +168,22 @@ _PRECISIONS = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED,
                "mixed_m2l": _lib.PREC_MIXED_M2L}
 
 
+# the current device's current stream as a raw handle: the direct accessor
+# costs a fraction of building a torch.cuda.Stream object (~4 us per call)
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+_raw_device = getattr(torch._C, "_cuda_getDevice", None)
+
+
 def _stream():
+    if _raw_stream is not None and _raw_device is not None:
+        return _lib.ctypes.c_void_p(_raw_stream(_raw_device()))
     return _lib.ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
 def _to_device(a, device, name):
+    if (isinstance(a, torch.Tensor) and a.is_cuda and a.dtype == torch.float64 and a.dim() == 1
+            and a.is_contiguous()):
+        return a.detach()
     if isinstance(a, torch.Tensor):
         t = a.detach()
         if t.device.type != "cuda":
@@ -205,7 +216,7 @@ class TreeInfo:
         self.row_off = [int(info[10 + 3 * d]) for d in range(L)] + [int(info[8 + 3 * L])]
         self.bbox = list(bbox)
         w0 = bbox[1] - bbox[0]
-        self.widths = [np.ldexp(w0, -d) for d in range(L)]
+        self.widths = [math.ldexp(w0, -d) for d in range(L)]
 
 
 class FMM:
@@ -287,11 +298,16 @@ class FMM:
                                              self.x.numel(), bb, self.Ncutoff, _stream()))
         info = (_lib.ctypes.c_int64 * (9 + 3 * 32))()
         _lib.check(self.lib.fmm2d_tree_info(self.handle, info, len(info)))
-        self.tree = TreeInfo(list(info), self.bbox)
-        self.print("....Tree formed in:             {:0.1f}".format((time.time() - st) * 1000))
+        self.tree = TreeInfo(info[:], self.bbox)
+        if self.verbose:
+            self.print("....Tree formed in:             {:0.1f}".format((time.time() - st) * 1000))
 
     def prepare(self):
         """Helper.prepare (helpers.py:93-108): operators of every level width."""
+        lw, widths = self._plan.level_width, self.tree.widths
+        if (all(lw.get(d) == widths[d] for d in range(2, self.tree.levels))
+                and all(self.helper.has_precomputation(w) for w in widths)):
+            return   # every width registered and every level's operators on the plan
         for w in self.tree.widths:
             self.helper.register_width(w)
         s = _stream()
