@@ -806,15 +806,26 @@ inline unsigned grid_for(int64_t n) { return (unsigned)(n <= 0 ? 1 : ceil_div(n,
 // kernel instead of the split lists, their scans and the two further read-backs.
 constexpr int kFullMaxDepth = 15;   // depth-D cell index tables of at most 4^15 ints
 
-// first sorted point of each nonempty depth-D cell (the others stay -1)
+// first sorted point of each nonempty depth-D cell, -1 for the empty ones:
+// the point that starts a cell also marks the empty cells between the
+// previous point's cell and its own (and the last point those after it).  In
+// a complete tree every depth D-1 cell has points, so such a run is at most
+// six cells.
 __global__ __launch_bounds__(BS) void k_full_starts(const uint64_t* __restrict__ keys,
                                                     const int2* __restrict__ sc, int n, int D,
                                                     int* __restrict__ start) {
   const int i = blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
   const int2 si = sc[i];
-  const uint64_t k = keys[i];   // loaded with sc, not behind its test
-  if (si.x < D) start[k >> (64 - 2 * D)] = i;
+  // loaded with sc, not behind its test
+  const uint64_t k = keys[i], kp = keys[i > 0 ? i - 1 : 0];
+  const int64_t cell = (int64_t)(k >> (64 - 2 * D));
+  if (si.x < D) {
+    for (int64_t e = i > 0 ? (int64_t)(kp >> (64 - 2 * D)) + 1 : 0; e < cell; ++e) start[e] = -1;
+    start[cell] = i;
+  }
+  if (i == n - 1)
+    for (int64_t e = cell + 1; e < ((int64_t)1 << (2 * D)); ++e) start[e] = -1;
 }
 
 // points of a depth-d cell c: [bot, top) from the depth-D starts (a cell
@@ -948,7 +959,6 @@ static void tree_build_full(Tree& t, int D, int npts, hipStream_t s) {
   const int64_t ncell = (int64_t)1 << (2 * D);
   t.w_flag.ensure((ncell + 1) * sizeof(int));
   int* start = t.w_flag.as<int>();
-  fill_i32(start, ncell, -1, s);
   hipLaunchKernelGGL(k_full_starts, dim3(grid_for(npts)), dim3(BS), 0, s, t.keys.as<uint64_t>(),
                      t.w_a.as<int2>(), npts, D, start);
   FMM_LAUNCH_CHECK();
