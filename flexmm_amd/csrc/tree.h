@@ -32,6 +32,7 @@ struct Tree {
   bool complete = false;  // the last build was such a tree: every leaf at depth L - 1, and
                           // leaf (depth L - 1, code c) is evaluation box c
   int64_t build_id = 0;   // incremented by every build
+  int full_D = 0;         // depth of the last build's complete tree (0: it was not one)
   int64_t n = 0;           // number of sources
   int ncut = 0;
   int L = 0;               // number of levels
@@ -67,6 +68,7 @@ struct Tree {
   // work areas
   PrimWork pw;
   DevBuf w_flag, w_rank, w_a, w_b, w_c, w_cnt;
+  DevBuf w_gate;   // i32: the split counts confirm the guessed complete tree
   HostBuf h_cnt;
   HostBuf h_up;    // pinned staging for small host->device uploads
   std::vector<int64_t> s_off;   // host offsets of the split-cell lists per depth

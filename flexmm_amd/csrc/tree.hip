@@ -180,17 +180,28 @@ __global__ __launch_bounds__(1024) void k_split_pos(const uint64_t* __restrict__
 
 // The host's share of the split counts: per depth d <= kMaxLevels the sum of
 // the kSplitRows rows, and the non-finite flag, into words 0 .. kMaxLevels + 1
-// of the host mailbox.
+// of the host mailbox.  With gate: *gate = whether they describe the complete
+// tree of depth D (every cell of depths < D split, none deeper, all points
+// finite), the test the host makes once it has the words.
 __global__ __launch_bounds__(64) void k_split_counts(const int* __restrict__ rows,
                                                      const int* __restrict__ bad,
-                                                     uint64_t* __restrict__ mbox, uint32_t seq) {
+                                                     uint64_t* __restrict__ mbox, uint32_t seq,
+                                                     int D, int* __restrict__ gate) {
   const int d = threadIdx.x;
+  int sum = 0;
   if (d <= kMaxLevels) {
-    int sum = 0;
     for (int r = 0; r < kSplitRows; ++r) sum += rows[r * (kMaxLevels + 1) + d];
     mailbox_put(mbox, d, seq, sum);
   } else if (d == kMaxLevels + 1) {
-    mailbox_put(mbox, d, seq, *bad);
+    sum = *bad;
+    mailbox_put(mbox, d, seq, sum);
+  }
+  if (gate) {
+    bool ok = true;
+    if (d < D) ok = sum == (1 << (2 * d));
+    else if (d < kMaxLevels || d == kMaxLevels + 1) ok = sum == 0;
+    const bool all = __ballot(!ok) == 0ull;
+    if (d == 0) *gate = all ? 1 : 0;
   }
 }
 
@@ -813,9 +824,10 @@ constexpr int kFullMaxDepth = 15;   // depth-D cell index tables of at most 4^15
 // six cells.
 __global__ __launch_bounds__(BS) void k_full_starts(const uint64_t* __restrict__ keys,
                                                     const int2* __restrict__ sc, int n, int D,
-                                                    int* __restrict__ start) {
+                                                    int* __restrict__ start,
+                                                    const int* __restrict__ gate) {
   const int i = blockIdx.x * BS + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (gate && *gate == 0)) return;
   const int2 si = sc[i];
   // loaded with sc, not behind its test
   const uint64_t k = keys[i], kp = keys[i > 0 ? i - 1 : 0];
@@ -839,7 +851,7 @@ __device__ __forceinline__ int full_pos(const int* __restrict__ start,
   return v >= 0 ? v : (int)lower_bound_u64(keys, n, leaf << (64 - 2 * D));
 }
 
-__device__ __forceinline__ int full_levoff(int d) { return (int)((((int64_t)1 << (2 * d)) - 1) / 3); }
+__host__ __device__ __forceinline__ int full_levoff(int d) { return (int)((((int64_t)1 << (2 * d)) - 1) / 3); }
 
 // Every node array of the complete tree of depth D (levels 0 .. D, level d =
 // the 4^d cells in Morton order from levoff(d) = (4^d - 1) / 3).  Sixteen
@@ -855,7 +867,8 @@ __global__ __launch_bounds__(BS) void k_full_nodes(
     int2* __restrict__ crange, double* __restrict__ xmid, double* __restrict__ ymid,
     int* __restrict__ row, int* __restrict__ m2l, int* __restrict__ evalid,
     int* __restrict__ evalnode, int* __restrict__ upnodes, int2* __restrict__ rowinfo,
-    int* __restrict__ levoff) {
+    int* __restrict__ levoff, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   const int64_t tid = (int64_t)blockIdx.x * BS + threadIdx.x;
   const int g = (int)(tid / kFullLanes), j = (int)(tid % kFullLanes);
   if (tid <= D + 1) levoff[tid] = full_levoff((int)tid);   // levels 0 .. D and the end
@@ -935,33 +948,15 @@ __global__ __launch_bounds__(BS) void k_full_nodes(
 
 // Node arrays of a complete tree of depth D from the sorted points (keys and
 // the common depths sc of neighbouring keys); see k_full_nodes.
-static void tree_build_full(Tree& t, int D, int npts, hipStream_t s) {
-  const int L = D + 1;
-  t.L = L;
-  t.lev.assign(L, TreeLevel());
-  std::vector<int> levoff(L + 1, 0);
-  for (int d = 0; d <= L; ++d) levoff[d] = (int)((((int64_t)1 << (2 * d)) - 1) / 3);
-  for (int d = 0; d < L; ++d) {
-    t.lev[d].node_off = levoff[d];
-    t.lev[d].n_nodes = (int64_t)1 << (2 * d);
-    t.lev[d].width = std::ldexp(t.w0, -d);
-  }
-  const int NN = levoff[L];
-  t.n_nodes = NN;
-  const int g0 = levoff[2];
-  t.n_rows = NN - g0;
-  t.row_off.assign(L + 1, 0);
-  for (int d = 0; d <= L; ++d) t.row_off[d] = d <= 2 ? 0 : levoff[d] - g0;
-  t.n_eval = (int64_t)1 << (2 * D);
-  t.n_up = t.n_eval;
-  t.nidx_off.assign(kMaxLevels, -1);
-  t.levoff.ensure((L + 1) * sizeof(int));   // written by k_full_nodes
-  const int64_t ncell = (int64_t)1 << (2 * D);
-  t.w_flag.ensure((ncell + 1) * sizeof(int));
-  int* start = t.w_flag.as<int>();
-  hipLaunchKernelGGL(k_full_starts, dim3(grid_for(npts)), dim3(BS), 0, s, t.keys.as<uint64_t>(),
-                     t.w_a.as<int2>(), npts, D, start);
-  FMM_LAUNCH_CHECK();
+// The complete tree of depth D, in three parts: its buffers, its node arrays
+// (k_full_starts, k_full_nodes; with gate, the kernels do nothing unless
+// *gate is set when they run) and the host's level tables.
+static void tree_full_buffers(Tree& t, int D) {
+  const int NN = full_levoff(D + 1);
+  const int64_t n_eval = (int64_t)1 << (2 * D);
+  const int64_t n_rows = NN - full_levoff(2);
+  t.levoff.ensure((D + 2) * sizeof(int));   // written by k_full_nodes
+  t.w_flag.ensure((n_eval + 1) * sizeof(int));
   t.code.ensure(NN * sizeof(uint64_t));
   t.nlevel.ensure(NN * sizeof(int));
   t.bot.ensure(NN * sizeof(int));
@@ -976,9 +971,17 @@ static void tree_build_full(Tree& t, int D, int npts, hipStream_t s) {
   t.row.ensure(NN * sizeof(int));
   t.m2l.ensure((int64_t)NN * kNumM2L * sizeof(int));
   t.evalid.ensure(NN * sizeof(int));
-  t.evalnode.ensure((t.n_eval + 1) * sizeof(int));
-  t.upnodes.ensure((t.n_up + 1) * sizeof(int));
-  t.rowinfo.ensure((t.n_rows + 1) * sizeof(int2));
+  t.evalnode.ensure((n_eval + 1) * sizeof(int));
+  t.upnodes.ensure((n_eval + 1) * sizeof(int));
+  t.rowinfo.ensure((n_rows + 1) * sizeof(int2));
+}
+
+static void tree_full_kernels(Tree& t, int D, int npts, hipStream_t s, const int* gate) {
+  const int NN = full_levoff(D + 1);
+  int* start = t.w_flag.as<int>();
+  hipLaunchKernelGGL(k_full_starts, dim3(grid_for(npts)), dim3(BS), 0, s, t.keys.as<uint64_t>(),
+                     t.w_a.as<int2>(), npts, D, start, gate);
+  FMM_LAUNCH_CHECK();
   // levels 0 and 1 have no M2L lists (their entries are never read), as in
   // the general build
   hipLaunchKernelGGL(k_full_nodes, dim3(grid_for((int64_t)NN * kFullLanes)), dim3(BS), 0, s, D, NN,
@@ -988,8 +991,32 @@ static void tree_build_full(Tree& t, int D, int npts, hipStream_t s) {
                      t.parent.as<int>(), t.child.as<int>(), t.coll.as<int>(), t.crange.as<int2>(),
                      t.xmid.as<double>(), t.ymid.as<double>(), t.row.as<int>(), t.m2l.as<int>(),
                      t.evalid.as<int>(), t.evalnode.as<int>(), t.upnodes.as<int>(),
-                     t.rowinfo.as<int2>(), t.levoff.as<int>());
+                     t.rowinfo.as<int2>(), t.levoff.as<int>(), gate);
   FMM_LAUNCH_CHECK();
+}
+
+static void tree_full_host(Tree& t, int D) {
+  const int L = D + 1;
+  t.L = L;
+  t.lev.assign(L, TreeLevel());
+  for (int d = 0; d < L; ++d) {
+    t.lev[d].node_off = full_levoff(d);
+    t.lev[d].n_nodes = (int64_t)1 << (2 * d);
+    t.lev[d].width = std::ldexp(t.w0, -d);
+  }
+  const int NN = full_levoff(L);
+  t.n_nodes = NN;
+  const int g0 = full_levoff(2);
+  t.n_rows = NN - g0;
+  t.row_off.assign(L + 1, 0);
+  for (int d = 0; d <= L; ++d) t.row_off[d] = d <= 2 ? 0 : full_levoff(d) - g0;
+  t.n_eval = (int64_t)1 << (2 * D);
+  t.n_up = t.n_eval;
+  t.nidx_off.assign(kMaxLevels, -1);
+  t.complete = true;
+  t.full_D = D;
+  ++t.build_id;
+  t.built = true;
 }
 
 // ---------------------------------------------------------------------------
@@ -1069,13 +1096,27 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   FMM_LAUNCH_CHECK();
   // per depth, then the split positions; then the non-finite flag
   int64_t dcount[kMaxLevels + 2];
+  // A complete tree of the previous build's depth is built at once, gated on
+  // the device by the split counts (its kernels then follow without waiting
+  // for the host to read them); that needs its buffers to leave the split
+  // flags in w_flag (still needed otherwise) where they are.
+  const int specD = t.full_D;
+  const bool spec = t.allow_full && specD >= 2 && specD <= kFullMaxDepth &&
+                    t.w_flag.bytes >= ((((size_t)1 << (2 * specD)) + 1) * sizeof(int));
   {
     HostMailbox& mb = t.pw.mbox;
     mb.ensure();
     const uint32_t seq = mb.next();
+    int* gate = nullptr;
+    if (spec) {
+      t.w_gate.ensure(sizeof(int));
+      gate = t.w_gate.as<int>();
+      tree_full_buffers(t, specD);
+    }
     hipLaunchKernelGGL(k_split_counts, dim3(1), dim3(64), 0, s, cnt + CNT_SPLIT, cnt + CNT_BAD,
-                       mb.dev, seq);
+                       mb.dev, seq, specD, gate);
     FMM_LAUNCH_CHECK();
+    if (spec) tree_full_kernels(t, specD, nn, s, gate);
     mb.take(kMaxLevels + 2, seq, s, dcount);
   }
   FMM_REQUIRE(dcount[kMaxLevels + 1] == 0, FMM_E_ARG,
@@ -1094,13 +1135,15 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
     bool complete = true;
     for (int d = 0; d <= L - 2 && complete; ++d) complete = dcount[d] == ((int64_t)1 << (2 * d));
     if (complete) {
-      tree_build_full(t, L - 1, nn, s);
-      t.complete = true;
-      ++t.build_id;
-      t.built = true;
+      if (!(spec && L - 1 == specD)) {   // else built already
+        tree_full_buffers(t, L - 1);
+        tree_full_kernels(t, L - 1, nn, s, nullptr);
+      }
+      tree_full_host(t, L - 1);
       return;
     }
   }
+  t.full_D = 0;
 
   // compacted split positions and the split-cell lists per depth (S_ref)
   exclusive_scan_i32(flag, rank, n, nullptr, t.pw, s);

@@ -177,3 +177,29 @@ def test_complete_tree_build_equals_general_build(case):
     f.build_tree()
     n = H.compare_trees_fast(f, _oracle_tree(x, y, ncut, f.bbox))
     print(case, "levels", f.tree.levels, "nodes", n)
+
+
+def test_tree_sequence_on_one_plan():
+    """Consecutive builds on one plan's tree: a complete tree of the previous
+    build's depth is launched before the host reads the split counts (its
+    kernels gated on the device by those counts); a build of another depth, an
+    adaptive build and a build after it take their own paths.  Every tree
+    equals the oracle's."""
+    seq = [
+        ("uniform_100k_ncut64", *H.uniform(100_000, 1)[:2], 64, None),
+        ("uniform_100k_ncut64 again", *H.uniform(100_000, 1)[:2], 64, None),
+        ("uniform_300k_ncut40", *H.uniform(300_000, 12)[:2], 40, [0, 1, 0, 1]),
+        ("clustered_30k", *H.clustered(30000, 5, 0.01, 3)[:2], 50, [0, 1, 0, 1]),
+        ("uniform_100k_ncut64 after", *H.uniform(100_000, 1)[:2], 64, None),
+        ("uniform_50k_ncut200", *H.uniform(50_000, 13)[:2], 200, [0, 1, 0, 1]),
+    ]
+    plans = set()
+    for name, x, y, ncut, bbox in seq:
+        print("build", name, flush=True)
+        f = _build(x, y, ncut, bbox=bbox)
+        plans.add(f.handle.value)
+        ora = _oracle_tree(x, y, ncut, f.bbox)
+        n = H.compare_canonical(H.gpu_canonical(f), ora.canonical())
+        assert n == sum(l.n_node for l in ora.Levels), name
+        f.close()
+    assert len(plans) == 1   # every build reused the one pooled plan
