@@ -69,6 +69,7 @@ struct Tree {
   PrimWork pw;
   DevBuf w_flag, w_rank, w_a, w_b, w_c, w_cnt;
   DevBuf w_gate;   // i32: the split counts confirm the guessed complete tree
+  DevBuf w_bad;    // i32 per key block: it holds a non-finite coordinate
   HostBuf h_cnt;
   HostBuf h_up;    // pinned staging for small host->device uploads
   std::vector<int64_t> s_off;   // host offsets of the split-cell lists per depth

@@ -80,8 +80,9 @@ __global__ __launch_bounds__(KEY_BS) void k_point_keys(const double* __restrict_
                                                    KeyScale ks, uint64_t* __restrict__ keys,
                                                    uint32_t* __restrict__ vals,
                                                    double2* __restrict__ xy,
-                                                   int* __restrict__ nonfinite, SortFirstRound fr,
-                                                   int with_hist) {
+                                                   int* __restrict__ bad_blk,
+                                                   int* __restrict__ cnt, int ncnt,
+                                                   SortFirstRound fr, int with_hist) {
   static_assert(kKeyPer * KEY_BS == kSortChunk, "one chunk per block");
   __shared__ int hist[256];
   if (with_hist && threadIdx.x < 256) hist[threadIdx.x] = 0;
@@ -109,11 +110,13 @@ __global__ __launch_bounds__(KEY_BS) void k_point_keys(const double* __restrict_
       if (with_hist) atomicAdd(&hist[(int)(key >> 56)], 1);
     }
   }
-  if (bad) atomicOr(nonfinite, 1);
-  if (with_hist) {
-    __syncthreads();
-    sort_first_round_block(fr, blockIdx.x, hist);
-  }
+  // every block writes its own flag (no zeroed counter to race with), and
+  // block 0 zeroes the build's counters, which only later kernels touch
+  const int any_bad = __syncthreads_or(bad);
+  if (threadIdx.x == 0) bad_blk[blockIdx.x] = any_bad;
+  if (blockIdx.x == 0)
+    for (int j = threadIdx.x; j < ncnt; j += KEY_BS) cnt[j] = 0;
+  if (with_hist) sort_first_round_block(fr, blockIdx.x, hist);
 }
 
 // --- refinement: split cells from the sorted keys ----------------------------
@@ -184,18 +187,19 @@ __global__ __launch_bounds__(1024) void k_split_pos(const uint64_t* __restrict__
 // tree of depth D (every cell of depths < D split, none deeper, all points
 // finite), the test the host makes once it has the words.
 __global__ __launch_bounds__(64) void k_split_counts(const int* __restrict__ rows,
-                                                     const int* __restrict__ bad,
+                                                     const int* __restrict__ bad_blk, int nblk,
                                                      uint64_t* __restrict__ mbox, uint32_t seq,
                                                      int D, int* __restrict__ gate) {
   const int d = threadIdx.x;
   int sum = 0;
-  if (d <= kMaxLevels) {
+  if (d <= kMaxLevels)
     for (int r = 0; r < kSplitRows; ++r) sum += rows[r * (kMaxLevels + 1) + d];
-    mailbox_put(mbox, d, seq, sum);
-  } else if (d == kMaxLevels + 1) {
-    sum = *bad;
-    mailbox_put(mbox, d, seq, sum);
-  }
+  // the key kernel's per-block non-finite flags, or-ed over the wave
+  int b = 0;
+  for (int j = d; j < nblk; j += 64) b |= bad_blk[j];
+  const int any_bad = __ballot(b != 0) != 0ull;
+  if (d == kMaxLevels + 1) sum = any_bad;
+  if (d <= kMaxLevels + 1) mailbox_put(mbox, d, seq, sum);
   if (gate) {
     bool ok = true;
     if (d < D) ok = sum == (1 << (2 * d));
@@ -1051,14 +1055,16 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
   t.h_cnt.ensure(CNT_ALL * sizeof(int));
   int* cnt = t.w_cnt.as<int>();
   int* hc = t.h_cnt.as<int>();
-  FMM_HIP(hipMemsetAsync(cnt, 0, CNT_ALL * sizeof(int), s));
+  const int nkey_blk = (int)ceil_div(n, kSortChunk);
+  t.w_bad.ensure(nkey_blk * sizeof(int));
   const int sort_T = ncut < kLocalSortMax ? ncut : kLocalSortMax;
   const bool first_round = n > sort_T;
   const SortFirstRound fr = first_round ? sort_first_round(t.pw, n) : SortFirstRound{};
   hipLaunchKernelGGL(k_point_keys, dim3((unsigned)ceil_div(n, kSortChunk)), dim3(KEY_BS), 0, s, x, y,
                      nn, xmin, ymin, w0, key_scale(xmin, ymin, w0),
                      first_round ? fr.keys : keys, first_round ? fr.vals : perm,
-                     t.xy.as<double2>(), cnt + CNT_BAD, fr, first_round ? 1 : 0);
+                     t.xy.as<double2>(), t.w_bad.as<int>(), cnt, (int)CNT_ALL, fr,
+                     first_round ? 1 : 0);
   FMM_LAUNCH_CHECK();
   // Segments of at most Ncutoff points that end in the last radix round sit
   // at the deepest radix depth D, which no cell of the tree goes below (2:1
@@ -1113,8 +1119,8 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
       gate = t.w_gate.as<int>();
       tree_full_buffers(t, specD);
     }
-    hipLaunchKernelGGL(k_split_counts, dim3(1), dim3(64), 0, s, cnt + CNT_SPLIT, cnt + CNT_BAD,
-                       mb.dev, seq, specD, gate);
+    hipLaunchKernelGGL(k_split_counts, dim3(1), dim3(64), 0, s, cnt + CNT_SPLIT,
+                       t.w_bad.as<int>(), nkey_blk, mb.dev, seq, specD, gate);
     FMM_LAUNCH_CHECK();
     if (spec) tree_full_kernels(t, specD, nn, s, gate);
     mb.take(kMaxLevels + 2, seq, s, dcount);

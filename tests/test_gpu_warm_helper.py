@@ -110,9 +110,10 @@ def test_nonfinite_sources_are_refused():
     from flexmm_amd.kifmm2d import FMM, the_help
     the_help.clear()
     x, y, _ = H.uniform(5000, 203)
-    for bad in (np.nan, np.inf, -np.inf):
+    # points 1234 and 4500 lie in different blocks of the key kernel
+    for bad, at in ((np.nan, 1234), (np.inf, 1234), (-np.inf, 1234), (np.nan, 4500)):
         xb = x.copy()
-        xb[1234] = bad
+        xb[at] = bad
         with pytest.raises(_lib.FMMError) as ei:
             FMM(xb, y, kernels.laplace, 64, 32, bbox=[0, 1, 0, 1])
         assert ei.value.code == _lib.E_ARG
