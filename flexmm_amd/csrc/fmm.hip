@@ -19,6 +19,7 @@
 #include <cstring>
 #include <exception>
 #include <mutex>
+#include <type_traits>
 
 #include "../../include/flexmm_amd.h"
 
@@ -1910,32 +1911,37 @@ struct Target {
   uint32_t dmin = 1u;
   double dip = 0.0;   // double layer: sum (q.d)/d^2 (Laplace dipole sources)
   // a Laplace source with charge w and dipole moment (qx, qy)
-  // (examples/pure_kifmm_laplace_dipoles.py:32-40)
+  // (examples/pure_kifmm_laplace_dipoles.py:32-40).  GUARD false: the source
+  // cannot coincide with the target (it lies outside the target's own box),
+  // so the coincidence test and its bookkeeping are left out.
+  template <bool GUARD = true>
   __device__ __forceinline__ void pair_dip(double sx, double sy, double w, double qx, double qy,
                                            const double2* __restrict__ tab) {
     const double dx = x - sx;
     const double dy = y - sy;
     const double d2 = __builtin_fma(dy, dy, dx * dx);
-    const bool z = d2 == 0.0;
-    if (CHECK) {
+    const bool z = GUARD && d2 == 0.0;
+    if constexpr (GUARD && CHECK) {
       w = z ? 0.0 : w;
       qx = z ? 0.0 : qx;
       qy = z ? 0.0 : qy;
-    } else {
+    } else if constexpr (GUARD) {
       dmin = min(dmin, (uint32_t)(__builtin_bit_cast(uint64_t, d2) >> 32));
     }
     u.add(w, d2, tab, kp);
     dip = __builtin_fma(__builtin_fma(qx, dx, qy * dy), rcp_nr(z ? 1.0 : d2), dip);
   }
-  // one source (or one equivalent-density point) at (sx, sy) with weight w
+  // one source (or one equivalent-density point) at (sx, sy) with weight w;
+  // GUARD as for pair_dip
+  template <bool GUARD = true>
   __device__ __forceinline__ void pair(double sx, double sy, double w,
                                        const double2* __restrict__ tab) {
     const double dx = x - sx;
     const double dy = y - sy;
     double d2 = __builtin_fma(dy, dy, dx * dx);
-    const bool z = d2 == 0.0;
-    if (CHECK) w = z ? 0.0 : w;   // coincident source skipped (helpers.py:214)
-    else dmin = min(dmin, (uint32_t)(__builtin_bit_cast(uint64_t, d2) >> 32));
+    const bool z = GUARD && d2 == 0.0;
+    if constexpr (GUARD && CHECK) w = z ? 0.0 : w;   // coincident source skipped (helpers.py:214)
+    else if constexpr (GUARD) dmin = min(dmin, (uint32_t)(__builtin_bit_cast(uint64_t, d2) >> 32));
     if constexpr (KER == KER_LAPLACE) {
       u.add(w, d2, tab, kp);
       if (NOUT == 3) {
@@ -2121,10 +2127,11 @@ __device__ __forceinline__ void eval_few(
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
-  for (int qn = 0; qn < 9 && (DIP || scap == 0); ++qn) {
-    const int b = __builtin_amdgcn_readlane(CL.b, qn);
-    const int e = __builtin_amdgcn_readlane(CL.e, qn);
-    if (e <= b) continue;
+  // colleague 4 is the target's own box, the only one that can hold a
+  // source at the target (a point and its box are found by the same
+  // midpoint tests for sources and targets)
+  auto colleague = [&](auto guard, int b, int e) {
+    constexpr bool G = decltype(guard)::value;
     int j = b + F.sg;
     if constexpr (!DIP) {
       // four sources per lane in flight: the loads are per lane here (lanes
@@ -2134,18 +2141,25 @@ __device__ __forceinline__ void eval_few(
 #pragma unroll
         for (int u = 0; u < 4; ++u) s4[u] = src[j + u * F.ng];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) T.pair(s4[u].x, s4[u].y, s4[u].z, tab);
+        for (int u = 0; u < 4; ++u) T.template pair<G>(s4[u].x, s4[u].y, s4[u].z, tab);
       }
     }
     for (; j < e; j += F.ng) {
       const double4 sj = src[j];
       if constexpr (DIP) {
         const double2 q = dip[j];
-        T.pair_dip(sj.x, sj.y, sj.z, q.x, q.y, tab);
+        T.template pair_dip<G>(sj.x, sj.y, sj.z, q.x, q.y, tab);
       } else {
-        T.pair(sj.x, sj.y, sj.z, tab);
+        T.template pair<G>(sj.x, sj.y, sj.z, tab);
       }
     }
+  };
+  for (int qn = 0; qn < 9 && (DIP || scap == 0); ++qn) {
+    const int b = __builtin_amdgcn_readlane(CL.b, qn);
+    const int e = __builtin_amdgcn_readlane(CL.e, qn);
+    if (e <= b) continue;
+    if (qn == 4) colleague(std::true_type{}, b, e);
+    else colleague(std::false_type{}, b, e);
   }
   double u = slot_sum(DIP ? __builtin_fma(kInv2Pi, T.dip, T.u.result()) : T.u.result() + far,
                       F.lg);
@@ -2225,32 +2239,40 @@ __attribute__((amdgpu_waves_per_eu(KER == KER_MODHELM && NOUT == 3 ? 3 : 1))) vo
       T.gx = L.gx;
       T.gy = L.gy;
     }
-    // P2P over the leaf's colleagues: helpers.py:180-217
+    // P2P over the leaf's colleagues: helpers.py:180-217.  Only colleague 4,
+    // the target's own box, can hold a source at the target (a point and its
+    // box are found by the same midpoint tests for sources and targets), so
+    // only its pairs test for coincidence.
     const CollLanes CL = coll_lanes(g, coll, crange);
-    for (int qn = 0; qn < 9; ++qn) {
-      const int b = __builtin_amdgcn_readlane(CL.b, qn);
-      const int e = __builtin_amdgcn_readlane(CL.e, qn);
-      if (e <= b) continue;
+    auto colleague = [&](auto guard, int b, int e) {
+      constexpr bool G = decltype(guard)::value;
       int j = b;
       if constexpr (DIP) {
         for (; j < e; ++j) {
           const double4 sj = src[j];
           const double2 q = dip[j];
-          T.pair_dip(sj.x, sj.y, sj.z, q.x, q.y, tab);
+          T.template pair_dip<G>(sj.x, sj.y, sj.z, q.x, q.y, tab);
         }
-        continue;
+        return;
       }
       for (; j + 4 <= e; j += 4) {
         double4 s4[4];
-  #pragma unroll
+#pragma unroll
         for (int u = 0; u < 4; ++u) s4[u] = src[j + u];
-  #pragma unroll
-        for (int u = 0; u < 4; ++u) T.pair(s4[u].x, s4[u].y, s4[u].z, tab);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) T.template pair<G>(s4[u].x, s4[u].y, s4[u].z, tab);
       }
       for (; j < e; ++j) {
         const double4 sj = src[j];
-        T.pair(sj.x, sj.y, sj.z, tab);
+        T.template pair<G>(sj.x, sj.y, sj.z, tab);
       }
+    };
+    for (int qn = 0; qn < 9; ++qn) {
+      const int b = __builtin_amdgcn_readlane(CL.b, qn);
+      const int e = __builtin_amdgcn_readlane(CL.e, qn);
+      if (e <= b) continue;
+      if (qn == 4) colleague(std::true_type{}, b, e);
+      else colleague(std::false_type{}, b, e);
     }
     if (act) {
       const int64_t o = tperm[t];
