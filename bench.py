@@ -53,8 +53,14 @@ def measured_traffic(what):
 
 
 def profiled_avg_us(kernel_prefix):
-    """Average duration of a kernel in the committed rocprofv3 --stats summary
-    of this workload (profiles/laplace_1M_fp64_kernels.txt); None if absent."""
+    """Average duration of a kernel in the committed rocprofv3 summary of this
+    workload (profiles/laplace_1M_fp64_kernels.txt): over the evented steps,
+    whose kernel times this script measures, when the summary gives it, else
+    over every call; None if absent."""
+    key = "evented_mean_us="
+    for line in _profile_lines("laplace_1M_fp64_kernels.txt"):
+        if key in line and line.strip().lstrip("# ").startswith(kernel_prefix):
+            return float(line.split(key)[1].split()[0])
     for line in _profile_lines("laplace_1M_fp64_kernels.txt"):
         if line.startswith(kernel_prefix):
             parts = line[len(kernel_prefix):].split()
@@ -672,8 +678,9 @@ def main():
             "profile_avg_us": prof_us,
             "frac_from_profile": (round(f_eval / (prof_us * 1e-6) / 1e12 / FP64_PEAK_TFLOPS, 4)
                                   if prof_us else None),
-            "profile_note": "average duration in profiles/laplace_1M_fp64_kernels.txt "
-                            "(rocprofv3 --kernel-trace --stats of bench.py --only fp64)",
+            "profile_note": "mean duration over the evented steps in "
+                            "profiles/laplace_1M_fp64_kernels.txt (rocprofv3 --kernel-trace "
+                            "--stats of bench.py --only fp64, the same steps as kernel_ms)",
             "traffic": measured_traffic("k_eval"),
             "traffic_note": "HBM bytes per launch of k_eval from rocprofv3 FETCH_SIZE/WRITE_SIZE "
                             "(profiles/laplace_1M_fp64_traffic.txt); algorithmic bytes ~6.9e7 "
