@@ -2762,20 +2762,24 @@ __device__ __forceinline__ void stokes_item(
     // A coincident pair (d2 = 0, so dx = dy = 0) continues with d2 = 1: every
     // term with a factor dx or dy then vanishes exactly, the stresslet
     // pressure -g/d2 + g/d2 cancels, and only the log term is zeroed (under
-    // CHECK; otherwise the target is flagged through dmin).  Forces and
+    // CHECK; otherwise the target is flagged through dmin).  Only colleague
+    // 4, the target's own box, can hold a source at the target (a point and
+    // its box are found by the same midpoint tests for sources and targets):
+    // the other colleagues' pairs skip the test (guard false).  Forces and
     // stresslet densities stay in the wave-uniform registers they load into.
-    auto pairS = [&](const double4& sj, int jj) {
+    auto pairS = [&](auto guard, const double4& sj, int jj) {
+      constexpr bool G = decltype(guard)::value;
       const double dx = x - sj.x, dy = y - sj.y;
       const double dxx = dx * dx;
       double d2 = __builtin_fma(dy, dy, dxx);
-      const bool z = d2 == 0.0;
-      if (!CHECK) dmin = __builtin_fmin(dmin, d2);
-      d2 = z ? 1.0 : d2;
+      const bool z = G && d2 == 0.0;
+      if (G && !CHECK) dmin = __builtin_fmin(dmin, d2);
+      if (G) d2 = z ? 1.0 : d2;
       const double id2 = rcp_nr(d2);
       double le_, lt_;
       log_split(d2, tab, le_, lt_);
       double lg = __builtin_fma(le_, kLn2, lt_);
-      if (CHECK) lg = z ? 0.0 : lg;
+      if (G && CHECK) lg = z ? 0.0 : lg;
       lfx = __builtin_fma(sj.z, lg, lfx);
       lfy = __builtin_fma(sj.w, lg, lfy);
       const double rf = __builtin_fma(dx, sj.z, dy * sj.w) * id2;
@@ -2796,17 +2800,21 @@ __device__ __forceinline__ void stokes_item(
         dp += __builtin_fma(-gg, id2, w2);
       }
     };
-    int j = b + F.sg;
-    if constexpr (FEW) {   // four per-lane loads in flight
-      for (; j + 3 * F.ng < e; j += 4 * F.ng) {
-        double4 s4[4];
+    auto colleague = [&](auto guard) {
+      int j = b + F.sg;
+      if constexpr (FEW) {   // four per-lane loads in flight
+        for (; j + 3 * F.ng < e; j += 4 * F.ng) {
+          double4 s4[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) s4[u] = src[j + u * F.ng];
+          for (int u = 0; u < 4; ++u) s4[u] = src[j + u * F.ng];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) pairS(s4[u], j + u * F.ng);
+          for (int u = 0; u < 4; ++u) pairS(guard, s4[u], j + u * F.ng);
+        }
       }
-    }
-    for (; j < e; j += F.ng) pairS(src[j], j);
+      for (; j < e; j += F.ng) pairS(guard, src[j], j);
+    };
+    if (qn == 4) colleague(std::true_type{});
+    else colleague(std::false_type{});
   }
   u += kInv4Pi * (-0.5 * lfx + su);
   v += kInv4Pi * (-0.5 * lfy + sv);
