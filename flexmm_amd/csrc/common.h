@@ -134,7 +134,7 @@ __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + 
 // a new sequence number and the host takes each word once it carries that
 // number; no order between words is assumed.
 struct HostMailbox {
-  static constexpr int kWords = 64;
+  static constexpr int kWords = 128;
   uint64_t* host = nullptr;   // host address of the words
   uint64_t* dev = nullptr;    // the same words as kernels address them
   uint32_t seq = 0;

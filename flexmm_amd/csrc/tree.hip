@@ -1023,6 +1023,26 @@ static void tree_full_host(Tree& t, int D) {
   t.built = true;
 }
 
+// Device counters c[0 .. n) into the host mailbox, for read_counts.
+__global__ __launch_bounds__(128) void k_counts_to_mailbox(const int* __restrict__ c, int n,
+                                                           uint64_t* __restrict__ mbox,
+                                                           uint32_t seq) {
+  for (int i = threadIdx.x; i < n; i += 128) mailbox_put(mbox, i, seq, c[i]);
+}
+
+// host[0 .. n) = the device counters dev[0 .. n) once the work queued on s
+// before this call has run, without draining s (HostMailbox).
+static void read_counts(Tree& t, const int* dev, int n, int* host, hipStream_t s) {
+  HostMailbox& mb = t.pw.mbox;
+  mb.ensure();
+  const uint32_t seq = mb.next();
+  hipLaunchKernelGGL(k_counts_to_mailbox, dim3(1), dim3(128), 0, s, dev, n, mb.dev, seq);
+  FMM_LAUNCH_CHECK();
+  int64_t v[HostMailbox::kWords];
+  mb.take(n, seq, s, v);
+  for (int i = 0; i < n; ++i) host[i] = (int)v[i];
+}
+
 // ---------------------------------------------------------------------------
 void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmin, double ymin,
                 double w0, int ncut, hipStream_t s) {
@@ -1186,8 +1206,7 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
       hipLaunchKernelGGL(k_balance_check, dim3(grid_for(ne)), dim3(BS), 0, s,
                          t.sref_all.as<uint64_t>(), d_soff, L, cnt + CNT_R);
       FMM_LAUNCH_CHECK();
-      FMM_HIP(hipMemcpyAsync(hc + CNT_R, cnt + CNT_R, sizeof(int), hipMemcpyDeviceToHost, s));
-      FMM_HIP(hipStreamSynchronize(s));
+      read_counts(t, cnt + CNT_R, 1, hc + CNT_R, s);
       balanced = hc[CNT_R] == 0;
     }
   }
@@ -1225,8 +1244,7 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
       hipLaunchKernelGGL(k_unique_flags, dim3(grid_for(nc)), dim3(BS), 0, s, candk, nc, fl);
       FMM_LAUNCH_CHECK();
       exclusive_scan_i32(fl, rk, nc, cnt + CNT_UNIQ, t.pw, s);
-      FMM_HIP(hipMemcpyAsync(hc + CNT_UNIQ, cnt + CNT_UNIQ, sizeof(int), hipMemcpyDeviceToHost, s));
-      FMM_HIP(hipStreamSynchronize(s));
+      read_counts(t, cnt + CNT_UNIQ, 1, hc + CNT_UNIQ, s);
       sz[d] = hc[CNT_UNIQ];
       lists[d].ensure((sz[d] + 1) * sizeof(uint64_t));
       hipLaunchKernelGGL(k_unique_scatter, dim3(grid_for(nc)), dim3(BS), 0, s, candk, fl, rk, nc,
@@ -1319,9 +1337,7 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
                      d_soff2, d_roff, L, rcode, hasc, kind, cnt + CNT_PC);
   FMM_LAUNCH_CHECK();
   exclusive_scan_i32(hasc, rrank, nr, nullptr, t.pw, s);
-  FMM_HIP(hipMemcpyAsync(hc + CNT_PC, cnt + CNT_PC, kMaxLevels * sizeof(int),
-                         hipMemcpyDeviceToHost, s));
-  FMM_HIP(hipStreamSynchronize(s));
+  read_counts(t, cnt + CNT_PC, kMaxLevels, hc + CNT_PC, s);
   t.p_off.assign(L + 1, 0);
   for (int d = 0; d < L; ++d) t.p_off[d + 1] = t.p_off[d] + hc[CNT_PC + d];
   FMM_REQUIRE(hc[CNT_PC + L - 1] == 0, FMM_E_INTERNAL, "tree: deepest level has children");
@@ -1441,9 +1457,7 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
                      t.row.as<int>(), NN, f2);
   FMM_LAUNCH_CHECK();
   exclusive_scan_i32(f2, r2, off, cnt + CNT_UP, t.pw, s);
-  FMM_HIP(hipMemcpyAsync(hc + CNT_ROWS, cnt + CNT_ROWS, (CNT_N - CNT_ROWS) * sizeof(int),
-                         hipMemcpyDeviceToHost, s));
-  FMM_HIP(hipStreamSynchronize(s));
+  read_counts(t, cnt + CNT_ROWS, CNT_N - CNT_ROWS, hc + CNT_ROWS, s);
   t.n_rows = ngroups > 0 ? 4 * (int64_t)hc[CNT_ROWS] : 0;
   t.row_off.assign(L + 1, 0);
   for (int d = 0; d < L; ++d) t.row_off[d + 1] = t.row_off[d] + hc[CNT_LR + d];
