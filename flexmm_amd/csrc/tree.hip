@@ -1292,10 +1292,8 @@ void tree_build(Tree& t, const double* x, const double* y, int64_t n, double xmi
                          t.bal_pc.as<int>(), cnt + CNT_BAL);
       FMM_LAUNCH_CHECK();
       exclusive_scan_i32(t.bal_pc.as<int>(), t.bal_pc.as<int>(), nwords, nullptr, t.pw, s);
-      FMM_HIP(hipMemcpyAsync(hc + CNT_BAL, cnt + CNT_BAL, kMaxLevels * sizeof(int),
-                             hipMemcpyDeviceToHost, s));
+      read_counts(t, cnt + CNT_BAL, kMaxLevels, hc + CNT_BAL, s);
     }
-    FMM_HIP(hipStreamSynchronize(s));
     for (int d = 1; d <= DM; ++d) sz[d] = hc[CNT_BAL + d];
     soff.assign(L + 1, 0);
     for (int d = 0; d < L; ++d) soff[d + 1] = soff[d] + (d <= L - 2 ? sz[d] : 0);
