@@ -7,7 +7,9 @@
  * are DEVICE pointers (for example torch.Tensor.data_ptr() of a CUDA/HIP
  * tensor); `stream` is a hipStream_t (torch.cuda.current_stream().cuda_stream)
  * or NULL for the default stream.  Work is enqueued on that stream; the tree
- * build synchronises it to size its per-level arrays.
+ * build waits on the host for the few counts it sizes its per-level arrays
+ * by, which its kernels hand over through pinned host memory as they are
+ * made, so the stream keeps running (it is not drained).
  *
  * Memory: a plan owns its device buffers (tree, expansions, target scratch).
  * They are allocated with hipMalloc when a call first needs them and grow
